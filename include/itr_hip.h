@@ -1,0 +1,165 @@
+/*
+ * itr_hip.h -- C ABI of the MI355X (gfx950) batch evaluator for the inverse terrain
+ * engine's f32 shallow-water candidate scoring.
+ *
+ * This is the drop-in boundary for the reference's population batch backend:
+ *   reference trait  BatchBackend<f32>        crates/itr-opt/src/evaluator.rs:131-135
+ *   reference impl   GpuSolver (wgpu/WGSL)    crates/itr-gpu/src/lib.rs:62-373
+ *   reference glue   Gpu(GpuSolver)           crates/itr-cli/src/backend.rs:44-64
+ * A Rust FFI crate binds these functions and implements BatchBackend<f32> with them.
+ *
+ * Conventions: plain pointers and sizes only; inputs are caller-owned and read only
+ * during the call (copied to the device); outputs go to caller-allocated buffers.
+ * A context is not thread-safe: use one per host thread.  Every entry point returns an
+ * itr_status; ITR_DECLINED means "not supported here" and maps to the reference's
+ * Err(...) decline, on which the evaluator falls back to the CPU for that level
+ * (evaluator.rs:231-237).  err/errlen receive a NUL-terminated message (may be NULL).
+ *
+ * Results are bitwise equal to the reference's CPU f32 fused kernel
+ * (crates/itr-hydro/src/kernel.rs:573-727 driven by solver.rs:346-660): state, step
+ * counts, t_end, monitor maxima and times, ledger rows and abort decisions.
+ */
+#ifndef ITR_HIP_H
+#define ITR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ITR_OK = 0,
+    ITR_DECLINED = 1, /* unsupported configuration: caller runs this level on the CPU */
+    ITR_EINVAL = 2,   /* invalid arguments */
+    ITR_EDEVICE = 3,  /* HIP runtime error */
+    ITR_ENOMEM = 4    /* device allocation failed */
+} itr_status;
+
+typedef struct itr_ctx itr_ctx;
+typedef struct itr_batch itr_batch;
+
+/* SolverParams (crates/itr-hydro/src/solver.rs:26-49).  Rain points are (t [s],
+ * rate [m/s]) of the piecewise-linear hyetograph (crates/itr-core/src/scenario.rs:527-589).
+ * locked_dt: baseline-locked dt schedule (solver.rs:41-44, 452-455) or NULL; a step
+ * whose locked dt fails the hard-CFL check ends the candidate with aborted = 4
+ * (the reference restarts it adaptively from the last sync point instead).
+ * max_steps: stop each candidate after this many steps (0 = no limit; test control). */
+typedef struct {
+    double duration_s, sync_interval_s, cfl, dt_max_s, h_dry, h_eps;
+    double infil_capacity; /* m/s */
+    double initial_depth;
+    double initial_stage; /* used when has_initial_stage */
+    double cfl_hard;
+    double inflow_scale; /* only 1.0 is supported (else ITR_DECLINED, backend.rs:53-55) */
+    const double* rain_t;
+    const double* rain_r;
+    const double* locked_dt;
+    int64_t n_locked;
+    int64_t max_steps;
+    int32_t n_rain;
+    int32_t has_initial_stage;
+    int32_t skip_dry; /* dry-row accounting rule of kernel.rs:605-612 (cell_updates) */
+    int32_t reserved;
+} itr_solver_params;
+
+/* One grid (a TerrainBase, crates/itr-hydro/src/terrain.rs:237-257) and the candidates
+ * run on it (PreparedTerrain overlays flattened as in crates/itr-gpu/src/lib.rs:177-194).
+ * Several grids of different shapes may be passed to one batch (fidelity ladder). */
+typedef struct {
+    int32_t nx, ny;
+    double dx, dy, g, z_ref;
+    int32_t crest;       /* FaceMode::Crest (coarse levels) */
+    int32_t wet_count;   /* TerrainBase::wet_count */
+    const float* wet;    /* [ny*nx] 1 = non-wall cell, 0 = wall (row-major, row 0 north) */
+    const float* gn2;    /* [ny*nx] g*n^2 rounded to f32 */
+    const int32_t* ghost; /* [2*nx + 2*ny] north[nx], south[nx], west[ny], east[ny];
+                             0 wall, 1 transmissive, 2 stage, 3 inflow (2, 3: declined) */
+    int32_t n_cand;
+    int32_t n_mon;
+    const float* dzx;    /* [n_cand][ny][nx+1] fx[0] per face (signed jump, or crest d_l) */
+    const float* dzy;    /* [n_cand][ny+1][nx] fy[0] */
+    const float* dzx_r;  /* crest: [n_cand][ny][nx+1] fx[1] (d_r); NULL for signed */
+    const float* dzy_r;  /* crest: [n_cand][ny+1][nx] fy[1] */
+    const double* z_level; /* [n_cand][ny*nx] level z incl. edits (initial_stage), or NULL */
+    const uint32_t* mon_cells; /* MonitorSet: sorted unique interior cells row*nx+col */
+    /* Exact guard early abort (Objective::guard_breach, crates/itr-core/src/objective.rs:132-144),
+     * evaluated on the device at every sync point.  baseline_max NULL: disabled. */
+    const double* baseline_max; /* [n_mon] */
+    const int32_t* guard_pos;   /* positions into mon_cells, in guard order */
+    int32_t n_guard;
+    int32_t reserved;
+    double guard_tol;
+} itr_grid;
+
+/* Per-candidate result: GpuRun (crates/itr-gpu/src/lib.rs:49-60) + BatchRun
+ * (evaluator.rs:118-125) + the RunSummary fields (crates/itr-core/src/model.rs:94-117). */
+typedef struct {
+    double t_end, volume_m3, dt_min, dt_max, dt_mean, lts, v0, rel_residual, abort_value;
+    uint64_t steps, cell_updates, rows_skipped;
+    int32_t bad;        /* Unhealthy (non-finite or beyond-roundoff negative depth) */
+    int32_t aborted;    /* 0 none, 1 guard, 2 sync callback, 3 unhealthy, 4 locked-dt check failed */
+    int32_t abort_cell; /* guard: offending cell (row*nx+col) */
+    int32_t abort_sync; /* sync index at the abort */
+    int32_t sync_count;
+    int32_t n_ledger;   /* ledger rows written */
+    double* monitor_max;  /* [n_mon] or NULL */
+    double* monitor_tmax; /* [n_mon] or NULL */
+    float* depth;         /* [ny*nx] final depth or NULL */
+    float* qx;            /* [ny*nx] or NULL (internal convention: +col east, +row south) */
+    float* qy;
+    double* ledger;       /* [max_ledger][8] rows (t, volume, rain, inflow, outflow, infil, fix, residual) */
+    int32_t max_ledger;
+    int32_t reserved;
+} itr_run_out;
+
+/* Observer::on_sync (crates/itr-core/src/model.rs:59-61) for one candidate, called on
+ * the calling thread whenever a candidate reaches a sync point (the candidate waits on
+ * the device until the callback returns).  Return 0 to continue or non-zero to abort
+ * the candidate (its result then reports aborted = 2 and *abort_value). */
+typedef int32_t (*itr_sync_fn)(void* user, int32_t grid, int32_t cand, double t, uint64_t step,
+                               uint32_t sync_index, const double* monitor_max, int32_t n_mon,
+                               double* abort_value);
+
+typedef struct {
+    double wall_s;       /* host wall time of the last itr_hip_batch_run */
+    double device_ms;    /* HIP-event time of the step loop on the library stream */
+    double cell_steps;   /* sum over candidates of steps * cells actually run */
+    uint64_t launches;   /* step-kernel launches */
+    uint64_t syncs;      /* host round trips */
+    int32_t band_rows, waves_per_task, steps_per_chunk, use_graph;
+} itr_stats;
+
+/* GpuSolver::new (lib.rs:85-124).  device: HIP device ordinal. */
+itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errlen);
+void itr_hip_destroy(itr_ctx* ctx);
+/* GpuSolver::adapter_info (lib.rs:126-128). */
+const char* itr_hip_device_info(itr_ctx* ctx);
+/* Tuning knobs: "band_rows" (rows per workgroup task), "steps_per_chunk" (step launches
+ * per host round trip), "use_graph" (0/1: capture the chunk in a hipGraph).  0 = default. */
+itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value);
+
+/* GpuSolver::run_batch (lib.rs:149-372) / BatchBackend::run (evaluator.rs:134):
+ * upload, run every candidate of every grid to its end (or abort), download.
+ * outs: one entry per candidate, grids concatenated in order. */
+itr_status itr_hip_run_batch(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                             itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen);
+
+/* The same, split so that inputs can stay resident in HBM across runs (serving loops,
+ * benchmarks): create uploads, run executes the step loop, results downloads. */
+itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                                itr_batch** out, char* err, size_t errlen);
+itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char* err, size_t errlen);
+itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, size_t errlen);
+/* Launch exactly n_steps step kernels (no sync callbacks; finished candidates idle) and
+ * wait for them: times a fixed number of steps of a resident batch.  itr_hip_batch_stats
+ * then reports the HIP-event time and the cell-steps actually run. */
+itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_t errlen);
+itr_status itr_hip_batch_stats(itr_batch* b, itr_stats* out);
+void itr_hip_batch_destroy(itr_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

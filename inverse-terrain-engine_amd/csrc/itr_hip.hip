@@ -1,0 +1,1556 @@
+// itr_hip.hip -- gfx950 batch evaluator for the f32 shallow-water candidate scoring,
+// behind the C ABI of include/itr_hip.h.
+//
+// Execution model
+//   * One step of every active candidate of every grid is ONE launch of itr_step_kernel.
+//     A workgroup owns a band of R full rows of one candidate and marches down it
+//     (register-resident sliding window in y); waves tile the row in x with 62 updated
+//     columns and one halo column on each side, exchanging neighbour values with DPP
+//     (no LDS traffic, no barrier inside the march).
+//   * The last workgroup of a candidate to finish a step (atomic ticket) runs that
+//     candidate's f64 control: ledger accounting in the reference's fixed order, CFL dt,
+//     sync clipping, exact hyetograph integral, guard abort, termination, and the f32
+//     constants of the next step.  The host only launches chunks of steps (optionally a
+//     captured hipGraph) and polls a status word per candidate between chunks.
+//   * State is ping-ponged per candidate (parity in its control block).
+//
+// HBM layout (per grid; P = row pitch in floats, multiple of 64):
+//   state  [cand][2][3][ny][P]  h, qx, qy (two copies)
+//   dzx    [cand][ny][P]        west-face bed jump of cell (col); col 0 = 0
+//   dzy    [cand][ny][P]        north-face bed jump of row (row); row 0 = 0
+//   dzx2/dzy2                   crest mode second rises, same layout
+//   wet, gn2, mon_map [ny][P]   shared by every candidate of the grid
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/itr_hip.h"
+#include "itr_numerics.h"
+
+using namespace itr;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kColsPerWave = 62;   // updated columns per wave (lanes 1..62)
+constexpr int kMaxWaves = 16;      // 1024 threads: rows up to 992 columns
+
+constexpr uint32_t F_DONE = 1u;
+constexpr uint32_t F_PAUSED = 2u;
+
+// Row partial results of one step, consumed by the candidate's control.
+struct RowAcc {
+    float infil, fix;      // FixedSum totals (real.rs:132-175) of the row
+    float bw, be;          // west / east boundary face mass flux (kernel.rs:436-450)
+    float vol, sum_s;      // sync steps: FixedSum of new depth and of the CFL rate
+    uint32_t wet;          // any new depth > 0 (kernel.rs:297)
+    uint32_t pad;
+};
+
+struct DevParams {
+    double duration, sync, cfl, dt_max, cfl_hard, infil_cap;
+    const double* rain_t;
+    const double* rain_r;
+    const double* locked;
+    int64_t n_locked;
+    int64_t max_steps;
+    int32_t n_rain;
+    uint32_t n_sync;
+    int32_t skip_dry;
+    int32_t has_cb;
+    int32_t max_ledger;
+    float h_dry, heps2, q_min;
+};
+
+struct DevGrid {
+    int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
+    size_t plane;              // ny * P
+    size_t plane_y;            // (ny + 1) * P: dzy rows 0..ny (rows 0 and ny are zero)
+    float* state;              // [ncand][2][3][plane]
+    const float* dzx;
+    const float* dzy;
+    const float* dzx2;
+    const float* dzy2;
+    const float* wet;
+    const float* gn2;
+    const int32_t* mon_map;
+    const uint8_t* band_mon;
+    const int32_t* ghost;
+    float* mon_max;            // [ncand][n_mon]
+    double* mon_tmax;          // [ncand][n_mon]
+    RowAcc* rowacc;            // [ncand][ny]
+    float* ybnd;               // [ncand][2][nx] top / bottom boundary y-face mass
+    float* sbuf;               // [ncand][plane] CFL rate at sync steps
+    double* ledger;            // [ncand][max_ledger][8]
+    const double* baseline_max;
+    const int32_t* guard_pos;
+    const uint32_t* mon_cells;
+    double guard_tol, dx, dy, g, area;
+    float g32, g_half32, inv_dx32, inv_dy32;
+};
+
+struct CandCtl {
+    float dt32, dtdx, dtdy, rain32, cap32;
+    int32_t hits_sync, parity, grid;
+    double t, t_new, dt, rain, dt_cfl, s_cur;
+    uint64_t step, cell_updates, rows_skipped;
+    uint32_t sync_index, flags, skipped_now, n_ledger;
+    double v0, acc_rain, inflow, outflow, infil, fix, dt_min, dt_max, dt_sum, lts;
+    int32_t abort_kind, abort_cell, abort_sync, local;
+    double abort_value;
+    int32_t smax_bits;
+    uint32_t bad, ticket, pad;
+};
+
+struct KArgs {
+    const DevGrid* grids;
+    const int4* tasks;         // (grid, local cand, band, -)
+    CandCtl* ctl;
+    const DevParams* prm;
+    uint32_t* status;          // [total cand] mirror of ctl flags for the host
+    int32_t ntasks;
+    int32_t pad;
+};
+
+// Dynamic LDS of the step kernel, in floats (offsets 16-byte aligned).
+struct LdsLayout { size_t infil, fix, bw, be, wet, red, acc, total; };
+__host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
+    auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
+    LdsLayout L;
+    L.infil = 0;
+    L.fix = al(L.infil + (size_t)R * nx);
+    L.bw = al(L.fix + (size_t)R * nx);
+    L.be = al(L.bw + R);
+    L.wet = al(L.be + R);
+    L.red = al(L.wet + R);
+    L.acc = al(L.red + 2 * kMaxWaves);
+    const size_t acc_n = (size_t)R * 64 > 96 ? (size_t)R * 64 : 96;   // >= 48 doubles of control scratch
+    L.total = al(L.acc + acc_n);
+    return L;
+}
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ float from_left(float x) {   // lane i <- lane i-1 (DPP wave_shr:1)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float x) {  // lane i <- lane i+1 (DPP wave_shl:1)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Pointers into device global memory (address space 1), so hot accesses compile to
+// global_load/global_store rather than flat instructions.
+#define ITR_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ ITR_GLOBAL T* gp(T* p) { return (ITR_GLOBAL T*)p; }
+template <class T> __device__ __forceinline__ const ITR_GLOBAL T* gp(const T* p) { return (const ITR_GLOBAL T*)p; }
+
+// Raw buffer access (V# with a byte bound): a scalar base + scalar row offset + 32-bit
+// lane offset, no 64-bit address arithmetic per access, and hardware bounds checking
+// (reads past `bytes` return 0, writes past it are dropped).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, size_t bytes) {
+    const uint32_t n = bytes > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)n, 0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ int bload_i(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+// Cell (row, col) of the old state with ghost cells built on the fly from the
+// boundary codes, as fill_ghosts does once per step (crates/itr-hydro/src/boundary.rs:12-87):
+// wall mirrors the normal discharge, transmissive copies, corners are zero.  code_n /
+// code_s: the lane's north / south codes; the west / east code is looked up per row.
+// Split in two so the loads can be issued a row ahead and the (value-dependent)
+// ghost transform applied only when the row is used.
+struct Cell { float h, qx, qy; int code_we; };
+__device__ __forceinline__ Cell fetch_cell(const ITR_GLOBAL float* __restrict__ H, const DevGrid& G,
+                                           const ITR_GLOBAL int32_t* __restrict__ ghost, int row, int col) {
+    const int nx = G.nx, ny = G.ny;
+    const int rr = min(max(row, 0), ny - 1);
+    const int cc = min(max(col, 0), nx - 1);
+    const size_t o = (size_t)rr * G.P + cc;
+    Cell v;
+    v.h = H[o];
+    v.qx = H[G.plane + o];
+    v.qy = H[2 * G.plane + o];
+    v.code_we = ghost[2 * nx + (col < 0 ? 0 : ny) + rr];
+    return v;
+}
+__device__ __forceinline__ Cell ghosted(Cell v, const DevGrid& G, int row, int col, int code_n, int code_s) {
+    const int nx = G.nx, ny = G.ny;
+    const bool rin = row >= 0 && row < ny;
+    const bool cin = col >= 0 && col < nx;
+    const bool wall_x = !cin && v.code_we == 0;
+    const bool wall_y = cin && !rin && (row < 0 ? code_n : code_s) == 0;
+    v.qx = wall_x ? -v.qx : v.qx;
+    v.qy = wall_y ? -v.qy : v.qy;
+    const bool corner = !rin && !cin;
+    v.h = corner ? 0.0f : v.h;
+    v.qx = corner ? 0.0f : v.qx;
+    v.qy = corner ? 0.0f : v.qy;
+    return v;
+}
+__device__ __forceinline__ Cell load_cell(const ITR_GLOBAL float* __restrict__ H, const DevGrid& G,
+                                          const ITR_GLOBAL int32_t* __restrict__ ghost, int row, int col,
+                                          int code_n, int code_s) {
+    return ghosted(fetch_cell(H, G, ghost, row, col), G, row, col, code_n, code_s);
+}
+
+// Hyetograph::rate / integral (crates/itr-core/src/scenario.rs:540-585), f64.
+__device__ __forceinline__ double hy_rate(const double* rt, const double* rr, int n, double x) {
+    if (n == 0) return 0.0;
+    if (x <= rt[0]) return rr[0];
+    for (int k = 0; k + 1 < n; k++) {
+        if (x <= rt[k + 1]) {
+            const double t0 = rt[k], r0 = rr[k], t1 = rt[k + 1], r1 = rr[k + 1];
+            if (t1 == t0) return r1;
+            return r0 + (r1 - r0) * (x - t0) / (t1 - t0);
+        }
+    }
+    return rr[n - 1];
+}
+__device__ __forceinline__ double hy_integral(const double* rt, const double* rr, int n, double a, double b) {
+    if (b <= a) return 0.0;
+    if (n == 0) return 0.0;
+    double total = 0.0, t = a, r = hy_rate(rt, rr, n, a);
+    for (int k = 0; k < n; k++) {
+        const double tk = rt[k];
+        if (tk > a && tk < b) {
+            const double rk = hy_rate(rt, rr, n, tk);
+            total += 0.5 * (r + rk) * (tk - t);
+            t = tk;
+            r = rk;
+        }
+    }
+    const double rb = hy_rate(rt, rr, n, b);
+    total += 0.5 * (r + rb) * (b - t);
+    return total;
+}
+
+// Head of one loop iteration of Solver::run_impl (solver.rs:449-489 and 269-285): the
+// step's dt (CFL, dt_max, locked schedule, sync clipping), exact rain depth, new time
+// and the f32 constants.  Returns false if the locked-dt check failed.
+__device__ __forceinline__ bool prepare_step(const DevParams& p, const DevGrid& G, CandCtl* C, const double* rt,
+                                             const double* rr) {
+    const double t = C->t;
+    const double t_sync = fmin((double)(C->sync_index + 1) * p.sync, p.duration);
+    double dt = fmin(C->dt_cfl, p.dt_max);
+    if (p.locked) {
+        if (C->step < (uint64_t)p.n_locked && p.locked[C->step] * C->s_cur <= p.cfl_hard) dt = p.locked[C->step];
+        else return false;
+    }
+    const bool hits = t + dt >= t_sync - 1e-9 * p.sync;
+    if (hits) dt = t_sync - t;
+    const double rain = hy_integral(rt, rr, p.n_rain, t, t + dt);
+    C->t_new = hits ? t_sync : t + dt;
+    C->dt = dt;
+    C->rain = rain;
+    C->hits_sync = hits ? 1 : 0;
+    C->dt32 = (float)dt;
+    C->dtdx = (float)(dt / G.dx);
+    C->dtdy = (float)(dt / G.dy);
+    C->rain32 = (float)rain;
+    C->cap32 = (float)(p.infil_cap * dt);
+    return true;
+}
+
+// Dry-row accounting of the coming step (kernel.rs:605-612): rows j (padded 1..ny)
+// with rows j-1, j, j+1 exactly dry and no rain.  Ghost rows mirror their neighbour for
+// wall and transmissive boundaries.  Computed by one wave.
+__device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const DevGrid& G, const CandCtl* C, const RowAcc* ra,
+                                    int lane) {
+    if (!p.skip_dry || C->rain32 > 0.0f) return 0;
+    const int ny = G.ny;
+    uint32_t n = 0;
+    for (int r = lane; r < ny; r += kWave) {
+        const bool d0 = r > 0 ? !ra[r - 1].wet : !ra[0].wet;
+        const bool d1 = !ra[r].wet;
+        const bool d2 = r + 1 < ny ? !ra[r + 1].wet : !ra[ny - 1].wet;
+        n += (d0 && d1 && d2) ? 1u : 0u;
+    }
+    for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o, 64);
+    return n;
+}
+
+// The candidate's control after a completed step: accounting (solver.rs:529-545) with
+// the reference's summation order (kernel.rs:453-500), sync point (solver.rs:554-603),
+// termination test and the next step's head.  Run by the whole last workgroup; wave 0
+// computes the row-ordered sums, thread 0 the scalar logic.
+__device__ __forceinline__ void candidate_control(const KArgs& a, const DevGrid& G, int cand, CandCtl* C, double* shd,
+                                                  float* stage, size_t stage_floats, const double* rt,
+                                                  const double* rr) {
+    const DevParams& p = *a.prm;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int local = C->local;
+    const RowAcc* ra = G.rowacc + (size_t)local * G.ny;
+    const int nx = G.nx, ny = G.ny;
+    const double dt = C->dt;
+    const double dt_dy = dt * G.dy;
+    const double dt_dx = dt * G.dx;
+    {
+        // y-boundary faces (kernel.rs:453-472): FixedSum<f64> over padded columns 1..nx,
+        // staged through LDS (2*nx floats fit in the stage buffer for nx <= 992).
+        const ITR_GLOBAL float* yb_src = gp(G.ybnd + (size_t)local * 2 * nx);
+#pragma unroll 8
+        for (int i = tid; i < 2 * nx; i += blockDim.x) stage[i] = yb_src[i];
+        __syncthreads();
+        if (tid < 32) {
+            const int L = tid & 15;
+            const float* row = stage + (tid >> 4) * nx;
+            double yb = 0.0;
+#pragma unroll 8
+            for (int i = (L == 0 ? 16 : L); i <= nx; i += 16) yb += (double)row[i - 1];
+            shd[tid] = yb;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const double* acc = shd + 16 * s;
+            double l8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) l8[k] = acc[2 * k] + acc[2 * k + 1];
+            const double l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
+            const double v = ((l4[0] + l4[1]) + (l4[2] + l4[3])) * dt_dx;
+            double fin = 0.0, fout = 0.0;
+            if (s == 0) { if (v > 0.0) fin = v; else fout = -v; }
+            else if (v > 0.0) fout = v;
+            else fin = -v;
+            shd[32 + 2 * s] = fin;
+            shd[33 + 2 * s] = fout;
+        }
+    }
+    __syncthreads();
+    {
+        // Row-ordered f64 sums of the row partials (finalize, kernel.rs:485-500).  The
+        // rows are staged through LDS in chunks by the whole workgroup; lanes 0-5 of wave
+        // 0 each run one sequential chain: infil, fix, bnd_in, bnd_out, volume, sum_s.
+        double s = 0.0;
+        if (tid == 2) s = shd[32] + shd[34];   // st.bnd_in = top.0 + bot.0
+        if (tid == 3) s = shd[33] + shd[35];   // st.bnd_out = top.1 + bot.1
+        const bool sync = C->hits_sync != 0;
+        const int chunk = max(1, (int)(stage_floats / 8));
+        const uint4* src = (const uint4*)ra;   // a RowAcc is two uint4
+        uint4* st4 = (uint4*)stage;
+        for (int r0 = 0; r0 < ny; r0 += chunk) {
+            const int nr = min(chunk, ny - r0);
+#pragma unroll 8
+            for (int i = tid; i < nr * 2; i += blockDim.x) st4[i] = src[(size_t)r0 * 2 + i];
+            __syncthreads();
+            if (tid <= 1 || (sync && (tid == 4 || tid == 5))) {
+                const int f = tid;   // RowAcc word: 0 infil, 1 fix, 4 vol, 5 sum_s
+#pragma unroll 8
+                for (int r = 0; r < nr; r++) s += (double)stage[r * 8 + f];
+            } else if (tid == 2 || tid == 3) {
+#pragma unroll 4
+                for (int r = 0; r < nr; r++) {
+                    const double w = (double)stage[r * 8 + 2] * dt_dy;
+                    const double e = (double)stage[r * 8 + 3] * dt_dy;
+                    double ai = 0.0, ao = 0.0;
+                    if (w > 0.0) ai += w; else ao -= w;
+                    if (e > 0.0) ao += e; else ai -= e;
+                    s += tid == 2 ? ai : ao;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid < 6) shd[40 + tid] = s;
+    }
+    __syncthreads();
+    __shared__ int s_go;
+    if (tid == 0) {
+        const double st_infil = shd[40], st_fix = shd[41], st_in = shd[42], st_out = shd[43];
+        const double st_vol = shd[44], st_sum_s = shd[45];
+        const double area = G.area;
+        const uint64_t cells = (uint64_t)nx * (uint64_t)ny;
+        C->step += 1;
+        C->t = C->t_new;
+        C->acc_rain += C->rain * area * (double)G.wet_count;
+        C->inflow += st_in;
+        C->outflow += st_out;
+        C->infil += st_infil * area;
+        C->fix += st_fix * area;
+        C->dt_min = fmin(C->dt_min, dt);
+        C->dt_max = fmax(C->dt_max, dt);
+        C->dt_sum += dt;
+        C->cell_updates += cells - (uint64_t)C->skipped_now * (uint64_t)nx;
+        C->rows_skipped += C->skipped_now;
+        C->parity ^= 1;
+        int go = 1;
+        uint32_t flags = C->flags;
+        if (C->bad) {
+            C->abort_kind = 3;
+            flags |= F_DONE;
+            go = 0;
+        } else {
+            const double smax = (double)__int_as_float(max(C->smax_bits, 0));
+            C->dt_cfl = smax > 0.0 ? p.cfl / smax : p.dt_max;
+            C->s_cur = smax;
+            if (C->hits_sync) {
+                C->sync_index += 1;
+                if (st_sum_s > 0.0) C->lts = fmax(C->lts, smax * (double)cells / st_sum_s);
+                const double v = st_vol * area;
+                const double expected = C->v0 + C->acc_rain + C->inflow - C->outflow - C->infil + C->fix;
+                if ((int)C->n_ledger < p.max_ledger) {
+                    double* row = G.ledger + ((size_t)local * p.max_ledger + C->n_ledger) * 8;
+                    row[0] = C->t; row[1] = v; row[2] = C->acc_rain; row[3] = C->inflow;
+                    row[4] = C->outflow; row[5] = C->infil; row[6] = C->fix; row[7] = v - expected;
+                }
+                C->n_ledger += 1;
+                if (G.baseline_max) {   // Objective::guard_breach (objective.rs:132-144)
+                    const float* mm = G.mon_max + (size_t)local * G.n_mon;
+                    bool have = false;
+                    double worst = 0.0;
+                    int wcell = 0;
+                    for (int q = 0; q < G.n_guard; q++) {
+                        const int k = G.guard_pos[q];
+                        const double w = (double)mm[k] - G.baseline_max[k] - G.guard_tol;
+                        if (w > 0.0 && (!have || w > worst)) { have = true; worst = w; wcell = (int)G.mon_cells[k]; }
+                    }
+                    if (have) {
+                        C->abort_kind = 1;
+                        C->abort_cell = wcell;
+                        C->abort_value = worst;
+                        C->abort_sync = (int)C->sync_index;
+                        flags |= F_DONE;
+                        go = 0;
+                    }
+                }
+                if (go && p.has_cb) flags |= F_PAUSED;
+            }
+            if (go && !(C->t < p.duration && C->sync_index < p.n_sync)) { flags |= F_DONE; go = 0; }
+            if (go && p.max_steps > 0 && C->step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = 0; }
+            if (go && !prepare_step(p, G, C, rt, rr)) {
+                C->abort_kind = 4;
+                flags |= F_DONE;
+                go = 0;
+            }
+        }
+        C->smax_bits = 0;
+        C->bad = 0;
+        C->ticket = 0;
+        C->flags = flags;
+        a.status[cand] = flags;
+        s_go = go;
+    }
+    __syncthreads();
+    if (s_go && tid < kWave) {
+        const uint32_t n = count_skippable(p, G, C, ra, lane);
+        if (lane == 0) C->skipped_now = n;
+    }
+}
+
+// One step of one band of one candidate (kernel.rs:590-727 semantics; the sweep order
+// differs but every face and cell value is the same IEEE expression of the old state).
+template <bool CREST>
+__global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
+    extern __shared__ float lds[];
+    // XCD-aware task order: consecutive bands of a candidate share one XCD's L2.
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int task = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (task >= a.ntasks) return;
+    const int4 tk = a.tasks[task];
+    const DevGrid& G = a.grids[tk.x];
+    const int local = tk.y;
+    const int cand = G.cand0 + local;
+    CandCtl* C = a.ctl + cand;
+    if (C->flags & (F_DONE | F_PAUSED)) return;
+
+    const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
+    const int a0 = tk.z * R;
+    const int b0 = min(a0 + R, ny);
+    const int nrows = b0 - a0;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int col = wave * kColsPerWave - 1 + lane;
+    const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
+    const bool sync = C->hits_sync != 0;
+    const bool band_mon = G.band_mon[tk.z] != 0;
+
+    StepK K;
+    K.g = G.g32;
+    K.g_half = G.g_half32;
+    K.inv_dx = G.inv_dx32;
+    K.inv_dy = G.inv_dy32;
+    K.h_dry = a.prm->h_dry;
+    K.heps2 = a.prm->heps2;
+    K.q_min = a.prm->q_min;
+    K.dt = C->dt32;
+    K.dtdx = C->dtdx;
+    K.dtdy = C->dtdy;
+    K.rain = C->rain32;
+    K.cap = C->cap32;
+    const double t_new = C->t_new;
+
+    const size_t cstride = 6 * G.plane;
+    float* base = G.state + (size_t)local * cstride;
+    const int par = C->parity;
+    ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)(par ^ 1) * 3 * G.plane);
+    const ITR_GLOBAL int32_t* __restrict__ MONMAP = gp(G.mon_map);
+    const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
+    ITR_GLOBAL float* __restrict__ mon_max = gp(G.mon_max + (size_t)local * G.n_mon);
+    ITR_GLOBAL double* __restrict__ mon_tmax = gp(G.mon_tmax + (size_t)local * G.n_mon);
+    ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
+    // Lane columns for addressing (unsigned, so loads use a scalar row base + 32-bit
+    // lane offset): ccol clamps to the interior, xcol to the x-face range 0..nx.
+    const uint32_t ccol = (uint32_t)min(max(col, 0), nx - 1);
+    const uint32_t xcol = (uint32_t)min(max(col, 0), nx);
+    const uint32_t cb = ccol * 4u, xb = xcol * 4u;   // lane byte offsets within a row
+    const uint32_t rowb = (uint32_t)P * 4u;           // row pitch in bytes
+    const uint32_t planeb = (uint32_t)(G.plane * 4u);
+    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(base + (size_t)par * 3 * G.plane, 3 * G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)(par ^ 1) * 3 * G.plane, 3 * G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzx = make_rsrc(G.dzx + (size_t)local * G.plane, G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzy = make_rsrc(G.dzy + (size_t)local * G.plane_y, G.plane_y * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzx2 = make_rsrc(CREST ? G.dzx2 + (size_t)local * G.plane : G.dzx, G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzy2 = make_rsrc(CREST ? G.dzy2 + (size_t)local * G.plane_y : G.dzy, G.plane_y * 4);
+    const __amdgpu_buffer_rsrc_t rs_wet = make_rsrc(G.wet, G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_gn2 = make_rsrc(G.gn2, G.plane * 4);
+    const int code_n = GHOST[ccol];
+    const int code_s = GHOST[nx + ccol];
+    const size_t plane = G.plane;
+    const int nwaves = (int)(blockDim.x >> 6);
+    const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
+
+    // LDS: per-row staging of infil / fix (R x nx each) + row scalars (lds_layout).
+    const LdsLayout LL = lds_layout(R, nx);
+    float* l_infil = lds + LL.infil;
+    float* l_fix = lds + LL.fix;
+    float* l_bw = lds + LL.bw;
+    float* l_be = lds + LL.be;
+    uint32_t* l_wet = (uint32_t*)(lds + LL.wet);
+    int* l_red = (int*)(lds + LL.red);            // [2 * kMaxWaves]
+    float* l_acc = lds + LL.acc;                  // [R][16][4]
+    if (tid < R) { l_wet[tid] = 0u; l_bw[tid] = 0.0f; l_be[tid] = 0.0f; }
+    __syncthreads();
+
+    // Old-state cell of this lane's column in `row`: raw loads (scalar row base + lane
+    // offset) and, on ghost rows or in the edge waves only, the boundary transform.
+    auto fetch = [&](int row) -> Cell {
+        const int rr = min(max(row, 0), ny - 1);
+        const uint32_t so = (uint32_t)rr * rowb;
+        Cell v;
+        v.h = bload(rs_h, cb, so);
+        v.qx = bload(rs_h, cb, so + planeb);
+        v.qy = bload(rs_h, cb, so + 2 * planeb);
+        v.code_we = edge_wave ? GHOST[2 * nx + (col < 0 ? 0 : ny) + rr] : 1;
+        return v;
+    };
+    auto fix = [&](const Cell& v, int row) -> Cell {
+        if (edge_wave || row < 0 || row >= ny) return ghosted(v, G, row, col, code_n, code_s);
+        return v;
+    };
+    // Face jumps are stored with zeros on boundary faces (row 0 and row ny of dzy, columns
+    // 0 and nx of dzx), so no masking is needed; lanes outside a face's role read an
+    // in-bounds clamped entry whose value is never used.
+    auto dzy_at = [&](__amdgpu_buffer_rsrc_t D, int row) -> float { return bload(D, cb, (uint32_t)min(row, ny) * rowb); };
+    auto dzx_at = [&](__amdgpu_buffer_rsrc_t D, int row) -> float { return bload(D, xb, (uint32_t)row * rowb); };
+    auto row_at = [&](__amdgpu_buffer_rsrc_t A, int row) -> float { return bload(A, cb, (uint32_t)min(row, ny - 1) * rowb); };
+
+    // Prologue: rows a0-1 and a0, and the y-face between them.
+    const Cell cp = fix(fetch(a0 - 1), a0 - 1);
+    const Cell c0 = fix(fetch(a0), a0);
+    float ch = c0.h, cqx = c0.qx, cqy = c0.qy;
+    const Vel vp = cell_velocity(cp.h, cp.qx, cp.qy, K);
+    Vel vc = cell_velocity(ch, cqx, cqy, K);
+    Flux fn;
+    if (CREST) fn = face_crest<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), dzy_at(rs_dzy2, a0), K);
+    else fn = face_signed<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), K);
+    const bool in_y = col >= 0 && col < nx;   // columns that own a y-face
+    if (a0 == 0 && in_y && lane >= 1 && lane <= kColsPerWave)
+        G.ybnd[((size_t)local * 2 + 0) * nx + col] = fn.m;
+
+    // Row inputs are loaded one row ahead of their use (software pipelining).
+    Cell pre_c = fetch(a0 + 1);                                   // row j+1 (raw)
+    float pre_dzs = dzy_at(rs_dzy, a0 + 1), pre_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 1) : 0.0f;
+    float pre_dzw = dzx_at(rs_dzx, a0), pre_dzw2 = CREST ? dzx_at(rs_dzx2, a0) : 0.0f;   // row j
+    float pre_wet = row_at(rs_wet, a0), pre_gn2 = row_at(rs_gn2, a0);
+
+    int smax_bits = 0;
+    bool bad = false;
+    for (int j = a0; j < b0; j++) {
+        const Cell cn = fix(pre_c, j + 1);
+        const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
+        const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
+        const float wet = pre_wet, gn2 = pre_gn2;
+        // Issue the next row's loads (raw values) before this row's arithmetic.
+        pre_c = fetch(j + 2);
+        pre_dzs = dzy_at(rs_dzy, j + 2);
+        if (CREST) pre_dzs2 = dzy_at(rs_dzy2, j + 2);
+        pre_dzw = dzx_at(rs_dzx, min(j + 1, ny - 1));
+        if (CREST) pre_dzw2 = dzx_at(rs_dzx2, min(j + 1, ny - 1));
+        pre_wet = row_at(rs_wet, j + 1);
+        pre_gn2 = row_at(rs_gn2, j + 1);
+
+        const Vel vn = cell_velocity(nh, nqx, nqy, K);
+        Flux fs;
+        if (CREST) fs = face_crest<false>(ch, vc, nh, vn, dzs, dzs2, K);
+        else fs = face_signed<false>(ch, vc, nh, vn, dzs, K);
+        // West face of this lane's column, from the left neighbour's old state.
+        Vel vw;
+        const float hw = from_left(ch);
+        vw.u = from_left(vc.u);
+        vw.v = from_left(vc.v);
+        vw.c = from_left(vc.c);
+        Flux fw;
+        if (CREST) fw = face_crest<true>(hw, vw, ch, vc, dzw, dzw2, K);
+        else fw = face_signed<true>(hw, vw, ch, vc, dzw, K);
+        const float em = from_right(fw.m);
+        const float enl = from_right(fw.nl);
+        const float et = from_right(fw.t);
+        const CellNew u = cell_update(ch, cqx, cqy, fw.m, fw.nr, fw.t, em, enl, et, fn.m, fn.nr, fn.t, fs.m, fs.nl,
+                                      fs.t, wet, gn2, K);
+        const int r = j - a0;
+        if (upd) {
+            const uint32_t so = (uint32_t)j * rowb;
+            bstore(u.h, rs_ho, cb, so);
+            bstore(u.qx, rs_ho, cb, so + planeb);
+            bstore(u.qy, rs_ho, cb, so + 2 * planeb);
+            l_infil[r * nx + col] = u.infil;
+            l_fix[r * nx + col] = u.fix;
+            smax_bits = max(smax_bits, __float_as_int(u.s));
+            bad = bad || u.bad;
+            if (sync) sbuf[(size_t)j * P + ccol] = u.s;
+            if (band_mon) {
+                const int mi = MONMAP[(size_t)j * P + ccol];
+                if (mi >= 0 && u.h > mon_max[mi]) {   // kernel.rs:311-321 (strict >)
+                    mon_max[mi] = u.h;
+                    mon_tmax[mi] = t_new;
+                }
+            }
+        }
+        if (edge_wave) {
+            if (col == 0 && lane >= 1) l_bw[r] = fw.m;
+            if (col == nx && lane >= 1) l_be[r] = fw.m;
+        }
+        const unsigned long long wb = __ballot(upd && u.h > 0.0f);
+        if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
+        ch = nh;
+        cqx = nqx;
+        cqy = nqy;
+        vc = vn;
+        fn = fs;
+    }
+    if (b0 == ny && in_y && lane >= 1 && lane <= kColsPerWave)
+        G.ybnd[((size_t)local * 2 + 1) * nx + col] = fn.m;
+
+    // Reductions of the band: CFL rate max (as ordered int bits; -0.0 never wins over
+    // +0.0) and the health flag.
+    smax_bits = wave_max_i32(smax_bits);
+    const bool bad_any = __any(bad);
+    if (lane == 0) {
+        l_red[wave] = smax_bits;
+        l_red[kMaxWaves + wave] = bad_any ? 1 : 0;
+    }
+    __syncthreads();
+
+    // Row FixedSums (real.rs:132-175): lane L = padded column & 15, sequential in column
+    // order inside a lane, then the fixed pairwise tree.
+    for (int t = tid; t < nrows * 16; t += blockDim.x) {
+        const int r = t >> 4, L = t & 15;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int j = a0 + r;
+        for (int pc = (L == 0 ? 16 : L); pc <= nx; pc += 16) {
+            const int c = pc - 1;
+            acc[0] += l_infil[(size_t)r * nx + c];
+            acc[1] += l_fix[(size_t)r * nx + c];
+            if (sync) {
+                acc[2] += Ho[(size_t)j * P + c];
+                acc[3] += sbuf[(size_t)j * P + c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) l_acc[((size_t)r * 16 + L) * 4 + q] = acc[q];
+    }
+    __syncthreads();
+    if (tid < nrows) {
+        const int r = tid;
+        float tot[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float* A = l_acc + (size_t)r * 64 + q;
+            float l8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) l8[k] = A[(2 * k) * 4] + A[(2 * k + 1) * 4];
+            const float l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
+            tot[q] = (l4[0] + l4[1]) + (l4[2] + l4[3]);
+        }
+        if (!sync) { tot[2] = 0.0f; tot[3] = 0.0f; }
+        RowAcc q;
+        q.infil = tot[0];
+        q.fix = tot[1];
+        q.bw = l_bw[r];
+        q.be = l_be[r];
+        q.vol = tot[2];
+        q.sum_s = tot[3];
+        q.wet = l_wet[r];
+        q.pad = 0;
+        G.rowacc[(size_t)local * ny + a0 + r] = q;
+    }
+    if (tid == 0) {
+        int m = 0, bd = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            m = max(m, l_red[w]);
+            bd |= l_red[kMaxWaves + w];
+        }
+        if (m > 0) atomicMax(&C->smax_bits, m);
+        if (bd) atomicOr(&C->bad, 1u);
+    }
+    // No device-scope fence here: on gfx950 that is an L2 write-back + invalidate per
+    // executing thread.  The candidate's control runs in the next kernel of the stream
+    // (itr_control_kernel), whose launch boundary makes these results visible.
+}
+
+// Per-candidate control after each step: one workgroup (one wave) per candidate.
+constexpr int kCtlThreads = 64;
+constexpr int kCtlStageRows = 256;
+constexpr int kCtlMaxRain = 128;
+__global__ void __launch_bounds__(kCtlThreads) itr_control_kernel(KArgs a, int32_t ncand_total) {
+    __shared__ double shd[64];
+    __shared__ float stage[kCtlStageRows * 8];
+    __shared__ double s_rt[kCtlMaxRain], s_rr[kCtlMaxRain];
+    const int cand = blockIdx.x;
+    if (cand >= ncand_total) return;
+    CandCtl* C = a.ctl + cand;
+    if (C->flags & (F_DONE | F_PAUSED)) return;
+    const DevGrid& G = a.grids[C->grid];
+    const DevParams& p = *a.prm;
+    // The hyetograph is walked point by point: stage it in LDS (short) or read it in place.
+    const bool in_lds = p.n_rain <= kCtlMaxRain;
+    if (in_lds)
+        for (int k = threadIdx.x; k < p.n_rain; k += blockDim.x) { s_rt[k] = p.rain_t[k]; s_rr[k] = p.rain_r[k]; }
+    __syncthreads();
+    candidate_control(a, G, cand, C, shd, stage, (size_t)kCtlStageRows * 8, in_lds ? s_rt : p.rain_t,
+                      in_lds ? s_rr : p.rain_r);
+}
+
+// Head of the first step of every candidate.
+__global__ void itr_init_kernel(KArgs a, int32_t ncand_total) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand_total) return;
+    CandCtl* C = a.ctl + c;
+    const DevGrid& G = a.grids[C->grid];
+    if (C->flags & F_DONE) return;
+    if (!prepare_step(*a.prm, G, C, a.prm->rain_t, a.prm->rain_r)) {
+        C->abort_kind = 4;
+        C->flags |= F_DONE;
+    }
+    a.status[c] = C->flags;
+}
+
+// Dry-row count of the first step (same rule as in candidate_control).
+__global__ void itr_init_skip_kernel(KArgs a) {
+    const int c = blockIdx.x;
+    CandCtl* C = a.ctl + c;
+    const DevGrid& G = a.grids[C->grid];
+    if (C->flags & F_DONE) return;
+    const uint32_t n = count_skippable(*a.prm, G, C, G.rowacc + (size_t)C->local * G.ny, threadIdx.x & 63);
+    if (threadIdx.x == 0) C->skipped_now = n;
+}
+
+// ------------------------------------------------------------------ host side
+void set_err(char* err, size_t errlen, const char* fmt, ...) {
+    if (!err || errlen == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, errlen, fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            set_err(err, errlen, "%s failed: %s", #expr, hipGetErrorString(e_));            \
+            return e_ == hipErrorOutOfMemory ? ITR_ENOMEM : ITR_EDEVICE;                     \
+        }                                                                                    \
+    } while (0)
+
+// FixedSum<f32> of one row, then the f64 sum over rows (solver.rs:250-262).
+double host_volume(const std::vector<float>& h, int nx, int ny, double area) {
+    double v = 0.0;
+    for (int r = 0; r < ny; r++) {
+        float acc[16];
+        for (auto& x : acc) x = 0.0f;
+        for (int i = 1; i <= nx; i++) acc[i & 15] += h[(size_t)r * nx + (i - 1)];
+        float l8[8];
+        for (int k = 0; k < 8; k++) l8[k] = acc[2 * k] + acc[2 * k + 1];
+        const float l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
+        v += (double)((l4[0] + l4[1]) + (l4[2] + l4[3]));
+    }
+    return v * area;
+}
+
+struct GridHost {
+    itr_grid in;
+    DevGrid d;
+    std::vector<void*> allocs;
+    int waves = 1;
+};
+
+}  // namespace
+
+struct itr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string info;
+    int band_rows = 0;
+    int steps_per_chunk = 0;
+    int use_graph = 1;
+};
+
+struct itr_batch {
+    itr_ctx* ctx = nullptr;
+    std::vector<GridHost> grids;
+    DevParams prm{};
+    KArgs args{};
+    void* d_grids = nullptr;
+    void* d_tasks = nullptr;
+    void* d_ctl = nullptr;
+    void* d_prm = nullptr;
+    void* d_status = nullptr;
+    void* d_rain_t = nullptr;
+    void* d_rain_r = nullptr;
+    void* d_locked = nullptr;
+    uint32_t* h_status = nullptr;   // pinned
+    int ncand = 0;
+    int ntasks = 0;
+    int block = 64;
+    size_t lds_bytes = 0;
+    int crest = 0;
+    int max_ledger = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    itr_stats stats{};
+    std::vector<std::vector<double>> cb_mon;   // scratch for callbacks
+};
+
+namespace {
+
+void free_batch_mem(itr_batch* b) {
+    for (auto& g : b->grids)
+        for (void* p : g.allocs) (void)hipFree(p);
+    void* ps[] = {b->d_grids, b->d_tasks, b->d_ctl, b->d_prm, b->d_status, b->d_rain_t, b->d_rain_r, b->d_locked};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (b->h_status) (void)hipHostFree(b->h_status);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+}
+
+template <class T>
+itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* err, size_t errlen) {
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    allocs.push_back(p);
+    *out = (T*)p;
+    return ITR_OK;
+}
+
+itr_status launch_steps(itr_batch* b, int nsteps, char* err, size_t errlen) {
+    const int nblk = ((b->ntasks + 7) / 8) * 8;
+    for (int s = 0; s < nsteps; s++) {
+        if (b->crest)
+            hipLaunchKernelGGL(itr_step_kernel<true>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args);
+        else
+            hipLaunchKernelGGL(itr_step_kernel<false>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args);
+        hipLaunchKernelGGL(itr_control_kernel, dim3(b->ncand), dim3(kCtlThreads), 0, b->ctx->stream, b->args, b->ncand);
+    }
+    HIP_TRY(hipGetLastError());
+    return ITR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errlen) {
+    if (!out) return ITR_EINVAL;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_err(err, errlen, "no HIP device %d (%d present)", device, n);
+        return ITR_DECLINED;
+    }
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
+        set_err(err, errlen, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+        return ITR_DECLINED;
+    }
+    auto* c = new itr_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_err(err, errlen, "hipStreamCreate: %s", hipGetErrorString(e));
+        return ITR_EDEVICE;
+    }
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s (%s, %d CUs, %.0f GiB HBM, HIP %d)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0), HIP_VERSION);
+    c->info = buf;
+    *out = c;
+    return ITR_OK;
+}
+
+void itr_hip_destroy(itr_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* itr_hip_device_info(itr_ctx* ctx) { return ctx ? ctx->info.c_str() : ""; }
+
+itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
+    if (!ctx || !name) return ITR_EINVAL;
+    std::string n(name);
+    if (n == "band_rows") ctx->band_rows = (int)value;
+    else if (n == "steps_per_chunk") ctx->steps_per_chunk = (int)value;
+    else if (n == "use_graph") ctx->use_graph = value != 0;
+    else return ITR_EINVAL;
+    return ITR_OK;
+}
+
+itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                                itr_batch** out, char* err, size_t errlen) {
+    if (!ctx || !p || !grids || n_grids <= 0 || !out) {
+        set_err(err, errlen, "null argument");
+        return ITR_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    // ---- validation and declines (lib.rs:155-171, backend.rs:53-55)
+    if (p->inflow_scale != 1.0) {
+        set_err(err, errlen, "inflow scaling is not supported on this backend");
+        return ITR_DECLINED;
+    }
+    if (!(p->sync_interval_s > 0.0) || !(p->duration_s >= 0.0) || p->n_rain < 0) {
+        set_err(err, errlen, "invalid solver parameters");
+        return ITR_EINVAL;
+    }
+    int crest = -1;
+    for (int g = 0; g < n_grids; g++) {
+        const itr_grid& G = grids[g];
+        if (G.nx <= 0 || G.ny <= 0 || G.n_cand < 0 || !G.wet || !G.gn2 || !G.ghost || !G.dzx || !G.dzy) {
+            set_err(err, errlen, "grid %d: missing arrays or empty shape", g);
+            return ITR_EINVAL;
+        }
+        if (G.nx > kMaxWaves * kColsPerWave) {
+            set_err(err, errlen, "grid %d: nx=%d exceeds %d columns", g, G.nx, kMaxWaves * kColsPerWave);
+            return ITR_DECLINED;
+        }
+        if (G.crest && (!G.dzx_r || !G.dzy_r)) {
+            set_err(err, errlen, "grid %d: crest mode needs dzx_r and dzy_r", g);
+            return ITR_EINVAL;
+        }
+        if (crest >= 0 && crest != (G.crest ? 1 : 0)) {
+            set_err(err, errlen, "grids of one batch must share the face mode");
+            return ITR_DECLINED;
+        }
+        crest = G.crest ? 1 : 0;
+        for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++)
+            if (G.ghost[i] == 2 || G.ghost[i] == 3) {
+                set_err(err, errlen, "Stage and Inflow boundaries are not supported on this backend");
+                return ITR_DECLINED;
+            } else if (G.ghost[i] != 0 && G.ghost[i] != 1) {
+                set_err(err, errlen, "grid %d: unknown boundary code %d", g, G.ghost[i]);
+                return ITR_EINVAL;
+            }
+        for (int q = 0; q < G.n_mon; q++) {
+            if (G.mon_cells[q] >= (uint32_t)(G.nx * G.ny) || (q > 0 && G.mon_cells[q] <= G.mon_cells[q - 1])) {
+                set_err(err, errlen, "grid %d: monitor cells must be sorted, unique and inside the grid", g);
+                return ITR_EINVAL;
+            }
+        }
+        if (G.baseline_max)
+            for (int q = 0; q < G.n_guard; q++)
+                if (G.guard_pos[q] < 0 || G.guard_pos[q] >= G.n_mon) {
+                    set_err(err, errlen, "grid %d: guard position out of range", g);
+                    return ITR_EINVAL;
+                }
+        if (p->has_initial_stage && !G.z_level) {
+            set_err(err, errlen, "grid %d: initial_stage needs z_level", g);
+            return ITR_EINVAL;
+        }
+    }
+    auto* b = new itr_batch();
+    b->ctx = ctx;
+    b->crest = crest;
+    auto fail = [&](itr_status s) {
+        free_batch_mem(b);
+        delete b;
+        return s;
+    };
+    // ---- shared parameters
+    const double dur = p->duration_s, syn = p->sync_interval_s;
+    const uint32_t n_sync = (uint32_t)std::ceil(dur / syn);
+    b->max_ledger = (int)std::min<uint64_t>(n_sync + 1, 1u << 20);
+    DevParams& P = b->prm;
+    P.duration = dur;
+    P.sync = syn;
+    P.cfl = p->cfl;
+    P.dt_max = p->dt_max_s;
+    P.cfl_hard = p->cfl_hard;
+    P.infil_cap = p->infil_capacity;
+    P.n_rain = p->n_rain;
+    P.n_sync = n_sync;
+    P.skip_dry = p->skip_dry;
+    P.max_steps = p->max_steps;
+    P.max_ledger = b->max_ledger;
+    P.h_dry = (float)p->h_dry;
+    P.heps2 = (float)(p->h_eps * p->h_eps);
+    P.q_min = (float)1e-30;
+    {
+        std::vector<void*> tmp;
+        double *rt = nullptr, *rr = nullptr;
+        if (dev_alloc(tmp, &rt, p->n_rain, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
+        b->d_rain_t = rt;
+        if (dev_alloc(tmp, &rr, p->n_rain, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
+        b->d_rain_r = rr;
+        if (p->n_rain > 0) {
+            if (hipMemcpy(rt, p->rain_t, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(rr, p->rain_r, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess) {
+                set_err(err, errlen, "rain upload failed");
+                return fail(ITR_EDEVICE);
+            }
+        }
+        P.rain_t = rt;
+        P.rain_r = rr;
+        if (p->locked_dt && p->n_locked > 0) {
+            double* ld = nullptr;
+            if (dev_alloc(tmp, &ld, p->n_locked, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
+            b->d_locked = ld;
+            if (hipMemcpy(ld, p->locked_dt, sizeof(double) * p->n_locked, hipMemcpyHostToDevice) != hipSuccess) {
+                set_err(err, errlen, "locked dt upload failed");
+                return fail(ITR_EDEVICE);
+            }
+            P.locked = ld;
+            P.n_locked = p->n_locked;
+        }
+    }
+    // ---- grids
+    int total = 0;
+    int max_waves = 1;
+    size_t max_lds = 0;
+    std::vector<int4> tasks;
+    std::vector<CandCtl> ctl;
+    std::vector<DevGrid> dgrids;
+    b->grids.resize(n_grids);
+    for (int g = 0; g < n_grids; g++) {
+        const itr_grid& I = grids[g];
+        GridHost& GH = b->grids[g];
+        GH.in = I;
+        DevGrid& D = GH.d;
+        const int nx = I.nx, ny = I.ny, nc = I.n_cand;
+        D.nx = nx;
+        D.ny = ny;
+        D.P = (nx + 1 + 63) / 64 * 64;   // >= nx + 1: column nx holds the east boundary face
+        D.waves = (nx + kColsPerWave - 1) / kColsPerWave;
+        max_waves = std::max(max_waves, D.waves);
+        int R = ctx->band_rows > 0 ? ctx->band_rows : 8;
+        R = std::max(1, std::min(R, ny));
+        D.R = R;
+        D.nbands = (ny + R - 1) / R;
+        D.cand0 = total;
+        D.ncand = nc;
+        D.n_mon = I.n_mon;
+        D.n_guard = I.baseline_max ? I.n_guard : 0;
+        D.crest = I.crest;
+        D.wet_count = I.wet_count;
+        D.plane = (size_t)ny * D.P;
+        D.plane_y = (size_t)(ny + 1) * D.P;
+        D.dx = I.dx;
+        D.dy = I.dy;
+        D.g = I.g;
+        D.area = I.dx * I.dy;
+        D.guard_tol = I.guard_tol;
+        D.g32 = (float)I.g;
+        D.g_half32 = (float)(0.5 * I.g);
+        D.inv_dx32 = (float)(1.0 / I.dx);
+        D.inv_dy32 = (float)(1.0 / I.dy);
+        const size_t plane = D.plane;
+        // host staging of the pitched arrays
+        std::vector<float> st((size_t)std::max(nc, 1) * 6 * plane, 0.0f);
+        const size_t plane_y = D.plane_y;
+        std::vector<float> dzx((size_t)std::max(nc, 1) * plane, 0.0f), dzy((size_t)std::max(nc, 1) * plane_y, 0.0f);
+        std::vector<float> dzx2, dzy2;
+        if (I.crest) {
+            dzx2.assign(dzx.size(), 0.0f);
+            dzy2.assign(dzy.size(), 0.0f);
+        }
+        std::vector<float> wet(plane, 0.0f), gn2(plane, 0.0f);
+        std::vector<int32_t> mon_map(plane, -1);
+        std::vector<uint8_t> band_mon(D.nbands, 0);
+        for (int r = 0; r < ny; r++)
+            for (int c = 0; c < nx; c++) {
+                wet[(size_t)r * D.P + c] = I.wet[(size_t)r * nx + c];
+                gn2[(size_t)r * D.P + c] = I.gn2[(size_t)r * nx + c];
+            }
+        for (int q = 0; q < I.n_mon; q++) {
+            const int r = (int)(I.mon_cells[q] / (uint32_t)nx), c = (int)(I.mon_cells[q] % (uint32_t)nx);
+            mon_map[(size_t)r * D.P + c] = q;
+            band_mon[r / R] = 1;
+        }
+        std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);
+        std::memset(rowacc.data(), 0, rowacc.size() * sizeof(RowAcc));
+        for (int k = 0; k < nc; k++) {
+            const float* sx = I.dzx + (size_t)k * ny * (nx + 1);
+            const float* sy = I.dzy + (size_t)k * (ny + 1) * nx;
+            for (int r = 0; r < ny; r++)
+                for (int c = 1; c < nx; c++) dzx[(size_t)k * plane + (size_t)r * D.P + c] = sx[(size_t)r * (nx + 1) + c];
+            for (int r = 1; r < ny; r++)
+                for (int c = 0; c < nx; c++) dzy[(size_t)k * plane_y + (size_t)r * D.P + c] = sy[(size_t)r * nx + c];
+            if (I.crest) {
+                const float* sx2 = I.dzx_r + (size_t)k * ny * (nx + 1);
+                const float* sy2 = I.dzy_r + (size_t)k * (ny + 1) * nx;
+                for (int r = 0; r < ny; r++)
+                    for (int c = 1; c < nx; c++)
+                        dzx2[(size_t)k * plane + (size_t)r * D.P + c] = sx2[(size_t)r * (nx + 1) + c];
+                for (int r = 1; r < ny; r++)
+                    for (int c = 0; c < nx; c++) dzy2[(size_t)k * plane_y + (size_t)r * D.P + c] = sy2[(size_t)r * nx + c];
+            }
+            // init_state (solver.rs:287-312) and the initial CFL rate (solver.rs:315-332)
+            std::vector<float> h0((size_t)nx * ny, 0.0f);
+            double s0 = 0.0;
+            for (int r = 0; r < ny; r++)
+                for (int c = 0; c < nx; c++) {
+                    const size_t q = (size_t)r * nx + c;
+                    if (I.wet[q] == 0.0f) continue;
+                    double d = p->initial_depth;
+                    if (p->has_initial_stage) d = std::fmax(d, p->initial_stage - I.z_ref - I.z_level[(size_t)k * nx * ny + q]);
+                    float hv = (float)std::fmax(d, 0.0);
+                    if (hv == 0.0f) hv = 0.0f;
+                    h0[q] = hv;
+                    st[(size_t)k * 6 * plane + (size_t)r * D.P + c] = hv;
+                    const double hd = (double)hv;
+                    if (hd <= p->h_dry) continue;
+                    const double cc = std::sqrt(I.g * hd);
+                    const double rate = (0.0 / hd + cc) / I.dx + (0.0 / hd + cc) / I.dy;
+                    s0 = std::fmax(s0, rate);
+                }
+            for (int r = 0; r < ny; r++) {
+                uint32_t w = 0;
+                for (int c = 0; c < nx; c++) w |= h0[(size_t)r * nx + c] > 0.0f ? 1u : 0u;
+                rowacc[(size_t)k * ny + r].wet = w;
+            }
+            CandCtl C;
+            std::memset(&C, 0, sizeof C);
+            C.grid = g;
+            C.local = k;
+            C.v0 = host_volume(h0, nx, ny, D.area);
+            C.t = 0.0;
+            C.s_cur = s0;
+            C.dt_cfl = s0 > 0.0 ? std::fmin(p->cfl / s0, p->dt_max_s) : p->dt_max_s;
+            C.dt_min = 1.79769313486231570815e+308;
+            C.flags = 0;
+            if (!(0.0 < dur && 0u < n_sync)) C.flags = F_DONE;
+            if (p->max_steps < 0) C.flags = F_DONE;
+            ctl.push_back(C);
+            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4(g, k, band, 0));
+        }
+        total += nc;
+        // device buffers
+        float *d_state, *d_dzx, *d_dzy, *d_dzx2 = nullptr, *d_dzy2 = nullptr, *d_wet, *d_gn2, *d_monmax, *d_ybnd, *d_sbuf;
+        double *d_montmax, *d_ledger, *d_base = nullptr;
+        int32_t *d_monmap, *d_ghost, *d_gpos = nullptr;
+        uint8_t* d_bandmon;
+        uint32_t* d_moncells;
+        RowAcc* d_rowacc;
+        const size_t ncx = (size_t)std::max(nc, 1);
+        itr_status s;
+#define ALLOC(ptr, n) \
+    if ((s = dev_alloc(GH.allocs, &ptr, (n), err, errlen)) != ITR_OK) return fail(s);
+        ALLOC(d_state, ncx * 6 * plane);
+        ALLOC(d_dzx, ncx * plane);
+        ALLOC(d_dzy, ncx * plane_y);
+        if (I.crest) {
+            ALLOC(d_dzx2, ncx * plane);
+            ALLOC(d_dzy2, ncx * plane_y);
+        }
+        ALLOC(d_wet, plane);
+        ALLOC(d_gn2, plane);
+        ALLOC(d_monmap, plane);
+        ALLOC(d_bandmon, (size_t)D.nbands);
+        ALLOC(d_ghost, (size_t)(2 * nx + 2 * ny));
+        ALLOC(d_monmax, ncx * std::max(I.n_mon, 1));
+        ALLOC(d_montmax, ncx * std::max(I.n_mon, 1));
+        ALLOC(d_rowacc, ncx * ny);
+        ALLOC(d_ybnd, ncx * 2 * nx);
+        ALLOC(d_sbuf, ncx * plane);
+        ALLOC(d_ledger, ncx * b->max_ledger * 8);
+        ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
+        if (I.baseline_max) {
+            ALLOC(d_base, (size_t)std::max(I.n_mon, 1));
+            ALLOC(d_gpos, (size_t)std::max(I.n_guard, 1));
+        }
+#undef ALLOC
+        auto up = [&](void* dst, const void* src, size_t bytes) -> bool {
+            return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+        };
+        bool ok = up(d_state, st.data(), st.size() * 4) && up(d_dzx, dzx.data(), dzx.size() * 4) &&
+                  up(d_dzy, dzy.data(), dzy.size() * 4) && up(d_wet, wet.data(), plane * 4) &&
+                  up(d_gn2, gn2.data(), plane * 4) && up(d_monmap, mon_map.data(), plane * 4) &&
+                  up(d_bandmon, band_mon.data(), band_mon.size()) &&
+                  up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
+                  up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
+                  up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
+        if (I.crest) ok = ok && up(d_dzx2, dzx2.data(), dzx2.size() * 4) && up(d_dzy2, dzy2.data(), dzy2.size() * 4);
+        if (I.baseline_max)
+            ok = ok && up(d_base, I.baseline_max, sizeof(double) * I.n_mon) &&
+                 up(d_gpos, I.guard_pos, sizeof(int32_t) * I.n_guard);
+        ok = ok && hipMemset(d_monmax, 0, ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
+             hipMemset(d_montmax, 0, ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
+             hipMemset(d_ybnd, 0, ncx * 2 * nx * 4) == hipSuccess;
+        if (!ok) {
+            set_err(err, errlen, "upload of grid %d failed", g);
+            return fail(ITR_EDEVICE);
+        }
+        D.state = d_state;
+        D.dzx = d_dzx;
+        D.dzy = d_dzy;
+        D.dzx2 = d_dzx2;
+        D.dzy2 = d_dzy2;
+        D.wet = d_wet;
+        D.gn2 = d_gn2;
+        D.mon_map = d_monmap;
+        D.band_mon = d_bandmon;
+        D.ghost = d_ghost;
+        D.mon_max = d_monmax;
+        D.mon_tmax = d_montmax;
+        D.rowacc = d_rowacc;
+        D.ybnd = d_ybnd;
+        D.sbuf = d_sbuf;
+        D.ledger = d_ledger;
+        D.baseline_max = d_base;
+        D.guard_pos = d_gpos;
+        D.mon_cells = d_moncells;
+        dgrids.push_back(D);
+        const size_t lds = lds_layout(R, nx).total * sizeof(float);
+        max_lds = std::max(max_lds, lds);
+    }
+    if (n_grids > 1) {
+        for (auto& D : dgrids)
+            if (D.waves != max_waves) {
+                set_err(err, errlen, "grids of different widths in one batch are not supported yet");
+                return fail(ITR_DECLINED);
+            }
+    }
+    b->ncand = total;
+    b->ntasks = (int)tasks.size();
+    b->block = max_waves * kWave;
+    b->lds_bytes = max_lds;
+    if (b->lds_bytes > 160 * 1024) {
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", b->lds_bytes);
+        return fail(ITR_DECLINED);
+    }
+    std::vector<void*> misc;
+    itr_status s;
+    DevGrid* d_grids;
+    int4* d_tasks;
+    CandCtl* d_ctl;
+    DevParams* d_prm;
+    uint32_t* d_status;
+    if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
+    b->d_grids = d_grids;
+    if ((s = dev_alloc(misc, &d_tasks, tasks.size(), err, errlen)) != ITR_OK) return fail(s);
+    b->d_tasks = d_tasks;
+    if ((s = dev_alloc(misc, &d_ctl, std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
+    b->d_ctl = d_ctl;
+    if ((s = dev_alloc(misc, &d_prm, 1, err, errlen)) != ITR_OK) return fail(s);
+    b->d_prm = d_prm;
+    if ((s = dev_alloc(misc, &d_status, std::max(total, 1), err, errlen)) != ITR_OK) return fail(s);
+    b->d_status = d_status;
+    if (hipHostMalloc((void**)&b->h_status, sizeof(uint32_t) * std::max(total, 1), hipHostMallocDefault) != hipSuccess) {
+        set_err(err, errlen, "pinned allocation failed");
+        return fail(ITR_ENOMEM);
+    }
+    bool ok = hipMemcpy(d_grids, dgrids.data(), dgrids.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
+              (tasks.empty() || hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
+              (ctl.empty() || hipMemcpy(d_ctl, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemcpy(d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        set_err(err, errlen, "control upload failed");
+        return fail(ITR_EDEVICE);
+    }
+    b->args.grids = d_grids;
+    b->args.tasks = d_tasks;
+    b->args.ctl = d_ctl;
+    b->args.prm = d_prm;
+    b->args.status = d_status;
+    b->args.ntasks = b->ntasks;
+    if (total > 0) {
+        hipLaunchKernelGGL(itr_init_kernel, dim3((total + 63) / 64), dim3(64), 0, ctx->stream, b->args, total);
+        hipLaunchKernelGGL(itr_init_skip_kernel, dim3(total), dim3(64), 0, ctx->stream, b->args);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    b->stats.band_rows = dgrids.empty() ? 0 : dgrids[0].R;
+    b->stats.waves_per_task = max_waves;
+    *out = b;
+    return ITR_OK;
+}
+
+itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char* err, size_t errlen) {
+    if (!b) return ITR_EINVAL;
+    itr_ctx* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int has_cb = on_sync ? 1 : 0;
+    if (b->prm.has_cb != has_cb) {
+        b->prm.has_cb = has_cb;
+        HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
+    }
+    const int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
+    const bool graph = ctx->use_graph != 0;
+    b->stats.steps_per_chunk = K;
+    b->stats.use_graph = graph ? 1 : 0;
+    b->stats.launches = 0;
+    b->stats.syncs = 0;
+    if (b->ncand == 0 || b->ntasks == 0) return ITR_OK;
+    if (graph && (!b->graph || b->graph_steps != K)) {
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        itr_status ls = launch_steps(b, K, err, errlen);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (ls != ITR_OK) return ls;
+        if (ce != hipSuccess) {
+            set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
+            return ITR_EDEVICE;
+        }
+        hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
+            return ITR_EDEVICE;
+        }
+        b->graph_steps = K;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    std::vector<CandCtl> snap;
+    for (;;) {
+        if (graph) {
+            HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
+        } else {
+            itr_status ls = launch_steps(b, K, err, errlen);
+            if (ls != ITR_OK) return ls;
+        }
+        b->stats.launches += (uint64_t)K;
+        HIP_TRY(hipMemcpyAsync(b->h_status, b->d_status, sizeof(uint32_t) * b->ncand, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        b->stats.syncs += 1;
+        bool any_paused = false;
+        for (int c = 0; c < b->ncand; c++)
+            if (b->h_status[c] & F_PAUSED) any_paused = true;
+        if (any_paused && on_sync) {
+            snap.resize(b->ncand);
+            HIP_TRY(hipMemcpy(snap.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+            for (int c = 0; c < b->ncand; c++) {
+                if (!(b->h_status[c] & F_PAUSED)) continue;
+                CandCtl& C = snap[c];
+                GridHost& GH = b->grids[C.grid];
+                const int nm = GH.d.n_mon;
+                std::vector<float> mm((size_t)std::max(nm, 1));
+                HIP_TRY(hipMemcpy(mm.data(), GH.d.mon_max + (size_t)C.local * nm, sizeof(float) * nm,
+                                  hipMemcpyDeviceToHost));
+                std::vector<double> md(mm.begin(), mm.begin() + nm);
+                double av = 0.0;
+                const int32_t r = on_sync(user, C.grid, C.local, C.t, C.step, C.sync_index, md.data(), nm, &av);
+                uint32_t f = C.flags & ~F_PAUSED;
+                if (r != 0) {
+                    f |= F_DONE;
+                    C.abort_kind = 2;
+                    C.abort_value = av;
+                    C.abort_sync = (int)C.sync_index;
+                    HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, abort_kind),
+                                      &C.abort_kind, sizeof(int32_t) * 3 + sizeof(int32_t), hipMemcpyHostToDevice));
+                    HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, abort_value),
+                                      &C.abort_value, sizeof(double), hipMemcpyHostToDevice));
+                }
+                HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, flags), &f, sizeof(uint32_t),
+                                  hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy((uint32_t*)b->d_status + c, &f, sizeof(uint32_t), hipMemcpyHostToDevice));
+                b->h_status[c] = f;
+            }
+        }
+        bool all_done = true;
+        for (int c = 0; c < b->ncand; c++)
+            if (!(b->h_status[c] & F_DONE)) all_done = false;
+        if (all_done) break;
+    }
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->stats.device_ms = ms;
+    b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
+    // useful cell-steps
+    std::vector<CandCtl> cc(b->ncand);
+    HIP_TRY(hipMemcpy(cc.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    double cs = 0.0;
+    for (auto& C : cc) cs += (double)C.step * (double)b->grids[C.grid].d.nx * (double)b->grids[C.grid].d.ny;
+    b->stats.cell_steps = cs;
+    return ITR_OK;
+}
+
+itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_t errlen) {
+    if (!b || n_steps < 0) return ITR_EINVAL;
+    itr_ctx* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (b->prm.has_cb) {
+        b->prm.has_cb = 0;
+        HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
+    }
+    std::vector<CandCtl> c0(b->ncand), c1(b->ncand);
+    if (b->ncand) HIP_TRY(hipMemcpy(c0.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    if (b->ntasks > 0 && n_steps > 0) {
+        itr_status ls = launch_steps(b, (int)n_steps, err, errlen);
+        if (ls != ITR_OK) return ls;
+    }
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (b->ncand) HIP_TRY(hipMemcpy(c1.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->h_status, b->d_status, sizeof(uint32_t) * b->ncand, hipMemcpyDeviceToHost));
+    double cs = 0.0;
+    for (int c = 0; c < b->ncand; c++)
+        cs += (double)(c1[c].step - c0[c].step) * (double)b->grids[c1[c].grid].d.nx * (double)b->grids[c1[c].grid].d.ny;
+    b->stats.device_ms = ms;
+    b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
+    b->stats.cell_steps = cs;
+    b->stats.launches = (uint64_t)n_steps;
+    b->stats.syncs = 1;
+    return ITR_OK;
+}
+
+itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, size_t errlen) {
+    if (!b || (!outs && b->ncand > 0)) return ITR_EINVAL;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::vector<CandCtl> cc(b->ncand);
+    if (b->ncand) HIP_TRY(hipMemcpy(cc.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    for (int c = 0; c < b->ncand; c++) {
+        const CandCtl& C = cc[c];
+        const GridHost& GH = b->grids[C.grid];
+        const DevGrid& D = GH.d;
+        itr_run_out& o = outs[c];
+        const int nx = D.nx, ny = D.ny, nm = D.n_mon;
+        o.t_end = C.t;
+        o.steps = C.step;
+        o.cell_updates = C.cell_updates;
+        o.rows_skipped = C.rows_skipped;
+        o.dt_min = C.dt_min == 1.79769313486231570815e+308 ? 0.0 : C.dt_min;
+        o.dt_max = C.dt_max;
+        o.dt_mean = C.step > 0 ? C.dt_sum / (double)C.step : 0.0;
+        o.lts = C.lts;
+        o.v0 = C.v0;
+        o.bad = C.abort_kind == 3 ? 1 : 0;
+        o.aborted = C.abort_kind;
+        o.abort_cell = C.abort_cell;
+        o.abort_sync = C.abort_sync;
+        o.abort_value = C.abort_value;
+        o.sync_count = (int32_t)C.sync_index;
+        const int nl = (int)std::min<uint32_t>(C.n_ledger, (uint32_t)b->max_ledger);
+        std::vector<double> led((size_t)std::max(nl, 1) * 8);
+        if (nl > 0)
+            HIP_TRY(hipMemcpy(led.data(), D.ledger + (size_t)C.local * b->max_ledger * 8, sizeof(double) * 8 * nl,
+                              hipMemcpyDeviceToHost));
+        const double gross = C.v0 + C.acc_rain + C.inflow;
+        const double last_res = nl > 0 ? led[(size_t)(nl - 1) * 8 + 7] : 0.0;
+        o.rel_residual = std::fabs(last_res) / std::fmax(gross, 1e-300);
+        o.n_ledger = nl;
+        if (o.ledger) {
+            const int m = std::min(nl, o.max_ledger);
+            std::memcpy(o.ledger, led.data(), sizeof(double) * 8 * (size_t)m);
+        }
+        if (nm > 0 && (o.monitor_max || o.monitor_tmax)) {
+            std::vector<float> mm(nm);
+            std::vector<double> mt(nm);
+            HIP_TRY(hipMemcpy(mm.data(), D.mon_max + (size_t)C.local * nm, sizeof(float) * nm, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(mt.data(), D.mon_tmax + (size_t)C.local * nm, sizeof(double) * nm, hipMemcpyDeviceToHost));
+            for (int q = 0; q < nm; q++) {
+                if (o.monitor_max) o.monitor_max[q] = (double)mm[q];
+                if (o.monitor_tmax) o.monitor_tmax[q] = mt[q];
+            }
+        }
+        std::vector<float> pl(3 * D.plane);
+        HIP_TRY(hipMemcpy(pl.data(), D.state + (size_t)C.local * 6 * D.plane + (size_t)C.parity * 3 * D.plane,
+                          sizeof(float) * 3 * D.plane, hipMemcpyDeviceToHost));
+        std::vector<float> hh((size_t)nx * ny);
+        for (int r = 0; r < ny; r++)
+            for (int col = 0; col < nx; col++) {
+                const size_t q = (size_t)r * D.P + col, k = (size_t)r * nx + col;
+                hh[k] = pl[q];
+                if (o.depth) o.depth[k] = pl[q];
+                if (o.qx) o.qx[k] = pl[D.plane + q];
+                if (o.qy) o.qy[k] = pl[2 * D.plane + q];
+            }
+        o.volume_m3 = host_volume(hh, nx, ny, D.area);
+    }
+    return ITR_OK;
+}
+
+itr_status itr_hip_batch_stats(itr_batch* b, itr_stats* out) {
+    if (!b || !out) return ITR_EINVAL;
+    *out = b->stats;
+    return ITR_OK;
+}
+
+void itr_hip_batch_destroy(itr_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    free_batch_mem(b);
+    delete b;
+}
+
+itr_status itr_hip_run_batch(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                             itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen) {
+    itr_batch* b = nullptr;
+    itr_status s = itr_hip_batch_create(ctx, p, grids, n_grids, &b, err, errlen);
+    if (s != ITR_OK) return s;
+    s = itr_hip_batch_run(b, on_sync, user, err, errlen);
+    if (s == ITR_OK) s = itr_hip_batch_results(b, outs, err, errlen);
+    itr_hip_batch_destroy(b);
+    return s;
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+"""Python host mirror of the reference's GPU batch backend, over the C ABI.
+
+``GpuSolver`` mirrors crates/itr-gpu/src/lib.rs (``new``, ``adapter_info``,
+``run_batch`` returning ``GpuRun`` per candidate) and ``BatchBackend.run`` mirrors the
+evaluator seam (crates/itr-opt/src/evaluator.rs:127-135, glue crates/itr-cli/src/backend.rs:48-63).
+Errors follow the reference: a declined configuration raises ``Declined`` (the
+reference's ``Err(String)``), which callers treat as "use the CPU for this level".
+Every call goes to the HIP library; there is no CPU fallback in this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from .scenario import SolverParams
+from .terrain import PreparedTerrain, TerrainBase
+
+
+class Declined(RuntimeError):
+    """The backend does not support this configuration (reference: Err(String))."""
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def _check(rc, err):
+    if rc == _abi.ITR_OK:
+        return
+    msg = err.value.decode(errors="replace") if err is not None else ""
+    if rc == _abi.ITR_DECLINED:
+        raise Declined(msg)
+    raise DeviceError(f"{_abi.STATUS_NAMES.get(rc, rc)}: {msg}")
+
+
+@dataclass
+class GpuRun:
+    """GpuRun (lib.rs:49-60) + BatchRun (evaluator.rs:118-125) + RunSummary fields."""
+    monitor_max: np.ndarray
+    monitor_tmax: np.ndarray
+    steps: int
+    t_end: float
+    volume_m3: float
+    depth: np.ndarray
+    qx: np.ndarray
+    qy: np.ndarray
+    bad: bool
+    cell_updates: int
+    rows_skipped: int
+    dt_min: float
+    dt_max: float
+    dt_mean: float
+    lts: float
+    v0: float
+    rel_residual: float
+    aborted: int
+    abort_cell: int
+    abort_sync: int
+    abort_value: float
+    sync_count: int
+    ledger: np.ndarray = field(repr=False, default=None)
+
+
+class _Keep(list):
+    def arr(self, a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        self.append(a)
+        return a
+
+
+def _params_c(p: SolverParams, keep: _Keep) -> _abi.SolverParamsC:
+    q = _abi.SolverParamsC()
+    q.duration_s, q.sync_interval_s, q.cfl, q.dt_max_s = p.duration_s, p.sync_interval_s, p.cfl, p.dt_max_s
+    q.h_dry, q.h_eps, q.infil_capacity, q.initial_depth = p.h_dry, p.h_eps, p.infil_capacity, p.initial_depth
+    q.has_initial_stage = 0 if p.initial_stage is None else 1
+    q.initial_stage = 0.0 if p.initial_stage is None else float(p.initial_stage)
+    q.cfl_hard, q.inflow_scale = p.cfl_hard, p.inflow_scale
+    rt = keep.arr([r[0] for r in p.rain] or [0.0], np.float64)
+    rr = keep.arr([r[1] for r in p.rain] or [0.0], np.float64)
+    q.rain_t, q.rain_r, q.n_rain = rt.ctypes.data_as(_abi._dp), rr.ctypes.data_as(_abi._dp), len(p.rain)
+    if p.locked_dt is not None:
+        ld = keep.arr(p.locked_dt, np.float64)
+        q.locked_dt, q.n_locked = ld.ctypes.data_as(_abi._dp), len(ld)
+    q.max_steps = int(p.max_steps)
+    q.skip_dry = 1 if p.skip_dry else 0
+    return q
+
+
+def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need_z=False) -> _abi.GridC:
+    g = _abi.GridC()
+    nx, ny = base.nx, base.ny
+    g.nx, g.ny, g.dx, g.dy, g.g, g.z_ref = nx, ny, base.dx, base.dy, base.g, base.z_ref
+    g.crest = 1 if base.crest else 0
+    g.wet_count = base.wet_count
+    f = lambda a: a.ctypes.data_as(_abi._fp)
+    g.wet = f(keep.arr(base.wet, np.float32))
+    g.gn2 = f(keep.arr(base.gn2, np.float32))
+    g.ghost = keep.arr(base.ghost_codes(), np.int32).ctypes.data_as(_abi._ip)
+    for t in terrains:
+        if t.base is not base:
+            raise Declined("all candidates must share the same TerrainBase")
+    g.n_cand = len(terrains)
+    g.dzx = f(keep.arr(np.stack([t.fx0 for t in terrains]) if terrains else np.zeros(1), np.float32))
+    g.dzy = f(keep.arr(np.stack([t.fy0 for t in terrains]) if terrains else np.zeros(1), np.float32))
+    if base.crest:
+        g.dzx_r = f(keep.arr(np.stack([t.fx1 for t in terrains]), np.float32))
+        g.dzy_r = f(keep.arr(np.stack([t.fy1 for t in terrains]), np.float32))
+    if need_z:
+        g.z_level = keep.arr(np.stack([t.z_level for t in terrains]), np.float64).ctypes.data_as(_abi._dp)
+    mon = keep.arr(np.asarray(monitors, np.uint32) if len(monitors) else np.zeros(1, np.uint32), np.uint32)
+    g.mon_cells = mon.ctypes.data_as(_abi._up)
+    g.n_mon = len(monitors)
+    if guard is not None:
+        g.baseline_max = keep.arr(guard["baseline_max"], np.float64).ctypes.data_as(_abi._dp)
+        g.guard_pos = keep.arr(guard["guard_pos"], np.int32).ctypes.data_as(_abi._ip)
+        g.n_guard = len(guard["guard_pos"])
+        g.guard_tol = float(guard["tol"])
+    return g
+
+
+class GpuSolver:
+    """GpuSolver (lib.rs:62-373) on one MI355X through the C ABI."""
+
+    def __init__(self, device: int = 0, band_rows: int = 0, steps_per_chunk: int = 0, use_graph: bool = True):
+        self._L = _abi.load()
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        _check(self._L.itr_hip_create(device, C.byref(h), err, 512), err)
+        self._h = h
+        for k, v in (("band_rows", band_rows), ("steps_per_chunk", steps_per_chunk),
+                     ("use_graph", 1 if use_graph else 0)):
+            self._L.itr_hip_set_option(self._h, k.encode(), int(v))
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.itr_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def adapter_info(self) -> str:
+        return self._L.itr_hip_device_info(self._h).decode()
+
+    def set_option(self, name: str, value: int):
+        _check(self._L.itr_hip_set_option(self._h, name.encode(), int(value)), None)
+
+    def session(self, p: SolverParams, groups, on_sync=None):
+        """Upload a batch once (inputs stay resident); ``groups`` is a list of
+        (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""
+        return BatchSession(self, p, groups, on_sync)
+
+    def run_batch(self, p: SolverParams, terrains, monitors, guard=None, on_sync=None, want_state=True):
+        """GpuSolver::run_batch: every candidate (sharing one TerrainBase) to its end."""
+        if len(terrains) == 0:
+            return []
+        s = BatchSession(self, p, [(terrains[0].base, terrains, monitors, guard)], on_sync)
+        try:
+            s.run()
+            return s.results(want_state)
+        finally:
+            s.close()
+
+
+class BatchSession:
+    def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None):
+        self.solver = solver
+        L = solver._L
+        self._L = L
+        self._keep = _Keep()
+        self.groups = groups
+        q = _params_c(p, self._keep)
+        grids = (_abi.GridC * len(groups))()
+        for i, (base, terrs, mons, guard) in enumerate(groups):
+            grids[i] = _grid_c(base, terrs, mons, self._keep, guard, p.initial_stage is not None)
+        self._q = q
+        self._grids = grids
+        err = C.create_string_buffer(1024)
+        b = C.c_void_p()
+        _check(L.itr_hip_batch_create(solver._h, C.byref(q), grids, len(groups), C.byref(b), err, 1024), err)
+        self._b = b
+        self._cb = None
+        if on_sync is not None:
+            def cb(user, grid, cand, t, step, sync_index, mon_ptr, n_mon, abort_value):
+                mon = np.ctypeslib.as_array(mon_ptr, shape=(n_mon,)).copy() if n_mon > 0 else np.zeros(0)
+                r = on_sync(int(grid), int(cand), float(t), int(step), int(sync_index), mon)
+                if r is None or r is False:
+                    return 0
+                abort_value[0] = float(r)
+                return 1
+            self._cb = _abi.SYNC_FN(cb)
+
+    def run(self):
+        err = C.create_string_buffer(1024)
+        cb = self._cb if self._cb is not None else C.cast(None, _abi.SYNC_FN)
+        _check(self._L.itr_hip_batch_run(self._b, cb, None, err, 1024), err)
+        st = _abi.StatsC()
+        self._L.itr_hip_batch_stats(self._b, C.byref(st))
+        self.stats = {k: getattr(st, k) for k, _ in _abi.StatsC._fields_}
+        self.solver.last_stats = self.stats
+        return self.stats
+
+    def advance(self, n_steps: int):
+        """Run exactly n_steps step launches (timing a resident batch); returns stats."""
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_batch_advance(self._b, int(n_steps), err, 1024), err)
+        st = _abi.StatsC()
+        self._L.itr_hip_batch_stats(self._b, C.byref(st))
+        self.stats = {k: getattr(st, k) for k, _ in _abi.StatsC._fields_}
+        return self.stats
+
+    def results(self, want_state=True):
+        n = sum(len(g[1]) for g in self.groups)
+        outs = (_abi.RunOutC * max(n, 1))()
+        bufs = []
+        k = 0
+        for base, terrs, mons, _ in self.groups:
+            for _ in terrs:
+                nm = max(len(mons), 1)
+                b = {"mm": np.zeros(nm), "mt": np.zeros(nm), "led": np.zeros((4096, 8))}
+                if want_state:
+                    b["h"] = np.zeros((base.ny, base.nx), np.float32)
+                    b["qx"] = np.zeros((base.ny, base.nx), np.float32)
+                    b["qy"] = np.zeros((base.ny, base.nx), np.float32)
+                o = outs[k]
+                o.monitor_max = b["mm"].ctypes.data_as(_abi._dp)
+                o.monitor_tmax = b["mt"].ctypes.data_as(_abi._dp)
+                o.ledger = b["led"].ctypes.data_as(_abi._dp)
+                o.max_ledger = 4096
+                if want_state:
+                    o.depth = b["h"].ctypes.data_as(_abi._fp)
+                    o.qx = b["qx"].ctypes.data_as(_abi._fp)
+                    o.qy = b["qy"].ctypes.data_as(_abi._fp)
+                bufs.append((b, len(mons)))
+                k += 1
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_batch_results(self._b, outs, err, 1024), err)
+        res = []
+        for i in range(n):
+            o = outs[i]
+            b, nm = bufs[i]
+            res.append(GpuRun(b["mm"][:nm].copy(), b["mt"][:nm].copy(), o.steps, o.t_end, o.volume_m3, b.get("h"),
+                              b.get("qx"), b.get("qy"), bool(o.bad), o.cell_updates, o.rows_skipped, o.dt_min,
+                              o.dt_max, o.dt_mean, o.lts, o.v0, o.rel_residual, o.aborted, o.abort_cell,
+                              o.abort_sync, o.abort_value, o.sync_count, b["led"][:min(o.n_ledger, 4096)].copy()))
+        return res
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._L.itr_hip_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class BatchRun:
+    """evaluator.rs:118-125."""
+    monitor_max: np.ndarray
+    steps: int
+    cell_updates: int
+    bad: bool
+
+
+class HipBatchBackend:
+    """BatchBackend<f32> over the HIP library (backend.rs:44-64 semantics)."""
+
+    def __init__(self, solver: GpuSolver | None = None):
+        self.solver = solver or GpuSolver()
+
+    def name(self) -> str:
+        return f"hip ({self.solver.adapter_info()})"
+
+    def run(self, p: SolverParams, terrains, monitors):
+        if p.inflow_scale != 1.0:
+            raise Declined("inflow scaling is not supported on the GPU backend")
+        cells = terrains[0].base.nx * terrains[0].base.ny if terrains else 0
+        return [BatchRun(r.monitor_max, r.steps, r.steps * cells, r.bad)
+                for r in self.solver.run_batch(p, terrains, monitors, want_state=False)]

@@ -1,0 +1,222 @@
+"""Solver parameters, forcing, and the synthetic workloads the evaluator is measured on.
+
+SolverParams mirrors crates/itr-hydro/src/solver.rs:26-49 with the defaults of
+validation::params (crates/itr-hydro/src/validation.rs:23-41).  The workload builders
+produce DEMs and candidate earthworks from seeded generators (SplitMix64), so every run
+of a configuration sees the same inputs.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .terrain import (FineTerrain, GeoTransform, Primitive, PreparedTerrain, TerrainBase, TerrainEdit,
+                      materialize)
+
+
+def mm_h(pts):
+    """Hyetograph::from_mm_h (crates/itr-core/src/scenario.rs:534-536): mm/h -> m/s."""
+    return [(float(t), float(r) / 1000.0 / 3600.0) for t, r in pts]
+
+
+@dataclass
+class SolverParams:
+    duration_s: float
+    sync_interval_s: float
+    cfl: float = 0.35
+    dt_max_s: float = 5.0
+    h_dry: float = 1e-6
+    h_eps: float = 1e-5
+    rain: list = field(default_factory=list)     # [(t s, rate m/s)]
+    infil_capacity: float = 0.0                   # m/s
+    initial_depth: float = 0.0
+    initial_stage: float | None = None
+    skip_dry: bool = True
+    locked_dt: np.ndarray | None = None
+    cfl_hard: float = 0.5
+    inflow_scale: float = 1.0
+    max_steps: int = 0                            # test control: stop after this many steps
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the seeded generator of the workloads."""
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, a=0.0, b=1.0) -> float:
+        return a + (b - a) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+def validation_terrain(nx, ny, dx, zfun, walls=None, n=0.035, segments=(), level=1) -> TerrainBase:
+    """validation::terrain (validation.rs:44-65): z(x, y) at cell centres, origin (0, ny*dx)."""
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    dem = np.zeros((ny, nx))
+    wm = np.zeros((ny, nx), bool)
+    for r in range(ny):
+        for c in range(nx):
+            x, y = geo.cell_center(c, r)
+            dem[r, c] = zfun(x, y)
+            if walls is not None:
+                wm[r, c] = walls(c, r)
+    return TerrainBase.build(FineTerrain.new(dem, wm, n, geo), level, segments)
+
+
+def bench_case_terrain(n: int) -> TerrainBase:
+    """The reference's `itr bench` case (crates/itr-cli/src/main.rs:682-702): dx = 2 m,
+    z = 10 + 0.002 (L - x) + 0.4 sin(x/23) cos(y/17), n = 0.035, closed walls."""
+    dx = 2.0
+    L = n * dx
+    return validation_terrain(n, n, dx, lambda x, y: 10.0 + 0.002 * (L - x) + 0.4 * math.sin(x / 23.0) * math.cos(y / 17.0))
+
+
+def bench_case_params() -> SolverParams:
+    """600 s, sync 60 s, initial depth 0.05 m, constant 50 mm/h (main.rs:689-691)."""
+    return SolverParams(600.0, 60.0, initial_depth=0.05, rain=mm_h([[0.0, 50.0]]))
+
+
+# ------------------------------------------------------------------ configurations
+def hills_dem(nx, ny, dx, rng: SplitMix64, n_hills, slope=0.01, amp=(0.5, 2.0), sigma=(15.0, 40.0),
+              noise=0.02, n_gullies=0, gully_depth=1.0, base=100.0) -> np.ndarray:
+    """Plane sloping `slope` down to the east + Gaussian hills + seeded Gaussian-section
+    gullies along random segments + uniform noise."""
+    L = nx * dx
+    x = (np.arange(nx) + 0.5) * dx
+    y = ((ny - np.arange(ny)) - 0.5) * dx
+    px, py = np.meshgrid(x, y)
+    z = base + slope * (L - px)
+    for _ in range(n_hills):
+        cx, cy = rng.uniform(0, nx * dx), rng.uniform(0, ny * dx)
+        a, s = rng.uniform(*amp), rng.uniform(*sigma)
+        z = z + a * np.exp(-((px - cx) ** 2 + (py - cy) ** 2) / (2 * s * s))
+    for _ in range(n_gullies):
+        x0, y0 = rng.uniform(0, nx * dx), rng.uniform(0, ny * dx)
+        ang = rng.uniform(0, math.pi)
+        ln = rng.uniform(0.3, 0.8) * max(nx, ny) * dx
+        x1, y1 = x0 + ln * math.cos(ang), y0 + ln * math.sin(ang)
+        vx, vy = x1 - x0, y1 - y0
+        t = np.clip(((px - x0) * vx + (py - y0) * vy) / (vx * vx + vy * vy), 0.0, 1.0)
+        d2 = (x0 + t * vx - px) ** 2 + (y0 + t * vy - py) ** 2
+        w = rng.uniform(3.0, 6.0) * dx
+        z = z - gully_depth * np.exp(-d2 / (w * w))
+    if noise > 0:
+        u = np.array([[rng.uniform(-noise, noise) for _ in range(nx)] for _ in range(ny)])
+        z = z + u
+    return z
+
+
+def random_primitives(rng: SplitMix64, k, geo: GeoTransform, nx, ny, kinds=("berm", "swale", "mound"), q=None):
+    """k primitives: centre uniform on the grid, length 10-100 m, width 2-6 m, angle
+    uniform in [0, pi), height uniform in [-0.5, 0.5] m; positions, lengths and widths
+    quantized to cells and angles to 1 degree (design.rs:259-296)."""
+    out = []
+    dx = geo.dx
+    q = dx if q is None else q
+    for i in range(k):
+        kind = kinds[i % len(kinds)]
+        cx = round(rng.uniform(0, nx * dx) / q) * q + geo.origin_x
+        cy = geo.origin_y - round(rng.uniform(0, ny * dx) / q) * q
+        length = round(rng.uniform(10.0, 100.0) / q) * q
+        width = max(round(rng.uniform(2.0, 6.0) / q) * q, 1e-6)
+        ang = float(round(math.degrees(rng.uniform(0.0, math.pi))))
+        h = rng.uniform(-0.5, 0.5)
+        h = round(h / 0.01) * 0.01
+        if kind == "mound":
+            length, ang = 0.0, 0.0
+        out.append(Primitive(kind, cx, cy, length, width, ang, h))
+    return out
+
+
+@dataclass
+class Workload:
+    name: str
+    base: TerrainBase
+    params: SolverParams
+    candidates: list           # PreparedTerrain
+    monitors: np.ndarray       # sorted unique cells
+    guard_pos: np.ndarray      # positions into monitors
+    asset_pos: list
+    edits: list = field(default_factory=list)
+
+
+def asset_and_guard_cells(nx, ny, boxes, guard_cols):
+    cells = set()
+    assets = []
+    for (c0, r0, w, h) in boxes:
+        a = [r * nx + c for r in range(r0, r0 + h) for c in range(c0, c0 + w)]
+        assets.append(a)
+        cells.update(a)
+    guard = [r * nx + c for r in range(ny) for c in guard_cols]
+    cells.update(guard)
+    mon = np.array(sorted(cells), np.uint32)
+    pos = {int(c): i for i, c in enumerate(mon)}
+    return mon, np.array([pos[c] for c in guard], np.int32), [[pos[c] for c in a] for a in assets]
+
+
+def config1(cfl=0.9) -> Workload:
+    """90x70 at 5 m; plane 1% + 6 hills + noise; 40 min triangular storm peaking at
+    60 mm/h at 15 min; n 0.035; infiltration 5 mm/h; walls N/S/W, free outflow east;
+    two 4x4 assets; 10x70 guard strip on the east side."""
+    nx, ny, dx = 90, 70, 5.0
+    rng = SplitMix64(1)
+    dem = hills_dem(nx, ny, dx, rng, 6)
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1,
+                             [{"kind": "transmissive", "side": "east"}])
+    p = SolverParams(2400.0, 60.0, cfl=cfl, rain=mm_h([[0, 0], [900, 60], [2400, 0]]),
+                     infil_capacity=5.0 / 1000.0 / 3600.0)
+    mon, gpos, apos = asset_and_guard_cells(nx, ny, [(30, 20, 4, 4), (55, 45, 4, 4)], range(nx - 10, nx))
+    return Workload("config1", base, p, [PreparedTerrain(base)], mon, gpos, apos)
+
+
+def config2(n_cand=64, seed=2) -> Workload:
+    """64 CMA-ES-like candidates on a 120x96 grid at 5 m (10 hills + 3 gullies), 3
+    primitives each; 60 min storm peaking at 80 mm/h; guard abort at 0.01 m."""
+    nx, ny, dx = 120, 96, 5.0
+    rng = SplitMix64(seed)
+    dem = hills_dem(nx, ny, dx, rng, 10, n_gullies=3)
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1,
+                             [{"kind": "transmissive", "side": "east"}])
+    p = SolverParams(3600.0, 60.0, rain=mm_h([[0, 0], [900, 80], [3600, 0]]),
+                     infil_capacity=5.0 / 1000.0 / 3600.0)
+    mon, gpos, apos = asset_and_guard_cells(nx, ny, [(40, 30, 4, 4), (70, 60, 4, 4)], range(nx - 10, nx))
+    cands, edits = [], []
+    for _ in range(n_cand):
+        prims = random_primitives(rng, 3, geo, nx, ny)
+        e = materialize(prims, geo, nx, ny, None, 1.0)
+        edits.append(e)
+        cands.append(PreparedTerrain(base).apply_edit(e))
+    return Workload("config2", base, p, cands, mon, gpos, apos, edits)
+
+
+def config5(n_cand=128, seed=5, n=512, duration=7200.0) -> Workload:
+    """128 candidates of 4 primitives on a 512x512 grid at 2 m (40 hills, 8 gullies),
+    2 h bimodal storm (peaks 50 and 90 mm/h)."""
+    nx = ny = n
+    dx = 2.0
+    rng = SplitMix64(seed)
+    dem = hills_dem(nx, ny, dx, rng, 40, n_gullies=8, sigma=(15.0, 60.0), noise=0.0)
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1,
+                             [{"kind": "transmissive", "side": "east"}])
+    p = SolverParams(duration, 60.0, rain=mm_h([[0, 0], [1800, 50], [3600, 10], [5400, 90], [7200, 0]]),
+                     infil_capacity=5.0 / 1000.0 / 3600.0)
+    mon, gpos, apos = asset_and_guard_cells(nx, ny, [(200, 200, 8, 8), (300, 350, 8, 8)], range(nx - 16, nx))
+    cands, edits = [], []
+    for _ in range(n_cand):
+        prims = random_primitives(rng, 4, geo, nx, ny)
+        e = materialize(prims, geo, nx, ny, None, 1.0)
+        edits.append(e)
+        cands.append(PreparedTerrain(base).apply_edit(e))
+    return Workload("config5", base, p, cands, mon, gpos, apos, edits)

@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every function include/itr_hip.h declares; the
+ctypes mirror of the ABI structs has the header's layout.  CPU only: no compute calls."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from inverse_terrain_engine_amd import _abi
+
+HEADER = os.path.join(ROOT, "include", "itr_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(itr_hip_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    assert "itr_hip_run_batch" in names and "itr_hip_create" in names
+    assert set(names) == set(_abi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("library not built")
+    L = C.CDLL(_abi.LIB_PATH)
+    for n in declared_functions():
+        assert hasattr(L, n), n
+    nm = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    for n in declared_functions():
+        assert re.search(rf"\bT {n}\b", nm), n
+
+
+def test_struct_layouts_match_header():
+    """Compile a probe against the header and compare sizeof/offsetof with ctypes."""
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "itr_hip.h"
+#define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
+int main(void) {
+  printf("itr_solver_params %zu\nitr_grid %zu\nitr_run_out %zu\nitr_stats %zu\n", sizeof(itr_solver_params),
+         sizeof(itr_grid), sizeof(itr_run_out), sizeof(itr_stats));
+  F(itr_solver_params, rain_t) F(itr_solver_params, max_steps) F(itr_solver_params, skip_dry)
+  F(itr_grid, wet) F(itr_grid, n_cand) F(itr_grid, dzx) F(itr_grid, mon_cells) F(itr_grid, guard_tol)
+  F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger)
+  F(itr_stats, launches) F(itr_stats, use_graph)
+  return 0;
+}
+"""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], capture_output=True, text=True).stdout.split("\n") if line)
+    ct = {"itr_solver_params": _abi.SolverParamsC, "itr_grid": _abi.GridC, "itr_run_out": _abi.RunOutC,
+          "itr_stats": _abi.StatsC}
+    for k, T in ct.items():
+        assert int(out[k]) == C.sizeof(T), k
+    for key, v in out.items():
+        if "." in key:
+            t, m = key.split(".")
+            assert getattr(ct[t], m).offset == int(v), key

@@ -1,0 +1,192 @@
+"""Bitwise parity of the HIP batch evaluator with the CPU oracle (the reference's f32
+fused kernel restated), through the C ABI."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import assert_run_equal, oracle_params, oracle_terrain, po
+from inverse_terrain_engine_amd import scenario as sc
+from inverse_terrain_engine_amd import terrain as tr
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def oracle_run(base, p, edit=None, monitors=(), observer=None):
+    return po.run(oracle_terrain(base), oracle_params(p), precision=32, edit=edit, monitors=monitors,
+                  observer=observer)
+
+
+def test_bench_case_short(solver):
+    _say("case: bench_case 64x64, 120 s")
+    base = sc.bench_case_terrain(64)
+    p = sc.bench_case_params()
+    p.duration_s, p.sync_interval_s = 120.0, 30.0
+    mon = np.arange(0, 64 * 64, 13, dtype=np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), "bench64")
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 5), (5, 1), (61, 7), (62, 9), (63, 4), (124, 6), (125, 11), (200, 3)])
+def test_shapes_at_wave_boundaries(solver, nx, ny):
+    _say(f"case: shape {nx}x{ny}")
+    base = sc.validation_terrain(nx, ny, 2.0, lambda x, y: 5.0 - 0.01 * x + 0.004 * y + 0.2 * math.sin(x / 7.0),
+                                 n=0.03, segments=[{"kind": "transmissive", "side": "east"}])
+    p = sc.SolverParams(200.0, 50.0, rain=sc.mm_h([[0, 120], [100, 40], [200, 0]]), initial_depth=0.01,
+                        infil_capacity=3e-6)
+    mon = np.arange(0, nx * ny, 3, dtype=np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), f"{nx}x{ny}")
+
+
+@pytest.mark.parametrize("band_rows", [1, 3, 8, 32])
+def test_band_shape_invariance(solver, band_rows):
+    _say(f"case: band_rows {band_rows}")
+    from inverse_terrain_engine_amd.gpu import GpuSolver
+    s = GpuSolver(0, band_rows=band_rows, steps_per_chunk=7)
+    base = sc.validation_terrain(70, 45, 2.0, lambda x, y: 3.0 - 0.01 * y + 0.2 * math.sin(x / 9.0),
+                                 n=0.035, segments=[{"kind": "transmissive", "side": "south"}])
+    p = sc.SolverParams(300.0, 50.0, rain=sc.mm_h([[0, 90], [200, 90], [250, 0]]))
+    mon = np.array(sorted([35 * 70 + 5, 20 * 70 + 20, 44 * 70 + 30]), np.uint32)
+    g = s.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    s.close()
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), f"band_rows={band_rows}")
+
+
+def test_config1_forward(solver):
+    """90x70 at 5 m, 40 min storm, infiltration, transmissive east, assets + guard."""
+    for cfl in (0.35, 0.9):
+        _say(f"case: config1 cfl={cfl}")
+        w = sc.config1(cfl)
+        g = solver.run_batch(w.params, w.candidates, w.monitors)[0]
+        assert_run_equal(g, oracle_run(w.base, w.params, monitors=w.monitors), f"config1 cfl={cfl}")
+
+
+def test_batch_matches_single_and_oracle(solver):
+    """Edited candidates in one batch: each bitwise equal to its own single run and to
+    the oracle (the reference's D2 test, crates/itr-gpu/tests/gpu.rs:81-110, made exact)."""
+    _say("case: batch of 6 edited candidates on 48x40")
+    base = sc.validation_terrain(48, 40, 2.0, lambda x, y: 3.0 - 0.01 * y + 0.2 * math.sin(x / 9.0),
+                                 n=0.035, segments=[{"kind": "transmissive", "side": "south"}])
+    p = sc.SolverParams(300.0, 50.0, rain=sc.mm_h([[0, 90], [200, 90], [250, 0]]))
+    rng = sc.SplitMix64(11)
+    geo = base.fine.geo
+    edits = [None] + [tr.materialize(sc.random_primitives(rng, 3, geo, 48, 40), geo, 48, 40, None, 1.0)
+                      for _ in range(5)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    mon = np.array(sorted([35 * 48 + 5, 20 * 48 + 20, 39 * 48 + 30]), np.uint32)
+    batch = solver.run_batch(p, terrs, mon)
+    for i, (t, e) in enumerate(zip(terrs, edits)):
+        single = solver.run_batch(p, [t], mon)[0]
+        assert_run_equal(batch[i], oracle_run(base, p, edit=None if e is None else e.as_dict(), monitors=mon),
+                         f"cand {i}")
+        assert np.array_equal(batch[i].depth.view(np.uint32), single.depth.view(np.uint32))
+        assert batch[i].steps == single.steps
+    assert not np.array_equal(batch[0].depth, batch[1].depth), "edits should change results"
+
+
+def test_guard_abort_matches_oracle(solver):
+    """Exact device-side guard abort (objective.rs:132-144) on config-2-like candidates."""
+    _say("case: guard abort, 10 candidates on 60x48")
+    nx, ny, dx = 60, 48, 5.0
+    rng = sc.SplitMix64(21)
+    dem = sc.hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
+    geo = tr.GeoTransform(0.0, ny * dx, dx, dx)
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.035, geo), 1, [{"kind": "transmissive", "side": "east"}])
+    p = sc.SolverParams(1200.0, 60.0, rain=sc.mm_h([[0, 0], [300, 80], [1200, 0]]), infil_capacity=1.4e-6)
+    mon, gpos, apos = sc.asset_and_guard_cells(nx, ny, [(20, 20, 4, 4)], range(nx - 6, nx))
+    base_run = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    guard = {"baseline_max": base_run.monitor_max, "guard_pos": gpos, "tol": 0.002}
+    edits = [tr.materialize(sc.random_primitives(rng, 3, geo, nx, ny), geo, nx, ny, None, 1.0) for _ in range(10)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    runs = solver.run_batch(p, terrs, mon, guard=guard)
+    n_abort = 0
+    for i, (r, e) in enumerate(zip(runs, edits)):
+        obs = {"baseline_max": base_run.monitor_max, "guard_pos": gpos, "tol": 0.002, "require": True}
+        o = oracle_run(base, p, edit=e.as_dict(), monitors=mon, observer=obs)
+        assert_run_equal(r, o, f"guard cand {i}")
+        n_abort += r.aborted == 1
+    _say(f"guard aborts: {n_abort} of 10")
+
+
+def test_sync_callback_abort(solver):
+    """Host sync callback (Observer::on_sync) aborts a candidate at a chosen sync index;
+    the result is the state at that sync point, as in the reference run loop."""
+    _say("case: sync callback")
+    base = sc.bench_case_terrain(40)
+    p = sc.bench_case_params()
+    p.duration_s, p.sync_interval_s = 300.0, 30.0
+    mon = np.arange(0, 1600, 11, dtype=np.uint32)
+    seen = []
+
+    def cb(grid, cand, t, step, sync_index, mon_max):
+        seen.append((cand, sync_index))
+        return 7.5 if (cand == 1 and sync_index == 4) else None
+
+    runs = solver.run_batch(p, [tr.PreparedTerrain(base), tr.PreparedTerrain(base)], mon, on_sync=cb)
+    assert runs[0].aborted == 0 and runs[0].sync_count == 10
+    assert runs[1].aborted == 2 and runs[1].sync_count == 4 and runs[1].abort_value == 7.5
+    q = sc.bench_case_params()
+    q.duration_s, q.sync_interval_s = 120.0, 30.0
+    o = oracle_run(base, q, monitors=mon)
+    assert runs[1].steps == o["steps"]
+    assert np.array_equal(runs[1].depth.astype(np.float64).view(np.uint64), o["h"].view(np.uint64))
+    assert (1, 4) in seen and (0, 10) in seen
+
+
+def test_dry_lead_in_accounting(solver):
+    """Rain starting after a dry lead-in: dry-row accounting (cell_updates, rows_skipped)
+    equals the reference rule, and the skip-free GPU sweep is bitwise the same state."""
+    _say("case: dry lead-in")
+    base = sc.validation_terrain(57, 43, 2.0, lambda x, y: 100.0 + 0.5 * math.sin(x / 9.0) * math.cos(y / 7.0)
+                                 + 0.004 * (120.0 - x), n=0.035, segments=[{"kind": "transmissive", "side": "east"}])
+    p = sc.SolverParams(900.0, 60.0, rain=sc.mm_h([[0, 0], [300, 0], [400, 80], [700, 0]]), infil_capacity=2.8e-6)
+    mon = np.array([c for c in range(57 * 43) if c % 7 == 3], np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    o = oracle_run(base, p, monitors=mon)
+    assert o["rows_skipped"] > 0
+    assert_run_equal(g, o, "dry lead-in")
+
+
+def test_crest_level(solver):
+    """Coarse fidelity level (FaceMode::Crest, level 2)."""
+    _say("case: crest level 2")
+    nx, ny = 64, 48
+    geo = tr.GeoTransform(0.0, float(ny), 1.0, 1.0)
+    dem = np.array([[0.2 * math.sin(c / 9.0) + (1.0 if c == 30 else 0.0) + 0.01 * r for c in range(nx)]
+                    for r in range(ny)])
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.03, geo), 2, [])
+    p = sc.SolverParams(120.0, 30.0, rain=sc.mm_h([[0, 60], [120, 0]]), initial_stage=0.4)
+    mon = np.arange(0, base.nx * base.ny, 5, dtype=np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), "crest")
+
+
+def test_walls_and_wide_grid(solver):
+    """512 columns (9 waves per row) with interior wall cells."""
+    _say("case: 512x24 with walls")
+    base = sc.validation_terrain(512, 24, 2.0, lambda x, y: 10.0 + 0.002 * (1024 - x) + 0.3 * math.sin(x / 23.0),
+                                 walls=lambda c, r: 200 <= c < 210 and 5 <= r < 15, n=0.035)
+    p = sc.SolverParams(60.0, 20.0, rain=sc.mm_h([[0, 80], [60, 80]]), initial_depth=0.02)
+    mon = np.arange(0, 512 * 24, 29, dtype=np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), "wide")
+
+
+def test_declines_unsupported(solver):
+    from inverse_terrain_engine_amd.gpu import Declined
+    base = sc.validation_terrain(16, 16, 1.0, lambda x, y: 0.0, n=0.03,
+                                 segments=[{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 1.0]]}])
+    with pytest.raises(Declined):
+        solver.run_batch(sc.SolverParams(10.0, 5.0), [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))
+    base = sc.validation_terrain(16, 16, 1.0, lambda x, y: 0.0, n=0.03)
+    p = sc.SolverParams(10.0, 5.0)
+    p.inflow_scale = 2.0
+    with pytest.raises(Declined):
+        solver.run_batch(p, [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))

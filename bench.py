@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--cands", type=int, default=128)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--band-rows", type=int, default=0)
+    ap.add_argument("--min-lds", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
@@ -101,6 +102,8 @@ def main():
     w = sc.config5(n_cand=a.cands, n=a.n)
     t_prep = time.time() - t0
     solver = GpuSolver(0, band_rows=a.band_rows)
+    if a.min_lds:
+        solver.set_option("min_lds", a.min_lds)
     sess = solver.session(w.params, [(w.base, w.candidates, w.monitors, None)])
     if a.warmup:
         sess.advance(a.warmup)

@@ -137,7 +137,8 @@ void itr_hip_destroy(itr_ctx* ctx);
 /* GpuSolver::adapter_info (lib.rs:126-128). */
 const char* itr_hip_device_info(itr_ctx* ctx);
 /* Tuning knobs: "band_rows" (rows per workgroup task), "steps_per_chunk" (step launches
- * per host round trip), "use_graph" (0/1: capture the chunk in a hipGraph).  0 = default. */
+ * per host round trip), "use_graph" (0/1: capture the chunk in a hipGraph), "min_lds"
+ * (bytes of LDS reserved per step workgroup, which caps occupancy).  0 = default. */
 itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value);
 
 /* GpuSolver::run_batch (lib.rs:149-372) / BatchBackend::run (evaluator.rs:134):

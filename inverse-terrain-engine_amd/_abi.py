@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "_lib", "libitr_hip.so")
+# ITR_HIP_LIB selects another build of the same library (for example a tuning variant).
+LIB_PATH = os.environ.get("ITR_HIP_LIB") or os.path.join(PKG, "_lib", "libitr_hip.so")
 
 ITR_OK, ITR_DECLINED, ITR_EINVAL, ITR_EDEVICE, ITR_ENOMEM = range(5)
 STATUS_NAMES = {0: "ok", 1: "declined", 2: "invalid argument", 3: "device error", 4: "out of memory"}

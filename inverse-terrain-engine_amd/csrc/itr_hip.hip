@@ -818,6 +818,7 @@ struct itr_ctx {
     int band_rows = 0;
     int steps_per_chunk = 0;
     int use_graph = 1;
+    int64_t min_lds = 0;   // reserve at least this much LDS per step workgroup (caps occupancy)
 };
 
 struct itr_batch {
@@ -908,7 +909,8 @@ itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errle
         return ITR_EDEVICE;
     }
     char buf[512];
-    snprintf(buf, sizeof buf, "%s (%s, %d CUs, %.0f GiB HBM, HIP %d)", prop.name, prop.gcnArchName,
+    snprintf(buf, sizeof buf, "%s (%s, %d CUs, %.0f GiB HBM, HIP %d)", prop.name[0] ? prop.name : "AMD GPU",
+             prop.gcnArchName,
              prop.multiProcessorCount, (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0), HIP_VERSION);
     c->info = buf;
     *out = c;
@@ -930,6 +932,7 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     if (n == "band_rows") ctx->band_rows = (int)value;
     else if (n == "steps_per_chunk") ctx->steps_per_chunk = (int)value;
     else if (n == "use_graph") ctx->use_graph = value != 0;
+    else if (n == "min_lds") ctx->min_lds = value;
     else return ITR_EINVAL;
     return ITR_OK;
 }
@@ -1069,7 +1072,7 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         D.P = (nx + 1 + 63) / 64 * 64;   // >= nx + 1: column nx holds the east boundary face
         D.waves = (nx + kColsPerWave - 1) / kColsPerWave;
         max_waves = std::max(max_waves, D.waves);
-        int R = ctx->band_rows > 0 ? ctx->band_rows : 8;
+        int R = ctx->band_rows > 0 ? ctx->band_rows : 12;
         R = std::max(1, std::min(R, ny));
         D.R = R;
         D.nbands = (ny + R - 1) / R;
@@ -1260,7 +1263,7 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     b->ncand = total;
     b->ntasks = (int)tasks.size();
     b->block = max_waves * kWave;
-    b->lds_bytes = max_lds;
+    b->lds_bytes = std::max(max_lds, (size_t)std::max<int64_t>(ctx->min_lds, 0));
     if (b->lds_bytes > 160 * 1024) {
         set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", b->lds_bytes);
         return fail(ITR_DECLINED);

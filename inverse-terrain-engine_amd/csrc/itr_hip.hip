@@ -2,20 +2,27 @@
 // behind the C ABI of include/itr_hip.h.
 //
 // Execution model
-//   * One step of every active candidate of every grid is ONE launch of itr_step_kernel.
-//     A workgroup owns a band of R full rows of one candidate and marches down it
-//     (register-resident sliding window in y); waves tile the row in x with 62 updated
-//     columns and one halo column on each side, exchanging neighbour values with DPP
-//     (no LDS traffic, no barrier inside the march).
-//   * The last workgroup of a candidate to finish a step (atomic ticket) runs that
-//     candidate's f64 control: ledger accounting in the reference's fixed order, CFL dt,
-//     sync clipping, exact hyetograph integral, guard abort, termination, and the f32
-//     constants of the next step.  The host only launches chunks of steps (optionally a
-//     captured hipGraph) and polls a status word per candidate between chunks.
-//   * State is ping-ponged per candidate (parity in its control block).
+//   * One step of every active candidate of every grid is ONE launch of itr_step_kernel
+//     (grids of different widths and face modes share the launch).  A workgroup owns a
+//     band of R full rows of one candidate and marches down it (register-resident
+//     sliding window in y); waves tile the row in x with 62 updated columns and one halo
+//     column on each side, exchanging neighbour values with DPP (no LDS traffic, no
+//     barrier inside the march).
+//   * The launch boundary is the only cross-workgroup synchronisation (no device-scope
+//     fences: on gfx950 those write back and invalidate the L2).  Each workgroup starts
+//     by completing its candidate's previous step from that step's results (CFL max and
+//     health flag, monitors): f64 time, dt, sync clipping, exact hyetograph integral,
+//     guard abort, termination and the next step's f32 constants.  Every workgroup of
+//     the candidate computes the same decision from the same inputs; the first band
+//     also does the row-ordered ledger sums and writes the new control block.
+//   * Everything a launch reads from the previous one is double-buffered (control
+//     blocks by launch parity; state, row partials, boundary masses and monitors by the
+//     candidate's state parity; per-step atomics in a ring of four by launch index).
+//   * The host launches chunks of steps (a captured hipGraph) and polls a status word
+//     per candidate between chunks (finished, or paused for a sync callback).
 //
 // HBM layout (per grid; P = row pitch in floats, multiple of 64):
-//   state  [cand][2][3][ny][P]  h, qx, qy (two copies)
+//   state  [cand][2][3][ny][P]  h, qx, qy (two copies, by state parity)
 //   dzx    [cand][ny][P]        west-face bed jump of cell (col); col 0 = 0
 //   dzy    [cand][ny][P]        north-face bed jump of row (row); row 0 = 0
 //   dzx2/dzy2                   crest mode second rises, same layout
@@ -72,6 +79,8 @@ struct DevParams {
 
 struct DevGrid {
     int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
+    int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
+    int32_t pad0;
     size_t plane;              // ny * P
     size_t plane_y;            // (ny + 1) * P: dzy rows 0..ny (rows 0 and ny are zero)
     float* state;              // [ncand][2][3][plane]
@@ -84,12 +93,12 @@ struct DevGrid {
     const int32_t* mon_map;
     const uint8_t* band_mon;
     const int32_t* ghost;
-    float* mon_max;            // [ncand][n_mon]
-    double* mon_tmax;          // [ncand][n_mon]
-    RowAcc* rowacc;            // [ncand][ny]
-    float* ybnd;               // [ncand][2][nx] top / bottom boundary y-face mass
-    float* sbuf;               // [ncand][plane] CFL rate at sync steps
-    double* ledger;            // [ncand][max_ledger][8]
+    float* mon_max;            // [2][nslot][n_mon]   by state parity
+    double* mon_tmax;          // [2][nslot][n_mon]
+    RowAcc* rowacc;            // [2][nslot][ny]      by state parity
+    float* ybnd;               // [2][nslot][2][nx]   top / bottom boundary y-face mass
+    float* sbuf;               // [nslot][plane] CFL rate at sync steps (within one workgroup)
+    double* ledger;            // [nslot][max_ledger][8]
     const double* baseline_max;
     const int32_t* guard_pos;
     const uint32_t* mon_cells;
@@ -97,8 +106,13 @@ struct DevGrid {
     float g32, g_half32, inv_dx32, inv_dy32;
 };
 
+__host__ __device__ inline size_t slot(const DevGrid& G, int par, int local) { return (size_t)par * G.nslot + local; }
+
+// Per-candidate control block.  Two copies per candidate, selected by launch parity:
+// launch k reads copy (k+1)&1, written by launch k-1, and writes copy k&1, so no
+// launch reads what it writes and the launch boundary orders the two.
 struct CandCtl {
-    float dt32, dtdx, dtdy, rain32, cap32;
+    float dt32, dtdx, dtdy, rain32, cap32;   // constants of the prepared step
     int32_t hits_sync, parity, grid;
     double t, t_new, dt, rain, dt_cfl, s_cur;
     uint64_t step, cell_updates, rows_skipped;
@@ -106,18 +120,26 @@ struct CandCtl {
     double v0, acc_rain, inflow, outflow, infil, fix, dt_min, dt_max, dt_sum, lts;
     int32_t abort_kind, abort_cell, abort_sync, local;
     double abort_value;
-    int32_t smax_bits;
-    uint32_t bad, ticket, pad;
+    uint32_t in_flight;   // a step was launched whose results are not yet accounted
+    uint32_t pad[3];
+};
+
+// Cross-workgroup results of one step of one candidate (atomics).  Four copies,
+// selected by launch index & 3: written in launch k, read in k+1, cleared in k+2.
+struct StepAcc {
+    int32_t smax_bits;   // max CFL rate as ordered int bits (rates are >= 0)
+    uint32_t bad;
 };
 
 struct KArgs {
     const DevGrid* grids;
     const int4* tasks;         // (grid, local cand, band, -)
-    CandCtl* ctl;
+    CandCtl* snap;             // [2][ncand]
+    StepAcc* acc;              // [4][ncand]
     const DevParams* prm;
-    uint32_t* status;          // [total cand] mirror of ctl flags for the host
+    uint32_t* status;          // [ncand] flags of the latest control block, for the host
     int32_t ntasks;
-    int32_t pad;
+    int32_t ncand;
 };
 
 // Dynamic LDS of the step kernel, in floats (offsets 16-byte aligned).
@@ -132,8 +154,11 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
     L.wet = al(L.be + R);
     L.red = al(L.wet + R);
     L.acc = al(L.red + 2 * kMaxWaves);
-    const size_t acc_n = (size_t)R * 64 > 96 ? (size_t)R * 64 : 96;   // >= 48 doubles of control scratch
+    const size_t acc_n = (size_t)R * 64;
     L.total = al(L.acc + acc_n);
+    // the control epilogue stages 2*nx floats and rows of 8 words in the same space
+    if (L.total < (size_t)2 * nx) L.total = al((size_t)2 * nx);
+    if (L.total < 64) L.total = 64;
     return L;
 }
 
@@ -143,6 +168,16 @@ __device__ __forceinline__ float from_left(float x) {   // lane i <- lane i-1 (D
 }
 __device__ __forceinline__ float from_right(float x) {  // lane i <- lane i+1 (DPP wave_shl:1)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+// Wave-uniform values into scalar registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ double uni(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 __device__ __forceinline__ int wave_max_i32(int v) {
@@ -193,8 +228,7 @@ __device__ __forceinline__ Cell fetch_cell(const ITR_GLOBAL float* __restrict__ 
     v.code_we = ghost[2 * nx + (col < 0 ? 0 : ny) + rr];
     return v;
 }
-__device__ __forceinline__ Cell ghosted(Cell v, const DevGrid& G, int row, int col, int code_n, int code_s) {
-    const int nx = G.nx, ny = G.ny;
+__device__ __forceinline__ Cell ghosted(Cell v, int nx, int ny, int row, int col, int code_n, int code_s) {
     const bool rin = row >= 0 && row < ny;
     const bool cin = col >= 0 && col < nx;
     const bool wall_x = !cin && v.code_we == 0;
@@ -210,7 +244,7 @@ __device__ __forceinline__ Cell ghosted(Cell v, const DevGrid& G, int row, int c
 __device__ __forceinline__ Cell load_cell(const ITR_GLOBAL float* __restrict__ H, const DevGrid& G,
                                           const ITR_GLOBAL int32_t* __restrict__ ghost, int row, int col,
                                           int code_n, int code_s) {
-    return ghosted(fetch_cell(H, G, ghost, row, col), G, row, col, code_n, code_s);
+    return ghosted(fetch_cell(H, G, ghost, row, col), G.nx, G.ny, row, col, code_n, code_s);
 }
 
 // Hyetograph::rate / integral (crates/itr-core/src/scenario.rs:540-585), f64.
@@ -289,229 +323,117 @@ __device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const De
     return n;
 }
 
-// The candidate's control after a completed step: accounting (solver.rs:529-545) with
-// the reference's summation order (kernel.rs:453-500), sync point (solver.rs:554-603),
-// termination test and the next step's head.  Run by the whole last workgroup; wave 0
-// computes the row-ordered sums, thread 0 the scalar logic.
-__device__ __forceinline__ void candidate_control(const KArgs& a, const DevGrid& G, int cand, CandCtl* C, double* shd,
-                                                  float* stage, size_t stage_floats, const double* rt,
-                                                  const double* rr) {
-    const DevParams& p = *a.prm;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int local = C->local;
-    const RowAcc* ra = G.rowacc + (size_t)local * G.ny;
-    const int nx = G.nx, ny = G.ny;
-    const double dt = C->dt;
-    const double dt_dy = dt * G.dy;
-    const double dt_dx = dt * G.dx;
-    {
-        // y-boundary faces (kernel.rs:453-472): FixedSum<f64> over padded columns 1..nx,
-        // staged through LDS (2*nx floats fit in the stage buffer for nx <= 992).
-        const ITR_GLOBAL float* yb_src = gp(G.ybnd + (size_t)local * 2 * nx);
-#pragma unroll 8
-        for (int i = tid; i < 2 * nx; i += blockDim.x) stage[i] = yb_src[i];
-        __syncthreads();
-        if (tid < 32) {
-            const int L = tid & 15;
-            const float* row = stage + (tid >> 4) * nx;
-            double yb = 0.0;
-#pragma unroll 8
-            for (int i = (L == 0 ? 16 : L); i <= nx; i += 16) yb += (double)row[i - 1];
-            shd[tid] = yb;
-        }
+
+// Objective::guard_breach (objective.rs:132-144) over the guard cells, one wave: the
+// first guard position (in guard order) of the largest positive exceedance
+// w = max_k - baseline_k - tol.  Returns the position, or -1 when nothing exceeds.
+__device__ __forceinline__ int guard_breach(const DevGrid& G, const float* mm, int lane, double* worst_out) {
+    bool have = false;
+    double worst = 0.0;
+    int wq = 0x7fffffff;
+    for (int q = lane; q < G.n_guard; q += kWave) {
+        const int k = G.guard_pos[q];
+        const double w = (double)mm[k] - G.baseline_max[k] - G.guard_tol;
+        if (w > 0.0 && (!have || w > worst)) { have = true; worst = w; wq = q; }
     }
-    __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const double* acc = shd + 16 * s;
-            double l8[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) l8[k] = acc[2 * k] + acc[2 * k + 1];
-            const double l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
-            const double v = ((l4[0] + l4[1]) + (l4[2] + l4[3])) * dt_dx;
-            double fin = 0.0, fout = 0.0;
-            if (s == 0) { if (v > 0.0) fin = v; else fout = -v; }
-            else if (v > 0.0) fout = v;
-            else fin = -v;
-            shd[32 + 2 * s] = fin;
-            shd[33 + 2 * s] = fout;
-        }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int oh = __shfl_xor(have ? 1 : 0, o, 64);
+        const double ow = __shfl_xor(worst, o, 64);
+        const int oq = __shfl_xor(wq, o, 64);
+        if (oh && (!have || ow > worst || (ow == worst && oq < wq))) { have = true; worst = ow; wq = oq; }
     }
-    __syncthreads();
-    {
-        // Row-ordered f64 sums of the row partials (finalize, kernel.rs:485-500).  The
-        // rows are staged through LDS in chunks by the whole workgroup; lanes 0-5 of wave
-        // 0 each run one sequential chain: infil, fix, bnd_in, bnd_out, volume, sum_s.
-        double s = 0.0;
-        if (tid == 2) s = shd[32] + shd[34];   // st.bnd_in = top.0 + bot.0
-        if (tid == 3) s = shd[33] + shd[35];   // st.bnd_out = top.1 + bot.1
-        const bool sync = C->hits_sync != 0;
-        const int chunk = max(1, (int)(stage_floats / 8));
-        const uint4* src = (const uint4*)ra;   // a RowAcc is two uint4
-        uint4* st4 = (uint4*)stage;
-        for (int r0 = 0; r0 < ny; r0 += chunk) {
-            const int nr = min(chunk, ny - r0);
-#pragma unroll 8
-            for (int i = tid; i < nr * 2; i += blockDim.x) st4[i] = src[(size_t)r0 * 2 + i];
-            __syncthreads();
-            if (tid <= 1 || (sync && (tid == 4 || tid == 5))) {
-                const int f = tid;   // RowAcc word: 0 infil, 1 fix, 4 vol, 5 sum_s
-#pragma unroll 8
-                for (int r = 0; r < nr; r++) s += (double)stage[r * 8 + f];
-            } else if (tid == 2 || tid == 3) {
-#pragma unroll 4
-                for (int r = 0; r < nr; r++) {
-                    const double w = (double)stage[r * 8 + 2] * dt_dy;
-                    const double e = (double)stage[r * 8 + 3] * dt_dy;
-                    double ai = 0.0, ao = 0.0;
-                    if (w > 0.0) ai += w; else ao -= w;
-                    if (e > 0.0) ao += e; else ai -= e;
-                    s += tid == 2 ? ai : ao;
-                }
-            }
-            __syncthreads();
-        }
-        if (tid < 6) shd[40 + tid] = s;
-    }
-    __syncthreads();
-    __shared__ int s_go;
-    if (tid == 0) {
-        const double st_infil = shd[40], st_fix = shd[41], st_in = shd[42], st_out = shd[43];
-        const double st_vol = shd[44], st_sum_s = shd[45];
-        const double area = G.area;
-        const uint64_t cells = (uint64_t)nx * (uint64_t)ny;
-        C->step += 1;
-        C->t = C->t_new;
-        C->acc_rain += C->rain * area * (double)G.wet_count;
-        C->inflow += st_in;
-        C->outflow += st_out;
-        C->infil += st_infil * area;
-        C->fix += st_fix * area;
-        C->dt_min = fmin(C->dt_min, dt);
-        C->dt_max = fmax(C->dt_max, dt);
-        C->dt_sum += dt;
-        C->cell_updates += cells - (uint64_t)C->skipped_now * (uint64_t)nx;
-        C->rows_skipped += C->skipped_now;
-        C->parity ^= 1;
-        int go = 1;
-        uint32_t flags = C->flags;
-        if (C->bad) {
-            C->abort_kind = 3;
+    *worst_out = worst;
+    return have ? wq : -1;
+}
+
+// Completion of the candidate's step launched by the previous launch (solver.rs:529-545
+// and the sync point, 554-603, except the ledger sums), then the termination test and
+// the head of the next step (prepare_step).  Every wave of every workgroup of the
+// candidate evaluates it on the same inputs, so all reach the same decision; only the
+// candidate's first band writes the result.  Returns true if this launch runs a step.
+__device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid& G, CandCtl& S, const StepAcc& A,
+                                              int lane) {
+    if (S.flags & F_DONE) return false;
+    if (S.in_flight) {
+        S.in_flight = 0;
+        S.step += 1;
+        S.t = S.t_new;
+        S.dt_min = fmin(S.dt_min, S.dt);
+        S.dt_max = fmax(S.dt_max, S.dt);
+        S.dt_sum += S.dt;
+        S.parity ^= 1;
+        uint32_t flags = S.flags;
+        bool go = true;
+        if (A.bad) {
+            S.abort_kind = 3;
             flags |= F_DONE;
-            go = 0;
+            go = false;
         } else {
-            const double smax = (double)__int_as_float(max(C->smax_bits, 0));
-            C->dt_cfl = smax > 0.0 ? p.cfl / smax : p.dt_max;
-            C->s_cur = smax;
-            if (C->hits_sync) {
-                C->sync_index += 1;
-                if (st_sum_s > 0.0) C->lts = fmax(C->lts, smax * (double)cells / st_sum_s);
-                const double v = st_vol * area;
-                const double expected = C->v0 + C->acc_rain + C->inflow - C->outflow - C->infil + C->fix;
-                if ((int)C->n_ledger < p.max_ledger) {
-                    double* row = G.ledger + ((size_t)local * p.max_ledger + C->n_ledger) * 8;
-                    row[0] = C->t; row[1] = v; row[2] = C->acc_rain; row[3] = C->inflow;
-                    row[4] = C->outflow; row[5] = C->infil; row[6] = C->fix; row[7] = v - expected;
-                }
-                C->n_ledger += 1;
-                if (G.baseline_max) {   // Objective::guard_breach (objective.rs:132-144)
-                    const float* mm = G.mon_max + (size_t)local * G.n_mon;
-                    bool have = false;
+            const double smax = (double)__int_as_float(max(A.smax_bits, 0));
+            S.dt_cfl = smax > 0.0 ? p.cfl / smax : p.dt_max;
+            S.s_cur = smax;
+            if (S.hits_sync) {
+                S.sync_index += 1;
+                if (G.baseline_max && G.n_guard > 0) {
                     double worst = 0.0;
-                    int wcell = 0;
-                    for (int q = 0; q < G.n_guard; q++) {
-                        const int k = G.guard_pos[q];
-                        const double w = (double)mm[k] - G.baseline_max[k] - G.guard_tol;
-                        if (w > 0.0 && (!have || w > worst)) { have = true; worst = w; wcell = (int)G.mon_cells[k]; }
-                    }
-                    if (have) {
-                        C->abort_kind = 1;
-                        C->abort_cell = wcell;
-                        C->abort_value = worst;
-                        C->abort_sync = (int)C->sync_index;
+                    const int q = guard_breach(G, G.mon_max + slot(G, S.parity, S.local) * G.n_mon, lane, &worst);
+                    if (q >= 0) {
+                        S.abort_kind = 1;
+                        S.abort_cell = (int)G.mon_cells[G.guard_pos[q]];
+                        S.abort_value = worst;
+                        S.abort_sync = (int)S.sync_index;
                         flags |= F_DONE;
-                        go = 0;
+                        go = false;
                     }
                 }
                 if (go && p.has_cb) flags |= F_PAUSED;
             }
-            if (go && !(C->t < p.duration && C->sync_index < p.n_sync)) { flags |= F_DONE; go = 0; }
-            if (go && p.max_steps > 0 && C->step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = 0; }
-            if (go && !prepare_step(p, G, C, rt, rr)) {
-                C->abort_kind = 4;
+            if (go && !(S.t < p.duration && S.sync_index < p.n_sync)) { flags |= F_DONE; go = false; }
+            if (go && p.max_steps > 0 && S.step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = false; }
+            if (go && !prepare_step(p, G, &S, p.rain_t, p.rain_r)) {
+                S.abort_kind = 4;
                 flags |= F_DONE;
-                go = 0;
+                go = false;
             }
         }
-        C->smax_bits = 0;
-        C->bad = 0;
-        C->ticket = 0;
-        C->flags = flags;
-        a.status[cand] = flags;
-        s_go = go;
+        S.flags = flags;
     }
-    __syncthreads();
-    if (s_go && tid < kWave) {
-        const uint32_t n = count_skippable(p, G, C, ra, lane);
-        if (lane == 0) C->skipped_now = n;
-    }
+    return !(S.flags & (F_DONE | F_PAUSED));
 }
 
 // One step of one band of one candidate (kernel.rs:590-727 semantics; the sweep order
 // differs but every face and cell value is the same IEEE expression of the old state).
+// Reads state copy `par`, writes copy par^1 and the row partials / monitors of parity
+// par^1; the band's CFL max and health flag go to the candidate's StepAcc.
 template <bool CREST>
-__global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
-    extern __shared__ float lds[];
-    // XCD-aware task order: consecutive bands of a candidate share one XCD's L2.
-    const int nblk = gridDim.x;
-    const int per = nblk >> 3;
-    const int task = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (task >= a.ntasks) return;
-    const int4 tk = a.tasks[task];
-    const DevGrid& G = a.grids[tk.x];
-    const int local = tk.y;
-    const int cand = G.cand0 + local;
-    CandCtl* C = a.ctl + cand;
-    if (C->flags & (F_DONE | F_PAUSED)) return;
-
+__device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
+                                          double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
-    const int a0 = tk.z * R;
+    const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
     const int nrows = b0 - a0;
     const int tid = threadIdx.x;
+    const int nthr = G.waves * kWave;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int col = wave * kColsPerWave - 1 + lane;
     const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
-    const bool sync = C->hits_sync != 0;
-    const bool band_mon = G.band_mon[tk.z] != 0;
-
-    StepK K;
-    K.g = G.g32;
-    K.g_half = G.g_half32;
-    K.inv_dx = G.inv_dx32;
-    K.inv_dy = G.inv_dy32;
-    K.h_dry = a.prm->h_dry;
-    K.heps2 = a.prm->heps2;
-    K.q_min = a.prm->q_min;
-    K.dt = C->dt32;
-    K.dtdx = C->dtdx;
-    K.dtdy = C->dtdy;
-    K.rain = C->rain32;
-    K.cap = C->cap32;
-    const double t_new = C->t_new;
+    const bool band_mon = G.band_mon[band] != 0;
+    const int npar = par ^ 1;
 
     const size_t cstride = 6 * G.plane;
     float* base = G.state + (size_t)local * cstride;
-    const int par = C->parity;
-    ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)(par ^ 1) * 3 * G.plane);
+    ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)npar * 3 * G.plane);
     const ITR_GLOBAL int32_t* __restrict__ MONMAP = gp(G.mon_map);
     const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
-    ITR_GLOBAL float* __restrict__ mon_max = gp(G.mon_max + (size_t)local * G.n_mon);
-    ITR_GLOBAL double* __restrict__ mon_tmax = gp(G.mon_tmax + (size_t)local * G.n_mon);
+    const ITR_GLOBAL float* __restrict__ mon_in = gp(G.mon_max + slot(G, par, local) * G.n_mon);
+    const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
+    ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
+    ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
+    const RowAcc* ra_in = G.rowacc + slot(G, par, local) * ny;
+    RowAcc* ra_out = G.rowacc + slot(G, npar, local) * ny;
+    float* yb_out = G.ybnd + slot(G, npar, local) * 2 * nx;
     // Lane columns for addressing (unsigned, so loads use a scalar row base + 32-bit
     // lane offset): ccol clamps to the interior, xcol to the x-face range 0..nx.
     const uint32_t ccol = (uint32_t)min(max(col, 0), nx - 1);
@@ -519,8 +441,13 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
     const uint32_t cb = ccol * 4u, xb = xcol * 4u;   // lane byte offsets within a row
     const uint32_t rowb = (uint32_t)P * 4u;           // row pitch in bytes
     const uint32_t planeb = (uint32_t)(G.plane * 4u);
+    const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)npar * 3 * G.plane, 3 * G.plane * 4);
+    const bool in_y = col >= 0 && col < nx;   // columns that own a y-face
+    const bool owner = in_y && lane >= 1 && lane <= kColsPerWave;
+    // Everything read from the grid descriptor is read here, before the first store:
+    // later loads of it could not be proven unclobbered and would become per-lane
+    // loads (and per-lane buffer descriptors).
     const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(base + (size_t)par * 3 * G.plane, 3 * G.plane * 4);
-    const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)(par ^ 1) * 3 * G.plane, 3 * G.plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzx = make_rsrc(G.dzx + (size_t)local * G.plane, G.plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzy = make_rsrc(G.dzy + (size_t)local * G.plane_y, G.plane_y * 4);
     const __amdgpu_buffer_rsrc_t rs_dzx2 = make_rsrc(CREST ? G.dzx2 + (size_t)local * G.plane : G.dzx, G.plane * 4);
@@ -530,8 +457,46 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
     const int code_n = GHOST[ccol];
     const int code_s = GHOST[nx + ccol];
     const size_t plane = G.plane;
-    const int nwaves = (int)(blockDim.x >> 6);
+    const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
+    const bool skip_dry = a.prm->skip_dry != 0;
+
+    // Dry band (kernel.rs:605-612 rule for every row of the band: rows a0-1 .. b0 exactly
+    // dry, ghost rows mirroring their neighbour, and no rain).  Every face of such a row
+    // is dry-dry (exact zero flux) and every cell stays (0, 0, 0), so the band's new
+    // state, row partials and boundary masses are zeros and its monitors carry over.
+    if (skip_dry && !(K.rain > 0.0f)) {
+        bool dry = true;
+        for (int r = max(a0 - 1, 0); r <= min(b0, ny - 1); r++) dry = dry && ra_in[r].wet == 0u;
+        if (dry) {
+            for (int j = a0; j < b0; j++) {
+                if (upd) {
+                    const uint32_t so = (uint32_t)j * rowb;
+                    bstore(0.0f, rs_ho, cb, so);
+                    bstore(0.0f, rs_ho, cb, so + planeb);
+                    bstore(0.0f, rs_ho, cb, so + 2 * planeb);
+                    if (band_mon) {
+                        const int mi = MONMAP[(size_t)j * P + ccol];
+                        if (mi >= 0) {
+                            mon_out[mi] = mon_in[mi];
+                            tmax_out[mi] = tmax_in[mi];
+                        }
+                    }
+                }
+            }
+            if (a0 == 0 && owner) yb_out[col] = 0.0f;
+            if (b0 == ny && owner) yb_out[nx + col] = 0.0f;
+            for (int r = tid; r < nrows; r += nthr) {
+                RowAcc q;
+                q.infil = q.fix = q.bw = q.be = q.vol = q.sum_s = 0.0f;
+                q.wet = 0u;
+                q.pad = 0u;
+                ra_out[a0 + r] = q;
+            }
+            return;
+        }
+    }
+
 
     // LDS: per-row staging of infil / fix (R x nx each) + row scalars (lds_layout).
     const LdsLayout LL = lds_layout(R, nx);
@@ -542,7 +507,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
     uint32_t* l_wet = (uint32_t*)(lds + LL.wet);
     int* l_red = (int*)(lds + LL.red);            // [2 * kMaxWaves]
     float* l_acc = lds + LL.acc;                  // [R][16][4]
-    if (tid < R) { l_wet[tid] = 0u; l_bw[tid] = 0.0f; l_be[tid] = 0.0f; }
+    for (int r = tid; r < R; r += nthr) { l_wet[r] = 0u; l_bw[r] = 0.0f; l_be[r] = 0.0f; }
     __syncthreads();
 
     // Old-state cell of this lane's column in `row`: raw loads (scalar row base + lane
@@ -558,7 +523,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
         return v;
     };
     auto fix = [&](const Cell& v, int row) -> Cell {
-        if (edge_wave || row < 0 || row >= ny) return ghosted(v, G, row, col, code_n, code_s);
+        if (edge_wave || row < 0 || row >= ny) return ghosted(v, nx, ny, row, col, code_n, code_s);
         return v;
     };
     // Face jumps are stored with zeros on boundary faces (row 0 and row ny of dzy, columns
@@ -577,9 +542,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
     Flux fn;
     if (CREST) fn = face_crest<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), dzy_at(rs_dzy2, a0), K);
     else fn = face_signed<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), K);
-    const bool in_y = col >= 0 && col < nx;   // columns that own a y-face
-    if (a0 == 0 && in_y && lane >= 1 && lane <= kColsPerWave)
-        G.ybnd[((size_t)local * 2 + 0) * nx + col] = fn.m;
+    if (a0 == 0 && owner) yb_out[col] = fn.m;
 
     // Row inputs are loaded one row ahead of their use (software pipelining).
     Cell pre_c = fetch(a0 + 1);                                   // row j+1 (raw)
@@ -634,9 +597,11 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
             if (sync) sbuf[(size_t)j * P + ccol] = u.s;
             if (band_mon) {
                 const int mi = MONMAP[(size_t)j * P + ccol];
-                if (mi >= 0 && u.h > mon_max[mi]) {   // kernel.rs:311-321 (strict >)
-                    mon_max[mi] = u.h;
-                    mon_tmax[mi] = t_new;
+                if (mi >= 0) {   // kernel.rs:311-321: strict >, time of the new state
+                    const float old = mon_in[mi];
+                    const bool up = u.h > old;
+                    mon_out[mi] = up ? u.h : old;
+                    tmax_out[mi] = up ? t_new : tmax_in[mi];
                 }
             }
         }
@@ -652,8 +617,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
         vc = vn;
         fn = fs;
     }
-    if (b0 == ny && in_y && lane >= 1 && lane <= kColsPerWave)
-        G.ybnd[((size_t)local * 2 + 1) * nx + col] = fn.m;
+    if (b0 == ny && owner) yb_out[nx + col] = fn.m;
 
     // Reductions of the band: CFL rate max (as ordered int bits; -0.0 never wins over
     // +0.0) and the health flag.
@@ -667,7 +631,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
 
     // Row FixedSums (real.rs:132-175): lane L = padded column & 15, sequential in column
     // order inside a lane, then the fixed pairwise tree.
-    for (int t = tid; t < nrows * 16; t += blockDim.x) {
+    for (int t = tid; t < nrows * 16; t += nthr) {
         const int r = t >> 4, L = t & 15;
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int j = a0 + r;
@@ -684,8 +648,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
         for (int q = 0; q < 4; q++) l_acc[((size_t)r * 16 + L) * 4 + q] = acc[q];
     }
     __syncthreads();
-    if (tid < nrows) {
-        const int r = tid;
+    for (int r = tid; r < nrows; r += nthr) {
         float tot[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -706,67 +669,247 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a) {
         q.sum_s = tot[3];
         q.wet = l_wet[r];
         q.pad = 0;
-        G.rowacc[(size_t)local * ny + a0 + r] = q;
+        ra_out[a0 + r] = q;
     }
     if (tid == 0) {
         int m = 0, bd = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        for (int w = 0; w < nwaves; w++) {
             m = max(m, l_red[w]);
             bd |= l_red[kMaxWaves + w];
         }
-        if (m > 0) atomicMax(&C->smax_bits, m);
-        if (bd) atomicOr(&C->bad, 1u);
+        if (m > 0) atomicMax(&acc_cur->smax_bits, m);
+        if (bd) atomicOr(&acc_cur->bad, 1u);
     }
-    // No device-scope fence here: on gfx950 that is an L2 write-back + invalidate per
-    // executing thread.  The candidate's control runs in the next kernel of the stream
-    // (itr_control_kernel), whose launch boundary makes these results visible.
+    // No device-scope fence: on gfx950 that is an L2 write-back + invalidate.  The
+    // candidate's next launch reads these results; the launch boundary orders them.
 }
 
-// Per-candidate control after each step: one workgroup (one wave) per candidate.
-constexpr int kCtlThreads = 64;
-constexpr int kCtlStageRows = 256;
-constexpr int kCtlMaxRain = 128;
-__global__ void __launch_bounds__(kCtlThreads) itr_control_kernel(KArgs a, int32_t ncand_total) {
-    __shared__ double shd[64];
-    __shared__ float stage[kCtlStageRows * 8];
-    __shared__ double s_rt[kCtlMaxRain], s_rr[kCtlMaxRain];
-    const int cand = blockIdx.x;
-    if (cand >= ncand_total) return;
-    CandCtl* C = a.ctl + cand;
-    if (C->flags & (F_DONE | F_PAUSED)) return;
-    const DevGrid& G = a.grids[C->grid];
+// Ledger accounting of the completed step (kernel.rs:453-500 sums, solver.rs:529-540,
+// 554-592), run by the candidate's first band after its own step.  Row-ordered f64
+// sums of the row partials and the FixedSum<f64> of the boundary y-faces, then thread
+// 0 adds them to the new control block (in LDS).  `Sin` is the block of the previous
+// launch (the completed step's dt, rain, dry-row count).
+__device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G, const CandCtl* Sin, const StepAcc& A,
+                                                CandCtl& Oc, float* stage, size_t stage_floats, double* shd) {
+    const CandCtl& Si = *Sin;
+    CandCtl* O = &Oc;
     const DevParams& p = *a.prm;
-    // The hyetograph is walked point by point: stage it in LDS (short) or read it in place.
-    const bool in_lds = p.n_rain <= kCtlMaxRain;
-    if (in_lds)
-        for (int k = threadIdx.x; k < p.n_rain; k += blockDim.x) { s_rt[k] = p.rain_t[k]; s_rr[k] = p.rain_r[k]; }
-    __syncthreads();
-    candidate_control(a, G, cand, C, shd, stage, (size_t)kCtlStageRows * 8, in_lds ? s_rt : p.rain_t,
-                      in_lds ? s_rr : p.rain_r);
-}
-
-// Head of the first step of every candidate.
-__global__ void itr_init_kernel(KArgs a, int32_t ncand_total) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncand_total) return;
-    CandCtl* C = a.ctl + c;
-    const DevGrid& G = a.grids[C->grid];
-    if (C->flags & F_DONE) return;
-    if (!prepare_step(*a.prm, G, C, a.prm->rain_t, a.prm->rain_r)) {
-        C->abort_kind = 4;
-        C->flags |= F_DONE;
+    const int tid = threadIdx.x;
+    const int nthr = G.waves * kWave;
+    const int local = Si.local;
+    const int par = Si.parity ^ 1;   // parity the completed step wrote
+    const RowAcc* ra = G.rowacc + slot(G, par, local) * G.ny;
+    const int nx = G.nx, ny = G.ny;
+    const double dt = Si.dt;
+    const double dt_dy = dt * G.dy;
+    const double dt_dx = dt * G.dx;
+    __syncthreads();   // the band step is done with the LDS
+    {
+        // y-boundary faces (kernel.rs:453-472): FixedSum<f64> over padded columns 1..nx.
+        const float* yb_src = G.ybnd + slot(G, par, local) * 2 * nx;
+        for (int i = tid; i < 2 * nx; i += nthr) stage[i] = yb_src[i];
+        __syncthreads();
+        if (tid < 32) {
+            const int L = tid & 15;
+            const float* row = stage + (tid >> 4) * nx;
+            double yb = 0.0;
+            for (int i = (L == 0 ? 16 : L); i <= nx; i += 16) yb += (double)row[i - 1];
+            shd[tid] = yb;
+        }
     }
-    a.status[c] = C->flags;
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const double* acc = shd + 16 * s;
+            double l8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) l8[k] = acc[2 * k] + acc[2 * k + 1];
+            const double l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
+            const double v = ((l4[0] + l4[1]) + (l4[2] + l4[3])) * dt_dx;
+            double fin = 0.0, fout = 0.0;
+            if (s == 0) { if (v > 0.0) fin = v; else fout = -v; }
+            else if (v > 0.0) fout = v;
+            else fin = -v;
+            shd[32 + 2 * s] = fin;
+            shd[33 + 2 * s] = fout;
+        }
+    }
+    __syncthreads();
+    const bool sync = Si.hits_sync != 0 && !A.bad;
+    {
+        // Row-ordered f64 sums of the row partials (finalize, kernel.rs:485-500), staged
+        // through LDS in chunks; lanes 0-5 of wave 0 each run one sequential chain:
+        // infil, fix, bnd_in, bnd_out, volume, sum_s.
+        double s = 0.0;
+        if (tid == 2) s = shd[32] + shd[34];   // st.bnd_in = top.0 + bot.0
+        if (tid == 3) s = shd[33] + shd[35];   // st.bnd_out = top.1 + bot.1
+        const int chunk = max(1, (int)(stage_floats / 8));
+        for (int r0 = 0; r0 < ny; r0 += chunk) {
+            const int nr = min(chunk, ny - r0);
+            const uint4* src = (const uint4*)(ra + r0);   // a RowAcc is two uint4
+            uint4* st4 = (uint4*)stage;
+            for (int i = tid; i < nr * 2; i += nthr) st4[i] = src[i];
+            __syncthreads();
+            if (tid <= 1 || (sync && (tid == 4 || tid == 5))) {
+                const int f = tid;   // RowAcc word: 0 infil, 1 fix, 4 vol, 5 sum_s
+#pragma unroll 8
+                for (int r = 0; r < nr; r++) s += (double)stage[r * 8 + f];
+            } else if (tid == 2 || tid == 3) {
+#pragma unroll 4
+                for (int r = 0; r < nr; r++) {
+                    const double w = (double)stage[r * 8 + 2] * dt_dy;
+                    const double e = (double)stage[r * 8 + 3] * dt_dy;
+                    double ai = 0.0, ao = 0.0;
+                    if (w > 0.0) ai += w; else ao -= w;
+                    if (e > 0.0) ao += e; else ai -= e;
+                    s += tid == 2 ? ai : ao;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid < 6) shd[40 + tid] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double st_infil = shd[40], st_fix = shd[41], st_in = shd[42], st_out = shd[43];
+        const double st_vol = shd[44], st_sum_s = shd[45];
+        const double area = G.area;
+        const uint64_t cells = (uint64_t)nx * (uint64_t)ny;
+        O->acc_rain += Si.rain * area * (double)G.wet_count;
+        O->inflow += st_in;
+        O->outflow += st_out;
+        O->infil += st_infil * area;
+        O->fix += st_fix * area;
+        O->cell_updates += cells - (uint64_t)Si.skipped_now * (uint64_t)nx;
+        O->rows_skipped += Si.skipped_now;
+        if (sync) {
+            const double smax = (double)__int_as_float(max(A.smax_bits, 0));
+            if (st_sum_s > 0.0) O->lts = fmax(O->lts, smax * (double)cells / st_sum_s);
+            const double v = st_vol * area;
+            const double expected = O->v0 + O->acc_rain + O->inflow - O->outflow - O->infil + O->fix;
+            if ((int)O->n_ledger < p.max_ledger) {
+                double* row = G.ledger + ((size_t)local * p.max_ledger + O->n_ledger) * 8;
+                row[0] = O->t; row[1] = v; row[2] = O->acc_rain; row[3] = O->inflow;
+                row[4] = O->outflow; row[5] = O->infil; row[6] = O->fix; row[7] = v - expected;
+            }
+            O->n_ledger += 1;
+        }
+    }
 }
 
-// Dry-row count of the first step (same rule as in candidate_control).
-__global__ void itr_init_skip_kernel(KArgs a) {
+// One launch = one step of every active candidate.  A workgroup first completes the
+// candidate's previous step from its results (identically in every workgroup of the
+// candidate), then, unless the candidate finished or waits for the host, runs its band
+// of the next step.  The candidate's first band also runs the ledger accounting and
+// writes the new control block.  The control block is worked on in LDS (it is 60 words;
+// held in registers it would set the kernel's register budget).  lk = launch index & 3.
+// MODES: face modes present in the batch (1 signed, 2 crest, 3 both); a batch of one
+// mode gets a kernel without the other mode's code and its register budget.
+constexpr int kCtlWords = (int)(sizeof(CandCtl) / 4);
+template <int MODES>
+__global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a, uint32_t lk) {
+    extern __shared__ float lds[];
+    __shared__ double shd[48];
+    __shared__ CandCtl sS;
+    __shared__ int s_go;
+    // XCD-aware task order: consecutive bands of a candidate share one XCD's L2.
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int task = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (task >= a.ntasks) return;
+    const int4 tk = a.tasks[task];
+    const DevGrid& G = a.grids[tk.x];
+    const int tid = threadIdx.x;
+    if ((tid >> 6) >= G.waves) return;   // the launch is sized for the widest grid
+    const int nthr = G.waves * kWave;
+    const int local = tk.y;
+    const int cand = G.cand0 + local;
+    const bool designated = tk.z == 0;
+    const DevParams& p = *a.prm;
+    const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
+    CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
+    if ((Sin->flags & F_DONE) && !designated) return;
+    const StepAcc A = a.acc[(size_t)((lk + 3) & 3) * a.ncand + cand];
+    const bool completed = Sin->in_flight != 0 && !(Sin->flags & F_DONE);
+    for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)&sS)[w] = ((const uint32_t*)Sin)[w];
+    __syncthreads();
+    if (tid < kWave) {
+        const bool go = complete_step(p, G, sS, A, tid);
+        // dry-row count of the prepared step (rows of the state just completed)
+        if (designated && completed && !(sS.flags & F_DONE))
+            sS.skipped_now = count_skippable(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
+        if (tid == 0) {
+            sS.in_flight = go ? 1u : 0u;
+            s_go = go ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    if (s_go) {
+        // The step's constants are wave-uniform: keep them in scalar registers.
+        StepK K;
+        K.g = G.g32;
+        K.g_half = G.g_half32;
+        K.inv_dx = G.inv_dx32;
+        K.inv_dy = G.inv_dy32;
+        K.h_dry = p.h_dry;
+        K.heps2 = p.heps2;
+        K.q_min = p.q_min;
+        K.dt = uni(sS.dt32);
+        K.dtdx = uni(sS.dtdx);
+        K.dtdy = uni(sS.dtdy);
+        K.rain = uni(sS.rain32);
+        K.cap = uni(sS.cap32);
+        const double t_new = uni(sS.t_new);
+        const bool sync = uni(sS.hits_sync) != 0;
+        const int par = uni(sS.parity);
+        StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
+        if (MODES == 3) {
+            if (G.crest) band_step<true>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            else band_step<false>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+        } else {
+            band_step<MODES == 2>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+        }
+    } else if (!designated) {
+        return;
+    }
+    if (!designated) return;
+    if (completed) {
+        const LdsLayout LL = lds_layout(G.R, G.nx);
+        ledger_epilogue(a, G, Sin, A, sS, lds, LL.total, shd);
+    }
+    __syncthreads();
+    for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)Sout)[w] = ((const uint32_t*)&sS)[w];
+    if (tid == 0) {
+        StepAcc z;
+        z.smax_bits = 0;
+        z.bad = 0u;
+        a.acc[(size_t)((lk + 2) & 3) * a.ncand + cand] = z;
+        a.status[cand] = sS.flags;
+    }
+}
+
+// Head of the first step of every candidate: control copy 1 (read by launch 0).
+__global__ void itr_init_kernel(KArgs a) {
     const int c = blockIdx.x;
-    CandCtl* C = a.ctl + c;
+    CandCtl* C = a.snap + (size_t)a.ncand + c;
     const DevGrid& G = a.grids[C->grid];
-    if (C->flags & F_DONE) return;
-    const uint32_t n = count_skippable(*a.prm, G, C, G.rowacc + (size_t)C->local * G.ny, threadIdx.x & 63);
-    if (threadIdx.x == 0) C->skipped_now = n;
+    if (C->flags & F_DONE) {
+        if (threadIdx.x == 0) a.status[c] = C->flags;
+        return;
+    }
+    CandCtl S = *C;
+    bool ok = prepare_step(*a.prm, G, &S, a.prm->rain_t, a.prm->rain_r);
+    if (!ok) {
+        S.abort_kind = 4;
+        S.flags |= F_DONE;
+    } else {
+        S.skipped_now = count_skippable(*a.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
+    }
+    if (threadIdx.x == 0) {
+        *C = S;
+        a.status[c] = S.flags;
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -828,7 +971,8 @@ struct itr_batch {
     KArgs args{};
     void* d_grids = nullptr;
     void* d_tasks = nullptr;
-    void* d_ctl = nullptr;
+    void* d_snap = nullptr;         // CandCtl [2][ncand]
+    void* d_acc = nullptr;          // StepAcc [4][ncand]
     void* d_prm = nullptr;
     void* d_status = nullptr;
     void* d_rain_t = nullptr;
@@ -839,8 +983,9 @@ struct itr_batch {
     int ntasks = 0;
     int block = 64;
     size_t lds_bytes = 0;
-    int crest = 0;
+    int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both
     int max_ledger = 0;
+    uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     itr_stats stats{};
@@ -852,7 +997,8 @@ namespace {
 void free_batch_mem(itr_batch* b) {
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
-    void* ps[] = {b->d_grids, b->d_tasks, b->d_ctl, b->d_prm, b->d_status, b->d_rain_t, b->d_rain_r, b->d_locked};
+    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_prm, b->d_status, b->d_rain_t, b->d_rain_r,
+                  b->d_locked};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (b->h_status) (void)hipHostFree(b->h_status);
@@ -868,17 +1014,93 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
     return ITR_OK;
 }
 
-itr_status launch_steps(itr_batch* b, int nsteps, char* err, size_t errlen) {
+// Enqueue n step launches whose launch indices start at lk0 (mod 4).
+itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
     const int nblk = ((b->ntasks + 7) / 8) * 8;
-    for (int s = 0; s < nsteps; s++) {
-        if (b->crest)
-            hipLaunchKernelGGL(itr_step_kernel<true>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args);
+    for (int s = 0; s < n; s++) {
+        const uint32_t lk = (uint32_t)((lk0 + (uint64_t)s) & 3u);
+        if (b->modes == 1)
+            hipLaunchKernelGGL(itr_step_kernel<1>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+        else if (b->modes == 2)
+            hipLaunchKernelGGL(itr_step_kernel<2>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
         else
-            hipLaunchKernelGGL(itr_step_kernel<false>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args);
-        hipLaunchKernelGGL(itr_control_kernel, dim3(b->ncand), dim3(kCtlThreads), 0, b->ctx->stream, b->args, b->ncand);
+            hipLaunchKernelGGL(itr_step_kernel<3>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
     }
     HIP_TRY(hipGetLastError());
     return ITR_OK;
+}
+
+// n launches continuing the batch's launch sequence: graph chunks of `K` launches when
+// a graph is captured and the sequence is at a multiple of 4, single launches otherwise.
+itr_status launch_steps(itr_batch* b, int64_t n, char* err, size_t errlen) {
+    while (n > 0) {
+        if (b->graph && b->graph_steps <= n && (b->launches & 3u) == 0) {
+            HIP_TRY(hipGraphLaunch(b->graph, b->ctx->stream));
+            b->launches += (uint64_t)b->graph_steps;
+            n -= b->graph_steps;
+        } else {
+            int64_t k = n;   // single launches: the remainder, or up to the next multiple of 4
+            if (b->graph && (b->launches & 3u) != 0) k = std::min<int64_t>(n, 4 - (int64_t)(b->launches & 3u));
+            itr_status s = enqueue_steps(b, (int)k, b->launches, err, errlen);
+            if (s != ITR_OK) return s;
+            b->launches += (uint64_t)k;
+            n -= k;
+        }
+    }
+    return ITR_OK;
+}
+
+// The current control copy of every candidate.
+itr_status read_ctl(itr_batch* b, std::vector<CandCtl>& out, char* err, size_t errlen) {
+    out.resize(b->ncand);
+    if (b->ncand == 0) return ITR_OK;
+    const size_t half = b->launches == 0 ? 1 : (size_t)((b->launches - 1) & 1u);
+    HIP_TRY(hipMemcpy(out.data(), (CandCtl*)b->d_snap + half * b->ncand, sizeof(CandCtl) * b->ncand,
+                      hipMemcpyDeviceToHost));
+    return ITR_OK;
+}
+CandCtl* cur_ctl_dev(itr_batch* b, int c) {
+    const size_t half = b->launches == 0 ? 1 : (size_t)((b->launches - 1) & 1u);
+    return (CandCtl*)b->d_snap + half * b->ncand + c;
+}
+
+// Capture K launches (launch indices 0..K-1 mod 4) in a hipGraph; K = 0 drops it.
+itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
+    if (K == b->graph_steps && (K == 0 || b->graph)) return ITR_OK;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->graph_steps = 0;
+    if (K == 0 || b->ntasks == 0) return ITR_OK;
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+    itr_status ls = enqueue_steps(b, K, 0, err, errlen);
+    hipError_t ce = hipStreamEndCapture(b->ctx->stream, &g);
+    if (ls != ITR_OK) return ls;
+    if (ce != hipSuccess) {
+        set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
+        return ITR_EDEVICE;
+    }
+    hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
+        return ITR_EDEVICE;
+    }
+    b->graph_steps = K;
+    return ITR_OK;
+}
+
+// Cell-steps run between two control snapshots: completed steps plus the step in
+// flight, each of nx*ny cells (the reference counts steps * cells, gpubench.rs:28).
+double cell_steps_between(const itr_batch* b, const std::vector<CandCtl>& c0, const std::vector<CandCtl>& c1) {
+    double cs = 0.0;
+    for (size_t c = 0; c < c1.size(); c++) {
+        const double n1 = (double)(c1[c].step + c1[c].in_flight), n0 = (double)(c0[c].step + c0[c].in_flight);
+        cs += (n1 - n0) * (double)b->grids[c1[c].grid].d.nx * (double)b->grids[c1[c].grid].d.ny;
+    }
+    return cs;
 }
 
 }  // namespace
@@ -953,7 +1175,6 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         set_err(err, errlen, "invalid solver parameters");
         return ITR_EINVAL;
     }
-    int crest = -1;
     for (int g = 0; g < n_grids; g++) {
         const itr_grid& G = grids[g];
         if (G.nx <= 0 || G.ny <= 0 || G.n_cand < 0 || !G.wet || !G.gn2 || !G.ghost || !G.dzx || !G.dzy) {
@@ -968,11 +1189,6 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
             set_err(err, errlen, "grid %d: crest mode needs dzx_r and dzy_r", g);
             return ITR_EINVAL;
         }
-        if (crest >= 0 && crest != (G.crest ? 1 : 0)) {
-            set_err(err, errlen, "grids of one batch must share the face mode");
-            return ITR_DECLINED;
-        }
-        crest = G.crest ? 1 : 0;
         for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++)
             if (G.ghost[i] == 2 || G.ghost[i] == 3) {
                 set_err(err, errlen, "Stage and Inflow boundaries are not supported on this backend");
@@ -1000,7 +1216,6 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     }
     auto* b = new itr_batch();
     b->ctx = ctx;
-    b->crest = crest;
     auto fail = [&](itr_status s) {
         free_batch_mem(b);
         delete b;
@@ -1078,6 +1293,7 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         D.nbands = (ny + R - 1) / R;
         D.cand0 = total;
         D.ncand = nc;
+        D.nslot = std::max(nc, 1);
         D.n_mon = I.n_mon;
         D.n_guard = I.baseline_max ? I.n_guard : 0;
         D.crest = I.crest;
@@ -1116,7 +1332,7 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
             mon_map[(size_t)r * D.P + c] = q;
             band_mon[r / R] = 1;
         }
-        std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);
+        std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);   // parity 0 half
         std::memset(rowacc.data(), 0, rowacc.size() * sizeof(RowAcc));
         for (int k = 0; k < nc; k++) {
             const float* sx = I.dzx + (size_t)k * ny * (nx + 1);
@@ -1197,10 +1413,10 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         ALLOC(d_monmap, plane);
         ALLOC(d_bandmon, (size_t)D.nbands);
         ALLOC(d_ghost, (size_t)(2 * nx + 2 * ny));
-        ALLOC(d_monmax, ncx * std::max(I.n_mon, 1));
-        ALLOC(d_montmax, ncx * std::max(I.n_mon, 1));
-        ALLOC(d_rowacc, ncx * ny);
-        ALLOC(d_ybnd, ncx * 2 * nx);
+        ALLOC(d_monmax, 2 * ncx * std::max(I.n_mon, 1));
+        ALLOC(d_montmax, 2 * ncx * std::max(I.n_mon, 1));
+        ALLOC(d_rowacc, 2 * ncx * ny);
+        ALLOC(d_ybnd, 2 * ncx * 2 * nx);
         ALLOC(d_sbuf, ncx * plane);
         ALLOC(d_ledger, ncx * b->max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
@@ -1223,9 +1439,10 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         if (I.baseline_max)
             ok = ok && up(d_base, I.baseline_max, sizeof(double) * I.n_mon) &&
                  up(d_gpos, I.guard_pos, sizeof(int32_t) * I.n_guard);
-        ok = ok && hipMemset(d_monmax, 0, ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
-             hipMemset(d_montmax, 0, ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
-             hipMemset(d_ybnd, 0, ncx * 2 * nx * 4) == hipSuccess;
+        ok = ok && hipMemset(d_monmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
+             hipMemset(d_montmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
+             hipMemset(d_ybnd, 0, 2 * ncx * 2 * nx * 4) == hipSuccess &&
+             hipMemset(d_rowacc + ncx * ny, 0, ncx * ny * sizeof(RowAcc)) == hipSuccess;
         if (!ok) {
             set_err(err, errlen, "upload of grid %d failed", g);
             return fail(ITR_EDEVICE);
@@ -1253,18 +1470,14 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         const size_t lds = lds_layout(R, nx).total * sizeof(float);
         max_lds = std::max(max_lds, lds);
     }
-    if (n_grids > 1) {
-        for (auto& D : dgrids)
-            if (D.waves != max_waves) {
-                set_err(err, errlen, "grids of different widths in one batch are not supported yet");
-                return fail(ITR_DECLINED);
-            }
-    }
     b->ncand = total;
     b->ntasks = (int)tasks.size();
+    b->modes = 0;
+    for (auto& D : dgrids) b->modes |= D.crest ? 2 : 1;
+    if (b->modes == 0) b->modes = 1;
     b->block = max_waves * kWave;
     b->lds_bytes = std::max(max_lds, (size_t)std::max<int64_t>(ctx->min_lds, 0));
-    if (b->lds_bytes > 160 * 1024) {
+    if (b->lds_bytes > 160 * 1024 - 1024) {   // static shared memory of the kernel: < 1 KiB
         set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", b->lds_bytes);
         return fail(ITR_DECLINED);
     }
@@ -1272,15 +1485,18 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     itr_status s;
     DevGrid* d_grids;
     int4* d_tasks;
-    CandCtl* d_ctl;
+    CandCtl* d_snap;
+    StepAcc* d_acc;
     DevParams* d_prm;
     uint32_t* d_status;
     if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
     b->d_grids = d_grids;
     if ((s = dev_alloc(misc, &d_tasks, tasks.size(), err, errlen)) != ITR_OK) return fail(s);
     b->d_tasks = d_tasks;
-    if ((s = dev_alloc(misc, &d_ctl, std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
-    b->d_ctl = d_ctl;
+    if ((s = dev_alloc(misc, &d_snap, 2 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
+    b->d_snap = d_snap;
+    if ((s = dev_alloc(misc, &d_acc, 4 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
+    b->d_acc = d_acc;
     if ((s = dev_alloc(misc, &d_prm, 1, err, errlen)) != ITR_OK) return fail(s);
     b->d_prm = d_prm;
     if ((s = dev_alloc(misc, &d_status, std::max(total, 1), err, errlen)) != ITR_OK) return fail(s);
@@ -1291,7 +1507,10 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     }
     bool ok = hipMemcpy(d_grids, dgrids.data(), dgrids.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
               (tasks.empty() || hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
-              (ctl.empty() || hipMemcpy(d_ctl, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess) &&
+              (ctl.empty() || (hipMemcpy(d_snap, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess &&
+                               hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
+                                         hipMemcpyHostToDevice) == hipSuccess)) &&
+              hipMemset(d_acc, 0, 4 * std::max<size_t>(ctl.size(), 1) * sizeof(StepAcc)) == hipSuccess &&
               hipMemcpy(d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         set_err(err, errlen, "control upload failed");
@@ -1299,13 +1518,14 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     }
     b->args.grids = d_grids;
     b->args.tasks = d_tasks;
-    b->args.ctl = d_ctl;
+    b->args.snap = d_snap;
+    b->args.acc = d_acc;
     b->args.prm = d_prm;
     b->args.status = d_status;
     b->args.ntasks = b->ntasks;
+    b->args.ncand = total;
     if (total > 0) {
-        hipLaunchKernelGGL(itr_init_kernel, dim3((total + 63) / 64), dim3(64), 0, ctx->stream, b->args, total);
-        hipLaunchKernelGGL(itr_init_skip_kernel, dim3(total), dim3(64), 0, ctx->stream, b->args);
+        hipLaunchKernelGGL(itr_init_kernel, dim3(total), dim3(64), 0, ctx->stream, b->args);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
@@ -1324,48 +1544,26 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
         b->prm.has_cb = has_cb;
         HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
     }
-    const int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
+    int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
+    K = (K + 3) / 4 * 4;   // chunks keep the launch sequence aligned to the 4-deep ring
     const bool graph = ctx->use_graph != 0;
     b->stats.steps_per_chunk = K;
     b->stats.use_graph = graph ? 1 : 0;
     b->stats.launches = 0;
     b->stats.syncs = 0;
     if (b->ncand == 0 || b->ntasks == 0) return ITR_OK;
-    if (graph && (!b->graph || b->graph_steps != K)) {
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-        }
-        hipGraph_t g;
-        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        itr_status ls = launch_steps(b, K, err, errlen);
-        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-        if (ls != ITR_OK) return ls;
-        if (ce != hipSuccess) {
-            set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
-            return ITR_EDEVICE;
-        }
-        hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
-            set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
-            return ITR_EDEVICE;
-        }
-        b->graph_steps = K;
-    }
+    itr_status s = prepare_graph(b, graph ? K : 0, err, errlen);
+    if (s != ITR_OK) return s;
+    std::vector<CandCtl> c0;
+    if ((s = read_ctl(b, c0, err, errlen)) != ITR_OK) return s;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(e0, ctx->stream));
-    std::vector<CandCtl> snap;
     for (;;) {
-        if (graph) {
-            HIP_TRY(hipGraphLaunch(b->graph, ctx->stream));
-        } else {
-            itr_status ls = launch_steps(b, K, err, errlen);
-            if (ls != ITR_OK) return ls;
-        }
+        // Complete every in-flight step first: a chunk always ends on the launch grid.
+        if ((s = launch_steps(b, K, err, errlen)) != ITR_OK) return s;
         b->stats.launches += (uint64_t)K;
         HIP_TRY(hipMemcpyAsync(b->h_status, b->d_status, sizeof(uint32_t) * b->ncand, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1374,34 +1572,30 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
         for (int c = 0; c < b->ncand; c++)
             if (b->h_status[c] & F_PAUSED) any_paused = true;
         if (any_paused && on_sync) {
-            snap.resize(b->ncand);
-            HIP_TRY(hipMemcpy(snap.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+            std::vector<CandCtl> snap;
+            if ((s = read_ctl(b, snap, err, errlen)) != ITR_OK) return s;
             for (int c = 0; c < b->ncand; c++) {
                 if (!(b->h_status[c] & F_PAUSED)) continue;
                 CandCtl& C = snap[c];
                 GridHost& GH = b->grids[C.grid];
                 const int nm = GH.d.n_mon;
                 std::vector<float> mm((size_t)std::max(nm, 1));
-                HIP_TRY(hipMemcpy(mm.data(), GH.d.mon_max + (size_t)C.local * nm, sizeof(float) * nm,
-                                  hipMemcpyDeviceToHost));
+                if (nm > 0)
+                    HIP_TRY(hipMemcpy(mm.data(), GH.d.mon_max + slot(GH.d, C.parity, C.local) * nm, sizeof(float) * nm,
+                                      hipMemcpyDeviceToHost));
                 std::vector<double> md(mm.begin(), mm.begin() + nm);
                 double av = 0.0;
                 const int32_t r = on_sync(user, C.grid, C.local, C.t, C.step, C.sync_index, md.data(), nm, &av);
-                uint32_t f = C.flags & ~F_PAUSED;
+                C.flags &= ~F_PAUSED;
                 if (r != 0) {
-                    f |= F_DONE;
+                    C.flags |= F_DONE;
                     C.abort_kind = 2;
                     C.abort_value = av;
                     C.abort_sync = (int)C.sync_index;
-                    HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, abort_kind),
-                                      &C.abort_kind, sizeof(int32_t) * 3 + sizeof(int32_t), hipMemcpyHostToDevice));
-                    HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, abort_value),
-                                      &C.abort_value, sizeof(double), hipMemcpyHostToDevice));
                 }
-                HIP_TRY(hipMemcpy((char*)b->d_ctl + sizeof(CandCtl) * c + offsetof(CandCtl, flags), &f, sizeof(uint32_t),
-                                  hipMemcpyHostToDevice));
-                HIP_TRY(hipMemcpy((uint32_t*)b->d_status + c, &f, sizeof(uint32_t), hipMemcpyHostToDevice));
-                b->h_status[c] = f;
+                HIP_TRY(hipMemcpy(cur_ctl_dev(b, c), &C, sizeof(CandCtl), hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy((uint32_t*)b->d_status + c, &C.flags, sizeof(uint32_t), hipMemcpyHostToDevice));
+                b->h_status[c] = C.flags;
             }
         }
         bool all_done = true;
@@ -1418,12 +1612,9 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     (void)hipEventDestroy(e1);
     b->stats.device_ms = ms;
     b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
-    // useful cell-steps
-    std::vector<CandCtl> cc(b->ncand);
-    HIP_TRY(hipMemcpy(cc.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
-    double cs = 0.0;
-    for (auto& C : cc) cs += (double)C.step * (double)b->grids[C.grid].d.nx * (double)b->grids[C.grid].d.ny;
-    b->stats.cell_steps = cs;
+    std::vector<CandCtl> c1;
+    if ((s = read_ctl(b, c1, err, errlen)) != ITR_OK) return s;
+    b->stats.cell_steps = cell_steps_between(b, c0, c1);
     return ITR_OK;
 }
 
@@ -1435,17 +1626,19 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
         b->prm.has_cb = 0;
         HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
     }
-    std::vector<CandCtl> c0(b->ncand), c1(b->ncand);
-    if (b->ncand) HIP_TRY(hipMemcpy(c0.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
+    K = (K + 3) / 4 * 4;
+    itr_status s = prepare_graph(b, ctx->use_graph ? K : 0, err, errlen);
+    if (s != ITR_OK) return s;
+    std::vector<CandCtl> c0, c1;
+    if ((s = read_ctl(b, c0, err, errlen)) != ITR_OK) return s;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(e0, ctx->stream));
-    if (b->ntasks > 0 && n_steps > 0) {
-        itr_status ls = launch_steps(b, (int)n_steps, err, errlen);
-        if (ls != ITR_OK) return ls;
-    }
+    if (b->ntasks > 0 && n_steps > 0)
+        if ((s = launch_steps(b, n_steps, err, errlen)) != ITR_OK) return s;
     HIP_TRY(hipEventRecord(e1, ctx->stream));
     HIP_TRY(hipEventSynchronize(e1));
     const auto t1 = std::chrono::steady_clock::now();
@@ -1453,24 +1646,24 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (b->ncand) HIP_TRY(hipMemcpy(c1.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    if ((s = read_ctl(b, c1, err, errlen)) != ITR_OK) return s;
     HIP_TRY(hipMemcpy(b->h_status, b->d_status, sizeof(uint32_t) * b->ncand, hipMemcpyDeviceToHost));
-    double cs = 0.0;
-    for (int c = 0; c < b->ncand; c++)
-        cs += (double)(c1[c].step - c0[c].step) * (double)b->grids[c1[c].grid].d.nx * (double)b->grids[c1[c].grid].d.ny;
     b->stats.device_ms = ms;
     b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
-    b->stats.cell_steps = cs;
+    b->stats.cell_steps = cell_steps_between(b, c0, c1);
     b->stats.launches = (uint64_t)n_steps;
     b->stats.syncs = 1;
+    b->stats.steps_per_chunk = K;
+    b->stats.use_graph = ctx->use_graph ? 1 : 0;
     return ITR_OK;
 }
 
 itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, size_t errlen) {
     if (!b || (!outs && b->ncand > 0)) return ITR_EINVAL;
     HIP_TRY(hipSetDevice(b->ctx->device));
-    std::vector<CandCtl> cc(b->ncand);
-    if (b->ncand) HIP_TRY(hipMemcpy(cc.data(), b->d_ctl, sizeof(CandCtl) * b->ncand, hipMemcpyDeviceToHost));
+    std::vector<CandCtl> cc;
+    itr_status s = read_ctl(b, cc, err, errlen);
+    if (s != ITR_OK) return s;
     for (int c = 0; c < b->ncand; c++) {
         const CandCtl& C = cc[c];
         const GridHost& GH = b->grids[C.grid];
@@ -1508,8 +1701,9 @@ itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, siz
         if (nm > 0 && (o.monitor_max || o.monitor_tmax)) {
             std::vector<float> mm(nm);
             std::vector<double> mt(nm);
-            HIP_TRY(hipMemcpy(mm.data(), D.mon_max + (size_t)C.local * nm, sizeof(float) * nm, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(mt.data(), D.mon_tmax + (size_t)C.local * nm, sizeof(double) * nm, hipMemcpyDeviceToHost));
+            const size_t off = slot(D, C.parity, C.local) * nm;
+            HIP_TRY(hipMemcpy(mm.data(), D.mon_max + off, sizeof(float) * nm, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(mt.data(), D.mon_tmax + off, sizeof(double) * nm, hipMemcpyDeviceToHost));
             for (int q = 0; q < nm; q++) {
                 if (o.monitor_max) o.monitor_max[q] = (double)mm[q];
                 if (o.monitor_tmax) o.monitor_tmax[q] = mt[q];

@@ -200,6 +200,69 @@ def config2(n_cand=64, seed=2) -> Workload:
     return Workload("config2", base, p, cands, mon, gpos, apos, edits)
 
 
+def refine_dem(dem: np.ndarray, f: int) -> np.ndarray:
+    """Upsample a DEM by an integer factor, bilinear between cell centres (the refinement
+    the reference's revalidation uses, crates/itr-cli/src/revalidate.rs:19-38; every
+    value here is finite, so its nearest-parent fallback never applies)."""
+    ny, nx = dem.shape
+    y = np.clip((np.arange(ny * f) + 0.5) / f - 0.5, 0.0, ny - 1.0)
+    x = np.clip((np.arange(nx * f) + 0.5) / f - 0.5, 0.0, nx - 1.0)
+    r0 = np.floor(y).astype(np.int64)
+    c0 = np.floor(x).astype(np.int64)
+    wy = (y - r0)[:, None]
+    wx = (x - c0)[None, :]
+    r1 = np.minimum(r0 + 1, ny - 1)
+    c1 = np.minimum(c0 + 1, nx - 1)
+    z00, z01 = dem[r0][:, c0], dem[r0][:, c1]
+    z10, z11 = dem[r1][:, c0], dem[r1][:, c1]
+    return (1.0 - wy) * ((1.0 - wx) * z00 + wx * z01) + wy * ((1.0 - wx) * z10 + wx * z11)
+
+
+@dataclass
+class Ladder:
+    """A mixed fidelity-ladder batch: one group per grid (TerrainBase, candidates,
+    monitors, guard-or-None), in the order the groups are uploaded."""
+    name: str
+    params: SolverParams
+    groups: list
+    edits: list
+
+
+def config3(n_inst=256, seed=3, duration=2400.0) -> Ladder:
+    """Uneven fidelity ladder: 45x35 at 10 m (face-crest coarse level 2), 90x70 at 5 m
+    and 180x140 at 2.5 m (bilinear refinement of the 5 m DEM) in ratio 4:2:1 over
+    n_inst instances (146/73/37 for 256); 2-4 primitives each, materialized on each
+    grid's own fine raster; config-1 storm and infiltration."""
+    nx, ny, dx = 90, 70, 5.0
+    rng = SplitMix64(seed)
+    dem = hills_dem(nx, ny, dx, rng, 6)
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    segs = [{"kind": "transmissive", "side": "east"}]
+    fine5 = FineTerrain.new(dem, None, 0.035, geo)
+    geo2 = GeoTransform(0.0, ny * dx, dx / 2, dx / 2)
+    fine25 = FineTerrain.new(refine_dem(dem, 2), None, 0.035, geo2)
+    levels = [(fine5, 2), (fine5, 1), (fine25, 1)]
+    n2 = round(n_inst * 2 / 7)
+    n1 = round(n_inst * 1 / 7)
+    counts = [n_inst - n2 - n1, n2, n1]
+    p = SolverParams(duration, 60.0, rain=mm_h([[0, 0], [900, 60], [2400, 0]]),
+                     infil_capacity=5.0 / 1000.0 / 3600.0)
+    groups, edits = [], []
+    for (fine, lv), n in zip(levels, counts):
+        base = TerrainBase.build(fine, lv, segs)
+        mon, _, _ = asset_and_guard_cells(base.nx, base.ny, [(base.nx // 3, base.ny // 3, 2, 2)],
+                                          range(base.nx - 2, base.nx))
+        cands = []
+        for _ in range(n):
+            k = 2 + int(rng.uniform() * 3) % 3
+            prims = random_primitives(rng, k, fine.geo, fine.nx, fine.ny)
+            e = materialize(prims, fine.geo, fine.nx, fine.ny, None, 1.0)
+            edits.append(e)
+            cands.append(PreparedTerrain(base).apply_edit(e))
+        groups.append((base, cands, mon, None))
+    return Ladder("config3", p, groups, edits)
+
+
 def config5(n_cand=128, seed=5, n=512, duration=7200.0) -> Workload:
     """128 candidates of 4 primitives on a 512x512 grid at 2 m (40 hills, 8 gullies),
     2 h bimodal storm (peaks 50 and 90 mm/h)."""

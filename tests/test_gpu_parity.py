@@ -190,3 +190,56 @@ def test_declines_unsupported(solver):
     p.inflow_scale = 2.0
     with pytest.raises(Declined):
         solver.run_batch(p, [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))
+
+
+def test_fidelity_ladder_one_launch(solver):
+    """Grids of three widths and both face modes (45x35 crest, 90x70 and 180x140 signed)
+    in one batch, one launch per step: every instance bitwise equal to the oracle."""
+    _say("case: fidelity ladder 4/2/1 instances, 900 s")
+    lad = sc.config3(n_inst=7, duration=900.0)
+    sess = solver.session(lad.params, lad.groups)
+    try:
+        sess.run()
+        res = sess.results()
+    finally:
+        sess.close()
+    k = 0
+    for base, cands, mon, _ in lad.groups:
+        for _ in cands:
+            o = oracle_run(base, lad.params, edit=lad.edits[k].as_dict(), monitors=mon)
+            assert_run_equal(res[k], o, f"ladder instance {k} ({base.nx}x{base.ny}, crest={base.crest})")
+            k += 1
+
+
+@pytest.mark.parametrize("use_graph,chunk", [(False, 5), (True, 4), (True, 12)])
+def test_launch_modes(use_graph, chunk):
+    """Single launches and graph chunks of several sizes give the same bits."""
+    _say(f"case: use_graph={use_graph} chunk={chunk}")
+    from inverse_terrain_engine_amd.gpu import GpuSolver
+    s = GpuSolver(0, steps_per_chunk=chunk, use_graph=use_graph)
+    base = sc.bench_case_terrain(48)
+    p = sc.bench_case_params()
+    p.duration_s, p.sync_interval_s = 90.0, 30.0
+    mon = np.arange(0, 48 * 48, 17, dtype=np.uint32)
+    g = s.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    s.close()
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), f"graph={use_graph} chunk={chunk}")
+
+
+def test_advance_then_run(solver):
+    """A resident batch advanced by a step count off the launch grid, then run to the
+    end, equals one uninterrupted run."""
+    _say("case: advance 13 then run")
+    base = sc.bench_case_terrain(40)
+    p = sc.bench_case_params()
+    p.duration_s, p.sync_interval_s = 90.0, 30.0
+    mon = np.arange(0, 1600, 9, dtype=np.uint32)
+    sess = solver.session(p, [(base, [tr.PreparedTerrain(base)], mon, None)])
+    try:
+        st = sess.advance(13)
+        assert st["cell_steps"] == 13 * 1600
+        sess.run()
+        g = sess.results()[0]
+    finally:
+        sess.close()
+    assert_run_equal(g, oracle_run(base, p, monitors=mon), "advance+run")

@@ -24,6 +24,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 BYTES_PER_CELL_STEP = 32   # read h, qx, qy, dzx, dzy + write h, qx, qy (f32)
+VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s: one per CU per cycle (packed FP32 not used)
+COPY_PROBE_BYTES = 1 << 30 # plain device-to-device copy used to calibrate the HBM counters
 METRIC = "batched f32 cell-steps/s + % HBM peak; adjoint grad cost vs fwd at tol"
 
 
@@ -74,6 +76,81 @@ def _run_cands(po, ot, params, edits, threads, monitors):
     return wall, steps, cu
 
 
+def _pmc_pass(counter, steps):
+    """One rocprofv3 counter pass over a short run of this benchmark (plus the copy
+    probe); returns {kernel name: [per-dispatch values]} or None if it did not run."""
+    import csv
+    import shutil
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    out = tempfile.mkdtemp(prefix="itr_pmc_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc",
+           "--", sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", "2", "--no-cpu",
+           "--no-traffic", "--copy-probe"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+    path = None
+    for root, _, files in os.walk(out):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if path is None:
+        print(f"pmc pass {counter} produced no counters (rc {r.returncode}): {r.stderr[-300:]}", file=sys.stderr)
+        return None
+    vals = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    shutil.rmtree(out, ignore_errors=True)
+    return vals
+
+
+def measure_traffic(kernel_us, steps=6):
+    """HBM bytes per step launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by what the same counter reads for a plain 1 GiB device copy; and the VALU
+    instruction count per launch (SQ_INSTS_VALU, wave64 instructions)."""
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
+        v = _pmc_pass(counter, steps)
+        if v is None:
+            return None
+        step = [x for k, xs in v.items() if "itr_step_kernel" in k for x in xs]
+        copy = [x for k, xs in v.items() if "copyBuffer" in k for x in xs]
+        if not step:
+            return None
+        res[counter] = (sum(step) / len(step), max(copy) if copy else None)
+    f_kib, f_copy = res["FETCH_SIZE"]
+    w_kib, w_copy = res["WRITE_SIZE"]
+    if not f_copy or not w_copy:
+        return None
+    kf = COPY_PROBE_BYTES / (f_copy * 1024.0)
+    kw = COPY_PROBE_BYTES / (w_copy * 1024.0)
+    fetch_b = f_kib * 1024.0 * kf
+    write_b = w_kib * 1024.0 * kw
+    valu = res["SQ_INSTS_VALU"][0]
+    return {"fetch_bytes": fetch_b, "write_bytes": write_b, "copy_scale_fetch": kf, "copy_scale_write": kw,
+            "valu_insts": valu, "valu_frac": valu / (kernel_us * 1e-6 * VALU_PEAK)}
+
+
+def copy_probe():
+    """A plain device-to-device copy of COPY_PROBE_BYTES through the HIP runtime."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(COPY_PROBE_BYTES)
+    ok = hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    ok = ok and hip.hipMemset(a, 1, n) == 0 and hip.hipMemcpy(b, a, n, 3) == 0   # 3: device to device
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    if not ok:
+        raise RuntimeError("copy probe failed")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,9 +161,11 @@ def main():
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--min-lds", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -107,11 +186,13 @@ def main():
     sess = solver.session(w.params, [(w.base, w.candidates, w.monitors, None)])
     if a.warmup:
         sess.advance(a.warmup)
-    try:
-        import torch
-        torch.cuda.synchronize()
-    except Exception:
-        torch = None
+    torch = None
+    if not a.copy_probe:   # the counter passes run without torch
+        try:
+            import torch
+            torch.cuda.synchronize()
+        except Exception:
+            torch = None
     t1 = time.perf_counter()
     st = sess.advance(a.steps)
     if torch is not None:
@@ -159,6 +240,21 @@ def main():
         },
     }
     solver.close()
+    if a.copy_probe:
+        copy_probe()
+    if not a.no_traffic:
+        t = measure_traffic(kernel_s * 1e6)
+        if t is not None:
+            cells = cells_per_launch
+            out["roofline"]["traffic"] = t["fetch_bytes"] + t["write_bytes"]
+            out["roofline"]["traffic_detail"] = {
+                "fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
+                "bytes_per_cell_step": (t["fetch_bytes"] + t["write_bytes"]) / cells,
+                "copy_scale_fetch": t["copy_scale_fetch"], "copy_scale_write": t["copy_scale_write"],
+                "dram_gbs": (t["fetch_bytes"] + t["write_bytes"]) / kernel_s / 1e9}
+            out["roofline"]["valu"] = {
+                "insts_per_launch": t["valu_insts"], "insts_per_cell_step_lane": t["valu_insts"] * 64 / cells,
+                "peak_insts_per_s": VALU_PEAK, "frac": t["valu_frac"]}
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         v, cw, cs = cpu_baseline(w, threads, a.cpu_cands, a.cpu_steps)

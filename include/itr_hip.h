@@ -160,6 +160,44 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
 itr_status itr_hip_batch_stats(itr_batch* b, itr_stats* out);
 void itr_hip_batch_destroy(itr_batch* b);
 
+/* ---------------------------------------------------------------- discrete adjoint
+ * No counterpart in the reference (its search is gradient-free): the gradient of the
+ * peak-depth risk term J_risk = sum_a w_a tau softplus((peak_a - eta_a) / tau)
+ * (crates/itr-core/src/objective.rs:70-128) with respect to the bed jump of every face
+ * and, mapped through z_R - z_L, of every cell's elevation, for each candidate of one
+ * grid.  The forward runs on the locked dt schedule p->locked_dt (solver.rs:452-455; a
+ * frozen schedule makes the discrete step a fixed function of the state), stores
+ * checkpoints every ceil(sqrt(N)) steps, replays each segment and sweeps it backwards;
+ * adjoint arithmetic is f32, face-gradient accumulation f64.  Signed face mode only
+ * (crest levels return ITR_DECLINED).  The earthwork term and the chain rule to
+ * primitive heights are host work on the returned cell gradient. */
+typedef struct {
+    int32_t n_assets;
+    int32_t reserved;
+    const int32_t* asset_start;   /* [n_assets + 1]: offsets into asset_pos */
+    const int32_t* asset_pos;     /* positions into the grid's monitor list (itr_grid.mon_cells) */
+    const double* threshold_m;    /* [n_assets] eta_a */
+    const double* weight;         /* [n_assets] w_a */
+    double tau_m;
+} itr_objective;
+
+typedef struct {
+    double j_risk;                /* J_risk of the forward run */
+    double* peak_m;               /* [n_assets] asset peaks, or NULL */
+    double* dJ_dz_face_x;         /* [ny][nx + 1] dJ_risk / d(z_R - z_L) of x-faces, or NULL */
+    double* dJ_dz_face_y;         /* [ny + 1][nx] of y-faces, or NULL */
+    double* dJ_dz_cell;           /* [ny][nx] dJ_risk / dz of each cell (0 on walls), or NULL */
+    uint64_t steps;               /* N */
+    int32_t segment_steps;        /* checkpoint spacing ceil(sqrt(N)) */
+    int32_t checkpoints;
+    uint64_t replay_mismatches;   /* replayed segment ends that differ bitwise from the checkpoint (0) */
+    double plain_forward_ms;      /* the same forward without checkpoints (device time) */
+    double forward_ms, replay_ms, reverse_ms;
+} itr_adjoint_out;
+
+itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,
+                           itr_adjoint_out* outs, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

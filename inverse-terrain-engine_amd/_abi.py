@@ -54,12 +54,24 @@ class StatsC(C.Structure):
                 ("waves_per_task", C.c_int32), ("steps_per_chunk", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class ObjectiveC(C.Structure):
+    _fields_ = [("n_assets", C.c_int32), ("reserved", C.c_int32), ("asset_start", _ip), ("asset_pos", _ip),
+                ("threshold_m", _dp), ("weight", _dp), ("tau_m", C.c_double)]
+
+
+class AdjointOutC(C.Structure):
+    _fields_ = [("j_risk", C.c_double), ("peak_m", _dp), ("dJ_dz_face_x", _dp), ("dJ_dz_face_y", _dp),
+                ("dJ_dz_cell", _dp), ("steps", C.c_uint64), ("segment_steps", C.c_int32), ("checkpoints", C.c_int32),
+                ("replay_mismatches", C.c_uint64), ("plain_forward_ms", C.c_double), ("forward_ms", C.c_double),
+                ("replay_ms", C.c_double), ("reverse_ms", C.c_double)]
+
+
 SYNC_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_uint32, _dp,
                       C.c_int32, _dp)
 
 EXPORTS = ["itr_hip_create", "itr_hip_destroy", "itr_hip_device_info", "itr_hip_set_option", "itr_hip_run_batch",
            "itr_hip_batch_create", "itr_hip_batch_run", "itr_hip_batch_results", "itr_hip_batch_stats",
-           "itr_hip_batch_destroy", "itr_hip_batch_advance"]
+           "itr_hip_batch_destroy", "itr_hip_batch_advance", "itr_hip_adjoint"]
 
 _lib = None
 
@@ -96,6 +108,9 @@ def load():
     L.itr_hip_batch_stats.argtypes = [vp, C.POINTER(StatsC)]
     L.itr_hip_batch_advance.restype = C.c_int32
     L.itr_hip_batch_advance.argtypes = [vp, C.c_int64, C.c_char_p, C.c_size_t]
+    L.itr_hip_adjoint.restype = C.c_int32
+    L.itr_hip_adjoint.argtypes = [vp, C.POINTER(SolverParamsC), C.POINTER(GridC), C.POINTER(ObjectiveC),
+                                  C.POINTER(AdjointOutC), C.c_char_p, C.c_size_t]
     L.itr_hip_batch_destroy.restype = None
     L.itr_hip_batch_destroy.argtypes = [vp]
     _lib = L

@@ -41,6 +41,7 @@
 
 #include "../../include/itr_hip.h"
 #include "itr_numerics.h"
+#include "itr_adjoint.h"
 
 using namespace itr;
 
@@ -80,7 +81,7 @@ struct DevParams {
 struct DevGrid {
     int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
     int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
-    int32_t pad0;
+    int32_t max_log;           // capacity of steplog per candidate (0: no log)
     size_t plane;              // ny * P
     size_t plane_y;            // (ny + 1) * P: dzy rows 0..ny (rows 0 and ny are zero)
     float* state;              // [ncand][2][3][plane]
@@ -99,6 +100,7 @@ struct DevGrid {
     float* ybnd;               // [2][nslot][2][nx]   top / bottom boundary y-face mass
     float* sbuf;               // [nslot][plane] CFL rate at sync steps (within one workgroup)
     double* ledger;            // [nslot][max_ledger][8]
+    struct StepRec* steplog;   // [nslot][max_log] constants of every step run (adjoint replay)
     const double* baseline_max;
     const int32_t* guard_pos;
     const uint32_t* mon_cells;
@@ -107,6 +109,12 @@ struct DevGrid {
 };
 
 __host__ __device__ inline size_t slot(const DevGrid& G, int par, int local) { return (size_t)par * G.nslot + local; }
+
+// The f32 constants and new time of one step, as the forward used them.
+struct StepRec {
+    float dt32, dtdx, dtdy, rain32, cap32, pad;
+    double t_new;
+};
 
 // Per-candidate control block.  Two copies per candidate, selected by launch parity:
 // launch k reads copy (k+1)&1, written by launch k-1, and writes copy k&1, so no
@@ -842,6 +850,17 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a, uint32_t lk) {
         if (tid == 0) {
             sS.in_flight = go ? 1u : 0u;
             s_go = go ? 1 : 0;
+            if (go && designated && G.steplog && sS.step < (uint64_t)G.max_log) {
+                StepRec q;
+                q.dt32 = sS.dt32;
+                q.dtdx = sS.dtdx;
+                q.dtdy = sS.dtdy;
+                q.rain32 = sS.rain32;
+                q.cap32 = sS.cap32;
+                q.pad = 0.0f;
+                q.t_new = sS.t_new;
+                G.steplog[(size_t)local * G.max_log + sS.step] = q;
+            }
         }
     }
     __syncthreads();
@@ -910,6 +929,222 @@ __global__ void itr_init_kernel(KArgs a) {
         *C = S;
         a.status[c] = S.flags;
     }
+}
+
+// ------------------------------------------------------------------ discrete adjoint
+// Per-cell kernels of the reverse sweep (one thread per cell, grid x = column blocks,
+// y = rows, z = candidates).  The forward replay recomputes a step cell by cell with
+// the same IEEE operations as the band kernel (every face value is a function of its
+// two cells only), so the replayed states are bitwise the forward's; the built-in
+// check compares each replayed segment end with the stored checkpoint.
+struct AdjArgs {
+    const float* U;        // old state [cand][3][plane] (h, qx, qy)
+    float* Uo;             // replay: new state
+    const float* lam;      // adjoint of the new state [cand][3][plane]
+    float* lam1;           // adjoint after the sources, before the flux divergence
+    float* lamo;           // adjoint of the old state
+    double* gx;            // [cand][ny][P] dJ/d(dzx) (west face of each cell)
+    double* gy;            // [cand][ny+1][P] dJ/d(dzy) (north face of each cell)
+    const float* dzx;      // [cand][ny][P]
+    const float* dzy;      // [cand][ny+1][P]
+    const float* wet;      // [ny][P]
+    const float* gn2;      // [ny][P]
+    const int32_t* ghost;  // [2nx + 2ny]
+    int32_t nx, ny, P, pad;
+    size_t plane, cstride; // plane = ny*P, cstride = 3*plane
+    size_t dzx_stride, dzy_stride;
+    StepK K;
+};
+
+struct Nb { float h, qx, qy; };
+
+// Old-state value of cell (r, c) or of the ghost next to the interior (boundary.rs:12-87:
+// wall mirrors the normal discharge, transmissive copies); never a corner here.
+__device__ __forceinline__ Nb nb_at(const float* U, const AdjArgs& a, int r, int c, bool* wall) {
+    int rr = r, cc = c, code = 1;
+    bool gx_ = false, gy_ = false;
+    if (c < 0 || c >= a.nx) {
+        cc = c < 0 ? 0 : a.nx - 1;
+        code = a.ghost[2 * a.nx + (c < 0 ? 0 : a.ny) + r];
+        gx_ = true;
+    } else if (r < 0 || r >= a.ny) {
+        rr = r < 0 ? 0 : a.ny - 1;
+        code = a.ghost[(r < 0 ? 0 : a.nx) + c];
+        gy_ = true;
+    }
+    const size_t o = (size_t)rr * a.P + cc;
+    Nb v{U[o], U[a.plane + o], U[2 * a.plane + o]};
+    if (gx_ && code == 0) v.qx = -v.qx;
+    if (gy_ && code == 0) v.qy = -v.qy;
+    if (wall) *wall = code == 0;
+    return v;
+}
+
+__device__ __forceinline__ adj::Consts<float> adj_consts(const StepK& K) {
+    adj::Consts<float> k;
+    k.g = K.g;
+    k.g_half = K.g_half;
+    k.heps2 = K.heps2;
+    k.h_dry = K.h_dry;
+    k.q_min = K.q_min;
+    k.dt = K.dt;
+    k.dtdx = K.dtdx;
+    k.dtdy = K.dtdy;
+    k.rain = K.rain;
+    k.cap = K.cap;
+    return k;
+}
+
+struct Stencil {
+    Nb C, W, E, N, S;
+    bool wW, wE, wN, wS;
+    Vel vC, vW, vE, vN, vS;
+    float dzW, dzE, dzN, dzS, wet, gn2;
+};
+
+__device__ __forceinline__ Stencil stencil(const AdjArgs& a, const float* U, int cand, int r, int c) {
+    Stencil s;
+    s.C = nb_at(U, a, r, c, nullptr);
+    s.W = nb_at(U, a, r, c - 1, &s.wW);
+    s.E = nb_at(U, a, r, c + 1, &s.wE);
+    s.N = nb_at(U, a, r - 1, c, &s.wN);
+    s.S = nb_at(U, a, r + 1, c, &s.wS);
+    s.vC = cell_velocity(s.C.h, s.C.qx, s.C.qy, a.K);
+    s.vW = cell_velocity(s.W.h, s.W.qx, s.W.qy, a.K);
+    s.vE = cell_velocity(s.E.h, s.E.qx, s.E.qy, a.K);
+    s.vN = cell_velocity(s.N.h, s.N.qx, s.N.qy, a.K);
+    s.vS = cell_velocity(s.S.h, s.S.qx, s.S.qy, a.K);
+    const float* dzx = a.dzx + (size_t)cand * a.dzx_stride;
+    const float* dzy = a.dzy + (size_t)cand * a.dzy_stride;
+    s.dzW = dzx[(size_t)r * a.P + c];
+    s.dzE = dzx[(size_t)r * a.P + c + 1];
+    s.dzN = dzy[(size_t)r * a.P + c];
+    s.dzS = dzy[(size_t)(r + 1) * a.P + c];
+    s.wet = a.wet[(size_t)r * a.P + c];
+    s.gn2 = a.gn2[(size_t)r * a.P + c];
+    return s;
+}
+
+// Forward replay of one step (kernel.rs:590-727 semantics, bitwise).
+__global__ void __launch_bounds__(256) itr_replay_kernel(AdjArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, cand = blockIdx.z;
+    if (c >= a.nx) return;
+    const float* U = a.U + (size_t)cand * a.cstride;
+    const Stencil s = stencil(a, U, cand, r, c);
+    const Flux fw = face_signed<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, a.K);
+    const Flux fe = face_signed<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, a.K);
+    const Flux fn = face_signed<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, a.K);
+    const Flux fs = face_signed<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, a.K);
+    const CellNew u = cell_update(s.C.h, s.C.qx, s.C.qy, fw.m, fw.nr, fw.t, fe.m, fe.nl, fe.t, fn.m, fn.nr, fn.t, fs.m,
+                                  fs.nl, fs.t, s.wet, s.gn2, a.K);
+    float* Uo = a.Uo + (size_t)cand * a.cstride;
+    const size_t o = (size_t)r * a.P + c;
+    Uo[o] = u.h;
+    Uo[a.plane + o] = u.qx;
+    Uo[2 * a.plane + o] = u.qy;
+}
+
+// Reverse of the sources (numerics.rs:158-195): lam1 = (d new / d (h1, qx1, qy1))^T lam.
+__global__ void __launch_bounds__(256) itr_adj_sources_kernel(AdjArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, cand = blockIdx.z;
+    if (c >= a.nx) return;
+    const float* U = a.U + (size_t)cand * a.cstride;
+    const Stencil s = stencil(a, U, cand, r, c);
+    const Flux fw = face_signed<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, a.K);
+    const Flux fe = face_signed<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, a.K);
+    const Flux fn = face_signed<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, a.K);
+    const Flux fs = face_signed<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, a.K);
+    const StepK& K = a.K;
+    const float h1 = s.C.h - K.dtdx * (fe.m - fw.m) - K.dtdy * (fs.m - fn.m);
+    const float qx1 = s.C.qx - K.dtdx * (fe.nl - fw.nr) - K.dtdy * (fs.t - fn.t);
+    const float qy1 = s.C.qy - K.dtdx * (fe.t - fw.t) - K.dtdy * (fs.nl - fn.nr);
+    const size_t o = (size_t)cand * a.cstride + (size_t)r * a.P + c;
+    const adj::S3<float> b{a.lam[o], a.lam[a.plane + o], a.lam[2 * a.plane + o]};
+    const adj::S3<float> l1 = adj::sources_adj(h1, qx1, qy1, s.wet, s.gn2, adj_consts(K), b);
+    a.lam1[o] = l1.h;
+    a.lam1[a.plane + o] = l1.qx;
+    a.lam1[2 * a.plane + o] = l1.qy;
+}
+
+// Reverse of the flux divergence, the four faces and the cell velocities: the adjoint
+// of the old state, and the west / north face jump gradients (each interior face is
+// owned by the cell to its east / south; boundary faces have fixed zero jumps).
+__global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, cand = blockIdx.z;
+    if (c >= a.nx) return;
+    const float* U = a.U + (size_t)cand * a.cstride;
+    const Stencil s = stencil(a, U, cand, r, c);
+    const float* L1 = a.lam1 + (size_t)cand * a.cstride;
+    auto lam1_at = [&](int rr, int cc) -> adj::S3<float> {
+        if (rr < 0 || rr >= a.ny || cc < 0 || cc >= a.nx) return adj::S3<float>{0.0f, 0.0f, 0.0f};   // ghosts are not updated
+        const size_t o = (size_t)rr * a.P + cc;
+        return adj::S3<float>{L1[o], L1[a.plane + o], L1[2 * a.plane + o]};
+    };
+    const adj::S3<float> lC = lam1_at(r, c), lW = lam1_at(r, c - 1), lE = lam1_at(r, c + 1), lN = lam1_at(r - 1, c),
+                         lS = lam1_at(r + 1, c);
+    const adj::Consts<float> k = adj_consts(a.K);
+    const float dtdx = a.K.dtdx, dtdy = a.K.dtdy;
+    auto V3 = [](const Vel& v) { return adj::V<float>{v.u, v.v, v.c}; };
+    adj::S3<float> bC = lC;   // h1 = h - ..., qx1 = qx - ..., qy1 = qy - ...
+    adj::V<float> bv{0.0f, 0.0f, 0.0f};
+    // A ghost's adjoint folds back into this cell (its mirror image).
+    auto fold = [&](const Nb& g, bool wall, bool xdir, float bh, const adj::V<float>& gv) {
+        adj::S3<float> q = adj::vel_adj(g.h, g.qx, g.qy, k, gv);
+        bC.h += bh + q.h;
+        bC.qx += (xdir && wall) ? -q.qx : q.qx;
+        bC.qy += (!xdir && wall) ? -q.qy : q.qy;
+    };
+    double* gx = a.gx + (size_t)cand * a.dzx_stride;
+    double* gy = a.gy + (size_t)cand * a.dzy_stride;
+    {   // west face: L = W, R = C
+        const adj::F<float> b{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
+        const adj::FaceBar<float> f = adj::face_adj<true>(s.W.h, V3(s.vW), s.C.h, V3(s.vC), s.dzW, k, b);
+        bC.h += f.hr;
+        bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
+        if (c == 0) fold(s.W, s.wW, true, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
+        else gx[(size_t)r * a.P + c] += (double)f.dz;
+    }
+    {   // east face: L = C, R = E
+        const adj::F<float> b{dtdx * (lE.h - lC.h), -dtdx * lC.qx, dtdx * lE.qx, dtdx * (lE.qy - lC.qy)};
+        const adj::FaceBar<float> f = adj::face_adj<true>(s.C.h, V3(s.vC), s.E.h, V3(s.vE), s.dzE, k, b);
+        bC.h += f.hl;
+        bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
+        if (c == a.nx - 1) fold(s.E, s.wE, true, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
+    }
+    {   // north face: L = N, R = C (normal velocity v)
+        const adj::F<float> b{dtdy * (lC.h - lN.h), -dtdy * lN.qy, dtdy * lC.qy, dtdy * (lC.qx - lN.qx)};
+        const adj::FaceBar<float> f = adj::face_adj<false>(s.N.h, V3(s.vN), s.C.h, V3(s.vC), s.dzN, k, b);
+        bC.h += f.hr;
+        bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
+        if (r == 0) fold(s.N, s.wN, false, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
+        else gy[(size_t)r * a.P + c] += (double)f.dz;
+    }
+    {   // south face: L = C, R = S
+        const adj::F<float> b{dtdy * (lS.h - lC.h), -dtdy * lC.qy, dtdy * lS.qy, dtdy * (lS.qx - lC.qx)};
+        const adj::FaceBar<float> f = adj::face_adj<false>(s.C.h, V3(s.vC), s.S.h, V3(s.vS), s.dzS, k, b);
+        bC.h += f.hl;
+        bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
+        if (r == a.ny - 1) fold(s.S, s.wS, false, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
+    }
+    const adj::S3<float> q = adj::vel_adj(s.C.h, s.C.qx, s.C.qy, k, bv);
+    const size_t o = (size_t)cand * a.cstride + (size_t)r * a.P + c;
+    a.lamo[o] = bC.h + q.h;
+    a.lamo[a.plane + o] = bC.qx + q.qx;
+    a.lamo[2 * a.plane + o] = bC.qy + q.qy;
+}
+
+// lam[cand][plane h][cell] += v (a peak-depth seed at the step that set the peak).
+__global__ void itr_adj_seed_kernel(float* lam, size_t cstride, int cand, size_t cell, float v) {
+    if (threadIdx.x == 0) lam[(size_t)cand * cstride + cell] += v;
+}
+
+// Count of cells where two states differ bitwise (replay check).
+__global__ void __launch_bounds__(256) itr_adj_compare_kernel(const float* A, const float* B, size_t n,
+                                                               unsigned long long* count) {
+    unsigned long long bad = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bad += __float_as_uint(A[i]) != __float_as_uint(B[i]) ? 1ull : 0ull;
+    if (bad) atomicAdd(count, bad);
 }
 
 // ------------------------------------------------------------------ host side
@@ -1159,8 +1394,18 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     return ITR_OK;
 }
 
+itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                             itr_batch** out, int64_t log_steps, char* err, size_t errlen);
+
 itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
                                 itr_batch** out, char* err, size_t errlen) {
+    return batch_create_impl(ctx, p, grids, n_grids, out, 0, err, errlen);
+}
+
+}  // extern "C"
+
+itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                             itr_batch** out, int64_t log_steps, char* err, size_t errlen) {
     if (!ctx || !p || !grids || n_grids <= 0 || !out) {
         set_err(err, errlen, "null argument");
         return ITR_EINVAL;
@@ -1420,6 +1665,8 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         ALLOC(d_sbuf, ncx * plane);
         ALLOC(d_ledger, ncx * b->max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
+        StepRec* d_steplog = nullptr;
+        if (log_steps > 0) ALLOC(d_steplog, ncx * (size_t)log_steps);
         if (I.baseline_max) {
             ALLOC(d_base, (size_t)std::max(I.n_mon, 1));
             ALLOC(d_gpos, (size_t)std::max(I.n_guard, 1));
@@ -1466,6 +1713,8 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
         D.baseline_max = d_base;
         D.guard_pos = d_gpos;
         D.mon_cells = d_moncells;
+        D.steplog = d_steplog;
+        D.max_log = log_steps > 0 ? (int32_t)std::min<int64_t>(log_steps, 1 << 30) : 0;
         dgrids.push_back(D);
         const size_t lds = lds_layout(R, nx).total * sizeof(float);
         max_lds = std::max(max_lds, lds);
@@ -1534,6 +1783,8 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
     *out = b;
     return ITR_OK;
 }
+
+extern "C" {
 
 itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char* err, size_t errlen) {
     if (!b) return ITR_EINVAL;
@@ -1748,6 +1999,298 @@ itr_status itr_hip_run_batch(itr_ctx* ctx, const itr_solver_params* p, const itr
     if (s == ITR_OK) s = itr_hip_batch_results(b, outs, err, errlen);
     itr_hip_batch_destroy(b);
     return s;
+}
+
+itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,
+                           itr_adjoint_out* outs, char* err, size_t errlen) {
+    if (!ctx || !p || !grid || !obj || !outs) {
+        set_err(err, errlen, "null argument");
+        return ITR_EINVAL;
+    }
+    if (!p->locked_dt || p->n_locked <= 0) {
+        set_err(err, errlen, "the adjoint runs on a locked dt schedule (locked_dt)");
+        return ITR_EINVAL;
+    }
+    if (grid->crest) {
+        set_err(err, errlen, "the adjoint supports signed-face levels only");
+        return ITR_DECLINED;
+    }
+    if (grid->n_cand <= 0 || !(obj->tau_m > 0.0) || obj->n_assets < 0 || (obj->n_assets > 0 && !obj->asset_start)) {
+        set_err(err, errlen, "invalid adjoint arguments");
+        return ITR_EINVAL;
+    }
+    for (int a = 0; a < obj->n_assets; a++)
+        for (int q = obj->asset_start[a]; q < obj->asset_start[a + 1]; q++)
+            if (obj->asset_pos[q] < 0 || obj->asset_pos[q] >= grid->n_mon) {
+                set_err(err, errlen, "asset position out of the monitor list");
+                return ITR_EINVAL;
+            }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int nc = grid->n_cand, nx = grid->nx, ny = grid->ny;
+    itr_solver_params q = *p;
+    q.max_steps = 0;
+    // 1. the plain forward: its device time and the step count N
+    double plain_ms = 0.0;
+    uint64_t N = 0;
+    {
+        itr_batch* b0 = nullptr;
+        itr_status s = batch_create_impl(ctx, &q, grid, 1, &b0, 0, err, errlen);
+        if (s != ITR_OK) return s;
+        s = itr_hip_batch_run(b0, nullptr, nullptr, err, errlen);
+        std::vector<CandCtl> cc;
+        if (s == ITR_OK) s = read_ctl(b0, cc, err, errlen);
+        plain_ms = b0->stats.device_ms;
+        itr_hip_batch_destroy(b0);
+        if (s != ITR_OK) return s;
+        for (int c = 0; c < nc; c++) {
+            if (cc[c].abort_kind != 0) {
+                set_err(err, errlen, "candidate %d: forward aborted (kind %d) on the locked schedule", c, cc[c].abort_kind);
+                return ITR_EINVAL;
+            }
+            if (c > 0 && cc[c].step != N) {
+                set_err(err, errlen, "candidates ran different step counts");
+                return ITR_EINVAL;
+            }
+            N = cc[c].step;
+        }
+    }
+    if (N == 0) {
+        set_err(err, errlen, "the forward ran no step");
+        return ITR_EINVAL;
+    }
+    const int C = (int)std::ceil(std::sqrt((double)N));
+    const int nseg = (int)((N + (uint64_t)C - 1) / (uint64_t)C);
+    // 2. the forward with checkpoints every C steps and the log of step constants
+    itr_batch* b = nullptr;
+    itr_status s = batch_create_impl(ctx, &q, grid, 1, &b, (int64_t)N + 1, err, errlen);
+    if (s != ITR_OK) return s;
+    std::vector<void*> mem;
+    auto cleanup = [&]() {
+        for (void* m : mem) (void)hipFree(m);
+        itr_hip_batch_destroy(b);
+    };
+    auto fail = [&](itr_status e) {
+        cleanup();
+        return e;
+    };
+#define ADJ_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_err(err, errlen, "%s failed: %s", #expr, hipGetErrorString(e_));           \
+            return fail(e_ == hipErrorOutOfMemory ? ITR_ENOMEM : ITR_EDEVICE);              \
+        }                                                                                   \
+    } while (0)
+    const DevGrid& D = b->grids[0].d;
+    const size_t plane = D.plane, cstride = 3 * plane, P = (size_t)D.P;
+    const size_t state_n = (size_t)nc * cstride;
+    auto dalloc = [&](size_t bytes, void** out) -> hipError_t {
+        hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 1));
+        if (e == hipSuccess) mem.push_back(*out);
+        return e;
+    };
+    float *ckpt, *seg, *lam, *lam1;
+    double *gx, *gy;
+    unsigned long long* d_mis;
+    ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(nseg + 1), (void**)&ckpt));
+    ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(C + 1), (void**)&seg));
+    ADJ_TRY(dalloc(sizeof(float) * state_n * 2, (void**)&lam));
+    ADJ_TRY(dalloc(sizeof(float) * state_n, (void**)&lam1));
+    ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * plane, (void**)&gx));
+    ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * D.plane_y, (void**)&gy));
+    ADJ_TRY(dalloc(sizeof(unsigned long long), (void**)&d_mis));
+    auto copy_states = [&](float* dst, int parity) -> hipError_t {
+        for (int c = 0; c < nc; c++) {
+            hipError_t e = hipMemcpyAsync(dst + (size_t)c * cstride, D.state + (size_t)c * 6 * plane + (size_t)parity * cstride,
+                                          sizeof(float) * cstride, hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    hipEvent_t ev[2];
+    ADJ_TRY(hipEventCreate(&ev[0]));
+    ADJ_TRY(hipEventCreate(&ev[1]));
+    auto elapsed = [&](float* ms) -> hipError_t {
+        hipError_t e = hipEventSynchronize(ev[1]);
+        return e == hipSuccess ? hipEventElapsedTime(ms, ev[0], ev[1]) : e;
+    };
+    float ms = 0.0f;
+    ADJ_TRY(hipEventRecord(ev[0], st));
+    ADJ_TRY(copy_states(ckpt, 0));
+    for (int k = 1; k <= nseg; k++) {
+        const uint64_t n0 = (uint64_t)(k - 1) * C, n1 = std::min<uint64_t>((uint64_t)k * C, N);
+        if ((s = launch_steps(b, (int64_t)(n1 - n0), err, errlen)) != ITR_OK) return fail(s);
+        ADJ_TRY(copy_states(ckpt + (size_t)k * state_n, (int)(n1 & 1u)));   // step n writes state copy n & 1
+    }
+    if ((s = launch_steps(b, 1, err, errlen)) != ITR_OK) return fail(s);   // completes step N
+    ADJ_TRY(hipEventRecord(ev[1], st));
+    ADJ_TRY(elapsed(&ms));
+    const double fwd_ms = ms;
+    std::vector<CandCtl> cc;
+    if ((s = read_ctl(b, cc, err, errlen)) != ITR_OK) return fail(s);
+    for (int c = 0; c < nc; c++)
+        if (cc[c].step != N || !(cc[c].flags & F_DONE)) {
+            set_err(err, errlen, "candidate %d: checkpointed forward differs from the plain forward", c);
+            return fail(ITR_EDEVICE);
+        }
+    std::vector<StepRec> log((size_t)N);
+    ADJ_TRY(hipMemcpy(log.data(), D.steplog, sizeof(StepRec) * N, hipMemcpyDeviceToHost));
+    // 3. J_risk and the peak seeds (objective.rs:95-101; softplus at :71-74)
+    struct Seed { uint64_t step; int cand; size_t cell; float v; };
+    std::vector<Seed> seeds;
+    const int nm = D.n_mon;
+    std::vector<float> mm((size_t)std::max(nm, 1));
+    std::vector<double> mt((size_t)std::max(nm, 1));
+    std::vector<uint32_t> mcells((size_t)std::max(nm, 1));
+    if (nm > 0) ADJ_TRY(hipMemcpy(mcells.data(), grid->mon_cells, sizeof(uint32_t) * nm, hipMemcpyHostToHost));
+    for (int c = 0; c < nc; c++) {
+        itr_adjoint_out& o = outs[c];
+        o.j_risk = 0.0;
+        if (nm > 0) {
+            const size_t off = slot(D, cc[c].parity, c) * nm;
+            ADJ_TRY(hipMemcpy(mm.data(), D.mon_max + off, sizeof(float) * nm, hipMemcpyDeviceToHost));
+            ADJ_TRY(hipMemcpy(mt.data(), D.mon_tmax + off, sizeof(double) * nm, hipMemcpyDeviceToHost));
+        }
+        for (int a = 0; a < obj->n_assets; a++) {
+            double pk = 0.0;
+            int kstar = -1;
+            for (int qq = obj->asset_start[a]; qq < obj->asset_start[a + 1]; qq++) {
+                const int kk = obj->asset_pos[qq];
+                if ((double)mm[kk] > pk) { pk = (double)mm[kk]; kstar = kk; }
+            }
+            const double x = (pk - obj->threshold_m[a]) / obj->tau_m;
+            const double sp = x > 30.0 ? x : (x < -30.0 ? std::exp(x) : std::log1p(std::exp(x)));
+            o.j_risk += obj->weight[a] * obj->tau_m * sp;
+            if (o.peak_m) o.peak_m[a] = pk;
+            if (kstar < 0) continue;   // never wet: the peak does not move to first order
+            uint64_t nstar = 0;
+            for (uint64_t i = 0; i < N; i++)
+                if (log[i].t_new == mt[kstar]) { nstar = i + 1; break; }
+            if (nstar == 0) continue;
+            const double dpk = obj->weight[a] / (1.0 + std::exp(-x));
+            const uint32_t cell = mcells[kstar];
+            seeds.push_back(Seed{nstar, c, (size_t)(cell / (uint32_t)nx) * P + cell % (uint32_t)nx, (float)dpk});
+        }
+    }
+    // 4. reverse sweep, segment by segment
+    ADJ_TRY(hipMemsetAsync(lam, 0, sizeof(float) * state_n * 2, st));
+    ADJ_TRY(hipMemsetAsync(gx, 0, sizeof(double) * (size_t)nc * plane, st));
+    ADJ_TRY(hipMemsetAsync(gy, 0, sizeof(double) * (size_t)nc * D.plane_y, st));
+    ADJ_TRY(hipMemsetAsync(d_mis, 0, sizeof(unsigned long long), st));
+    AdjArgs A{};
+    A.dzx = D.dzx;
+    A.dzy = D.dzy;
+    A.wet = D.wet;
+    A.gn2 = D.gn2;
+    A.ghost = D.ghost;
+    A.nx = nx;
+    A.ny = ny;
+    A.P = D.P;
+    A.plane = plane;
+    A.cstride = cstride;
+    A.dzx_stride = plane;
+    A.dzy_stride = D.plane_y;
+    A.gx = gx;
+    A.gy = gy;
+    A.lam1 = lam1;
+    auto consts = [&](uint64_t n) {   // constants of step n (1-based)
+        StepK K;
+        const StepRec& r = log[(size_t)(n - 1)];
+        K.g = D.g32;
+        K.g_half = D.g_half32;
+        K.inv_dx = D.inv_dx32;
+        K.inv_dy = D.inv_dy32;
+        K.h_dry = b->prm.h_dry;
+        K.heps2 = b->prm.heps2;
+        K.q_min = b->prm.q_min;
+        K.dt = r.dt32;
+        K.dtdx = r.dtdx;
+        K.dtdy = r.dtdy;
+        K.rain = r.rain32;
+        K.cap = r.cap32;
+        return K;
+    };
+    const dim3 grd((unsigned)((nx + 63) / 64), (unsigned)ny, (unsigned)nc), blk(64);
+    hipEvent_t er[4];
+    for (auto& e : er) ADJ_TRY(hipEventCreate(&e));
+    double replay_ms = 0.0, reverse_ms = 0.0;
+    int li = 0;
+    for (int k = nseg; k >= 1; k--) {
+        const uint64_t n0 = (uint64_t)(k - 1) * C, n1 = std::min<uint64_t>((uint64_t)k * C, N);
+        ADJ_TRY(hipEventRecord(er[0], st));
+        ADJ_TRY(hipMemcpyAsync(seg, ckpt + (size_t)(k - 1) * state_n, sizeof(float) * state_n, hipMemcpyDeviceToDevice, st));
+        for (uint64_t n = n0 + 1; n <= n1; n++) {   // replay; the segment's last step is the check
+            A.U = seg + (size_t)(n - 1 - n0) * state_n;
+            A.Uo = seg + (size_t)(n - n0) * state_n;
+            A.K = consts(n);
+            hipLaunchKernelGGL(itr_replay_kernel, grd, blk, 0, st, A);
+        }
+        hipLaunchKernelGGL(itr_adj_compare_kernel, dim3(256), dim3(256), 0, st, seg + (size_t)(n1 - n0) * state_n,
+                           ckpt + (size_t)k * state_n, state_n, d_mis);
+        ADJ_TRY(hipGetLastError());
+        ADJ_TRY(hipEventRecord(er[1], st));
+        for (uint64_t n = n1; n > n0; n--) {
+            float* lam_n = lam + (size_t)li * state_n;
+            for (const Seed& sd : seeds)
+                if (sd.step == n) hipLaunchKernelGGL(itr_adj_seed_kernel, dim3(1), dim3(64), 0, st, lam_n, cstride, sd.cand, sd.cell, sd.v);
+            A.U = seg + (size_t)(n - 1 - n0) * state_n;
+            A.lam = lam_n;
+            A.lamo = lam + (size_t)(li ^ 1) * state_n;
+            A.K = consts(n);
+            hipLaunchKernelGGL(itr_adj_sources_kernel, grd, blk, 0, st, A);
+            hipLaunchKernelGGL(itr_adj_faces_kernel, grd, blk, 0, st, A);
+            li ^= 1;
+        }
+        ADJ_TRY(hipGetLastError());
+        ADJ_TRY(hipEventRecord(er[2], st));
+        ADJ_TRY(hipEventSynchronize(er[2]));
+        float a_ms = 0.0f, b_ms = 0.0f;
+        ADJ_TRY(hipEventElapsedTime(&a_ms, er[0], er[1]));
+        ADJ_TRY(hipEventElapsedTime(&b_ms, er[1], er[2]));
+        replay_ms += a_ms;
+        reverse_ms += b_ms;
+    }
+    unsigned long long mis = 0;
+    ADJ_TRY(hipMemcpy(&mis, d_mis, sizeof mis, hipMemcpyDeviceToHost));
+    // 5. face and cell gradients
+    std::vector<double> hx((size_t)nc * plane), hy((size_t)nc * D.plane_y);
+    ADJ_TRY(hipMemcpy(hx.data(), gx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost));
+    ADJ_TRY(hipMemcpy(hy.data(), gy, sizeof(double) * hy.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < nc; c++) {
+        itr_adjoint_out& o = outs[c];
+        const double* X = hx.data() + (size_t)c * plane;
+        const double* Y = hy.data() + (size_t)c * D.plane_y;
+        auto fx = [&](int r, int f) { return (f >= 1 && f <= nx - 1) ? X[(size_t)r * P + f] : 0.0; };
+        auto fy = [&](int f, int col) { return (f >= 1 && f <= ny - 1) ? Y[(size_t)f * P + col] : 0.0; };
+        if (o.dJ_dz_face_x)
+            for (int r = 0; r < ny; r++)
+                for (int f = 0; f <= nx; f++) o.dJ_dz_face_x[(size_t)r * (nx + 1) + f] = fx(r, f);
+        if (o.dJ_dz_face_y)
+            for (int f = 0; f <= ny; f++)
+                for (int col = 0; col < nx; col++) o.dJ_dz_face_y[(size_t)f * nx + col] = fy(f, col);
+        if (o.dJ_dz_cell)
+            for (int r = 0; r < ny; r++)
+                for (int col = 0; col < nx; col++) {
+                    const size_t i = (size_t)r * nx + col;
+                    o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0
+                                      : fx(r, col) - fx(r, col + 1) + fy(r, col) - fy(r + 1, col);
+                }
+        o.steps = N;
+        o.segment_steps = C;
+        o.checkpoints = nseg + 1;
+        o.replay_mismatches = mis;
+        o.plain_forward_ms = plain_ms;
+        o.forward_ms = fwd_ms;
+        o.replay_ms = replay_ms;
+        o.reverse_ms = reverse_ms;
+    }
+    for (auto& e : er) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+#undef ADJ_TRY
+    cleanup();
+    return ITR_OK;
 }
 
 }  // extern "C"

@@ -152,6 +152,55 @@ class GpuSolver:
     def set_option(self, name: str, value: int):
         _check(self._L.itr_hip_set_option(self._h, name.encode(), int(value)), None)
 
+    def adjoint(self, p: SolverParams, base: TerrainBase, terrains, monitors, assets, tau_m: float):
+        """Discrete adjoint of J_risk for each candidate on the locked schedule p.locked_dt.
+
+        ``assets``: list of (positions into ``monitors``, threshold_m, weight).  Returns one
+        dict per candidate: j_risk, peak (per asset), dJ_dz_cell (ny, nx), dJ_dz_face_x
+        (ny, nx+1), dJ_dz_face_y (ny+1, nx), steps, segment_steps, checkpoints,
+        replay_mismatches and the device times of the plain forward, the checkpointed
+        forward, the replay and the reverse sweep (ms)."""
+        if p.locked_dt is None:
+            raise ValueError("the adjoint needs p.locked_dt")
+        keep = _Keep()
+        q = _params_c(p, keep)
+        g = _grid_c(base, terrains, monitors, keep)
+        starts = np.zeros(len(assets) + 1, np.int32)
+        pos = []
+        for i, (ps, _, _) in enumerate(assets):
+            pos.extend(int(x) for x in ps)
+            starts[i + 1] = len(pos)
+        ob = _abi.ObjectiveC()
+        ob.n_assets = len(assets)
+        ob.asset_start = keep.arr(starts, np.int32).ctypes.data_as(_abi._ip)
+        ob.asset_pos = keep.arr(np.array(pos if pos else [0], np.int32), np.int32).ctypes.data_as(_abi._ip)
+        ob.threshold_m = keep.arr(np.array([a[1] for a in assets] or [0.0]), np.float64).ctypes.data_as(_abi._dp)
+        ob.weight = keep.arr(np.array([a[2] for a in assets] or [0.0]), np.float64).ctypes.data_as(_abi._dp)
+        ob.tau_m = float(tau_m)
+        n = len(terrains)
+        nx, ny = base.nx, base.ny
+        outs = (_abi.AdjointOutC * n)()
+        bufs = []
+        for i in range(n):
+            b = {"peak": np.zeros(max(len(assets), 1)), "cell": np.zeros((ny, nx)), "fx": np.zeros((ny, nx + 1)),
+                 "fy": np.zeros((ny + 1, nx))}
+            outs[i].peak_m = b["peak"].ctypes.data_as(_abi._dp)
+            outs[i].dJ_dz_cell = b["cell"].ctypes.data_as(_abi._dp)
+            outs[i].dJ_dz_face_x = b["fx"].ctypes.data_as(_abi._dp)
+            outs[i].dJ_dz_face_y = b["fy"].ctypes.data_as(_abi._dp)
+            bufs.append(b)
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_adjoint(self._h, C.byref(q), C.byref(g), C.byref(ob), outs, err, 1024), err)
+        res = []
+        for i in range(n):
+            o, b = outs[i], bufs[i]
+            res.append({"j_risk": o.j_risk, "peak": b["peak"][:len(assets)].copy(), "dJ_dz_cell": b["cell"],
+                        "dJ_dz_face_x": b["fx"], "dJ_dz_face_y": b["fy"], "steps": int(o.steps),
+                        "segment_steps": o.segment_steps, "checkpoints": o.checkpoints,
+                        "replay_mismatches": int(o.replay_mismatches), "plain_forward_ms": o.plain_forward_ms,
+                        "forward_ms": o.forward_ms, "replay_ms": o.replay_ms, "reverse_ms": o.reverse_ms})
+        return res
+
     def session(self, p: SolverParams, groups, on_sync=None):
         """Upload a batch once (inputs stay resident); ``groups`` is a list of
         (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""

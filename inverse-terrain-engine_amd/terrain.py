@@ -433,3 +433,35 @@ def materialize(prims, geo: GeoTransform, nx, ny, editable=None, max_abs_dz=1e9)
         else:
             cut -= v * a
     return TerrainEdit(c0, r0, dz, cut, fill)
+
+def materialize_jacobian(prims, geo: GeoTransform, nx, ny, editable=None, max_abs_dz=1e9):
+    """d dz / d height of every primitive, dense (ny, nx) per primitive: the primitive's
+    4x4-quadrature mean profile (design.rs:356-371 is linear in each height), zero where
+    the sum is clamped to +-max_abs_dz or the cell is not editable."""
+    total = materialize(prims, geo, nx, ny, None, 1e300).dense(nx, ny)
+    free = np.abs(total) < max_abs_dz
+    if editable is not None:
+        free &= np.asarray(editable, bool)
+    S = 4
+    out = []
+    for p in prims:
+        J = np.zeros((ny, nx))
+        if p.height != 0.0:
+            x0, y0, x1, y1 = p.bbox()
+            fc0, fr1 = geo.to_grid(x0, y0)
+            fc1, fr0 = geo.to_grid(x1, y1)
+            pa, pb = min(int(max(math.floor(fc0), 0.0)), nx), min(int(max(math.floor(fr0), 0.0)), ny)
+            pc, pd = min(int(max(math.ceil(fc1), 0.0)), nx), min(int(max(math.ceil(fr1), 0.0)), ny)
+            if pa < pc and pb < pd:
+                cols = np.arange(pa, pc, dtype=np.float64)[None, :]
+                rows = np.arange(pb, pd, dtype=np.float64)[:, None]
+                hh, ww = pd - pb, pc - pa
+                s = np.zeros((hh, ww))
+                for sy in range(S):
+                    for sx in range(S):
+                        x = geo.origin_x + (cols + (sx + 0.5) / S) * geo.dx
+                        y = geo.origin_y - (rows + (sy + 0.5) / S) * geo.dy
+                        s = s + p.profile(np.broadcast_to(x, (hh, ww)), np.broadcast_to(y, (hh, ww)))
+                J[pb:pd, pa:pc] = s / (S * S)
+        out.append(np.where(free, J, 0.0))
+    return out

@@ -1,0 +1,118 @@
+"""Writes tests/golden/adjoint_fd_config4.json: f64 central finite differences of the
+objective J = J_risk + lambda * J_earth on the 120x96 adjoint workload (scenario.config4)
+with the dt schedule frozen, at 32 seeded cells (d J / d dz of the cell) and for every
+primitive height.  The schedule is the f32 forward's own (the oracle, which is bitwise
+the reference's f32 fused kernel); every differenced run is the oracle in f64 on that
+schedule.
+
+Steps 1e-4, 1e-6 and 1e-7 m.  At 1e-4 m the differences cross kinks of the objective:
+rain-fed films of a few mm switch the hydrostatic reconstruction max(h - dz, 0) and the
+dry-cell test within +-1e-4 m of the bed, so that difference is not the derivative
+(it is kept for reference).  1e-6 and 1e-7 agree to ~1e-7 relative: J is smooth at that
+scale and the 1e-6 difference is the derivative the adjoint is checked against.
+Usage: python tests/golden/make_adjoint_fd.py  (about a minute on 8 cores)."""
+from __future__ import annotations
+
+import json
+import os
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from __graft_entry__ import load_package  # noqa: E402
+
+load_package()
+from inverse_terrain_engine_amd import objective as ob  # noqa: E402
+from inverse_terrain_engine_amd import scenario as sc  # noqa: E402
+from inverse_terrain_engine_amd import terrain as tr  # noqa: E402
+import conftest as cf  # noqa: E402
+
+STEPS = (1e-4, 1e-6, 1e-7)   # FD steps on dz and on heights (m)
+
+
+def fd_cells(case, per_asset=16, seed=41):
+    """16 seeded cells within 4 cells of each 4x4 asset box (rows and columns
+    first-4 .. first+7)."""
+    rng = np.random.default_rng(seed)
+    nx, ny = case.base.nx, case.base.ny
+    cells = []
+    for pos, _, _ in case.assets:
+        cs = case.monitors[pos]
+        r0, c0 = int(cs.min() // nx), int((cs % nx).min())
+        picked = []
+        while len(picked) < per_asset:
+            r = int(np.clip(r0 + rng.integers(-4, 8), 0, ny - 1))
+            c = int(np.clip(c0 + rng.integers(-4, 8), 0, nx - 1))
+            if (r, c) not in cells and (r, c) not in picked:
+                picked.append((r, c))
+        cells += picked
+    return cells
+
+
+def main():
+    case = sc.config4()
+    base, p = case.base, case.params
+    nx, ny = base.nx, base.ny
+    area = base.dx * base.dy
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
+    terr = cf.oracle_terrain(base)
+    f32 = cf.po.run(terr, cf.oracle_params(p), precision=32, edit=case.edit.as_dict(), monitors=case.monitors,
+                    want_dts=True)
+    sched = f32["dts"]
+    q = cf.oracle_params(p)
+    q.locked_dt = sched
+    dz0 = case.edit.dense(nx, ny)
+
+    def J(dz):
+        o = cf.po.run(terr, q, precision=64, edit={"c0": 0, "r0": 0, "dz": dz}, monitors=case.monitors)
+        assert o["aborted"] == 0, o["aborted"]
+        cut, fill = ob.volumes(dz, area)
+        return objv.j(o["monitor_max"], cut, fill), o["steps"]
+
+    j0, steps = J(dz0)
+    cells = fd_cells(case)
+    prims = case.prims
+    jobs = []
+    for H in STEPS:
+        for (r, c) in cells:
+            for sgn in (1.0, -1.0):
+                d = dz0.copy()
+                d[r, c] += sgn * H
+                jobs.append(d)
+        for i in range(len(prims)):
+            for sgn in (1.0, -1.0):
+                pp = [tr.Primitive(**{**vars(x)}) for x in prims]
+                pp[i].height += sgn * H
+                jobs.append(tr.materialize(pp, base.fine.geo, nx, ny, None, 1.0).dense(nx, ny))
+    with ThreadPoolExecutor(max_workers=os.cpu_count() or 4) as ex:
+        vals = list(ex.map(lambda d: J(d)[0], jobs))
+    per = 2 * (len(cells) + len(prims))
+    fd = {}
+    for s, H in enumerate(STEPS):
+        v = vals[s * per:(s + 1) * per]
+        g_cells = [(v[2 * k] - v[2 * k + 1]) / (2 * H) for k in range(len(cells))]
+        off = 2 * len(cells)
+        g_h = [(v[off + 2 * i] - v[off + 2 * i + 1]) / (2 * H) for i in range(len(prims))]
+        fd[repr(H)] = {"dJ_dz_cell": g_cells, "dJ_dheight": g_h}
+    out = {
+        "workload": "scenario.config4(): 120x96 at 5 m, 40 min storm, CFL 0.9, locked dt from the f32 forward",
+        "steps": int(steps), "j0": j0, "locked_dt": [float(x) for x in sched],
+        "cells": [[int(r), int(c)] for r, c in cells], "heights": [float(x.height) for x in prims],
+        "fd": fd, "fd_step_checked": repr(1e-6),
+    }
+    with open(os.path.join(HERE, "adjoint_fd_config4.json"), "w") as f:
+        json.dump(out, f)
+    for H in STEPS:
+        g = fd[repr(H)]
+        print(f"step {H:g}: cells {np.round(g['dJ_dz_cell'], 6).tolist()}\n  heights {g['dJ_dheight']}")
+    print(f"J0 {j0:.10g}, {steps} steps")
+
+
+if __name__ == "__main__":
+    main()

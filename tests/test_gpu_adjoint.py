@@ -1,0 +1,78 @@
+"""The discrete adjoint through the C ABI (itr_hip_adjoint) against f64 central finite
+differences of the oracle on the frozen dt schedule (tests/golden/adjoint_fd_config4.json,
+made by tests/golden/make_adjoint_fd.py).  The check uses the 1e-6 m differences where
+they agree with the 1e-7 m ones (the objective is differentiable at that scale); the
+1e-4 m differences cross reconstruction kinks of mm-thick rain films and are printed
+for reference only."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from inverse_terrain_engine_amd import objective as ob
+from inverse_terrain_engine_amd import scenario as sc
+from inverse_terrain_engine_amd import terrain as tr
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+REL_TOL = 1e-3   # relative error bound against f64 central differences
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _fixture():
+    with open(os.path.join(HERE, "golden", "adjoint_fd_config4.json")) as f:
+        return json.load(f)
+
+
+def _rel(a, b, scale):
+    return abs(a - b) / max(abs(b), scale)
+
+
+def test_adjoint_config4_against_f64_fd(solver):
+    fx = _fixture()
+    case = sc.config4()
+    p = case.params
+    p.locked_dt = np.array(fx["locked_dt"])
+    _say(f"case: adjoint config4, {len(p.locked_dt)} locked steps")
+    res = solver.adjoint(p, case.base, [case.candidate], case.monitors, case.assets, case.tau_m)[0]
+    _say({k: v for k, v in res.items() if not isinstance(v, np.ndarray)})
+    assert res["steps"] == fx["steps"]
+    assert res["replay_mismatches"] == 0, "replayed segments must equal the checkpoints bitwise"
+    nx, ny = case.base.nx, case.base.ny
+    area = case.base.dx * case.base.dy
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
+    dz0 = case.edit.dense(nx, ny)
+    cut, fill = ob.volumes(dz0, area)
+    j = res["j_risk"] + case.earth_weight * objv.j_earth(cut, fill)
+    assert abs(j - fx["j0"]) <= 1e-4 * abs(fx["j0"]), (j, fx["j0"])
+    g = res["dJ_dz_cell"] + objv.earth_gradient(dz0, area)
+    fd6, fd7, fd4 = (fx["fd"][repr(h)] for h in (1e-6, 1e-7, 1e-4))
+    scale = 1e-3 * np.abs(np.array(fd6["dJ_dz_cell"])).max()
+    worst, checked = 0.0, 0
+    for (r, c), v6, v7, v4 in zip(fx["cells"], fd6["dJ_dz_cell"], fd7["dJ_dz_cell"], fd4["dJ_dz_cell"]):
+        smooth = _rel(v6, v7, scale) <= 1e-4   # J differentiable at this scale
+        e = _rel(g[r, c], v6, scale)
+        if smooth:
+            worst = max(worst, e)
+            checked += 1
+        _say(f"cell ({r:3d},{c:3d}) adjoint {g[r, c]: .6e} fd(1e-6) {v6: .6e} rel {e:.2e}"
+             f"{'' if smooth else ' (kink within 1e-6: not checked)'}  fd(1e-4) {v4: .6e}")
+    jac = tr.materialize_jacobian(case.prims, case.base.fine.geo, nx, ny, None, 1.0)
+    for i, (Jp, v6, v7, v4) in enumerate(zip(jac, fd6["dJ_dheight"], fd7["dJ_dheight"], fd4["dJ_dheight"])):
+        gh = float(np.sum(g * Jp))
+        e = _rel(gh, v6, 1e-9)
+        smooth = _rel(v6, v7, 1e-9) <= 1e-3
+        if smooth:
+            worst = max(worst, e)
+            checked += 1
+        _say(f"height {i}: adjoint {gh: .8e} fd(1e-6) {v6: .8e} rel {e:.2e}  fd(1e-7) {v7: .8e}  fd(1e-4) {v4: .8e}")
+    cost = (res["forward_ms"] + res["replay_ms"] + res["reverse_ms"]) / res["plain_forward_ms"]
+    _say(f"worst rel {worst:.2e} over {checked} checked values; cost {cost:.2f}x the plain forward")
+    assert checked >= 30
+    assert worst <= REL_TOL

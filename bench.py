@@ -151,6 +151,66 @@ def copy_probe():
         raise RuntimeError("copy probe failed")
 
 
+def adjoint_config4(solver, sc, ob):
+    """The 120x96 adjoint workload: gradient error against the committed f64 central
+    differences (1e-6 m steps where they agree with 1e-7 m) and the cost relative to the
+    plain forward on the same locked schedule."""
+    import numpy as np
+    with open(os.path.join(ROOT, "tests", "golden", "adjoint_fd_config4.json")) as f:
+        fx = json.load(f)
+    case = sc.config4()
+    p = case.params
+    p.locked_dt = np.array(fx["locked_dt"])
+    res = solver.adjoint(p, case.base, [case.candidate], case.monitors, case.assets, case.tau_m)[0]
+    nx, ny = case.base.nx, case.base.ny
+    area = case.base.dx * case.base.dy
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
+    dz0 = case.edit.dense(nx, ny)
+    g = res["dJ_dz_cell"] + objv.earth_gradient(dz0, area)
+    fd6, fd7 = fx["fd"][repr(1e-6)], fx["fd"][repr(1e-7)]
+    scale = 1e-3 * max(abs(v) for v in fd6["dJ_dz_cell"])
+    rel = lambda a, b, sc_: abs(a - b) / max(abs(b), sc_)
+    errs = [rel(g[r, c], v6, scale) for (r, c), v6, v7 in zip(fx["cells"], fd6["dJ_dz_cell"], fd7["dJ_dz_cell"])
+            if rel(v6, v7, scale) <= 1e-4]
+    from inverse_terrain_engine_amd import terrain as tr
+    jac = tr.materialize_jacobian(case.prims, case.base.fine.geo, nx, ny, None, 1.0)
+    for Jp, v6, v7 in zip(jac, fd6["dJ_dheight"], fd7["dJ_dheight"]):
+        if rel(v6, v7, 1e-9) <= 1e-3:
+            errs.append(rel(float(np.sum(g * Jp)), v6, 1e-9))
+    total = res["forward_ms"] + res["replay_ms"] + res["reverse_ms"]
+    return {"workload": "120x96 at 5 m, 40 min storm, one design of 3 primitives, locked dt (CFL 0.9)",
+            "steps": res["steps"], "segment_steps": res["segment_steps"], "checkpoints": res["checkpoints"],
+            "cost_vs_forward": total / res["plain_forward_ms"], "worst_rel_err_vs_f64_fd": max(errs),
+            "values_checked": len(errs), "tol": 1e-3, "replay_mismatches": res["replay_mismatches"],
+            "plain_forward_ms": res["plain_forward_ms"], "forward_ms": res["forward_ms"],
+            "replay_ms": res["replay_ms"], "reverse_ms": res["reverse_ms"]}
+
+
+def adjoint_best16(solver, sc, ob, w, res, horizon):
+    """Adjoint of the 16 best candidates of the batched forward (J from its monitor
+    maxima), on the best candidate's dt schedule over a bounded horizon."""
+    import numpy as np
+    base = w.base
+    area = base.dx * base.dy
+    objv = ob.Objective([ob.Asset(np.asarray(a), 0.05, 1.0) for a in w.asset_pos], 0.01, 1e-3)
+    js = [objv.j(r.monitor_max, e.cut_m3, e.fill_m3) for r, e in zip(res, w.edits)]
+    best = [int(i) for i in np.argsort(js)[:16]]
+    p = w.params
+    p.cfl_hard = 1.0
+    sched = solver.dt_schedule(p, base, w.candidates[best[0]], w.monitors, max_steps=horizon)
+    p.locked_dt = sched
+    p.max_steps = len(sched)
+    assets = [(a.positions, a.threshold_m, a.weight) for a in objv.assets]
+    out = solver.adjoint(p, base, [w.candidates[i] for i in best], w.monitors, assets, objv.tau_m)
+    r0 = out[0]
+    total = r0["forward_ms"] + r0["replay_ms"] + r0["reverse_ms"]
+    return {"workload": f"16 best of the batch, {base.nx}x{base.ny}, first {len(sched)} steps on the best one's dt",
+            "steps": r0["steps"], "cost_vs_forward": total / r0["plain_forward_ms"],
+            "replay_mismatches": r0["replay_mismatches"], "plain_forward_ms": r0["plain_forward_ms"],
+            "forward_ms": r0["forward_ms"], "replay_ms": r0["replay_ms"], "reverse_ms": r0["reverse_ms"],
+            "cell_steps_per_s_reverse": 16 * base.nx * base.ny * r0["steps"] / (r0["reverse_ms"] * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +222,8 @@ def main():
     ap.add_argument("--min-lds", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
+    ap.add_argument("--adjoint-horizon", type=int, default=120)
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
@@ -186,17 +248,10 @@ def main():
     sess = solver.session(w.params, [(w.base, w.candidates, w.monitors, None)])
     if a.warmup:
         sess.advance(a.warmup)
-    torch = None
-    if not a.copy_probe:   # the counter passes run without torch
-        try:
-            import torch
-            torch.cuda.synchronize()
-        except Exception:
-            torch = None
+    # advance() returns after synchronizing the library's stream (HIP event), so the
+    # timed region is bracketed by device synchronization on both sides.
     t1 = time.perf_counter()
     st = sess.advance(a.steps)
-    if torch is not None:
-        torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     cell_steps = st["cell_steps"]
     value = cell_steps / wall
@@ -205,6 +260,12 @@ def main():
     achieved = BYTES_PER_CELL_STEP * cells_per_launch / kernel_s
     res = sess.results(want_state=False)
     sess.close()
+    adj = adj16 = None
+    if not a.no_adjoint and not a.copy_probe:
+        from inverse_terrain_engine_amd import objective as ob
+        adj = adjoint_config4(solver, sc, ob)
+        if a.cands >= 16:
+            adj16 = adjoint_best16(solver, sc, ob, w, res, a.adjoint_horizon)
 
     out = {
         "metric": METRIC,
@@ -239,6 +300,10 @@ def main():
             "cells_per_launch": cells_per_launch,
         },
     }
+    if adj is not None:
+        out["adjoint"] = adj
+    if adj16 is not None:
+        out["adjoint_best16"] = adj16
     solver.close()
     if a.copy_probe:
         copy_probe()

@@ -168,7 +168,8 @@ void itr_hip_batch_destroy(itr_batch* b);
  * grid.  The forward runs on the locked dt schedule p->locked_dt (solver.rs:452-455; a
  * frozen schedule makes the discrete step a fixed function of the state), stores
  * checkpoints every ceil(sqrt(N)) steps, replays each segment and sweeps it backwards;
- * adjoint arithmetic is f32, face-gradient accumulation f64.  Signed face mode only
+ * adjoint arithmetic is f32, face-gradient accumulation f64.  p->max_steps > 0 bounds the
+ * horizon (the run then ends after that many steps).  Signed face mode only
  * (crest levels return ITR_DECLINED).  The earthwork term and the chain rule to
  * primitive heights are host work on the returned cell gradient. */
 typedef struct {
@@ -197,6 +198,12 @@ typedef struct {
 
 itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,
                            itr_adjoint_out* outs, char* err, size_t errlen);
+
+/* No counterpart in the reference: the dt sequence of one candidate's free forward (at
+ * most `capacity` steps, or p->max_steps), i.e. the schedule solver.rs:452-455 locks to
+ * when a baseline's dt is reused.  Writes *n_out values to dts. */
+itr_status itr_hip_dt_schedule(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, int32_t cand,
+                               double* dts, int64_t capacity, int64_t* n_out, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

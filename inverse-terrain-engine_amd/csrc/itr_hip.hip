@@ -113,7 +113,7 @@ __host__ __device__ inline size_t slot(const DevGrid& G, int par, int local) { r
 // The f32 constants and new time of one step, as the forward used them.
 struct StepRec {
     float dt32, dtdx, dtdy, rain32, cap32, pad;
-    double t_new;
+    double t_new, dt;
 };
 
 // Per-candidate control block.  Two copies per candidate, selected by launch parity:
@@ -859,6 +859,7 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a, uint32_t lk) {
                 q.cap32 = sS.cap32;
                 q.pad = 0.0f;
                 q.t_new = sS.t_new;
+                q.dt = sS.dt;
                 G.steplog[(size_t)local * G.max_log + sS.step] = q;
             }
         }
@@ -2029,7 +2030,6 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     hipStream_t st = ctx->stream;
     const int nc = grid->n_cand, nx = grid->nx, ny = grid->ny;
     itr_solver_params q = *p;
-    q.max_steps = 0;
     // 1. the plain forward: its device time and the step count N
     double plain_ms = 0.0;
     uint64_t N = 0;
@@ -2291,6 +2291,48 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
 #undef ADJ_TRY
     cleanup();
     return ITR_OK;
+}
+
+itr_status itr_hip_dt_schedule(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, int32_t cand,
+                               double* dts, int64_t capacity, int64_t* n_out, char* err, size_t errlen) {
+    if (!ctx || !p || !grid || !dts || !n_out || capacity <= 0 || cand < 0 || cand >= grid->n_cand) {
+        set_err(err, errlen, "invalid argument");
+        return ITR_EINVAL;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    itr_grid one = *grid;
+    one.n_cand = 1;
+    const size_t sx = (size_t)grid->ny * (grid->nx + 1), sy = (size_t)(grid->ny + 1) * grid->nx;
+    one.dzx = grid->dzx + (size_t)cand * sx;
+    one.dzy = grid->dzy + (size_t)cand * sy;
+    if (grid->crest) {
+        one.dzx_r = grid->dzx_r + (size_t)cand * sx;
+        one.dzy_r = grid->dzy_r + (size_t)cand * sy;
+    }
+    if (grid->z_level) one.z_level = grid->z_level + (size_t)cand * grid->nx * grid->ny;
+    one.baseline_max = nullptr;
+    one.n_guard = 0;
+    itr_solver_params q = *p;
+    if (q.max_steps <= 0 || q.max_steps > capacity) q.max_steps = capacity;
+    itr_batch* b = nullptr;
+    itr_status s = batch_create_impl(ctx, &q, &one, 1, &b, capacity, err, errlen);
+    if (s != ITR_OK) return s;
+    s = itr_hip_batch_run(b, nullptr, nullptr, err, errlen);
+    std::vector<CandCtl> cc;
+    if (s == ITR_OK) s = read_ctl(b, cc, err, errlen);
+    if (s == ITR_OK) {
+        const uint64_t n = std::min<uint64_t>(cc[0].step, (uint64_t)capacity);
+        std::vector<StepRec> log(std::max<uint64_t>(n, 1));
+        if (n > 0 && hipMemcpy(log.data(), b->grids[0].d.steplog, sizeof(StepRec) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_err(err, errlen, "step log read failed");
+            s = ITR_EDEVICE;
+        } else {
+            for (uint64_t i = 0; i < n; i++) dts[i] = log[i].dt;
+            *n_out = (int64_t)n;
+        }
+    }
+    itr_hip_batch_destroy(b);
+    return s;
 }
 
 }  // extern "C"

@@ -201,6 +201,22 @@ class GpuSolver:
                         "forward_ms": o.forward_ms, "replay_ms": o.replay_ms, "reverse_ms": o.reverse_ms})
         return res
 
+    def dt_schedule(self, p: SolverParams, base: TerrainBase, terrain, monitors=(), max_steps: int = 0,
+                    capacity: int = 1 << 20) -> np.ndarray:
+        """dt of every step of one candidate's free forward (the schedule to lock to)."""
+        keep = _Keep()
+        q = _params_c(p, keep)
+        if max_steps:
+            q.max_steps = int(max_steps)
+        g = _grid_c(base, [terrain], monitors, keep)
+        cap = int(max_steps) if max_steps else int(capacity)
+        out = np.zeros(cap)
+        n = C.c_int64(0)
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_dt_schedule(self._h, C.byref(q), C.byref(g), 0, out.ctypes.data_as(_abi._dp), cap,
+                                           C.byref(n), err, 1024), err)
+        return out[:n.value].copy()
+
     def session(self, p: SolverParams, groups, on_sync=None):
         """Upload a batch once (inputs stay resident); ``groups`` is a list of
         (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""

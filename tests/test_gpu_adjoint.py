@@ -76,3 +76,29 @@ def test_adjoint_config4_against_f64_fd(solver):
     _say(f"worst rel {worst:.2e} over {checked} checked values; cost {cost:.2f}x the plain forward")
     assert checked >= 30
     assert worst <= REL_TOL
+
+
+def test_autograd_function_backward(solver):
+    """The torch.autograd.Function wrapper: J.backward() gives the adjoint gradient, and
+    it matches the f64 differences of the committed fixture.  The tensors live on the
+    host: the HIP library owns the device work (torch's bundled HIP runtime and the
+    system one are not both initialised in one process)."""
+    import torch
+    from inverse_terrain_engine_amd.autograd import peak_risk_objective
+    fx = _fixture()
+    case = sc.config4()
+    p = case.params
+    p.locked_dt = np.array(fx["locked_dt"])
+    nx, ny = case.base.nx, case.base.ny
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
+    dz = torch.tensor(case.edit.dense(nx, ny), dtype=torch.float64, requires_grad=True)
+    _say("case: autograd wrapper on config4")
+    J = peak_risk_objective(dz, solver, p, case.base, case.monitors, objv)
+    J.backward()
+    assert abs(J.item() - fx["j0"]) <= 1e-4 * abs(fx["j0"])
+    g = dz.grad.cpu().numpy()
+    fd6 = fx["fd"][repr(1e-6)]["dJ_dz_cell"]
+    scale = 1e-3 * max(abs(v) for v in fd6)
+    worst = max(_rel(g[r, c], v, scale) for (r, c), v in zip(fx["cells"], fd6))
+    _say(f"autograd worst rel {worst:.2e}")
+    assert worst <= REL_TOL

@@ -161,6 +161,7 @@ def adjoint_config4(solver, sc, ob):
     case = sc.config4()
     p = case.params
     p.locked_dt = np.array(fx["locked_dt"])
+    solver.autotune(p, [(case.base, [case.candidate], case.monitors, None)], trial_steps=16)
     res = solver.adjoint(p, case.base, [case.candidate], case.monitors, case.assets, case.tau_m)[0]
     nx, ny = case.base.nx, case.base.ny
     area = case.base.dx * case.base.dy
@@ -201,7 +202,9 @@ def adjoint_best16(solver, sc, ob, w, res, horizon):
     p.locked_dt = sched
     p.max_steps = len(sched)
     assets = [(a.positions, a.threshold_m, a.weight) for a in objv.assets]
-    out = solver.adjoint(p, base, [w.candidates[i] for i in best], w.monitors, assets, objv.tau_m)
+    cands = [w.candidates[i] for i in best]
+    solver.autotune(p, [(base, cands, w.monitors, None)], trial_steps=16)
+    out = solver.adjoint(p, base, cands, w.monitors, assets, objv.tau_m)
     r0 = out[0]
     total = r0["forward_ms"] + r0["replay_ms"] + r0["reverse_ms"]
     return {"workload": f"16 best of the batch, {base.nx}x{base.ny}, first {len(sched)} steps on the best one's dt",
@@ -209,6 +212,22 @@ def adjoint_best16(solver, sc, ob, w, res, horizon):
             "replay_mismatches": r0["replay_mismatches"], "plain_forward_ms": r0["plain_forward_ms"],
             "forward_ms": r0["forward_ms"], "replay_ms": r0["replay_ms"], "reverse_ms": r0["reverse_ms"],
             "cell_steps_per_s_reverse": 16 * base.nx * base.ny * r0["steps"] / (r0["reverse_ms"] * 1e-3)}
+
+
+def ladder(solver_cls, sc, steps):
+    """The fidelity-ladder batch (256 instances: 45x35 crest, 90x70 and 180x140 signed,
+    4:2:1) in one launch per step, with band heights from the autotuner."""
+    lad = sc.config3()
+    s = solver_cls(0)
+    tuned = s.autotune(lad.params, lad.groups, trial_steps=16)
+    sess = s.session(lad.params, lad.groups)
+    sess.advance(8)
+    st = sess.advance(steps)
+    sess.close()
+    s.close()
+    return {"workload": "256 instances: 146 x 45x35 at 10 m (face-crest), 73 x 90x70 at 5 m, 37 x 180x140 at 2.5 m",
+            "cell_steps_per_s": st["cell_steps"] / st["wall_s"], "us_per_step": st["device_ms"] * 1e3 / steps,
+            "band_rows": [r for r, _ in tuned], "steps": steps}
 
 
 def main():
@@ -224,6 +243,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
     ap.add_argument("--adjoint-horizon", type=int, default=120)
+    ap.add_argument("--no-ladder", action="store_true", help="skip the fidelity-ladder leg")
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
@@ -300,6 +320,8 @@ def main():
             "cells_per_launch": cells_per_launch,
         },
     }
+    if not a.no_ladder and not a.copy_probe:
+        out["ladder"] = ladder(GpuSolver, sc, 200)
     if adj is not None:
         out["adjoint"] = adj
     if adj16 is not None:

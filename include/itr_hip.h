@@ -199,6 +199,20 @@ typedef struct {
 itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,
                            itr_adjoint_out* outs, char* err, size_t errlen);
 
+/* No counterpart in the reference (its wgpu kernel has fixed workgroup shapes): times
+ * the step on these grids for several band heights (rows per workgroup) per grid shape,
+ * by coordinate descent over shapes (largest work first), `trial_steps` launches per
+ * trial.  The chosen heights are kept in the context and used by later batches whose
+ * grids have the same shape (unless the "band_rows" option fixes one height). */
+typedef struct {
+    int32_t band_rows;
+    int32_t reserved;
+    double us_per_step;   /* device time per step of the whole batch at the chosen shapes */
+} itr_tune_result;
+
+itr_status itr_hip_autotune(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                            int32_t trial_steps, itr_tune_result* out, char* err, size_t errlen);
+
 /* No counterpart in the reference: the dt sequence of one candidate's free forward (at
  * most `capacity` steps, or p->max_steps), i.e. the schedule solver.rs:452-455 locks to
  * when a baseline's dt is reused.  Writes *n_out values to dts. */

@@ -92,7 +92,7 @@ struct DevGrid {
     const float* wet;
     const float* gn2;
     const int32_t* mon_map;
-    const uint8_t* band_mon;
+    const uint16_t* band_mon;  // [ny] per band: bit w set if wave w updates a monitored cell
     const int32_t* ghost;
     float* mon_max;            // [2][nslot][n_mon]   by state parity
     double* mon_tmax;          // [2][nslot][n_mon]
@@ -426,7 +426,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const int wave = tid >> 6;
     const int col = wave * kColsPerWave - 1 + lane;
     const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
-    const bool band_mon = G.band_mon[band] != 0;
+    const bool band_mon = ((G.band_mon[band] >> wave) & 1u) != 0;   // this wave's columns hold monitors
     const int npar = par ^ 1;
 
     const size_t cstride = 6 * G.plane;
@@ -1186,7 +1186,20 @@ struct GridHost {
     DevGrid d;
     std::vector<void*> allocs;
     int waves = 1;
+    std::vector<uint32_t> mon;   // monitored cells (row * nx + col)
+    uint16_t* d_bandmon = nullptr;
 };
+
+// Per band of R rows: bit w set if wave w (updated columns 62w .. 62w+61) holds a
+// monitored cell in the band.
+std::vector<uint16_t> band_masks(const std::vector<uint32_t>& mon, int nx, int ny, int R) {
+    std::vector<uint16_t> m((size_t)ny, 0);
+    for (uint32_t cell : mon) {
+        const int r = (int)(cell / (uint32_t)nx), c = (int)(cell % (uint32_t)nx);
+        m[(size_t)(r / R)] |= (uint16_t)(1u << (c / kColsPerWave));
+    }
+    return m;
+}
 
 }  // namespace
 
@@ -1198,7 +1211,31 @@ struct itr_ctx {
     int steps_per_chunk = 0;
     int use_graph = 1;
     int64_t min_lds = 0;   // reserve at least this much LDS per step workgroup (caps occupancy)
+    struct Tuned { int nx, ny, crest, R; };
+    std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
 };
+
+namespace {
+
+// Band rows of a grid: the fixed option, else the tuned value for the shape, else enough
+// bands to give the whole batch a few workgroups per CU (capped at 12 rows).
+int choose_band_rows(const itr_ctx* ctx, const itr_grid* grids, int n_grids, int g) {
+    const itr_grid& G = grids[g];
+    int R = 0;
+    if (ctx->band_rows > 0) R = ctx->band_rows;
+    else {
+        for (const auto& t : ctx->tuned)
+            if (t.nx == G.nx && t.ny == G.ny && t.crest == (G.crest ? 1 : 0)) R = t.R;
+        if (R == 0) {
+            double rows = 0.0;
+            for (int k = 0; k < n_grids; k++) rows += (double)grids[k].ny * (double)std::max(grids[k].n_cand, 0);
+            R = (int)std::lround(std::min(12.0, std::max(2.0, rows / 768.0)));
+        }
+    }
+    return std::max(1, std::min(R, G.ny));
+}
+
+}  // namespace
 
 struct itr_batch {
     itr_ctx* ctx = nullptr;
@@ -1533,8 +1570,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         D.P = (nx + 1 + 63) / 64 * 64;   // >= nx + 1: column nx holds the east boundary face
         D.waves = (nx + kColsPerWave - 1) / kColsPerWave;
         max_waves = std::max(max_waves, D.waves);
-        int R = ctx->band_rows > 0 ? ctx->band_rows : 12;
-        R = std::max(1, std::min(R, ny));
+        const int R = choose_band_rows(ctx, grids, n_grids, g);
         D.R = R;
         D.nbands = (ny + R - 1) / R;
         D.cand0 = total;
@@ -1567,7 +1603,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         }
         std::vector<float> wet(plane, 0.0f), gn2(plane, 0.0f);
         std::vector<int32_t> mon_map(plane, -1);
-        std::vector<uint8_t> band_mon(D.nbands, 0);
+        GH.mon.assign(I.mon_cells, I.mon_cells + I.n_mon);
+        const std::vector<uint16_t> band_mon = band_masks(GH.mon, nx, ny, R);
         for (int r = 0; r < ny; r++)
             for (int c = 0; c < nx; c++) {
                 wet[(size_t)r * D.P + c] = I.wet[(size_t)r * nx + c];
@@ -1576,7 +1613,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         for (int q = 0; q < I.n_mon; q++) {
             const int r = (int)(I.mon_cells[q] / (uint32_t)nx), c = (int)(I.mon_cells[q] % (uint32_t)nx);
             mon_map[(size_t)r * D.P + c] = q;
-            band_mon[r / R] = 1;
         }
         std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);   // parity 0 half
         std::memset(rowacc.data(), 0, rowacc.size() * sizeof(RowAcc));
@@ -1640,7 +1676,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         float *d_state, *d_dzx, *d_dzy, *d_dzx2 = nullptr, *d_dzy2 = nullptr, *d_wet, *d_gn2, *d_monmax, *d_ybnd, *d_sbuf;
         double *d_montmax, *d_ledger, *d_base = nullptr;
         int32_t *d_monmap, *d_ghost, *d_gpos = nullptr;
-        uint8_t* d_bandmon;
+        uint16_t* d_bandmon;
         uint32_t* d_moncells;
         RowAcc* d_rowacc;
         const size_t ncx = (size_t)std::max(nc, 1);
@@ -1657,7 +1693,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         ALLOC(d_wet, plane);
         ALLOC(d_gn2, plane);
         ALLOC(d_monmap, plane);
-        ALLOC(d_bandmon, (size_t)D.nbands);
+        ALLOC(d_bandmon, (size_t)ny);
         ALLOC(d_ghost, (size_t)(2 * nx + 2 * ny));
         ALLOC(d_monmax, 2 * ncx * std::max(I.n_mon, 1));
         ALLOC(d_montmax, 2 * ncx * std::max(I.n_mon, 1));
@@ -1679,7 +1715,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         bool ok = up(d_state, st.data(), st.size() * 4) && up(d_dzx, dzx.data(), dzx.size() * 4) &&
                   up(d_dzy, dzy.data(), dzy.size() * 4) && up(d_wet, wet.data(), plane * 4) &&
                   up(d_gn2, gn2.data(), plane * 4) && up(d_monmap, mon_map.data(), plane * 4) &&
-                  up(d_bandmon, band_mon.data(), band_mon.size()) &&
+                  up(d_bandmon, band_mon.data(), band_mon.size() * sizeof(uint16_t)) &&
                   up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
                   up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
                   up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
@@ -1704,6 +1740,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         D.gn2 = d_gn2;
         D.mon_map = d_monmap;
         D.band_mon = d_bandmon;
+        GH.d_bandmon = d_bandmon;
         D.ghost = d_ghost;
         D.mon_max = d_monmax;
         D.mon_tmax = d_montmax;
@@ -1741,7 +1778,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
     uint32_t* d_status;
     if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
     b->d_grids = d_grids;
-    if ((s = dev_alloc(misc, &d_tasks, tasks.size(), err, errlen)) != ITR_OK) return fail(s);
+    size_t task_cap = 0;   // room for any band height (re-tiling never reallocates)
+    for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)D.ny;
+    if ((s = dev_alloc(misc, &d_tasks, std::max(task_cap, tasks.size()), err, errlen)) != ITR_OK) return fail(s);
     b->d_tasks = d_tasks;
     if ((s = dev_alloc(misc, &d_snap, 2 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_snap = d_snap;
@@ -2291,6 +2330,126 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
 #undef ADJ_TRY
     cleanup();
     return ITR_OK;
+}
+
+}  // extern "C"
+
+// Re-tile a resident batch: new band heights per grid (tasks, band monitor masks and
+// LDS size follow).  State and control blocks are untouched.
+itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t errlen) {
+    size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0);
+    for (size_t g = 0; g < b->grids.size(); g++) {
+        const DevGrid& D = b->grids[g].d;
+        lds = std::max(lds, lds_layout(std::max(1, std::min(Rs[g], D.ny)), D.nx).total * sizeof(float));
+    }
+    if (lds > 160 * 1024 - 1024) {
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit", lds);
+        return ITR_DECLINED;
+    }
+    std::vector<int4> tasks;
+    std::vector<DevGrid> dg;
+    for (size_t g = 0; g < b->grids.size(); g++) {
+        GridHost& GH = b->grids[g];
+        DevGrid& D = GH.d;
+        const int R = std::max(1, std::min(Rs[g], D.ny));
+        D.R = R;
+        D.nbands = (D.ny + R - 1) / R;
+        const std::vector<uint16_t> bm = band_masks(GH.mon, D.nx, D.ny, R);
+        HIP_TRY(hipMemcpy(GH.d_bandmon, bm.data(), bm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        for (int k = 0; k < D.ncand; k++)
+            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4((int)g, k, band, 0));
+        dg.push_back(D);
+    }
+    HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
+    if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    b->ntasks = (int)tasks.size();
+    b->args.ntasks = b->ntasks;
+    b->lds_bytes = lds;
+    b->stats.band_rows = dg.empty() ? 0 : dg[0].R;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->graph_steps = 0;
+    return ITR_OK;
+}
+
+extern "C" {
+
+itr_status itr_hip_autotune(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                            int32_t trial_steps, itr_tune_result* out, char* err, size_t errlen) {
+    if (!ctx || !p || !grids || n_grids <= 0 || !out) {
+        set_err(err, errlen, "null argument");
+        return ITR_EINVAL;
+    }
+    const int T = std::max(4, (int)trial_steps) / 4 * 4;
+    const int saved = ctx->band_rows;
+    ctx->band_rows = 0;
+    itr_batch* b = nullptr;
+    itr_solver_params q = *p;
+    q.max_steps = 0;
+    itr_status s = batch_create_impl(ctx, &q, grids, n_grids, &b, 0, err, errlen);
+    ctx->band_rows = saved;
+    if (s != ITR_OK) return s;
+    std::vector<int> Rs;
+    for (auto& GH : b->grids) Rs.push_back(GH.d.R);
+    auto timed = [&](double* ms) -> itr_status {
+        itr_status e = itr_hip_batch_advance(b, T, err, errlen);   // warm: graph capture, caches
+        if (e == ITR_OK) e = itr_hip_batch_advance(b, T, err, errlen);
+        *ms = b->stats.device_ms;
+        return e;
+    };
+    double best_all = 0.0;
+    s = timed(&best_all);
+    // Coordinate descent over grid shapes, the largest work first.
+    std::vector<int> order(n_grids);
+    for (int g = 0; g < n_grids; g++) order[g] = g;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        return (double)grids[x].nx * grids[x].ny * grids[x].n_cand > (double)grids[y].nx * grids[y].ny * grids[y].n_cand;
+    });
+    static const int cand_R[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+    std::vector<char> done(n_grids, 0);
+    for (int gi = 0; gi < n_grids && s == ITR_OK; gi++) {
+        const int g = order[gi];
+        if (done[g]) continue;
+        std::vector<int> same;   // grids of the same shape share one band height
+        for (int k = 0; k < n_grids; k++)
+            if (grids[k].nx == grids[g].nx && grids[k].ny == grids[g].ny && grids[k].crest == grids[g].crest) same.push_back(k);
+        int bestR = Rs[g];
+        for (int R : cand_R) {
+            if (R > grids[g].ny || R == bestR) continue;
+            std::vector<int> trial = Rs;
+            for (int k : same) trial[k] = R;
+            if (retile(b, trial, err, errlen) != ITR_OK) continue;   // does not fit the LDS
+            double ms = 0.0;
+            if ((s = timed(&ms)) != ITR_OK) break;
+            if (ms < best_all) {
+                best_all = ms;
+                bestR = R;
+            }
+        }
+        for (int k : same) {
+            Rs[k] = bestR;
+            done[k] = 1;
+        }
+        if (s == ITR_OK) s = retile(b, Rs, err, errlen);
+    }
+    if (s == ITR_OK) {
+        for (int g = 0; g < n_grids; g++) {
+            out[g].band_rows = Rs[g];
+            out[g].reserved = 0;
+            out[g].us_per_step = best_all * 1e3 / T;
+            bool found = false;
+            for (auto& t : ctx->tuned)
+                if (t.nx == grids[g].nx && t.ny == grids[g].ny && t.crest == (grids[g].crest ? 1 : 0)) {
+                    t.R = Rs[g];
+                    found = true;
+                }
+            if (!found) ctx->tuned.push_back(itr_ctx::Tuned{grids[g].nx, grids[g].ny, grids[g].crest ? 1 : 0, Rs[g]});
+        }
+    }
+    itr_hip_batch_destroy(b);
+    return s;
 }
 
 itr_status itr_hip_dt_schedule(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, int32_t cand,

@@ -217,6 +217,21 @@ class GpuSolver:
                                            C.byref(n), err, 1024), err)
         return out[:n.value].copy()
 
+    def autotune(self, p: SolverParams, groups, trial_steps: int = 16):
+        """Choose band heights per grid shape by timing (itr_hip_autotune); later batches
+        of this solver use them.  ``groups`` as for ``session``.  Returns [(band_rows,
+        us_per_step)] per group."""
+        keep = _Keep()
+        q = _params_c(p, keep)
+        grids = (_abi.GridC * len(groups))()
+        for i, (base, terrs, mons, guard) in enumerate(groups):
+            grids[i] = _grid_c(base, terrs, mons, keep, guard, p.initial_stage is not None)
+        out = (_abi.TuneResultC * len(groups))()
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_autotune(self._h, C.byref(q), grids, len(groups), int(trial_steps), out, err, 1024),
+               err)
+        return [(o.band_rows, o.us_per_step) for o in out]
+
     def session(self, p: SolverParams, groups, on_sync=None):
         """Upload a batch once (inputs stay resident); ``groups`` is a list of
         (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""

@@ -46,15 +46,15 @@ def test_struct_layouts_match_header():
 #include "itr_hip.h"
 #define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
 int main(void) {
-  printf("itr_solver_params %zu\nitr_grid %zu\nitr_run_out %zu\nitr_stats %zu\nitr_objective %zu\nitr_adjoint_out %zu\n",
+  printf("itr_solver_params %zu\nitr_grid %zu\nitr_run_out %zu\nitr_stats %zu\nitr_objective %zu\nitr_adjoint_out %zu\nitr_tune_result %zu\n",
          sizeof(itr_solver_params), sizeof(itr_grid), sizeof(itr_run_out), sizeof(itr_stats), sizeof(itr_objective),
-         sizeof(itr_adjoint_out));
+         sizeof(itr_adjoint_out), sizeof(itr_tune_result));
   F(itr_solver_params, rain_t) F(itr_solver_params, max_steps) F(itr_solver_params, skip_dry)
   F(itr_grid, wet) F(itr_grid, n_cand) F(itr_grid, dzx) F(itr_grid, mon_cells) F(itr_grid, guard_tol)
   F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger)
   F(itr_stats, launches) F(itr_stats, use_graph)
   F(itr_objective, asset_start) F(itr_objective, tau_m) F(itr_adjoint_out, dJ_dz_cell) F(itr_adjoint_out, steps)
-  F(itr_adjoint_out, replay_mismatches) F(itr_adjoint_out, reverse_ms)
+  F(itr_adjoint_out, replay_mismatches) F(itr_adjoint_out, reverse_ms) F(itr_tune_result, us_per_step)
   return 0;
 }
 """
@@ -66,7 +66,8 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         out = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], capture_output=True, text=True).stdout.split("\n") if line)
     ct = {"itr_solver_params": _abi.SolverParamsC, "itr_grid": _abi.GridC, "itr_run_out": _abi.RunOutC,
-          "itr_stats": _abi.StatsC, "itr_objective": _abi.ObjectiveC, "itr_adjoint_out": _abi.AdjointOutC}
+          "itr_stats": _abi.StatsC, "itr_objective": _abi.ObjectiveC, "itr_adjoint_out": _abi.AdjointOutC,
+          "itr_tune_result": _abi.TuneResultC}
     for k, T in ct.items():
         assert int(out[k]) == C.sizeof(T), k
     for key, v in out.items():

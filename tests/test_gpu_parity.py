@@ -243,3 +243,29 @@ def test_advance_then_run(solver):
     finally:
         sess.close()
     assert_run_equal(g, oracle_run(base, p, monitors=mon), "advance+run")
+
+
+def test_autotune_then_run_ladder():
+    """The autotuner picks band heights per grid shape on a fidelity ladder; a batch
+    created afterwards uses them and stays bitwise equal to the oracle."""
+    _say("case: autotune on a 14-instance ladder")
+    from inverse_terrain_engine_amd.gpu import GpuSolver
+    s = GpuSolver(0)
+    lad = sc.config3(n_inst=14, duration=600.0)
+    tuned = s.autotune(lad.params, lad.groups, trial_steps=8)
+    _say(f"tuned band rows {tuned}")
+    assert len(tuned) == 3 and all(r >= 1 for r, _ in tuned)
+    sess = s.session(lad.params, lad.groups)
+    try:
+        st = sess.run()
+        res = sess.results()
+    finally:
+        sess.close()
+        s.close()
+    k = 0
+    for base, cands, mon, _ in lad.groups:
+        for i in range(len(cands)):
+            if i < 2:
+                o = oracle_run(base, lad.params, edit=lad.edits[k].as_dict(), monitors=mon)
+                assert_run_equal(res[k], o, f"tuned ladder instance {k} ({base.nx}x{base.ny})")
+            k += 1

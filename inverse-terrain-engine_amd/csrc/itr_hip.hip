@@ -2386,8 +2386,10 @@ itr_status itr_hip_autotune(itr_ctx* ctx, const itr_solver_params* p, const itr_
     const int saved = ctx->band_rows;
     ctx->band_rows = 0;
     itr_batch* b = nullptr;
-    itr_solver_params q = *p;
+    itr_solver_params q = *p;   // trials time the kernel: free dt, no step bound
     q.max_steps = 0;
+    q.locked_dt = nullptr;
+    q.n_locked = 0;
     itr_status s = batch_create_impl(ctx, &q, grids, n_grids, &b, 0, err, errlen);
     ctx->band_rows = saved;
     if (s != ITR_OK) return s;

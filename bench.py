@@ -76,7 +76,7 @@ def _run_cands(po, ot, params, edits, threads, monitors):
     return wall, steps, cu
 
 
-def _pmc_pass(counter, steps):
+def _pmc_pass(counter, steps, band_rows):
     """One rocprofv3 counter pass over a short run of this benchmark (plus the copy
     probe); returns {kernel name: [per-dispatch values]} or None if it did not run."""
     import csv
@@ -88,7 +88,7 @@ def _pmc_pass(counter, steps):
     out = tempfile.mkdtemp(prefix="itr_pmc_", dir="/tmp")
     cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc",
            "--", sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", "2", "--no-cpu",
-           "--no-traffic", "--copy-probe"]
+           "--no-traffic", "--copy-probe", "--no-adjoint", "--no-ladder", "--band-rows", str(band_rows)]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
     path = None
@@ -108,13 +108,13 @@ def _pmc_pass(counter, steps):
     return vals
 
 
-def measure_traffic(kernel_us, steps=6):
+def measure_traffic(kernel_us, band_rows, steps=6):
     """HBM bytes per step launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
     scaled by what the same counter reads for a plain 1 GiB device copy; and the VALU
     instruction count per launch (SQ_INSTS_VALU, wave64 instructions)."""
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
-        v = _pmc_pass(counter, steps)
+        v = _pmc_pass(counter, steps, band_rows)
         if v is None:
             return None
         step = [x for k, xs in v.items() if "itr_step_kernel" in k for x in xs]
@@ -244,6 +244,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
     ap.add_argument("--adjoint-horizon", type=int, default=120)
     ap.add_argument("--no-ladder", action="store_true", help="skip the fidelity-ladder leg")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default band height")
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
@@ -265,7 +266,11 @@ def main():
     solver = GpuSolver(0, band_rows=a.band_rows)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
-    sess = solver.session(w.params, [(w.base, w.candidates, w.monitors, None)])
+    groups = [(w.base, w.candidates, w.monitors, None)]
+    tuned = None
+    if not a.band_rows and not a.no_autotune:
+        tuned = solver.autotune(w.params, groups, trial_steps=16)
+    sess = solver.session(w.params, groups)
     if a.warmup:
         sess.advance(a.warmup)
     # advance() returns after synchronizing the library's stream (HIP event), so the
@@ -305,6 +310,7 @@ def main():
                         "seeded DEM (40 hills, 8 gullies), bimodal 2 h hyetograph (50/90 mm/h), infiltration 5 mm/h",
             "candidates": a.cands, "nx": a.n, "ny": a.n, "band_rows": st["band_rows"],
             "waves_per_task": st["waves_per_task"], "device": solver.adapter_info(),
+            "autotuned": tuned is not None,
             "hbm_fraction": achieved / HBM_PEAK, "prep_s": round(t_prep, 2),
             "steps_run_per_candidate": int(res[0].steps),
         },
@@ -330,7 +336,7 @@ def main():
     if a.copy_probe:
         copy_probe()
     if not a.no_traffic:
-        t = measure_traffic(kernel_s * 1e6)
+        t = measure_traffic(kernel_s * 1e6, st["band_rows"])
         if t is not None:
             cells = cells_per_launch
             out["roofline"]["traffic"] = t["fetch_bytes"] + t["write_bytes"]

@@ -200,7 +200,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
                            itr_adjoint_out* outs, char* err, size_t errlen);
 
 /* No counterpart in the reference (its wgpu kernel has fixed workgroup shapes): times
- * the step on these grids for several band heights (rows per workgroup) per grid shape,
+ * the step on these grids for band heights 1..16, 20, 24, 28, 32 (rows per workgroup) per grid shape,
  * by coordinate descent over shapes (largest work first), `trial_steps` launches per
  * trial.  The chosen heights are kept in the context and used by later batches whose
  * grids have the same shape (unless the "band_rows" option fixes one height). */
@@ -212,6 +212,12 @@ typedef struct {
 
 itr_status itr_hip_autotune(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
                             int32_t trial_steps, itr_tune_result* out, char* err, size_t errlen);
+
+/* No counterpart in the reference: runs the step kernel's short correctly-rounded
+ * reciprocal and square-root sequences on every f32 bit pattern and counts results that
+ * differ from the IEEE operations (both counts are 0 for a correct build). */
+itr_status itr_hip_math_selfcheck(itr_ctx* ctx, uint64_t* recip_mismatches, uint64_t* sqrt_mismatches, char* err,
+                                  size_t errlen);
 
 /* No counterpart in the reference: the dt sequence of one candidate's free forward (at
  * most `capacity` steps, or p->max_steps), i.e. the schedule solver.rs:452-455 locks to

@@ -76,7 +76,7 @@ SYNC_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C
 EXPORTS = ["itr_hip_create", "itr_hip_destroy", "itr_hip_device_info", "itr_hip_set_option", "itr_hip_run_batch",
            "itr_hip_batch_create", "itr_hip_batch_run", "itr_hip_batch_results", "itr_hip_batch_stats",
            "itr_hip_batch_destroy", "itr_hip_batch_advance", "itr_hip_adjoint",
-           "itr_hip_dt_schedule", "itr_hip_autotune"]
+           "itr_hip_dt_schedule", "itr_hip_autotune", "itr_hip_math_selfcheck"]
 
 _lib = None
 
@@ -122,6 +122,8 @@ def load():
     L.itr_hip_autotune.restype = C.c_int32
     L.itr_hip_autotune.argtypes = [vp, C.POINTER(SolverParamsC), C.POINTER(GridC), C.c_int32, C.c_int32,
                                    C.POINTER(TuneResultC), C.c_char_p, C.c_size_t]
+    L.itr_hip_math_selfcheck.restype = C.c_int32
+    L.itr_hip_math_selfcheck.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
     L.itr_hip_batch_destroy.restype = None
     L.itr_hip_batch_destroy.argtypes = [vp]
     _lib = L

@@ -6,7 +6,8 @@
 // equal to the reference's f32 fused kernel.  Rules that keep it so:
 //   * built with -ffp-contract=off: no a*b+c -> fma contraction;
 //   * '/' and sqrt stay IEEE correctly rounded (hipcc's default
-//     -fhip-fp32-correctly-rounded-divide-sqrt), f32 denormals preserved;
+//     -fhip-fp32-correctly-rounded-divide-sqrt, or the exhaustively verified short
+//     sequences crecip / csqrt below), f32 denormals preserved;
 //   * max/min are selects with the reference's NaN rule (a > b ? a : b), never
 //     fmaxf/fminf; abs is a select that keeps -0.0, never fabsf;
 //   * every expression keeps the reference's association order (left to right).
@@ -30,7 +31,36 @@ __device__ __forceinline__ float smax(float a, float b) { return a > b ? a : b; 
 __device__ __forceinline__ float smin(float a, float b) { return a < b ? a : b; }   // real.rs:48-50
 __device__ __forceinline__ float sabs(float x) { return x < 0.0f ? -x : x; }        // real.rs:39-41
 __device__ __forceinline__ float canon(float x) { return x == 0.0f ? 0.0f : x; }    // real.rs:56-58
-__device__ __forceinline__ float csqrt(float x) { return __builtin_sqrtf(x); }
+// Correctly rounded f32 square root and reciprocal in few instructions.  Each takes the
+// hardware estimate and one Newton step whose residual is computed with a fused
+// multiply-add (Markstein's construction, "IA-64 and Elementary Functions", 2000); an
+// exhaustive run over all 2^32 inputs (itr_hip_math_selfcheck) shows the short sequence
+// equals the IEEE operation on the ranges guarded below, and every other input (denormal
+// or tiny, huge, inf, NaN, negative) takes the IEEE operation itself, which the step
+// never needs on a healthy state.
+//
+// sqrt: y = rsq(x) (clamped so that x = +0 gives +0), s = x y, s + (x - s^2) y / 2;
+// exact for x = +0 and 2^-102 <= x < inf.
+__device__ __forceinline__ float csqrt(float x) {
+    float y = __builtin_amdgcn_rsqf(x);
+    y = y < 0x1p64f ? y : 0x1p64f;
+    const float s = x * y;
+    const float e = __builtin_fmaf(-s, s, x);
+    float r = __builtin_fmaf(e, 0.5f * y, s);
+    const uint32_t b = __float_as_uint(x);
+    if (__builtin_expect(b - 0x0C800000u >= 0x7F800000u - 0x0C800000u && b != 0u, 0)) r = __builtin_sqrtf(x);
+    return r;
+}
+
+// 1/x: r = rcp(x), r + (1 - x r) r; exact for 2^-126 <= x < 2^126.
+__device__ __forceinline__ float crecip(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    float q = __builtin_fmaf(e, r, r);
+    const uint32_t b = __float_as_uint(x);
+    if (__builtin_expect(b - 0x00800000u >= 0x7E000000u - 0x00800000u, 0)) q = 1.0f / x;
+    return q;
+}
 
 // x^(-1/3): integer seed then three multiplication-only Newton steps (real.rs:116-129).
 __device__ __forceinline__ float inv_cbrt(float x) {
@@ -49,7 +79,7 @@ __device__ __forceinline__ float inv_cbrt(float x) {
 // Desingularised velocity and celerity of one cell (numerics.rs:36-43).
 __device__ __forceinline__ Vel cell_velocity(float h, float qx, float qy, const StepK& k) {
     const float h2 = h * h;
-    const float inv = 1.0f / (h2 + smax(h2, k.heps2));
+    const float inv = crecip(h2 + smax(h2, k.heps2));
     const float t = (h + h) * inv;
     Vel o;
     o.u = qx * t;
@@ -80,7 +110,7 @@ __device__ __forceinline__ Flux hll_flux(float hl, float ul, float tl, float cl,
     const bool use_l = sl >= 0.0f;
     const bool use_r = sr <= 0.0f;
     const float den = (both_dry || use_l || use_r) ? 1.0f : sr - sl;
-    const float inv = 1.0f / den;
+    const float inv = crecip(den);
     const float slsr = sl * sr;
     const float fm_star = (sr * ml - sl * mr + slsr * (hr - hl)) * inv;
     const float fn_star = (sr * pl - sl * pr + slsr * (mr - ml)) * inv;
@@ -146,7 +176,7 @@ __device__ __forceinline__ CellNew cell_update(float h, float qx, float qy, floa
     const float r3 = r * r * r;
     const float r7 = r3 * r3 * r;
     const float qm = csqrt(qx1 * qx1 + qy1 * qy1);
-    const float inv = 1.0f / (1.0f + k.dt * gn2 * qm * r7);
+    const float inv = crecip(1.0f + k.dt * gn2 * qm * r7);
     const float keep = dry ? 0.0f : 1.0f;
     float qx2 = qx1 * inv * keep;
     float qy2 = qy1 * inv * keep;

@@ -232,6 +232,14 @@ class GpuSolver:
                err)
         return [(o.band_rows, o.us_per_step) for o in out]
 
+    def math_selfcheck(self):
+        """(reciprocal, sqrt) mismatch counts of the step's short sequences against the
+        IEEE operations over all 2^32 inputs; (0, 0) for a correct build."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        err = C.create_string_buffer(512)
+        _check(self._L.itr_hip_math_selfcheck(self._h, C.byref(a), C.byref(b), err, 512), err)
+        return int(a.value), int(b.value)
+
     def session(self, p: SolverParams, groups, on_sync=None):
         """Upload a batch once (inputs stay resident); ``groups`` is a list of
         (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""

@@ -269,3 +269,10 @@ def test_autotune_then_run_ladder():
                 o = oracle_run(base, lad.params, edit=lad.edits[k].as_dict(), monitors=mon)
                 assert_run_equal(res[k], o, f"tuned ladder instance {k} ({base.nx}x{base.ny})")
             k += 1
+
+
+def test_short_sqrt_and_reciprocal_are_ieee(solver):
+    """The step's short correctly-rounded sequences equal the IEEE operations on every
+    one of the 2^32 f32 inputs (the guarded ranges plus the IEEE fallbacks)."""
+    _say("case: exhaustive reciprocal / sqrt check")
+    assert solver.math_selfcheck() == (0, 0)

@@ -117,7 +117,7 @@ def measure_traffic(kernel_us, band_rows, steps=6):
         v = _pmc_pass(counter, steps, band_rows)
         if v is None:
             return None
-        step = [x for k, xs in v.items() if "itr_step_kernel" in k for x in xs]
+        step = [x for k, xs in v.items() if "itr_step_" in k for x in xs]
         copy = [x for k, xs in v.items() if "copyBuffer" in k for x in xs]
         if not step:
             return None

@@ -2,7 +2,7 @@
 // behind the C ABI of include/itr_hip.h.
 //
 // Execution model
-//   * One step of every active candidate of every grid is ONE launch of itr_step_kernel
+//   * One step of every active candidate of every grid is ONE launch of the step kernel
 //     (grids of different widths and face modes share the launch).  A workgroup owns a
 //     band of R full rows of one candidate and marches down it (register-resident
 //     sliding window in y); waves tile the row in x with 62 updated columns and one halo
@@ -99,6 +99,7 @@ struct DevGrid {
     RowAcc* rowacc;            // [2][nslot][ny]      by state parity
     float* ybnd;               // [2][nslot][2][nx]   top / bottom boundary y-face mass
     float* sbuf;               // [nslot][plane] CFL rate at sync steps (within one workgroup)
+    float* fixbuf;             // [nslot][plane] positivity-fix values of rows that have one
     double* ledger;            // [nslot][max_ledger][8]
     struct StepRec* steplog;   // [nslot][max_log] constants of every step run (adjoint replay)
     const double* baseline_max;
@@ -151,13 +152,16 @@ struct KArgs {
 };
 
 // Dynamic LDS of the step kernel, in floats (offsets 16-byte aligned).
-struct LdsLayout { size_t infil, fix, bw, be, wet, red, acc, total; };
+// Positivity-fix values are zero almost everywhere: a wave that has one in a row writes
+// its lanes' values to the grid's fix buffer and sets its bit in the row's mask, so only
+// infiltration is staged per cell.
+struct LdsLayout { size_t infil, fixmask, bw, be, wet, red, acc, total; };
 __host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
     auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
     LdsLayout L;
     L.infil = 0;
-    L.fix = al(L.infil + (size_t)R * nx);
-    L.bw = al(L.fix + (size_t)R * nx);
+    L.fixmask = al(L.infil + (size_t)R * nx);
+    L.bw = al(L.fixmask + (size_t)R);
     L.be = al(L.bw + R);
     L.wet = al(L.be + R);
     L.red = al(L.wet + R);
@@ -439,6 +443,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
+    ITR_GLOBAL float* __restrict__ fixbuf = gp(G.fixbuf + (size_t)local * G.plane);
     const RowAcc* ra_in = G.rowacc + slot(G, par, local) * ny;
     RowAcc* ra_out = G.rowacc + slot(G, npar, local) * ny;
     float* yb_out = G.ybnd + slot(G, npar, local) * 2 * nx;
@@ -506,16 +511,16 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     }
 
 
-    // LDS: per-row staging of infil / fix (R x nx each) + row scalars (lds_layout).
+    // LDS: per-row staging of infil (R x nx) + row scalars (lds_layout).
     const LdsLayout LL = lds_layout(R, nx);
     float* l_infil = lds + LL.infil;
-    float* l_fix = lds + LL.fix;
+    uint32_t* l_fixmask = (uint32_t*)(lds + LL.fixmask);
     float* l_bw = lds + LL.bw;
     float* l_be = lds + LL.be;
     uint32_t* l_wet = (uint32_t*)(lds + LL.wet);
     int* l_red = (int*)(lds + LL.red);            // [2 * kMaxWaves]
     float* l_acc = lds + LL.acc;                  // [R][16][4]
-    for (int r = tid; r < R; r += nthr) { l_wet[r] = 0u; l_bw[r] = 0.0f; l_be[r] = 0.0f; }
+    for (int r = tid; r < R; r += nthr) { l_wet[r] = 0u; l_fixmask[r] = 0u; l_bw[r] = 0.0f; l_be[r] = 0.0f; }
     __syncthreads();
 
     // Old-state cell of this lane's column in `row`: raw loads (scalar row base + lane
@@ -599,7 +604,6 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             bstore(u.qx, rs_ho, cb, so + planeb);
             bstore(u.qy, rs_ho, cb, so + 2 * planeb);
             l_infil[r * nx + col] = u.infil;
-            l_fix[r * nx + col] = u.fix;
             smax_bits = max(smax_bits, __float_as_int(u.s));
             bad = bad || u.bad;
             if (sync) sbuf[(size_t)j * P + ccol] = u.s;
@@ -619,6 +623,10 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         }
         const unsigned long long wb = __ballot(upd && u.h > 0.0f);
         if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
+        if (__builtin_expect(__ballot(upd && u.fix != 0.0f) != 0ull, 0)) {   // rare: a cell went negative
+            if (upd) fixbuf[(size_t)j * P + ccol] = u.fix;
+            if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
+        }
         ch = nh;
         cqx = nqx;
         cqy = nqy;
@@ -643,10 +651,11 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         const int r = t >> 4, L = t & 15;
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int j = a0 + r;
+        const uint32_t fm = l_fixmask[r];   // waves whose fix values are in fixbuf (others hold +0)
         for (int pc = (L == 0 ? 16 : L); pc <= nx; pc += 16) {
             const int c = pc - 1;
             acc[0] += l_infil[(size_t)r * nx + c];
-            acc[1] += l_fix[(size_t)r * nx + c];
+            if (fm != 0u) acc[1] += ((fm >> (c / kColsPerWave)) & 1u) ? fixbuf[(size_t)j * P + c] : 0.0f;
             if (sync) {
                 acc[2] += Ho[(size_t)j * P + c];
                 acc[3] += sbuf[(size_t)j * P + c];
@@ -816,7 +825,7 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
 // mode gets a kernel without the other mode's code and its register budget.
 constexpr int kCtlWords = (int)(sizeof(CandCtl) / 4);
 template <int MODES>
-__global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a, uint32_t lk) {
+__device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     extern __shared__ float lds[];
     __shared__ double shd[48];
     __shared__ CandCtl sS;
@@ -908,6 +917,14 @@ __global__ void __launch_bounds__(1024) itr_step_kernel(KArgs a, uint32_t lk) {
         a.status[cand] = sS.flags;
     }
 }
+
+// The launch entry points.  The signed-face kernel (the common case) is held to 72 VGPRs
+// (7 waves per SIMD: three 9-wave workgroups per CU); the crest variants keep their
+// natural budget.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7)))
+itr_step_signed(KArgs a, uint32_t lk) { step_body<1>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3>(a, lk); }
 
 // Head of the first step of every candidate: control copy 1 (read by launch 0).
 __global__ void itr_init_kernel(KArgs a) {
@@ -1310,11 +1327,11 @@ itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t er
     for (int s = 0; s < n; s++) {
         const uint32_t lk = (uint32_t)((lk0 + (uint64_t)s) & 3u);
         if (b->modes == 1)
-            hipLaunchKernelGGL(itr_step_kernel<1>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+            hipLaunchKernelGGL(itr_step_signed, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
         else if (b->modes == 2)
-            hipLaunchKernelGGL(itr_step_kernel<2>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+            hipLaunchKernelGGL(itr_step_crest, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
         else
-            hipLaunchKernelGGL(itr_step_kernel<3>, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+            hipLaunchKernelGGL(itr_step_mixed, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
     }
     HIP_TRY(hipGetLastError());
     return ITR_OK;
@@ -1690,7 +1707,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         }
         total += nc;
         // device buffers
-        float *d_state, *d_dzx, *d_dzy, *d_dzx2 = nullptr, *d_dzy2 = nullptr, *d_wet, *d_gn2, *d_monmax, *d_ybnd, *d_sbuf;
+        float *d_state, *d_dzx, *d_dzy, *d_dzx2 = nullptr, *d_dzy2 = nullptr, *d_wet, *d_gn2, *d_monmax, *d_ybnd, *d_sbuf,
+            *d_fixbuf;
         double *d_montmax, *d_ledger, *d_base = nullptr;
         int32_t *d_monmap, *d_ghost, *d_gpos = nullptr;
         uint16_t* d_bandmon;
@@ -1717,6 +1735,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         ALLOC(d_rowacc, 2 * ncx * ny);
         ALLOC(d_ybnd, 2 * ncx * 2 * nx);
         ALLOC(d_sbuf, ncx * plane);
+        ALLOC(d_fixbuf, ncx * plane);
         ALLOC(d_ledger, ncx * b->max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
         StepRec* d_steplog = nullptr;
@@ -1764,6 +1783,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         D.rowacc = d_rowacc;
         D.ybnd = d_ybnd;
         D.sbuf = d_sbuf;
+        D.fixbuf = d_fixbuf;
         D.ledger = d_ledger;
         D.baseline_max = d_base;
         D.guard_pos = d_gpos;

@@ -175,11 +175,12 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
 }
 
 // ------------------------------------------------------------------ device helpers
+// Lanes without a source (lane 0 / lane 63) read 0 (bound_ctrl), as the halo lanes expect.
 __device__ __forceinline__ float from_left(float x) {   // lane i <- lane i-1 (DPP wave_shr:1)
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float x) {  // lane i <- lane i+1 (DPP wave_shl:1)
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 
 // Wave-uniform values into scalar registers.
@@ -557,11 +558,15 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     else fn = face_signed<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), K);
     if (a0 == 0 && owner) yb_out[col] = fn.m;
 
-    // Row inputs are loaded one row ahead of their use (software pipelining).
-    Cell pre_c = fetch(a0 + 1);                                   // row j+1 (raw)
+    // Row inputs are loaded two rows ahead of their use (software pipelining deep enough
+    // to cover HBM latency at two resident workgroups per CU).
+    Cell pre_c = fetch(a0 + 1), pre_c2 = fetch(a0 + 2);
     float pre_dzs = dzy_at(rs_dzy, a0 + 1), pre_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 1) : 0.0f;
-    float pre_dzw = dzx_at(rs_dzx, a0), pre_dzw2 = CREST ? dzx_at(rs_dzx2, a0) : 0.0f;   // row j
+    float nxt_dzs = dzy_at(rs_dzy, a0 + 2), nxt_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 2) : 0.0f;
+    float pre_dzw = dzx_at(rs_dzx, a0), pre_dzw2 = CREST ? dzx_at(rs_dzx2, a0) : 0.0f;
+    float nxt_dzw = dzx_at(rs_dzx, min(a0 + 1, ny - 1)), nxt_dzw2 = CREST ? dzx_at(rs_dzx2, min(a0 + 1, ny - 1)) : 0.0f;
     float pre_wet = row_at(rs_wet, a0), pre_gn2 = row_at(rs_gn2, a0);
+    float nxt_wet = row_at(rs_wet, a0 + 1), nxt_gn2 = row_at(rs_gn2, a0 + 1);
 
     int smax_bits = 0;
     bool bad = false;
@@ -570,14 +575,20 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
         const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
         const float wet = pre_wet, gn2 = pre_gn2;
-        // Issue the next row's loads (raw values) before this row's arithmetic.
-        pre_c = fetch(j + 2);
-        pre_dzs = dzy_at(rs_dzy, j + 2);
-        if (CREST) pre_dzs2 = dzy_at(rs_dzy2, j + 2);
-        pre_dzw = dzx_at(rs_dzx, min(j + 1, ny - 1));
-        if (CREST) pre_dzw2 = dzx_at(rs_dzx2, min(j + 1, ny - 1));
-        pre_wet = row_at(rs_wet, j + 1);
-        pre_gn2 = row_at(rs_gn2, j + 1);
+        pre_c = pre_c2;
+        pre_dzs = nxt_dzs;
+        pre_dzs2 = nxt_dzs2;
+        pre_dzw = nxt_dzw;
+        pre_dzw2 = nxt_dzw2;
+        pre_wet = nxt_wet;
+        pre_gn2 = nxt_gn2;
+        pre_c2 = fetch(j + 3);
+        nxt_dzs = dzy_at(rs_dzy, j + 3);
+        if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, j + 3);
+        nxt_dzw = dzx_at(rs_dzx, min(j + 2, ny - 1));
+        if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(j + 2, ny - 1));
+        nxt_wet = row_at(rs_wet, j + 2);
+        nxt_gn2 = row_at(rs_gn2, j + 2);
 
         const Vel vn = cell_velocity(nh, nqx, nqy, K);
         Flux fs;
@@ -918,10 +929,9 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     }
 }
 
-// The launch entry points.  The signed-face kernel (the common case) is held to 72 VGPRs
-// (7 waves per SIMD: three 9-wave workgroups per CU); the crest variants keep their
-// natural budget.
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7)))
+// The launch entry points.  The signed-face kernel (the common case) is held to 80 VGPRs
+// (6 waves per SIMD); the crest variants keep their natural budget.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 itr_step_signed(KArgs a, uint32_t lk) { step_body<1>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3>(a, lk); }

@@ -58,6 +58,22 @@ def cpu_baseline(workload, threads, n_cand, n_steps):
     return float(cu.sum()) / wall, wall, int(steps.sum())
 
 
+def cpu_single_thread_bench_case(n=512, steps=100):
+    """The oracle on the reference's own bench case (main.rs:686-694) at n x n, f32, one
+    thread, a bounded number of steps: the figure to set beside the published
+    single-thread 1.531e8 cell-steps/s (docs/data/bench.json, M4 Pro)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po  # the checker, timed here as the CPU baseline only
+    from inverse_terrain_engine_amd import scenario as sc
+    base = sc.bench_case_terrain(n)
+    p = sc.bench_case_params()
+    ot = po.Terrain(base.fine.dem, manning=0.035, dx=2.0)
+    t0 = time.perf_counter()
+    o = po.run(ot, po.SolverParams(p.duration_s, p.sync_interval_s, initial_depth=0.05, rain=p.rain), precision=32,
+               max_steps=steps)
+    return o["cell_updates"] / (time.perf_counter() - t0)
+
+
 def _run_cands(po, ot, params, edits, threads, monitors):
     import ctypes as C
     import numpy as np
@@ -351,8 +367,10 @@ def main():
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         v, cw, cs = cpu_baseline(w, threads, a.cpu_cands, a.cpu_steps)
+        st1 = cpu_single_thread_bench_case()
         out["cpu_baseline"] = {
             "value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "single_thread_bench_case_512": st1, "published_single_thread_m4pro": 1.531e8,
             "sample": f"{a.cpu_cands} candidates of the same workload x {a.cpu_steps} steps, one candidate per "
                       f"thread (C++ restatement of the reference f32 fused kernel), {cw:.1f} s wall, CPU {cpu_model()}",
         }

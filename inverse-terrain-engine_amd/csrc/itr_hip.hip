@@ -418,7 +418,7 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
 // differs but every face and cell value is the same IEEE expression of the old state).
 // Reads state copy `par`, writes copy par^1 and the row partials / monitors of parity
 // par^1; the band's CFL max and health flag go to the candidate's StepAcc.
-template <bool CREST>
+template <bool CREST, int DEPTH>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
                                           double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
@@ -558,15 +558,23 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     else fn = face_signed<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), K);
     if (a0 == 0 && owner) yb_out[col] = fn.m;
 
-    // Row inputs are loaded two rows ahead of their use (software pipelining deep enough
-    // to cover HBM latency at two resident workgroups per CU).
-    Cell pre_c = fetch(a0 + 1), pre_c2 = fetch(a0 + 2);
+    // Row inputs are loaded DEPTH rows ahead of their use (software pipelining; two rows
+    // cover HBM latency at two resident workgroups per CU, one row keeps the register
+    // budget of the crest variants).
+    Cell pre_c = fetch(a0 + 1), pre_c2{};
     float pre_dzs = dzy_at(rs_dzy, a0 + 1), pre_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 1) : 0.0f;
-    float nxt_dzs = dzy_at(rs_dzy, a0 + 2), nxt_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 2) : 0.0f;
     float pre_dzw = dzx_at(rs_dzx, a0), pre_dzw2 = CREST ? dzx_at(rs_dzx2, a0) : 0.0f;
-    float nxt_dzw = dzx_at(rs_dzx, min(a0 + 1, ny - 1)), nxt_dzw2 = CREST ? dzx_at(rs_dzx2, min(a0 + 1, ny - 1)) : 0.0f;
     float pre_wet = row_at(rs_wet, a0), pre_gn2 = row_at(rs_gn2, a0);
-    float nxt_wet = row_at(rs_wet, a0 + 1), nxt_gn2 = row_at(rs_gn2, a0 + 1);
+    float nxt_dzs = 0.0f, nxt_dzs2 = 0.0f, nxt_dzw = 0.0f, nxt_dzw2 = 0.0f, nxt_wet = 0.0f, nxt_gn2 = 0.0f;
+    if (DEPTH == 2) {
+        pre_c2 = fetch(a0 + 2);
+        nxt_dzs = dzy_at(rs_dzy, a0 + 2);
+        if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, a0 + 2);
+        nxt_dzw = dzx_at(rs_dzx, min(a0 + 1, ny - 1));
+        if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(a0 + 1, ny - 1));
+        nxt_wet = row_at(rs_wet, a0 + 1);
+        nxt_gn2 = row_at(rs_gn2, a0 + 1);
+    }
 
     int smax_bits = 0;
     bool bad = false;
@@ -575,20 +583,30 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
         const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
         const float wet = pre_wet, gn2 = pre_gn2;
-        pre_c = pre_c2;
-        pre_dzs = nxt_dzs;
-        pre_dzs2 = nxt_dzs2;
-        pre_dzw = nxt_dzw;
-        pre_dzw2 = nxt_dzw2;
-        pre_wet = nxt_wet;
-        pre_gn2 = nxt_gn2;
-        pre_c2 = fetch(j + 3);
-        nxt_dzs = dzy_at(rs_dzy, j + 3);
-        if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, j + 3);
-        nxt_dzw = dzx_at(rs_dzx, min(j + 2, ny - 1));
-        if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(j + 2, ny - 1));
-        nxt_wet = row_at(rs_wet, j + 2);
-        nxt_gn2 = row_at(rs_gn2, j + 2);
+        if (DEPTH == 2) {
+            pre_c = pre_c2;
+            pre_dzs = nxt_dzs;
+            pre_dzs2 = nxt_dzs2;
+            pre_dzw = nxt_dzw;
+            pre_dzw2 = nxt_dzw2;
+            pre_wet = nxt_wet;
+            pre_gn2 = nxt_gn2;
+            pre_c2 = fetch(j + 3);
+            nxt_dzs = dzy_at(rs_dzy, j + 3);
+            if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, j + 3);
+            nxt_dzw = dzx_at(rs_dzx, min(j + 2, ny - 1));
+            if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(j + 2, ny - 1));
+            nxt_wet = row_at(rs_wet, j + 2);
+            nxt_gn2 = row_at(rs_gn2, j + 2);
+        } else {
+            pre_c = fetch(j + 2);
+            pre_dzs = dzy_at(rs_dzy, j + 2);
+            if (CREST) pre_dzs2 = dzy_at(rs_dzy2, j + 2);
+            pre_dzw = dzx_at(rs_dzx, min(j + 1, ny - 1));
+            if (CREST) pre_dzw2 = dzx_at(rs_dzx2, min(j + 1, ny - 1));
+            pre_wet = row_at(rs_wet, j + 1);
+            pre_gn2 = row_at(rs_gn2, j + 1);
+        }
 
         const Vel vn = cell_velocity(nh, nqx, nqy, K);
         Flux fs;
@@ -905,10 +923,10 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const int par = uni(sS.parity);
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
         if (MODES == 3) {
-            if (G.crest) band_step<true>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
-            else band_step<false>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            if (G.crest) band_step<true, 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            else band_step<false, 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
         } else {
-            band_step<MODES == 2>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            band_step<MODES == 2, MODES == 1 ? 2 : 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
         }
     } else if (!designated) {
         return;

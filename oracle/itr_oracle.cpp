@@ -39,10 +39,12 @@ template <class T> inline T rmin(T a, T b) { return a < b ? a : b; }   // real.r
 template <class T> inline T rabs(T x) { return x < T(0) ? -x : x; }    // real.rs:39-41
 template <class T> inline T canon(T x) { return x == T(0) ? T(0) : x; } // real.rs:56-58
 
-inline float rcbrt(float x) {  // real.rs:116-129
+__attribute__((always_inline)) inline float rcbrt(float x) {  // real.rs:116-129
     uint32_t bits;
     std::memcpy(&bits, &x, 4);
-    uint32_t s = 0x54A23280u - bits / 3u;
+    // bits / 3 as a multiply-high (exact for every 32-bit value: 0xAAAAAAAB = ceil(2^33 / 3)),
+    // which vectorises where a vector integer divide does not exist.
+    uint32_t s = 0x54A23280u - (uint32_t)(((uint64_t)bits * 0xAAAAAAABull) >> 33);
     float r;
     std::memcpy(&r, &s, 4);
     const float x3 = x * (float)(1.0 / 3.0);
@@ -134,7 +136,7 @@ template <class T> struct StepConsts {
 template <class T> struct Vel { T u, v, c; };
 template <class T> struct Face { T m, n_l, n_r, t; };
 
-template <class T> inline Vel<T> cell_vel(T h, T qx, T qy, const StepConsts<T>& k) {  // numerics.rs:36-43
+template <class T> __attribute__((always_inline)) inline Vel<T> cell_vel(T h, T qx, T qy, const StepConsts<T>& k) {  // numerics.rs:36-43
     T h2 = h * h;
     T inv = T(1) / (h2 + rmax(h2, k.heps2));
     T t = (h + h) * inv;
@@ -142,39 +144,41 @@ template <class T> inline Vel<T> cell_vel(T h, T qx, T qy, const StepConsts<T>& 
 }
 
 template <class T>
-inline Face<T> hll(T hl, T ul, T tl, T cl, T hr, T ur, T tr, T cr, const StepConsts<T>& k) {  // numerics.rs:58-93
+__attribute__((always_inline)) inline Face<T> hll(T hl, T ul, T tl, T cl, T hr, T ur, T tr, T cr, const StepConsts<T>& k) {  // numerics.rs:58-93
     const T z = T(0);
     const T half = T(0.5), quarter = T(0.25), two = T(2.0);
-    bool dry_l = !(hl > z);
-    bool dry_r = !(hr > z);
+    // The selects are written as nested selects on the ordered comparisons hl > 0 and
+    // hr > 0 (dry_x = !(h_x > 0): same values, NaN depths count as dry) so that they
+    // vectorise.
     T ustar = half * (ul + ur) + cl - cr;
     T cstar = half * (cl + cr) + quarter * (ul - ur);
     T slw = rmin(ul - cl, ustar - cstar);
     T srw = rmax(ur + cr, ustar + cstar);
-    T sl = dry_l ? ur - two * cr : (dry_r ? ul - cl : slw);
-    T sr = dry_l ? ur + cr : (dry_r ? ul + two * cl : srw);
+    T sl = hl > z ? (hr > z ? slw : ul - cl) : ur - two * cr;
+    T sr = hl > z ? (hr > z ? srw : ul + two * cl) : ur + cr;
     T ml = hl * ul;
     T mr = hr * ur;
     T pl_ = k.g_half * hl * hl;
     T pr_ = k.g_half * hr * hr;
     T pl = ml * ul + pl_;
     T pr = mr * ur + pr_;
-    bool both_dry = dry_l && dry_r;
-    bool use_l = sl >= z;
-    bool use_r = sr <= z;
-    T den = (both_dry || use_l || use_r) ? T(1) : sr - sl;
+    const T any_wet = hl > z ? T(1) : (hr > z ? T(1) : z);   // 0 when both sides are dry
+    const T span = any_wet > z ? sr - sl : T(1);
+    T den = sl >= z ? T(1) : (sr <= z ? T(1) : span);
     T inv = T(1) / den;
     T slsr = sl * sr;
     T fm_star = (sr * ml - sl * mr + slsr * (hr - hl)) * inv;
     T fn_star = (sr * pl - sl * pr + slsr * (mr - ml)) * inv;
-    T fm = both_dry ? z : (use_l ? ml : (use_r ? mr : fm_star));
-    T fn_ = both_dry ? z : (use_l ? pl : (use_r ? pr : fn_star));
+    const T fm_w = sl >= z ? ml : (sr <= z ? mr : fm_star);
+    const T fn_w = sl >= z ? pl : (sr <= z ? pr : fn_star);
+    T fm = any_wet > z ? fm_w : z;
+    T fn_ = any_wet > z ? fn_w : z;
     T ft = fm * (fm > z ? tl : tr);
     return Face<T>{fm, fn_ - pl_, fn_ - pr_, ft};
 }
 
 template <class T>
-inline Face<T> face_signed(T hl, Vel<T> vl, T hr, Vel<T> vr, T dz, bool normal_x, const StepConsts<T>& k) {  // numerics.rs:97-112
+__attribute__((always_inline)) inline Face<T> face_signed(T hl, Vel<T> vl, T hr, Vel<T> vr, T dz, bool normal_x, const StepConsts<T>& k) {  // numerics.rs:97-112
     const T z = T(0);
     bool pos = dz > z;
     T hsl = rmax(hl - rmax(dz, z), z);
@@ -188,7 +192,7 @@ inline Face<T> face_signed(T hl, Vel<T> vl, T hr, Vel<T> vr, T dz, bool normal_x
 }
 
 template <class T>
-inline Face<T> face_crest(T hl, Vel<T> vl, T hr, Vel<T> vr, T d_l, T d_r, bool normal_x, const StepConsts<T>& k) {  // numerics.rs:117-129
+__attribute__((always_inline)) inline Face<T> face_crest(T hl, Vel<T> vl, T hr, Vel<T> vr, T d_l, T d_r, bool normal_x, const StepConsts<T>& k) {  // numerics.rs:117-129
     const T z = T(0);
     T hsl = rmax(hl - d_l, z);
     T hsr = rmax(hr - d_r, z);
@@ -198,10 +202,10 @@ inline Face<T> face_crest(T hl, Vel<T> vl, T hr, Vel<T> vr, T d_l, T d_r, bool n
     return hll(hsl, vl.v, vl.u, csl, hsr, vr.v, vr.u, csr, k);
 }
 
-template <class T> struct CellOut { T h, qx, qy, s, infil, fix; bool bad; };
+template <class T> struct CellOut { T h, qx, qy, s, infil, fix; bool ok; };   // ok: !bad
 
 template <class T>
-inline void flux_update(T h, T qx, T qy, const Face<T>& w, const Face<T>& e, const Face<T>& n, const Face<T>& s,
+__attribute__((always_inline)) inline void flux_update(T h, T qx, T qy, const Face<T>& w, const Face<T>& e, const Face<T>& n, const Face<T>& s,
                         const StepConsts<T>& k, T& h1, T& qx1, T& qy1) {  // numerics.rs:148-153
     h1 = h - k.dtdx * (e.m - w.m) - k.dtdy * (s.m - n.m);
     qx1 = qx - k.dtdx * (e.n_l - w.n_r) - k.dtdy * (s.t - n.t);
@@ -211,11 +215,11 @@ inline void flux_update(T h, T qx, T qy, const Face<T>& w, const Face<T>& e, con
 template <class T> inline T rcbrt_t(T x) { return rcbrt(x); }
 
 template <class T>
-inline CellOut<T> sources(T h1, T qx1, T qy1, T wet, T gn2, const StepConsts<T>& k) {  // numerics.rs:158-195
+__attribute__((always_inline)) inline CellOut<T> sources(T h1, T qx1, T qy1, T wet, T gn2, const StepConsts<T>& k) {  // numerics.rs:158-195
     const T z = T(0);
-    bool bad = !(h1 >= -k.h_dry);
+    bool ok = h1 >= -k.h_dry;   // bad = !ok (ordered comparisons vectorise)
     T fix = h1 < z ? -h1 : z;
-    if (h1 < z) h1 = z;
+    h1 = h1 < z ? z : h1;
     T h2 = h1 + k.rain * wet;
     T infil = rmin(k.cap * wet, h2);
     T h3 = h2 - infil;
@@ -228,12 +232,12 @@ inline CellOut<T> sources(T h1, T qx1, T qy1, T wet, T gn2, const StepConsts<T>&
     T keep = dry ? z : T(1);
     T qx2 = qx1 * inv * keep;
     T qy2 = qy1 * inv * keep;
-    if (rabs(qx2) < k.q_min) qx2 = z;
-    if (rabs(qy2) < k.q_min) qy2 = z;
+    qx2 = rabs(qx2) < k.q_min ? z : qx2;
+    qy2 = rabs(qy2) < k.q_min ? z : qy2;
     T c = std::sqrt(k.g * h3);
     T s_rate = (rabs(qx2) * r3 + c) * k.inv_dx + (rabs(qy2) * r3 + c) * k.inv_dy;
-    bad = bad || !(qm == qm);
-    return CellOut<T>{canon(h3), canon(qx2), canon(qy2), s_rate, infil, fix, bad};
+    ok = ok & (qm == qm);
+    return CellOut<T>{canon(h3), canon(qx2), canon(qy2), s_rate, infil, fix, ok};
 }
 
 // ---------------------------------------------------------------- terrain.rs
@@ -603,33 +607,59 @@ struct Monitors {  // kernel.rs:86-113
     }
 };
 
+// The row loops take raw __restrict__ pointers and compile-time face modes so that the
+// compiler vectorises them (IEEE single operations lane by lane: the same results as the
+// scalar loop, since nothing is contracted or reassociated).
 template <class T> void vel_row(const T* h, const T* qx, const T* qy, VelRow<T>& out, size_t n, const StepConsts<T>& k) {  // kernel.rs:115-126
+    const StepConsts<T> K = k;
+    const T* __restrict__ H = h;
+    const T* __restrict__ QX = qx;
+    const T* __restrict__ QY = qy;
+    T* __restrict__ U = out.u.data();
+    T* __restrict__ V = out.v.data();
+    T* __restrict__ Cc = out.c.data();
+#pragma GCC ivdep
     for (size_t i = 0; i < n; i++) {
-        Vel<T> o = cell_vel(h[i], qx[i], qy[i], k);
-        out.u[i] = o.u; out.v[i] = o.v; out.c[i] = o.c;
+        Vel<T> o = cell_vel(H[i], QX[i], QY[i], K);
+        U[i] = o.u; V[i] = o.v; Cc[i] = o.c;
+    }
+}
+
+template <class T, bool CREST, bool NX>
+void face_loop(const T* __restrict__ hl, const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ cl,
+               const T* __restrict__ hr, const T* __restrict__ ur, const T* __restrict__ vr, const T* __restrict__ cr,
+               const T* __restrict__ f0, const T* __restrict__ f1, int i0, int i1, T* __restrict__ om,
+               T* __restrict__ onl, T* __restrict__ onr, T* __restrict__ ot, const StepConsts<T> K) {
+#pragma GCC ivdep
+    for (int i = i0; i <= i1; i++) {
+        Vel<T> a{ul[i], vl[i], cl[i]}, b{ur[i], vr[i], cr[i]};
+        Face<T> f = CREST ? face_crest(hl[i], a, hr[i], b, f0[i], f1[i], NX, K)
+                          : face_signed(hl[i], a, hr[i], b, f0[i], NX, K);
+        om[i] = f.m; onl[i] = f.n_l; onr[i] = f.n_r; ot[i] = f.t;
     }
 }
 
 template <class T>
 void xface_row(bool crest, const T* h, const VelRow<T>& v, const T* f0, const T* f1, int nx, FaceRow<T>& out,
-               const StepConsts<T>& k) {  // kernel.rs:128-149
-    for (int i = 0; i <= nx; i++) {
-        Vel<T> a{v.u[i], v.v[i], v.c[i]}, b{v.u[i + 1], v.v[i + 1], v.c[i + 1]};
-        Face<T> f = crest ? face_crest(h[i], a, h[i + 1], b, f0[i], f1[i], true, k)
-                          : face_signed(h[i], a, h[i + 1], b, f0[i], true, k);
-        out.m[i] = f.m; out.nl[i] = f.n_l; out.nr[i] = f.n_r; out.t[i] = f.t;
-    }
+               const StepConsts<T>& k) {  // kernel.rs:128-149: face i joins padded cells i and i+1
+    auto* o = &out;
+    if (crest)
+        face_loop<T, true, true>(h, v.u.data(), v.v.data(), v.c.data(), h + 1, v.u.data() + 1, v.v.data() + 1,
+                                 v.c.data() + 1, f0, f1, 0, nx, o->m.data(), o->nl.data(), o->nr.data(), o->t.data(), k);
+    else
+        face_loop<T, false, true>(h, v.u.data(), v.v.data(), v.c.data(), h + 1, v.u.data() + 1, v.v.data() + 1,
+                                  v.c.data() + 1, f0, f1, 0, nx, o->m.data(), o->nl.data(), o->nr.data(), o->t.data(), k);
 }
 
 template <class T>
 void yface_row(bool crest, const T* ht, const VelRow<T>& vt, const T* hb, const VelRow<T>& vb, const T* f0,
                const T* f1, int nx, FaceRow<T>& out, const StepConsts<T>& k) {  // kernel.rs:151-179
-    for (int i = 1; i <= nx; i++) {
-        Vel<T> a{vt.u[i], vt.v[i], vt.c[i]}, b{vb.u[i], vb.v[i], vb.c[i]};
-        Face<T> f = crest ? face_crest(ht[i], a, hb[i], b, f0[i], f1[i], false, k)
-                          : face_signed(ht[i], a, hb[i], b, f0[i], false, k);
-        out.m[i] = f.m; out.nl[i] = f.n_l; out.nr[i] = f.n_r; out.t[i] = f.t;
-    }
+    if (crest)
+        face_loop<T, true, false>(ht, vt.u.data(), vt.v.data(), vt.c.data(), hb, vb.u.data(), vb.v.data(), vb.c.data(),
+                                  f0, f1, 1, nx, out.m.data(), out.nl.data(), out.nr.data(), out.t.data(), k);
+    else
+        face_loop<T, false, false>(ht, vt.u.data(), vt.v.data(), vt.c.data(), hb, vb.u.data(), vb.v.data(), vb.c.data(),
+                                   f0, f1, 1, nx, out.m.data(), out.nl.data(), out.nr.data(), out.t.data(), k);
 }
 
 template <class T> struct RowScratch {
@@ -642,22 +672,48 @@ template <class T>
 RowAcc update_row(T* h, T* qx, T* qy, const FaceRow<T>& fx, const FaceRow<T>& gn, const FaceRow<T>& gs, const T* wet,
                   const T* gn2, int nx, const StepConsts<T>& k, RowScratch<T>& rs, bool want_sum_s, double& sum_s) {
     const T z = T(0);
-    for (int c = 1; c <= nx; c++) {   // loop 1 (kernel.rs:214-223)
-        Face<T> fw{fx.m[c - 1], z, fx.nr[c - 1], fx.t[c - 1]};
-        Face<T> fe{fx.m[c], fx.nl[c], z, fx.t[c]};
-        Face<T> fnn{gn.m[c], z, gn.nr[c], gn.t[c]};
-        Face<T> fs{gs.m[c], gs.nl[c], z, gs.t[c]};
-        T a, b, cc;
-        flux_update(h[c], qx[c], qy[c], fw, fe, fnn, fs, k, a, b, cc);
-        h[c] = a; qx[c] = b; qy[c] = cc;
+    const StepConsts<T> K = k;
+    T* __restrict__ H = h;
+    T* __restrict__ QX = qx;
+    T* __restrict__ QY = qy;
+    {
+        const T* __restrict__ xm = fx.m.data();
+        const T* __restrict__ xnl = fx.nl.data();
+        const T* __restrict__ xnr = fx.nr.data();
+        const T* __restrict__ xt = fx.t.data();
+        const T* __restrict__ nm = gn.m.data();
+        const T* __restrict__ nnr = gn.nr.data();
+        const T* __restrict__ nt = gn.t.data();
+        const T* __restrict__ sm = gs.m.data();
+        const T* __restrict__ snl = gs.nl.data();
+        const T* __restrict__ st = gs.t.data();
+#pragma GCC ivdep
+        for (int c = 1; c <= nx; c++) {   // loop 1 (kernel.rs:214-223)
+            Face<T> fw{xm[c - 1], z, xnr[c - 1], xt[c - 1]};
+            Face<T> fe{xm[c], xnl[c], z, xt[c]};
+            Face<T> fnn{nm[c], z, nnr[c], nt[c]};
+            Face<T> fs{sm[c], snl[c], z, st[c]};
+            T a, b, cc;
+            flux_update(H[c], QX[c], QY[c], fw, fe, fnn, fs, K, a, b, cc);
+            H[c] = a; QX[c] = b; QY[c] = cc;
+        }
     }
-    bool bad = false;
-    for (int c = 1; c <= nx; c++) {   // loop 2 (kernel.rs:225-234)
-        CellOut<T> o = sources(h[c], qx[c], qy[c], wet[c], gn2[c], k);
-        h[c] = o.h; qx[c] = o.qx; qy[c] = o.qy;
-        rs.infil[c] = o.infil; rs.fix[c] = o.fix; rs.s[c] = o.s;
-        bad = bad || o.bad;
+    int ok_all = 1;
+    {
+        const T* __restrict__ WET = wet;
+        const T* __restrict__ GN2 = gn2;
+        T* __restrict__ RI = rs.infil.data();
+        T* __restrict__ RF = rs.fix.data();
+        T* __restrict__ RS = rs.s.data();
+#pragma GCC ivdep
+        for (int c = 1; c <= nx; c++) {   // loop 2 (kernel.rs:225-234)
+            CellOut<T> o = sources(H[c], QX[c], QY[c], WET[c], GN2[c], K);
+            H[c] = o.h; QX[c] = o.qx; QY[c] = o.qy;
+            RI[c] = o.infil; RF[c] = o.fix; RS[c] = o.s;
+            ok_all &= o.ok ? 1 : 0;
+        }
     }
+    const bool bad = ok_all == 0;
     size_t n = (size_t)nx + 2;
     FixedSum<T> si, sf;
     si.add_slice(rs.infil.data(), n);

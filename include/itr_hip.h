@@ -52,7 +52,7 @@ typedef struct {
     double initial_depth;
     double initial_stage; /* used when has_initial_stage */
     double cfl_hard;
-    double inflow_scale; /* only 1.0 is supported (else ITR_DECLINED, backend.rs:53-55) */
+    double inflow_scale; /* ensemble scaling of every inflow hydrograph (solver.rs:47-48, boundary.rs:26) */
     const double* rain_t;
     const double* rain_r;
     const double* locked_dt;
@@ -75,7 +75,7 @@ typedef struct {
     const float* wet;    /* [ny*nx] 1 = non-wall cell, 0 = wall (row-major, row 0 north) */
     const float* gn2;    /* [ny*nx] g*n^2 rounded to f32 */
     const int32_t* ghost; /* [2*nx + 2*ny] north[nx], south[nx], west[ny], east[ny];
-                             0 wall, 1 transmissive, 2 stage, 3 inflow (2, 3: declined) */
+                             0 wall, 1 transmissive, 2 stage, 3 inflow */
     int32_t n_cand;
     int32_t n_mon;
     const float* dzx;    /* [n_cand][ny][nx+1] fx[0] per face (signed jump, or crest d_l) */
@@ -91,6 +91,22 @@ typedef struct {
     int32_t n_guard;
     int32_t reserved;
     double guard_tol;
+    /* Stage and Inflow boundary cells (Ghost::Stage / Ghost::Inflow as built by
+     * Boundaries::from_segments, crates/itr-hydro/src/terrain.rs:122-187, and filled as
+     * boundary.rs:43-47, 66-70).  May be NULL when no ghost code is 2 or 3.
+     * ghost_value: [2*nx + 2*ny] in the order of `ghost`: stage level minus z_ref [m] for
+     *              code 2 (the ghost depth uses z_level, which must then be given), or
+     *              1 / segment length [1/m] for code 3.
+     * ghost_hydro: [2*nx + 2*ny] hydrograph index of each code-3 cell.
+     * hydro_start: [n_hydro + 1] offsets into hydro_t / hydro_q, the piecewise-linear
+     *              hydrographs (t [s], Q [m^3/s]; Hyetograph rules, scenario.rs:540-563). */
+    const double* ghost_value;
+    const int32_t* ghost_hydro;
+    const int32_t* hydro_start;
+    const double* hydro_t;
+    const double* hydro_q;
+    int32_t n_hydro;
+    int32_t reserved2;
 } itr_grid;
 
 /* Per-candidate result: GpuRun (crates/itr-gpu/src/lib.rs:49-60) + BatchRun

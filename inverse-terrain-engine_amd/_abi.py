@@ -35,7 +35,9 @@ class GridC(C.Structure):
                 ("z_ref", C.c_double), ("crest", C.c_int32), ("wet_count", C.c_int32), ("wet", _fp), ("gn2", _fp),
                 ("ghost", _ip), ("n_cand", C.c_int32), ("n_mon", C.c_int32), ("dzx", _fp), ("dzy", _fp),
                 ("dzx_r", _fp), ("dzy_r", _fp), ("z_level", _dp), ("mon_cells", _up), ("baseline_max", _dp),
-                ("guard_pos", _ip), ("n_guard", C.c_int32), ("reserved", C.c_int32), ("guard_tol", C.c_double)]
+                ("guard_pos", _ip), ("n_guard", C.c_int32), ("reserved", C.c_int32), ("guard_tol", C.c_double),
+                ("ghost_value", _dp), ("ghost_hydro", _ip), ("hydro_start", _ip), ("hydro_t", _dp), ("hydro_q", _dp),
+                ("n_hydro", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class RunOutC(C.Structure):

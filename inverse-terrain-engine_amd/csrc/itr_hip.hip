@@ -64,7 +64,7 @@ struct RowAcc {
 };
 
 struct DevParams {
-    double duration, sync, cfl, dt_max, cfl_hard, infil_cap;
+    double duration, sync, cfl, dt_max, cfl_hard, infil_cap, inflow_scale;
     const double* rain_t;
     const double* rain_r;
     const double* locked;
@@ -94,6 +94,14 @@ struct DevGrid {
     const int32_t* mon_map;
     const uint16_t* band_mon;  // [ny] per band: bit w set if wave w updates a monitored cell
     const int32_t* ghost;
+    // Stage / Inflow ghosts (codes 2 / 3); special = 0 when the grid has none.
+    int32_t special, n_hydro;
+    const float* gstage;       // [nslot][2nx + 2ny] f32 stage depth of code-2 cells per candidate
+    const double* ginv;        // [2nx + 2ny] 1 / segment length of code-3 cells
+    const int32_t* ghydro;     // [2nx + 2ny] hydrograph of code-3 cells
+    const int32_t* hstart;     // [n_hydro + 1] offsets into hyd_t / hyd_q
+    const double* hyd_t;
+    const double* hyd_q;
     float* mon_max;            // [2][nslot][n_mon]   by state parity
     double* mon_tmax;          // [2][nslot][n_mon]
     RowAcc* rowacc;            // [2][nslot][ny]      by state parity
@@ -154,9 +162,12 @@ struct KArgs {
 // Dynamic LDS of the step kernel, in floats (offsets 16-byte aligned).
 // Positivity-fix values are zero almost everywhere: a wave that has one in a row writes
 // its lanes' values to the grid's fix buffer and sets its bit in the row's mask, so only
-// infiltration is staged per cell.
-struct LdsLayout { size_t infil, fixmask, bw, be, wet, red, acc, total; };
-__host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
+// infiltration is staged per cell.  Grids with Stage / Inflow ghosts add the band's
+// special ghost values: (hv, qv) pairs for the west / east ghosts of rows a0-1 .. b0 and
+// the north / south ghost rows.
+struct LdsLayout { size_t infil, fixmask, bw, be, wet, red, acc, gsp, total; };
+__host__ __device__ inline size_t gsp_pairs(int R, int nx) { return (size_t)2 * (R + 2) + (size_t)2 * nx; }
+__host__ __device__ inline LdsLayout lds_layout(int R, int nx, bool special) {
     auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
     LdsLayout L;
     L.infil = 0;
@@ -167,7 +178,8 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nx) {
     L.red = al(L.wet + R);
     L.acc = al(L.red + 2 * kMaxWaves);
     const size_t acc_n = (size_t)R * 64;
-    L.total = al(L.acc + acc_n);
+    L.gsp = al(L.acc + acc_n);
+    L.total = al(L.gsp + (special ? 2 * gsp_pairs(R, nx) : 0));
     // the control epilogue stages 2*nx floats and rows of 8 words in the same space
     if (L.total < (size_t)2 * nx) L.total = al((size_t)2 * nx);
     if (L.total < 64) L.total = 64;
@@ -291,6 +303,55 @@ __device__ __forceinline__ double hy_integral(const double* rt, const double* rr
     return total;
 }
 
+// Real::rcbrt for f64 (crates/itr-core/src/real.rs:79-94): exponent-bias seed minus
+// bits / 3, then 4 steps r <- r*(4/3) - (x/3)*r^4, no fused operations.
+__device__ __forceinline__ double rcbrt64(double x) {
+    const uint64_t K = 0x5540000000000000ull - (0x8782Bull << 29);
+    const uint64_t bits = (uint64_t)__double_as_longlong(x);
+    double r = __longlong_as_double((long long)(K - bits / 3u));
+    const double third = 1.0 / 3.0;
+    const double x3 = x * third;
+    const double c43 = 4.0 * third;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double r2 = r * r;
+        r = r * c43 - x3 * (r2 * r2);
+    }
+    return r;
+}
+
+// Stage / Inflow ghost of boundary cell k (index into the grid's ghost codes) of
+// candidate `local` at step start t (boundary.rs:43-47, 66-70).  Stage: hv is the ghost
+// depth f32(max(stage - z, 0)) (prepared on the host), the discharge copies the
+// interior.  Inflow: q = rate(t) / length * inflow_scale per unit width, hv is the
+// critical depth (q^2/g)^(1/3) = x * rcbrt(x) * rcbrt(x) rounded to f32 (the ghost depth
+// is max(h_interior, critical)), qv = f32(inward * q) is the normal discharge and the
+// tangential one is 0.  Both kinds make the ghost wet exactly when hv > 0 (or the
+// interior is wet).
+struct GhostSp { float hv, qv; };
+__device__ __forceinline__ GhostSp special_ghost(const DevGrid& G, double inflow_scale, int code, int k, int local,
+                                                 double t, bool inward_pos) {
+    GhostSp s;
+    s.hv = 0.0f;
+    s.qv = 0.0f;
+    if (code == 2) {
+        s.hv = G.gstage[(size_t)local * (2 * G.nx + 2 * G.ny) + k];
+    } else if (code == 3) {
+        const int hi = G.ghydro[k];
+        const int a = G.hstart[hi], n = G.hstart[hi + 1] - a;
+        const double q = hy_rate(G.hyd_t + a, G.hyd_q + a, n, t) * G.ginv[k] * inflow_scale;
+        const double x = q * q / G.g;
+        const double crit = x > 0.0 ? x * rcbrt64(x) * rcbrt64(x) : 0.0;
+        s.hv = (float)crit;
+        s.qv = (float)((inward_pos ? 1.0 : -1.0) * q);
+    }
+    return s;
+}
+// f32(max(f64(h), crit)) with f64::max's NaN rule (a NaN operand yields the other one),
+// from hc = f32(crit): rounding is monotone and h is an f32, so the f64 comparison and
+// the f32 one agree.  Interior depths are never -0.
+__device__ __forceinline__ float inflow_depth(float h, float hc) { return (h != h || hc > h) ? hc : h; }
+
 // Head of one loop iteration of Solver::run_impl (solver.rs:449-489 and 269-285): the
 // step's dt (CFL, dt_max, locked schedule, sync clipping), exact rain depth, new time
 // and the f32 constants.  Returns false if the locked-dt check failed.
@@ -320,17 +381,62 @@ __device__ __forceinline__ bool prepare_step(const DevParams& p, const DevGrid& 
 
 // Dry-row accounting of the coming step (kernel.rs:605-612): rows j (padded 1..ny)
 // with rows j-1, j, j+1 exactly dry and no rain.  Ghost rows mirror their neighbour for
-// wall and transmissive boundaries.  Computed by one wave.
+// wall and transmissive boundaries; a wet Stage / Inflow ghost makes its row (west /
+// east) or the ghost row (north / south) wet (boundary.rs:50-52, 73-75).  Computed by one
+// wave at the step's start time C->t.
+__device__ __forceinline__ bool we_ghost_wet(const DevParams& p, const DevGrid& G, int local, double t, int r) {
+    const int nx = G.nx, ny = G.ny;
+    bool w = false;
+    for (int side = 0; side < 2; side++) {
+        const int k = 2 * nx + side * ny + r;
+        const int code = G.ghost[k];
+        if (code >= 2) w = w || special_ghost(G, p.inflow_scale, code, k, local, t, side == 0).hv > 0.0f;
+    }
+    return w;
+}
+template <bool SPECIAL>
 __device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const DevGrid& G, const CandCtl* C, const RowAcc* ra,
                                     int lane) {
     if (!p.skip_dry || C->rain32 > 0.0f) return 0;
     const int ny = G.ny;
     uint32_t n = 0;
-    for (int r = lane; r < ny; r += kWave) {
-        const bool d0 = r > 0 ? !ra[r - 1].wet : !ra[0].wet;
-        const bool d1 = !ra[r].wet;
-        const bool d2 = r + 1 < ny ? !ra[r + 1].wet : !ra[ny - 1].wet;
-        n += (d0 && d1 && d2) ? 1u : 0u;
+    if (!SPECIAL || !G.special) {
+        for (int r = lane; r < ny; r += kWave) {
+            const bool d0 = r > 0 ? !ra[r - 1].wet : !ra[0].wet;
+            const bool d1 = !ra[r].wet;
+            const bool d2 = r + 1 < ny ? !ra[r + 1].wet : !ra[ny - 1].wet;
+            n += (d0 && d1 && d2) ? 1u : 0u;
+        }
+    } else {
+        // Each lane walks a run of consecutive rows, so every row's ghosts are evaluated
+        // about once.  The north / south ghost rows count as dry unless a special ghost
+        // cell of theirs is wet (a wall / transmissive ghost is wet only with its row).
+        const int nx = G.nx, local = C->local;
+        const double t = C->t;
+        bool top = false, bot = false;
+        for (int c = lane; c < nx; c += kWave) {
+            const int cn = G.ghost[c], cs = G.ghost[nx + c];
+            if (cn >= 2) top = top || special_ghost(G, p.inflow_scale, cn, c, local, t, true).hv > 0.0f;
+            if (cs >= 2) bot = bot || special_ghost(G, p.inflow_scale, cs, nx + c, local, t, false).hv > 0.0f;
+        }
+        top = __any(top);
+        bot = __any(bot);
+        const int per = (ny + kWave - 1) / kWave;
+        const int r0 = lane * per, r1 = min(r0 + per, ny);
+        if (r0 < r1) {
+            auto dry = [&](int r) -> bool {
+                if (r < 0) return !top;
+                if (r >= ny) return !bot;
+                return !ra[r].wet && !we_ghost_wet(p, G, local, t, r);
+            };
+            bool d0 = dry(r0 - 1), d1 = dry(r0);
+            for (int r = r0; r < r1; r++) {
+                const bool d2 = dry(r + 1);
+                n += (d0 && d1 && d2) ? 1u : 0u;
+                d0 = d1;
+                d1 = d2;
+            }
+        }
     }
     for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o, 64);
     return n;
@@ -418,9 +524,9 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
 // differs but every face and cell value is the same IEEE expression of the old state).
 // Reads state copy `par`, writes copy par^1 and the row partials / monitors of parity
 // par^1; the band's CFL max and health flag go to the candidate's StepAcc.
-template <bool CREST, int DEPTH>
+template <bool CREST, int DEPTH, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
-                                          double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
+                                          double t, double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -474,12 +580,42 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
     const bool skip_dry = a.prm->skip_dry != 0;
+    const bool special = SPECIAL && G.special != 0;
+    const LdsLayout LL = lds_layout(R, nx, G.special != 0);
+    float2* l_gsp = (float2*)(lds + LL.gsp);
+
+    // Stage / Inflow ghosts of the band at the step start: west / east of rows a0-1 .. b0
+    // (pairs 0 .. 2R+3), then the north / south ghost rows when the band touches them.
+    bool forced_wet = false;
+    if (special) {
+        const int nwe = 2 * (R + 2);
+        bool fw = false;
+        for (int i = tid; i < nwe + 2 * nx; i += nthr) {
+            int k = -1;
+            bool inward = true;
+            if (i < nwe) {
+                const int side = i / (R + 2), row = a0 - 1 + i % (R + 2);
+                if (row >= 0 && row < ny) { k = 2 * nx + side * ny + row; inward = side == 0; }
+            } else {
+                const int side = (i - nwe) / nx, c = (i - nwe) % nx;
+                if (side == 0 ? a0 == 0 : b0 == ny) { k = side * nx + c; inward = side == 0; }
+            }
+            GhostSp gs;
+            gs.hv = 0.0f;
+            gs.qv = 0.0f;
+            if (k >= 0) gs = special_ghost(G, a.prm->inflow_scale, GHOST[k], k, local, t, inward);
+            l_gsp[i] = make_float2(gs.hv, gs.qv);
+            fw = fw || gs.hv > 0.0f;
+        }
+        forced_wet = __syncthreads_or(fw ? 1 : 0) != 0;
+    }
 
     // Dry band (kernel.rs:605-612 rule for every row of the band: rows a0-1 .. b0 exactly
-    // dry, ghost rows mirroring their neighbour, and no rain).  Every face of such a row
-    // is dry-dry (exact zero flux) and every cell stays (0, 0, 0), so the band's new
-    // state, row partials and boundary masses are zeros and its monitors carry over.
-    if (skip_dry && !(K.rain > 0.0f)) {
+    // dry, ghost rows mirroring their neighbour or holding no wet special ghost, and no
+    // rain).  Every face of such a row is dry-dry (exact zero flux) and every cell stays
+    // (0, 0, 0), so the band's new state, row partials and boundary masses are zeros and
+    // its monitors carry over.
+    if (skip_dry && !(K.rain > 0.0f) && !forced_wet) {
         bool dry = true;
         for (int r = max(a0 - 1, 0); r <= min(b0, ny - 1); r++) dry = dry && ra_in[r].wet == 0u;
         if (dry) {
@@ -513,7 +649,6 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
 
 
     // LDS: per-row staging of infil (R x nx) + row scalars (lds_layout).
-    const LdsLayout LL = lds_layout(R, nx);
     float* l_infil = lds + LL.infil;
     uint32_t* l_fixmask = (uint32_t*)(lds + LL.fixmask);
     float* l_bw = lds + LL.bw;
@@ -536,8 +671,41 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         v.code_we = edge_wave ? GHOST[2 * nx + (col < 0 ? 0 : ny) + rr] : 1;
         return v;
     };
+    // Stage / Inflow ghosts over the wall / transmissive transform (boundary.rs:43-47, 66-70).
+    auto ghost_special = [&](Cell w, const Cell& v, int row) -> Cell {
+        const bool rin = row >= 0 && row < ny;
+        const bool cin = col >= 0 && col < nx;
+        if (rin && !cin) {
+            if (v.code_we >= 2) {
+                const float2 g = l_gsp[(col < 0 ? 0 : R + 2) + (row - (a0 - 1))];
+                if (v.code_we == 2) {
+                    w.h = g.x;
+                } else {
+                    w.h = inflow_depth(v.h, g.x);
+                    w.qx = g.y;
+                    w.qy = 0.0f;
+                }
+            }
+        } else if (!rin && cin) {
+            const int code = row < 0 ? code_n : code_s;
+            if (code >= 2) {
+                const float2 g = l_gsp[2 * (R + 2) + (row < 0 ? 0 : nx) + col];
+                if (code == 2) {
+                    w.h = g.x;
+                } else {
+                    w.h = inflow_depth(v.h, g.x);
+                    w.qx = 0.0f;
+                    w.qy = g.y;
+                }
+            }
+        }
+        return w;
+    };
     auto fix = [&](const Cell& v, int row) -> Cell {
-        if (edge_wave || row < 0 || row >= ny) return ghosted(v, nx, ny, row, col, code_n, code_s);
+        if (edge_wave || row < 0 || row >= ny) {
+            const Cell w = ghosted(v, nx, ny, row, col, code_n, code_s);
+            return special ? ghost_special(w, v, row) : w;
+        }
         return v;
     };
     // Face jumps are stored with zeros on boundary faces (row 0 and row ny of dzy, columns
@@ -851,9 +1019,10 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
 // writes the new control block.  The control block is worked on in LDS (it is 60 words;
 // held in registers it would set the kernel's register budget).  lk = launch index & 3.
 // MODES: face modes present in the batch (1 signed, 2 crest, 3 both); a batch of one
-// mode gets a kernel without the other mode's code and its register budget.
+// mode gets a kernel without the other mode's code and its register budget.  SPECIAL:
+// the batch has Stage / Inflow ghosts (their code stays out of the other kernels).
 constexpr int kCtlWords = (int)(sizeof(CandCtl) / 4);
-template <int MODES>
+template <int MODES, bool SPECIAL>
 __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     extern __shared__ float lds[];
     __shared__ double shd[48];
@@ -884,7 +1053,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const bool go = complete_step(p, G, sS, A, tid);
         // dry-row count of the prepared step (rows of the state just completed)
         if (designated && completed && !(sS.flags & F_DONE))
-            sS.skipped_now = count_skippable(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
+            sS.skipped_now = count_skippable<SPECIAL>(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
         if (tid == 0) {
             sS.in_flight = go ? 1u : 0u;
             s_go = go ? 1 : 0;
@@ -919,21 +1088,23 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         K.rain = uni(sS.rain32);
         K.cap = uni(sS.cap32);
         const double t_new = uni(sS.t_new);
+        const double t_now = uni(sS.t);
         const bool sync = uni(sS.hits_sync) != 0;
         const int par = uni(sS.parity);
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
         if (MODES == 3) {
-            if (G.crest) band_step<true, 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
-            else band_step<false, 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            if (G.crest) band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+            else band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
         } else {
-            band_step<MODES == 2, MODES == 1 ? 2 : 1>(a, G, local, tk.z, K, t_new, sync, par, acc_cur, lds);
+            band_step<MODES == 2, MODES == 1 ? 2 : 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur,
+                                                              lds);
         }
     } else if (!designated) {
         return;
     }
     if (!designated) return;
     if (completed) {
-        const LdsLayout LL = lds_layout(G.R, G.nx);
+        const LdsLayout LL = lds_layout(G.R, G.nx, G.special != 0);
         ledger_epilogue(a, G, Sin, A, sS, lds, LL.total, shd);
     }
     __syncthreads();
@@ -950,9 +1121,10 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
 // The launch entry points.  The signed-face kernel (the common case) is held to 80 VGPRs
 // (6 waves per SIMD); the crest variants keep their natural budget.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
-itr_step_signed(KArgs a, uint32_t lk) { step_body<1>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3>(a, lk); }
+itr_step_signed(KArgs a, uint32_t lk) { step_body<1, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_special(KArgs a, uint32_t lk) { step_body<3, true>(a, lk); }
 
 // Head of the first step of every candidate: control copy 1 (read by launch 0).
 __global__ void itr_init_kernel(KArgs a) {
@@ -969,7 +1141,7 @@ __global__ void itr_init_kernel(KArgs a) {
         S.abort_kind = 4;
         S.flags |= F_DONE;
     } else {
-        S.skipped_now = count_skippable(*a.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
+        S.skipped_now = count_skippable<true>(*a.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
     }
     if (threadIdx.x == 0) {
         *C = S;
@@ -1318,7 +1490,7 @@ struct itr_batch {
     int ntasks = 0;
     int block = 64;
     size_t lds_bytes = 0;
-    int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both
+    int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts
     int max_ledger = 0;
     uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
     hipGraphExec_t graph = nullptr;
@@ -1354,7 +1526,9 @@ itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t er
     const int nblk = ((b->ntasks + 7) / 8) * 8;
     for (int s = 0; s < n; s++) {
         const uint32_t lk = (uint32_t)((lk0 + (uint64_t)s) & 3u);
-        if (b->modes == 1)
+        if (b->modes & 4)
+            hipLaunchKernelGGL(itr_step_special, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+        else if (b->modes == 1)
             hipLaunchKernelGGL(itr_step_signed, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
         else if (b->modes == 2)
             hipLaunchKernelGGL(itr_step_crest, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
@@ -1511,11 +1685,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         return ITR_EINVAL;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    // ---- validation and declines (lib.rs:155-171, backend.rs:53-55)
-    if (p->inflow_scale != 1.0) {
-        set_err(err, errlen, "inflow scaling is not supported on this backend");
-        return ITR_DECLINED;
-    }
+    // ---- validation and declines (lib.rs:155-171)
     if (!(p->sync_interval_s > 0.0) || !(p->duration_s >= 0.0) || p->n_rain < 0) {
         set_err(err, errlen, "invalid solver parameters");
         return ITR_EINVAL;
@@ -1534,14 +1704,35 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
             set_err(err, errlen, "grid %d: crest mode needs dzx_r and dzy_r", g);
             return ITR_EINVAL;
         }
-        for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++)
-            if (G.ghost[i] == 2 || G.ghost[i] == 3) {
-                set_err(err, errlen, "Stage and Inflow boundaries are not supported on this backend");
-                return ITR_DECLINED;
-            } else if (G.ghost[i] != 0 && G.ghost[i] != 1) {
-                set_err(err, errlen, "grid %d: unknown boundary code %d", g, G.ghost[i]);
+        for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++) {
+            const int code = G.ghost[i];
+            if (code < 0 || code > 3) {
+                set_err(err, errlen, "grid %d: unknown boundary code %d", g, code);
                 return ITR_EINVAL;
             }
+            if (code == 2 && (!G.ghost_value || !G.z_level)) {
+                set_err(err, errlen, "grid %d: stage boundary cells need ghost_value and z_level", g);
+                return ITR_EINVAL;
+            }
+            if (code == 3) {
+                if (!G.ghost_value || !G.ghost_hydro || !G.hydro_start || G.n_hydro <= 0 || G.ghost_hydro[i] < 0 ||
+                    G.ghost_hydro[i] >= G.n_hydro) {
+                    set_err(err, errlen, "grid %d: inflow boundary cell %d needs ghost_value and a hydrograph", g, i);
+                    return ITR_EINVAL;
+                }
+            }
+        }
+        if (G.n_hydro > 0) {
+            if (!G.hydro_start || G.hydro_start[0] != 0 || (G.hydro_start[G.n_hydro] > 0 && (!G.hydro_t || !G.hydro_q))) {
+                set_err(err, errlen, "grid %d: invalid hydrograph arrays", g);
+                return ITR_EINVAL;
+            }
+            for (int h = 0; h < G.n_hydro; h++)
+                if (G.hydro_start[h + 1] < G.hydro_start[h]) {
+                    set_err(err, errlen, "grid %d: hydrograph offsets must not decrease", g);
+                    return ITR_EINVAL;
+                }
+        }
         for (int q = 0; q < G.n_mon; q++) {
             if (G.mon_cells[q] >= (uint32_t)(G.nx * G.ny) || (q > 0 && G.mon_cells[q] <= G.mon_cells[q - 1])) {
                 set_err(err, errlen, "grid %d: monitor cells must be sorted, unique and inside the grid", g);
@@ -1577,6 +1768,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
     P.dt_max = p->dt_max_s;
     P.cfl_hard = p->cfl_hard;
     P.infil_cap = p->infil_capacity;
+    P.inflow_scale = p->inflow_scale;
     P.n_rain = p->n_rain;
     P.n_sync = n_sync;
     P.skip_dry = p->skip_dry;
@@ -1653,6 +1845,27 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         D.g_half32 = (float)(0.5 * I.g);
         D.inv_dx32 = (float)(1.0 / I.dx);
         D.inv_dy32 = (float)(1.0 / I.dy);
+        // Stage / Inflow ghosts: the stage depth of every code-2 cell per candidate
+        // (boundary.rs:43, 66: f32(max(stage - z_level(interior neighbour), 0)), with the
+        // candidate's own levels); inflow cells keep 1 / length and their hydrograph.
+        const int nb = 2 * nx + 2 * ny;
+        D.special = 0;
+        for (int i = 0; i < nb; i++) D.special |= I.ghost[i] >= 2 ? 1 : 0;
+        std::vector<float> gstage;
+        if (D.special) {
+            gstage.assign((size_t)std::max(nc, 1) * nb, 0.0f);
+            for (int k = 0; k < nc; k++)
+                for (int i = 0; i < nb; i++) {
+                    if (I.ghost[i] != 2) continue;
+                    int r, c;
+                    if (i < nx) { r = 0; c = i; }
+                    else if (i < 2 * nx) { r = ny - 1; c = i - nx; }
+                    else if (i < 2 * nx + ny) { r = i - 2 * nx; c = 0; }
+                    else { r = i - 2 * nx - ny; c = nx - 1; }
+                    const double zl = I.z_level[(size_t)k * nx * ny + (size_t)r * nx + c];
+                    gstage[(size_t)k * nb + i] = (float)std::fmax(I.ghost_value[i] - zl, 0.0);
+                }
+        }
         const size_t plane = D.plane;
         // host staging of the pitched arrays
         std::vector<float> st((size_t)std::max(nc, 1) * 6 * plane, 0.0f);
@@ -1768,6 +1981,19 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
         StepRec* d_steplog = nullptr;
         if (log_steps > 0) ALLOC(d_steplog, ncx * (size_t)log_steps);
+        float* d_gstage = nullptr;
+        double *d_ginv = nullptr, *d_hyd_t = nullptr, *d_hyd_q = nullptr;
+        int32_t *d_ghydro = nullptr, *d_hstart = nullptr;
+        const int n_hydro = D.special && I.n_hydro > 0 ? I.n_hydro : 0;
+        const size_t n_pts = n_hydro > 0 ? (size_t)I.hydro_start[n_hydro] : 0;
+        if (D.special) {
+            ALLOC(d_gstage, gstage.size());
+            ALLOC(d_ginv, (size_t)nb);
+            ALLOC(d_ghydro, (size_t)nb);
+            ALLOC(d_hstart, (size_t)n_hydro + 1);
+            ALLOC(d_hyd_t, std::max<size_t>(n_pts, 1));
+            ALLOC(d_hyd_q, std::max<size_t>(n_pts, 1));
+        }
         if (I.baseline_max) {
             ALLOC(d_base, (size_t)std::max(I.n_mon, 1));
             ALLOC(d_gpos, (size_t)std::max(I.n_guard, 1));
@@ -1787,6 +2013,19 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         if (I.baseline_max)
             ok = ok && up(d_base, I.baseline_max, sizeof(double) * I.n_mon) &&
                  up(d_gpos, I.guard_pos, sizeof(int32_t) * I.n_guard);
+        if (D.special) {
+            std::vector<double> ginv(nb, 0.0);
+            std::vector<int32_t> ghydro(nb, 0), hstart(n_hydro + 1, 0);
+            for (int i = 0; i < nb; i++)
+                if (I.ghost[i] == 3) {
+                    ginv[i] = I.ghost_value[i];
+                    ghydro[i] = I.ghost_hydro[i];
+                }
+            for (int h = 0; h <= n_hydro; h++) hstart[h] = I.hydro_start[h];
+            ok = ok && up(d_gstage, gstage.data(), gstage.size() * 4) && up(d_ginv, ginv.data(), nb * 8) &&
+                 up(d_ghydro, ghydro.data(), nb * 4) && up(d_hstart, hstart.data(), hstart.size() * 4) &&
+                 up(d_hyd_t, I.hydro_t, n_pts * 8) && up(d_hyd_q, I.hydro_q, n_pts * 8);
+        }
         ok = ok && hipMemset(d_monmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
              hipMemset(d_montmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
              hipMemset(d_ybnd, 0, 2 * ncx * 2 * nx * 4) == hipSuccess &&
@@ -1817,15 +2056,22 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         D.guard_pos = d_gpos;
         D.mon_cells = d_moncells;
         D.steplog = d_steplog;
+        D.n_hydro = n_hydro;
+        D.gstage = d_gstage;
+        D.ginv = d_ginv;
+        D.ghydro = d_ghydro;
+        D.hstart = d_hstart;
+        D.hyd_t = d_hyd_t;
+        D.hyd_q = d_hyd_q;
         D.max_log = log_steps > 0 ? (int32_t)std::min<int64_t>(log_steps, 1 << 30) : 0;
         dgrids.push_back(D);
-        const size_t lds = lds_layout(R, nx).total * sizeof(float);
+        const size_t lds = lds_layout(R, nx, D.special != 0).total * sizeof(float);
         max_lds = std::max(max_lds, lds);
     }
     b->ncand = total;
     b->ntasks = (int)tasks.size();
     b->modes = 0;
-    for (auto& D : dgrids) b->modes |= D.crest ? 2 : 1;
+    for (auto& D : dgrids) b->modes |= (D.crest ? 2 : 1) | (D.special ? 4 : 0);
     if (b->modes == 0) b->modes = 1;
     b->block = max_waves * kWave;
     b->lds_bytes = std::max(max_lds, (size_t)std::max<int64_t>(ctx->min_lds, 0));
@@ -2120,6 +2366,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "the adjoint supports signed-face levels only");
         return ITR_DECLINED;
     }
+    if (grid->ghost)
+        for (int i = 0; i < 2 * grid->nx + 2 * grid->ny; i++)
+            if (grid->ghost[i] >= 2) {
+                set_err(err, errlen, "the adjoint supports wall and transmissive boundaries only");
+                return ITR_DECLINED;
+            }
     if (grid->n_cand <= 0 || !(obj->tau_m > 0.0) || obj->n_assets < 0 || (obj->n_assets > 0 && !obj->asset_start)) {
         set_err(err, errlen, "invalid adjoint arguments");
         return ITR_EINVAL;
@@ -2405,7 +2657,7 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
     size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0);
     for (size_t g = 0; g < b->grids.size(); g++) {
         const DevGrid& D = b->grids[g].d;
-        lds = std::max(lds, lds_layout(std::max(1, std::min(Rs[g], D.ny)), D.nx).total * sizeof(float));
+        lds = std::max(lds, lds_layout(std::max(1, std::min(Rs[g], D.ny)), D.nx, D.special != 0).total * sizeof(float));
     }
     if (lds > 160 * 1024 - 1024) {
         set_err(err, errlen, "band of %zu bytes of LDS does not fit", lds);

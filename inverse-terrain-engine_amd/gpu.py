@@ -98,7 +98,22 @@ def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need
     f = lambda a: a.ctypes.data_as(_abi._fp)
     g.wet = f(keep.arr(base.wet, np.float32))
     g.gn2 = f(keep.arr(base.gn2, np.float32))
-    g.ghost = keep.arr(base.ghost_codes(), np.int32).ctypes.data_as(_abi._ip)
+    codes = base.ghost_codes()
+    g.ghost = keep.arr(codes, np.int32).ctypes.data_as(_abi._ip)
+    if (codes >= 2).any():
+        # Stage / Inflow cells (boundary.rs:43-47, 66-70): per-cell parameter, hydrograph
+        # index, and the hydrographs flattened with offsets.
+        g.ghost_value = keep.arr(base.ghost_params(), np.float64).ctypes.data_as(_abi._dp)
+        g.ghost_hydro = keep.arr(base.ghost_hydro(), np.int32).ctypes.data_as(_abi._ip)
+        hyd = [np.asarray(h, np.float64).reshape(-1, 2) for h in base.ghosts.get("hydrographs", [])]
+        if hyd:
+            start = np.concatenate([[0], np.cumsum([len(h) for h in hyd])]).astype(np.int32)
+            pts = np.concatenate(hyd) if start[-1] > 0 else np.zeros((1, 2))
+            g.hydro_start = keep.arr(start, np.int32).ctypes.data_as(_abi._ip)
+            g.hydro_t = keep.arr(pts[:, 0], np.float64).ctypes.data_as(_abi._dp)
+            g.hydro_q = keep.arr(pts[:, 1], np.float64).ctypes.data_as(_abi._dp)
+            g.n_hydro = len(hyd)
+        need_z = need_z or base.has_stage
     for t in terrains:
         if t.base is not base:
             raise Declined("all candidates must share the same TerrainBase")
@@ -371,8 +386,9 @@ class HipBatchBackend:
         return f"hip ({self.solver.adapter_info()})"
 
     def run(self, p: SolverParams, terrains, monitors):
-        if p.inflow_scale != 1.0:
-            raise Declined("inflow scaling is not supported on the GPU backend")
+        """The reference's wgpu backend declines inflow scaling (backend.rs:53-55) and Stage /
+        Inflow boundaries (lib.rs:160-171); this one runs them (ensemble members, boundary
+        hydrographs).  cell_updates = steps * cells as backend.rs:56,61 count them."""
         cells = terrains[0].base.nx * terrains[0].base.ny if terrains else 0
         return [BatchRun(r.monitor_max, r.steps, r.steps * cells, r.bad)
                 for r in self.solver.run_batch(p, terrains, monitors, want_state=False)]

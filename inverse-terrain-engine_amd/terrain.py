@@ -189,6 +189,7 @@ class TerrainBase:
     wet_count: int
     ghosts: dict          # side -> int32 codes
     g: float = G
+    segments: tuple = ()  # the BoundarySegments the ghosts were built from
 
     @property
     def z_ref(self):
@@ -205,10 +206,22 @@ class TerrainBase:
         gn2 = (G * nval * nval).astype(np.float32)
         ghosts = boundaries_from_segments(segments, nx, ny, fine.geo.dx * lv, fine.geo.dy * lv, fine.z_ref)
         return TerrainBase(fine, lv, nx, ny, fine.geo.dx * lv, fine.geo.dy * lv, crest, z, wall, nval, fx0, fx1,
-                           fy0, fy1, wet, gn2, int((~wall).sum()), ghosts)
+                           fy0, fy1, wet, gn2, int((~wall).sum()), ghosts, segments=tuple(segments))
 
     def ghost_codes(self) -> np.ndarray:
         return np.concatenate([self.ghosts[s]["code"] for s in SIDES]).astype(np.int32)
+
+    def ghost_params(self) -> np.ndarray:
+        """Per boundary cell, in ghost-code order: stage level minus z_ref (Stage) or
+        1 / segment length (Inflow)."""
+        return np.concatenate([self.ghosts[s]["param"] for s in SIDES]).astype(np.float64)
+
+    def ghost_hydro(self) -> np.ndarray:
+        return np.concatenate([self.ghosts[s]["hydro"] for s in SIDES]).astype(np.int32)
+
+    @property
+    def has_stage(self) -> bool:
+        return bool((self.ghost_codes() == GHOST_STAGE).any())
 
 
 def _faces(fine, lv, crest, z, zfine, nx, ny):
@@ -235,7 +248,8 @@ def boundaries_from_segments(segments, nx, ny, dx, dy, z_ref):
     """Boundaries::from_segments (terrain.rs:151-187).  segment: dict(kind, side, range_m,
     stage_m, hydrograph)."""
     out = {s: {"code": np.zeros(nx if s in ("north", "south") else ny, np.int32),
-               "param": np.zeros(nx if s in ("north", "south") else ny)} for s in SIDES}
+               "param": np.zeros(nx if s in ("north", "south") else ny),
+               "hydro": np.zeros(nx if s in ("north", "south") else ny, np.int32)} for s in SIDES}
     hydro = []
     for seg in segments:
         side = seg["side"]
@@ -255,6 +269,7 @@ def boundaries_from_segments(segments, nx, ny, dx, dy, z_ref):
         else:
             hydro.append(seg["hydrograph"])
             code, param = GHOST_INFLOW, (1.0 / len_m if len_m > 0.0 else 0.0)
+            out[side]["hydro"][a:e] = len(hydro) - 1
         out[side]["code"][a:e] = code
         out[side]["param"][a:e] = param
     out["hydrographs"] = hydro

@@ -35,8 +35,8 @@ def solver():
 def oracle_terrain(base):
     """The oracle's own TerrainBase built from the same DEM, walls, roughness, segments."""
     f = base.fine
-    segs = []
-    for side in ("north", "south", "west", "east"):
+    segs = [dict(sg) for sg in base.segments]
+    for side in ("north", "south", "west", "east") if not segs else ():
         codes = base.ghosts[side]["code"]
         if np.all(codes == 1):
             segs.append({"kind": "transmissive", "side": side})
@@ -52,7 +52,7 @@ def oracle_params(p):
     return po.SolverParams(p.duration_s, p.sync_interval_s, cfl=p.cfl, dt_max_s=p.dt_max_s, h_dry=p.h_dry,
                            h_eps=p.h_eps, rain=list(p.rain), infil_capacity=p.infil_capacity,
                            initial_depth=p.initial_depth, initial_stage=p.initial_stage, skip_dry=p.skip_dry,
-                           locked_dt=p.locked_dt, cfl_hard=p.cfl_hard)
+                           locked_dt=p.locked_dt, cfl_hard=p.cfl_hard, inflow_scale=p.inflow_scale)
 
 
 def bits64(a):

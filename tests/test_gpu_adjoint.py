@@ -102,3 +102,15 @@ def test_autograd_function_backward(solver):
     worst = max(_rel(g[r, c], v, scale) for (r, c), v in zip(fx["cells"], fd6))
     _say(f"autograd worst rel {worst:.2e}")
     assert worst <= REL_TOL
+
+
+def test_adjoint_declines_stage_boundaries(solver):
+    """The adjoint covers wall and transmissive boundaries; a Stage ghost's depth depends on
+    the bed, so such grids return ITR_DECLINED."""
+    from inverse_terrain_engine_amd.gpu import Declined
+    base = sc.validation_terrain(20, 12, 2.0, lambda x, y: 1.0 + 0.01 * x, n=0.03,
+                                 segments=[{"kind": "stage", "side": "west", "stage_m": 1.2}])
+    p = sc.SolverParams(20.0, 10.0)
+    p.locked_dt = [0.1] * 200
+    with pytest.raises(Declined):
+        solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)

@@ -179,17 +179,86 @@ def test_walls_and_wide_grid(solver):
     assert_run_equal(g, oracle_run(base, p, monitors=mon), "wide")
 
 
+def _equivalence_base():
+    """The boundary fixture of crates/itr-hydro/tests/equivalence.rs:38-53: a west inflow
+    segment (hydrograph rising from 0), a transmissive east side, a south stage segment
+    above the bed, and interior walls."""
+    segs = [{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [200.0, 1.5], [500.0, 0.2]],
+             "range_m": (20.0, 40.0)},
+            {"kind": "transmissive", "side": "east"},
+            {"kind": "stage", "side": "south", "stage_m": 100.3, "range_m": (5.0, 25.0)}]
+    zf = lambda x, y: (100.0 + 0.5 * math.sin(x / 9.0) * math.cos(y / 7.0) + 0.004 * (120.0 - x)
+                       + (0.6 if 60.0 <= x < 64.0 else 0.0))
+    return sc.validation_terrain(57, 43, 2.0, zf, walls=lambda c, r: 30 <= c < 34 and 10 <= r < 18, n=0.035,
+                                 segments=segs)
+
+
+@pytest.mark.parametrize("band_rows", [0, 3])
+def test_stage_and_inflow_boundaries(band_rows):
+    """Stage and Inflow ghosts (boundary.rs:43-47, 66-70), which the reference GPU declines:
+    the equivalence fixture with a dry lead-in (dry-row rule with wet ghosts), and edited
+    candidates whose edits reach the stage cells (per-candidate ghost depth)."""
+    _say(f"case: stage + inflow boundaries, band_rows {band_rows}")
+    from inverse_terrain_engine_amd.gpu import GpuSolver
+    s = GpuSolver(0, band_rows=band_rows, steps_per_chunk=5)
+    base = _equivalence_base()
+    p = sc.SolverParams(900.0, 60.0, infil_capacity=10.0 / 1000.0 / 3600.0,
+                        rain=sc.mm_h([[0.0, 0.0], [300.0, 0.0], [400.0, 80.0], [700.0, 0.0]]))
+    mon = np.array([c for c in range(57 * 43) if c % 7 == 3], np.uint32)
+    dz = np.array([0.05 * (1 + k % 5) for k in range(18)]).reshape(3, 6)
+    edits = [None, tr.TerrainEdit(c0=40, r0=5, dz=dz, fill_m3=float(dz.sum()) * 4.0),
+             tr.TerrainEdit(c0=2, r0=40, dz=-0.4 * np.ones((3, 10)), cut_m3=12 * 4.0)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    runs = s.run_batch(p, terrs, mon)
+    s.close()
+    for i, (r, e) in enumerate(zip(runs, edits)):
+        o = oracle_run(base, p, edit=None if e is None else e.as_dict(), monitors=mon)
+        assert o["rows_skipped"] > 0 and o["rows"][-1][3] > 0.0
+        assert_run_equal(r, o, f"stage+inflow candidate {i}")
+    assert not np.array_equal(runs[0].depth, runs[2].depth)
+
+
+@pytest.mark.parametrize("scale", [0.5, 2.0])
+def test_inflow_scale_and_other_sides(solver, scale):
+    """Ensemble inflow scaling (solver.rs:47-48; the reference GPU declines it,
+    backend.rs:53-55) with inflow through the north side, stage on the east side and a
+    transmissive south: the north / south ghost rows and the east ghost column."""
+    _say(f"case: inflow scale {scale}, north inflow, east stage")
+    segs = [{"kind": "inflow", "side": "north", "hydrograph": [[0.0, 0.4], [150.0, 2.0], [400.0, 0.0]],
+             "range_m": (30.0, 70.0)},
+            {"kind": "stage", "side": "east", "stage_m": 50.35, "range_m": (10.0, 60.0)},
+            {"kind": "transmissive", "side": "south"}]
+    base = sc.validation_terrain(75, 40, 2.0, lambda x, y: 50.0 + 0.003 * (y - 40.0) + 0.15 * math.sin(x / 11.0),
+                                 n=0.03, segments=segs)
+    p = sc.SolverParams(420.0, 60.0, rain=sc.mm_h([[0.0, 30.0], [420.0, 0.0]]), infil_capacity=1e-6)
+    p.inflow_scale = scale
+    mon = np.arange(0, 75 * 40, 11, dtype=np.uint32)
+    g = solver.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    o = oracle_run(base, p, monitors=mon)
+    assert o["rows"][-1][3] > 0.0
+    assert_run_equal(g, o, f"inflow scale {scale}")
+
+
+def test_batch_backend_runs_ensembles(solver):
+    """HipBatchBackend.run (BatchBackend<f32>) takes inflow-scaled ensemble members and
+    boundary segments instead of declining them."""
+    from inverse_terrain_engine_amd.gpu import HipBatchBackend
+    base = _equivalence_base()
+    p = sc.SolverParams(240.0, 60.0, rain=sc.mm_h([[0.0, 40.0]]))
+    p.inflow_scale = 1.4
+    mon = np.arange(0, 57 * 43, 17, dtype=np.uint32)
+    runs = HipBatchBackend(solver).run(p, [tr.PreparedTerrain(base)], mon)
+    o = oracle_run(base, p, monitors=mon)
+    assert runs[0].steps == o["steps"] and runs[0].cell_updates == o["steps"] * 57 * 43
+    assert np.array_equal(runs[0].monitor_max.view(np.uint64), o["monitor_max"].view(np.uint64))
+
+
 def test_declines_unsupported(solver):
+    """Rows wider than one workgroup (992 columns) decline, as the reference's Err does."""
     from inverse_terrain_engine_amd.gpu import Declined
-    base = sc.validation_terrain(16, 16, 1.0, lambda x, y: 0.0, n=0.03,
-                                 segments=[{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 1.0]]}])
+    base = sc.validation_terrain(1000, 2, 1.0, lambda x, y: 0.0, n=0.03)
     with pytest.raises(Declined):
         solver.run_batch(sc.SolverParams(10.0, 5.0), [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))
-    base = sc.validation_terrain(16, 16, 1.0, lambda x, y: 0.0, n=0.03)
-    p = sc.SolverParams(10.0, 5.0)
-    p.inflow_scale = 2.0
-    with pytest.raises(Declined):
-        solver.run_batch(p, [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))
 
 
 def test_fidelity_ladder_one_launch(solver):

@@ -107,6 +107,12 @@ typedef struct {
     const double* hydro_q;
     int32_t n_hydro;
     int32_t reserved2;
+    /* This grid's own solver parameters, or NULL for the call's.  An ensemble member
+     * (crates/itr-opt/src/problem.rs:42-65: scaled rain, inflow_scale, its own locked dt
+     * schedule and guard baseline) is one grid entry with the member's parameters, so all
+     * members of a level run in one batch (the reference runs them on the CPU,
+     * evaluator.rs:212-215, 494-560). */
+    const itr_solver_params* params;
 } itr_grid;
 
 /* Per-candidate result: GpuRun (crates/itr-gpu/src/lib.rs:49-60) + BatchRun

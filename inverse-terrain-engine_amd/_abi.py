@@ -37,7 +37,7 @@ class GridC(C.Structure):
                 ("dzx_r", _fp), ("dzy_r", _fp), ("z_level", _dp), ("mon_cells", _up), ("baseline_max", _dp),
                 ("guard_pos", _ip), ("n_guard", C.c_int32), ("reserved", C.c_int32), ("guard_tol", C.c_double),
                 ("ghost_value", _dp), ("ghost_hydro", _ip), ("hydro_start", _ip), ("hydro_t", _dp), ("hydro_q", _dp),
-                ("n_hydro", C.c_int32), ("reserved2", C.c_int32)]
+                ("n_hydro", C.c_int32), ("reserved2", C.c_int32), ("params", C.POINTER(SolverParamsC))]
 
 
 class RunOutC(C.Structure):

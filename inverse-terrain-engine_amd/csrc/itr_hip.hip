@@ -79,6 +79,7 @@ struct DevParams {
 };
 
 struct DevGrid {
+    const struct DevParams* prm;   // this grid's solver parameters (ensemble members differ)
     int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
     int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
     int32_t max_log;           // capacity of steplog per candidate (0: no log)
@@ -153,7 +154,6 @@ struct KArgs {
     const int4* tasks;         // (grid, local cand, band, -)
     CandCtl* snap;             // [2][ncand]
     StepAcc* acc;              // [4][ncand]
-    const DevParams* prm;
     uint32_t* status;          // [ncand] flags of the latest control block, for the host
     int32_t ntasks;
     int32_t ncand;
@@ -579,7 +579,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const size_t plane = G.plane;
     const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
-    const bool skip_dry = a.prm->skip_dry != 0;
+    const bool skip_dry = G.prm->skip_dry != 0;
     const bool special = SPECIAL && G.special != 0;
     const LdsLayout LL = lds_layout(R, nx, G.special != 0);
     float2* l_gsp = (float2*)(lds + LL.gsp);
@@ -603,7 +603,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             GhostSp gs;
             gs.hv = 0.0f;
             gs.qv = 0.0f;
-            if (k >= 0) gs = special_ghost(G, a.prm->inflow_scale, GHOST[k], k, local, t, inward);
+            if (k >= 0) gs = special_ghost(G, G.prm->inflow_scale, GHOST[k], k, local, t, inward);
             l_gsp[i] = make_float2(gs.hv, gs.qv);
             fw = fw || gs.hv > 0.0f;
         }
@@ -907,7 +907,7 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
                                                 CandCtl& Oc, float* stage, size_t stage_floats, double* shd) {
     const CandCtl& Si = *Sin;
     CandCtl* O = &Oc;
-    const DevParams& p = *a.prm;
+    const DevParams& p = *G.prm;
     const int tid = threadIdx.x;
     const int nthr = G.waves * kWave;
     const int local = Si.local;
@@ -1041,7 +1041,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     const int local = tk.y;
     const int cand = G.cand0 + local;
     const bool designated = tk.z == 0;
-    const DevParams& p = *a.prm;
+    const DevParams& p = *G.prm;
     const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
     if ((Sin->flags & F_DONE) && !designated) return;
@@ -1136,12 +1136,12 @@ __global__ void itr_init_kernel(KArgs a) {
         return;
     }
     CandCtl S = *C;
-    bool ok = prepare_step(*a.prm, G, &S, a.prm->rain_t, a.prm->rain_r);
+    bool ok = prepare_step(*G.prm, G, &S, G.prm->rain_t, G.prm->rain_r);
     if (!ok) {
         S.abort_kind = 4;
         S.flags |= F_DONE;
     } else {
-        S.skipped_now = count_skippable<true>(*a.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
+        S.skipped_now = count_skippable<true>(*G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
     }
     if (threadIdx.x == 0) {
         *C = S;
@@ -1418,6 +1418,8 @@ double host_volume(const std::vector<float>& h, int nx, int ny, double area) {
 struct GridHost {
     itr_grid in;
     DevGrid d;
+    DevParams prm{};             // host copy of this grid's parameters
+    int max_ledger = 0;
     std::vector<void*> allocs;
     int waves = 1;
     std::vector<uint32_t> mon;   // monitored cells (row * nx + col)
@@ -1474,7 +1476,7 @@ int choose_band_rows(const itr_ctx* ctx, const itr_grid* grids, int n_grids, int
 struct itr_batch {
     itr_ctx* ctx = nullptr;
     std::vector<GridHost> grids;
-    DevParams prm{};
+    DevParams prm{};                // the first grid's parameters (single-grid users: adjoint, schedules)
     KArgs args{};
     void* d_grids = nullptr;
     void* d_tasks = nullptr;
@@ -1482,16 +1484,12 @@ struct itr_batch {
     void* d_acc = nullptr;          // StepAcc [4][ncand]
     void* d_prm = nullptr;
     void* d_status = nullptr;
-    void* d_rain_t = nullptr;
-    void* d_rain_r = nullptr;
-    void* d_locked = nullptr;
     uint32_t* h_status = nullptr;   // pinned
     int ncand = 0;
     int ntasks = 0;
     int block = 64;
     size_t lds_bytes = 0;
     int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts
-    int max_ledger = 0;
     uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -1504,8 +1502,7 @@ namespace {
 void free_batch_mem(itr_batch* b) {
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
-    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_prm, b->d_status, b->d_rain_t, b->d_rain_r,
-                  b->d_locked};
+    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_prm, b->d_status};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (b->h_status) (void)hipHostFree(b->h_status);
@@ -1518,6 +1515,67 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
     HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
     allocs.push_back(p);
     *out = (T*)p;
+    return ITR_OK;
+}
+
+// Device parameters of one grid (SolverParams, solver.rs:26-49, with the f32 constants of
+// Solver::consts, solver.rs:269-285), the hyetograph and locked schedule uploaded into
+// `allocs`.  max_ledger: ledger rows per candidate (one per sync point plus one).
+itr_status set_params(const itr_solver_params* p, DevParams& P, int& max_ledger, std::vector<void*>& allocs, char* err,
+                      size_t errlen) {
+    const double dur = p->duration_s, syn = p->sync_interval_s;
+    const uint32_t n_sync = (uint32_t)std::ceil(dur / syn);
+    max_ledger = (int)std::min<uint64_t>(n_sync + 1, 1u << 20);
+    P = DevParams{};
+    P.duration = dur;
+    P.sync = syn;
+    P.cfl = p->cfl;
+    P.dt_max = p->dt_max_s;
+    P.cfl_hard = p->cfl_hard;
+    P.infil_cap = p->infil_capacity;
+    P.inflow_scale = p->inflow_scale;
+    P.n_rain = p->n_rain;
+    P.n_sync = n_sync;
+    P.skip_dry = p->skip_dry;
+    P.max_steps = p->max_steps;
+    P.max_ledger = max_ledger;
+    P.h_dry = (float)p->h_dry;
+    P.heps2 = (float)(p->h_eps * p->h_eps);
+    P.q_min = (float)1e-30;
+    double *rt = nullptr, *rr = nullptr;
+    itr_status s;
+    if ((s = dev_alloc(allocs, &rt, (size_t)p->n_rain, err, errlen)) != ITR_OK) return s;
+    if ((s = dev_alloc(allocs, &rr, (size_t)p->n_rain, err, errlen)) != ITR_OK) return s;
+    if (p->n_rain > 0 && (hipMemcpy(rt, p->rain_t, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess ||
+                          hipMemcpy(rr, p->rain_r, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess)) {
+        set_err(err, errlen, "rain upload failed");
+        return ITR_EDEVICE;
+    }
+    P.rain_t = rt;
+    P.rain_r = rr;
+    if (p->locked_dt && p->n_locked > 0) {
+        double* ld = nullptr;
+        if ((s = dev_alloc(allocs, &ld, (size_t)p->n_locked, err, errlen)) != ITR_OK) return s;
+        if (hipMemcpy(ld, p->locked_dt, sizeof(double) * p->n_locked, hipMemcpyHostToDevice) != hipSuccess) {
+            set_err(err, errlen, "locked dt upload failed");
+            return ITR_EDEVICE;
+        }
+        P.locked = ld;
+        P.n_locked = p->n_locked;
+    }
+    return ITR_OK;
+}
+
+// The sync-callback flag in every grid's parameters: candidates pause at sync points for
+// the host only while a callback is attached.
+itr_status set_callback_flag(itr_batch* b, int has_cb, char* err, size_t errlen) {
+    std::vector<DevParams> h(b->grids.size());
+    for (size_t g = 0; g < b->grids.size(); g++) {
+        b->grids[g].prm.has_cb = has_cb;
+        h[g] = b->grids[g].prm;
+    }
+    b->prm.has_cb = has_cb;
+    if (!h.empty()) HIP_TRY(hipMemcpy(b->d_prm, h.data(), sizeof(DevParams) * h.size(), hipMemcpyHostToDevice));
     return ITR_OK;
 }
 
@@ -1678,20 +1736,23 @@ itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const 
 
 }  // extern "C"
 
-itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, const itr_grid* grids, int32_t n_grids,
                              itr_batch** out, int64_t log_steps, char* err, size_t errlen) {
+    const itr_solver_params* p = p_call;
     if (!ctx || !p || !grids || n_grids <= 0 || !out) {
         set_err(err, errlen, "null argument");
         return ITR_EINVAL;
     }
     HIP_TRY(hipSetDevice(ctx->device));
     // ---- validation and declines (lib.rs:155-171)
-    if (!(p->sync_interval_s > 0.0) || !(p->duration_s >= 0.0) || p->n_rain < 0) {
-        set_err(err, errlen, "invalid solver parameters");
-        return ITR_EINVAL;
-    }
     for (int g = 0; g < n_grids; g++) {
         const itr_grid& G = grids[g];
+        const itr_solver_params* q = G.params ? G.params : p;
+        if (!(q->sync_interval_s > 0.0) || !(q->duration_s >= 0.0) || q->n_rain < 0 ||
+            (q->n_rain > 0 && (!q->rain_t || !q->rain_r)) || q->n_locked < 0) {
+            set_err(err, errlen, "grid %d: invalid solver parameters", g);
+            return ITR_EINVAL;
+        }
         if (G.nx <= 0 || G.ny <= 0 || G.n_cand < 0 || !G.wet || !G.gn2 || !G.ghost || !G.dzx || !G.dzy) {
             set_err(err, errlen, "grid %d: missing arrays or empty shape", g);
             return ITR_EINVAL;
@@ -1745,7 +1806,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
                     set_err(err, errlen, "grid %d: guard position out of range", g);
                     return ITR_EINVAL;
                 }
-        if (p->has_initial_stage && !G.z_level) {
+        if (q->has_initial_stage && !G.z_level) {
             set_err(err, errlen, "grid %d: initial_stage needs z_level", g);
             return ITR_EINVAL;
         }
@@ -1757,54 +1818,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         delete b;
         return s;
     };
-    // ---- shared parameters
-    const double dur = p->duration_s, syn = p->sync_interval_s;
-    const uint32_t n_sync = (uint32_t)std::ceil(dur / syn);
-    b->max_ledger = (int)std::min<uint64_t>(n_sync + 1, 1u << 20);
-    DevParams& P = b->prm;
-    P.duration = dur;
-    P.sync = syn;
-    P.cfl = p->cfl;
-    P.dt_max = p->dt_max_s;
-    P.cfl_hard = p->cfl_hard;
-    P.infil_cap = p->infil_capacity;
-    P.inflow_scale = p->inflow_scale;
-    P.n_rain = p->n_rain;
-    P.n_sync = n_sync;
-    P.skip_dry = p->skip_dry;
-    P.max_steps = p->max_steps;
-    P.max_ledger = b->max_ledger;
-    P.h_dry = (float)p->h_dry;
-    P.heps2 = (float)(p->h_eps * p->h_eps);
-    P.q_min = (float)1e-30;
-    {
-        std::vector<void*> tmp;
-        double *rt = nullptr, *rr = nullptr;
-        if (dev_alloc(tmp, &rt, p->n_rain, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
-        b->d_rain_t = rt;
-        if (dev_alloc(tmp, &rr, p->n_rain, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
-        b->d_rain_r = rr;
-        if (p->n_rain > 0) {
-            if (hipMemcpy(rt, p->rain_t, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(rr, p->rain_r, sizeof(double) * p->n_rain, hipMemcpyHostToDevice) != hipSuccess) {
-                set_err(err, errlen, "rain upload failed");
-                return fail(ITR_EDEVICE);
-            }
-        }
-        P.rain_t = rt;
-        P.rain_r = rr;
-        if (p->locked_dt && p->n_locked > 0) {
-            double* ld = nullptr;
-            if (dev_alloc(tmp, &ld, p->n_locked, err, errlen) != ITR_OK) return fail(ITR_ENOMEM);
-            b->d_locked = ld;
-            if (hipMemcpy(ld, p->locked_dt, sizeof(double) * p->n_locked, hipMemcpyHostToDevice) != hipSuccess) {
-                set_err(err, errlen, "locked dt upload failed");
-                return fail(ITR_EDEVICE);
-            }
-            P.locked = ld;
-            P.n_locked = p->n_locked;
-        }
-    }
     // ---- grids
     int total = 0;
     int max_waves = 1;
@@ -1818,6 +1831,14 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         GridHost& GH = b->grids[g];
         GH.in = I;
         DevGrid& D = GH.d;
+        // ---- this grid's parameters (the call's, or an ensemble member's own)
+        const itr_solver_params* p = I.params ? I.params : p_call;
+        const double dur = p->duration_s, syn = p->sync_interval_s;
+        const uint32_t n_sync = (uint32_t)std::ceil(dur / syn);
+        {
+            itr_status ps = set_params(p, GH.prm, GH.max_ledger, GH.allocs, err, errlen);
+            if (ps != ITR_OK) return fail(ps);
+        }
         const int nx = I.nx, ny = I.ny, nc = I.n_cand;
         D.nx = nx;
         D.ny = ny;
@@ -1977,7 +1998,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
         ALLOC(d_ybnd, 2 * ncx * 2 * nx);
         ALLOC(d_sbuf, ncx * plane);
         ALLOC(d_fixbuf, ncx * plane);
-        ALLOC(d_ledger, ncx * b->max_ledger * 8);
+        ALLOC(d_ledger, ncx * GH.max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
         StepRec* d_steplog = nullptr;
         if (log_steps > 0) ALLOC(d_steplog, ncx * (size_t)log_steps);
@@ -2097,8 +2118,15 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
     b->d_snap = d_snap;
     if ((s = dev_alloc(misc, &d_acc, 4 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_acc = d_acc;
-    if ((s = dev_alloc(misc, &d_prm, 1, err, errlen)) != ITR_OK) return fail(s);
+    if ((s = dev_alloc(misc, &d_prm, (size_t)n_grids, err, errlen)) != ITR_OK) return fail(s);
     b->d_prm = d_prm;
+    std::vector<DevParams> hprm(n_grids);
+    for (int g = 0; g < n_grids; g++) {
+        hprm[g] = b->grids[g].prm;
+        dgrids[g].prm = d_prm + g;
+        b->grids[g].d.prm = d_prm + g;
+    }
+    b->prm = hprm[0];
     if ((s = dev_alloc(misc, &d_status, std::max(total, 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_status = d_status;
     if (hipHostMalloc((void**)&b->h_status, sizeof(uint32_t) * std::max(total, 1), hipHostMallocDefault) != hipSuccess) {
@@ -2111,7 +2139,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
                                hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
                                          hipMemcpyHostToDevice) == hipSuccess)) &&
               hipMemset(d_acc, 0, 4 * std::max<size_t>(ctl.size(), 1) * sizeof(StepAcc)) == hipSuccess &&
-              hipMemcpy(d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(d_prm, hprm.data(), sizeof(DevParams) * n_grids, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         set_err(err, errlen, "control upload failed");
         return fail(ITR_EDEVICE);
@@ -2120,7 +2148,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p, const itr
     b->args.tasks = d_tasks;
     b->args.snap = d_snap;
     b->args.acc = d_acc;
-    b->args.prm = d_prm;
     b->args.status = d_status;
     b->args.ntasks = b->ntasks;
     b->args.ncand = total;
@@ -2143,8 +2170,8 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     HIP_TRY(hipSetDevice(ctx->device));
     const int has_cb = on_sync ? 1 : 0;
     if (b->prm.has_cb != has_cb) {
-        b->prm.has_cb = has_cb;
-        HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
+        itr_status ps = set_callback_flag(b, has_cb, err, errlen);
+        if (ps != ITR_OK) return ps;
     }
     int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
     K = (K + 3) / 4 * 4;   // chunks keep the launch sequence aligned to the 4-deep ring
@@ -2225,8 +2252,8 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     itr_ctx* ctx = b->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     if (b->prm.has_cb) {
-        b->prm.has_cb = 0;
-        HIP_TRY(hipMemcpy(b->d_prm, &b->prm, sizeof(DevParams), hipMemcpyHostToDevice));
+        itr_status ps = set_callback_flag(b, 0, err, errlen);
+        if (ps != ITR_OK) return ps;
     }
     int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
     K = (K + 3) / 4 * 4;
@@ -2287,10 +2314,10 @@ itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, siz
         o.abort_sync = C.abort_sync;
         o.abort_value = C.abort_value;
         o.sync_count = (int32_t)C.sync_index;
-        const int nl = (int)std::min<uint32_t>(C.n_ledger, (uint32_t)b->max_ledger);
+        const int nl = (int)std::min<uint32_t>(C.n_ledger, (uint32_t)GH.max_ledger);
         std::vector<double> led((size_t)std::max(nl, 1) * 8);
         if (nl > 0)
-            HIP_TRY(hipMemcpy(led.data(), D.ledger + (size_t)C.local * b->max_ledger * 8, sizeof(double) * 8 * nl,
+            HIP_TRY(hipMemcpy(led.data(), D.ledger + (size_t)C.local * GH.max_ledger * 8, sizeof(double) * 8 * nl,
                               hipMemcpyDeviceToHost));
         const double gross = C.v0 + C.acc_rain + C.inflow;
         const double last_res = nl > 0 ? led[(size_t)(nl - 1) * 8 + 7] : 0.0;
@@ -2358,6 +2385,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "null argument");
         return ITR_EINVAL;
     }
+    if (grid->params) p = grid->params;   // the grid's own parameters win, as in a batch
     if (!p->locked_dt || p->n_locked <= 0) {
         set_err(err, errlen, "the adjoint runs on a locked dt schedule (locked_dt)");
         return ITR_EINVAL;
@@ -2810,6 +2838,7 @@ itr_status itr_hip_dt_schedule(itr_ctx* ctx, const itr_solver_params* p, const i
         set_err(err, errlen, "invalid argument");
         return ITR_EINVAL;
     }
+    if (grid->params) p = grid->params;   // the grid's own parameters win, as in a batch
     HIP_TRY(hipSetDevice(ctx->device));
     itr_grid one = *grid;
     one.n_cand = 1;
@@ -2823,6 +2852,7 @@ itr_status itr_hip_dt_schedule(itr_ctx* ctx, const itr_solver_params* p, const i
     if (grid->z_level) one.z_level = grid->z_level + (size_t)cand * grid->nx * grid->ny;
     one.baseline_max = nullptr;
     one.n_guard = 0;
+    one.params = nullptr;   // q below carries them
     itr_solver_params q = *p;
     if (q.max_steps <= 0 || q.max_steps > capacity) q.max_steps = capacity;
     itr_batch* b = nullptr;

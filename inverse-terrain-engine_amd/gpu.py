@@ -260,6 +260,37 @@ class GpuSolver:
         (TerrainBase, [PreparedTerrain], monitors, guard-or-None)."""
         return BatchSession(self, p, groups, on_sync)
 
+    def run_ensemble(self, members, terrains, monitors, guards=None, want_state=True):
+        """Every candidate under every ensemble member's parameters in one batch (one grid
+        entry per member, each with its own forcing and guard baseline).  The reference
+        runs members on the CPU (evaluator.rs:212-215, 494-560), each member a
+        ``scaled`` copy of the level's parameters (problem.rs:56-63).  Returns
+        runs[member][candidate]."""
+        if len(terrains) == 0 or len(members) == 0:
+            return [[] for _ in members]
+        base = terrains[0].base
+        groups = [(base, terrains, monitors, None if guards is None else guards[m]) for m in range(len(members))]
+        s = BatchSession(self, members[0], groups, None, group_params=list(members))
+        try:
+            s.run()
+            flat = s.results(want_state)
+        finally:
+            s.close()
+        n = len(terrains)
+        return [flat[m * n:(m + 1) * n] for m in range(len(members))]
+
+    def full_run(self, p: SolverParams, terrain):
+        """Full-domain run for the result rasters (crates/itr-cli/src/run.rs:73-93): every
+        cell monitored; returns (run, peak, tpeak) with peak NaN on walls and tpeak NaN on
+        walls and where the peak is not positive."""
+        base = terrain.base
+        mon = np.arange(base.nx * base.ny, dtype=np.uint32)
+        run = self.run_batch(p, [terrain], mon)[0]
+        wall = base.wall.reshape(-1)
+        peak = np.where(wall, np.nan, run.monitor_max).reshape(base.ny, base.nx)
+        tpeak = np.where(wall | (run.monitor_max <= 0.0), np.nan, run.monitor_tmax).reshape(base.ny, base.nx)
+        return run, peak, tpeak
+
     def run_batch(self, p: SolverParams, terrains, monitors, guard=None, on_sync=None, want_state=True):
         """GpuSolver::run_batch: every candidate (sharing one TerrainBase) to its end."""
         if len(terrains) == 0:
@@ -273,7 +304,10 @@ class GpuSolver:
 
 
 class BatchSession:
-    def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None):
+    """A resident batch.  ``group_params``: optional per-group SolverParams (None = ``p``),
+    e.g. one group per ensemble member."""
+
+    def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None, group_params=None):
         self.solver = solver
         L = solver._L
         self._L = L
@@ -282,7 +316,13 @@ class BatchSession:
         q = _params_c(p, self._keep)
         grids = (_abi.GridC * len(groups))()
         for i, (base, terrs, mons, guard) in enumerate(groups):
-            grids[i] = _grid_c(base, terrs, mons, self._keep, guard, p.initial_stage is not None)
+            gp = group_params[i] if group_params is not None else None
+            eff = gp if gp is not None else p
+            grids[i] = _grid_c(base, terrs, mons, self._keep, guard, eff.initial_stage is not None)
+            if gp is not None:
+                gq = _params_c(gp, self._keep)
+                self._keep.append(gq)
+                grids[i].params = C.pointer(gq)
         self._q = q
         self._grids = grids
         err = C.create_string_buffer(1024)

@@ -7,6 +7,7 @@ of a configuration sees the same inputs.
 """
 from __future__ import annotations
 
+import copy
 import math
 from dataclasses import dataclass, field
 
@@ -38,6 +39,15 @@ class SolverParams:
     cfl_hard: float = 0.5
     inflow_scale: float = 1.0
     max_steps: int = 0                            # test control: stop after this many steps
+
+
+def scaled(p: SolverParams, rain: float, inflow: float) -> SolverParams:
+    """Forcing of an ensemble member (crates/itr-opt/src/problem.rs:56-63): every rain rate
+    times ``rain`` (f64), the inflow scale times ``inflow``."""
+    q = copy.copy(p)
+    q.rain = [(t, r * rain) for t, r in p.rain]
+    q.inflow_scale = p.inflow_scale * inflow
+    return q
 
 
 class SplitMix64:

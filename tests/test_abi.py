@@ -51,7 +51,7 @@ int main(void) {
          sizeof(itr_adjoint_out), sizeof(itr_tune_result));
   F(itr_solver_params, rain_t) F(itr_solver_params, max_steps) F(itr_solver_params, skip_dry)
   F(itr_grid, wet) F(itr_grid, n_cand) F(itr_grid, dzx) F(itr_grid, mon_cells) F(itr_grid, guard_tol)
-  F(itr_grid, ghost_value) F(itr_grid, hydro_q) F(itr_grid, n_hydro)
+  F(itr_grid, ghost_value) F(itr_grid, hydro_q) F(itr_grid, n_hydro) F(itr_grid, params)
   F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger)
   F(itr_stats, launches) F(itr_stats, use_graph)
   F(itr_objective, asset_start) F(itr_objective, tau_m) F(itr_adjoint_out, dJ_dz_cell) F(itr_adjoint_out, steps)

@@ -345,3 +345,46 @@ def test_short_sqrt_and_reciprocal_are_ieee(solver):
     one of the 2^32 f32 inputs (the guarded ranges plus the IEEE fallbacks)."""
     _say("case: exhaustive reciprocal / sqrt check")
     assert solver.math_selfcheck() == (0, 0)
+
+
+def test_ensemble_members_one_batch(solver):
+    """Ensemble members (problem.rs:56-63 scaled forcing; evaluated on the CPU by the
+    reference, evaluator.rs:494-560) as grid entries of one batch with their own
+    parameters, including a member with another duration and sync interval: every
+    (member, candidate) bitwise equal to the oracle run under that member's parameters."""
+    _say("case: ensemble of 4 members x 2 candidates, one batch")
+    base = _equivalence_base()
+    p = sc.SolverParams(600.0, 60.0, infil_capacity=5.0 / 1000.0 / 3600.0,
+                        rain=sc.mm_h([[0.0, 20.0], [200.0, 70.0], [500.0, 0.0]]))
+    members = [sc.scaled(p, 1.0, 1.0), sc.scaled(p, 1.4, 1.0), sc.scaled(p, 0.6, 2.0), sc.scaled(p, 1.0, 0.5)]
+    members[3].duration_s, members[3].sync_interval_s = 420.0, 45.0
+    mon = np.array([c for c in range(57 * 43) if c % 5 == 1], np.uint32)
+    dz = np.array([0.05 * (1 + k % 5) for k in range(18)]).reshape(3, 6)
+    edits = [None, tr.TerrainEdit(c0=40, r0=5, dz=dz, fill_m3=float(dz.sum()) * 4.0)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    runs = solver.run_ensemble(members, terrs, mon)
+    for m, mp in enumerate(members):
+        for c, e in enumerate(edits):
+            o = oracle_run(base, mp, edit=None if e is None else e.as_dict(), monitors=mon)
+            assert_run_equal(runs[m][c], o, f"member {m} candidate {c}")
+    assert runs[1][0].steps != runs[0][0].steps or not np.array_equal(runs[1][0].depth, runs[0][0].depth)
+
+
+def test_full_domain_rasters(solver):
+    """Full-domain run (crates/itr-cli/src/run.rs:73-93): every cell monitored, peak and
+    time-to-peak rasters (NaN on walls; tpeak NaN where the peak is not positive)."""
+    _say("case: full-domain peak rasters 70x45")
+    base = sc.validation_terrain(70, 45, 2.0, lambda x, y: 3.0 - 0.01 * y + 0.2 * math.sin(x / 9.0),
+                                 walls=lambda c, r: 20 <= c < 26 and 10 <= r < 30, n=0.035,
+                                 segments=[{"kind": "transmissive", "side": "south"}])
+    p = sc.SolverParams(300.0, 50.0, rain=sc.mm_h([[0, 0], [60, 0], [90, 90], [200, 90], [250, 0]]))
+    run, peak, tpeak = solver.full_run(p, tr.PreparedTerrain(base))
+    mon = np.arange(70 * 45, dtype=np.uint32)
+    o = oracle_run(base, p, monitors=mon)
+    assert_run_equal(run, o, "full domain")
+    wall = base.wall
+    assert np.isnan(peak[wall]).all() and np.isnan(tpeak[wall]).all()
+    assert np.array_equal(peak[~wall], o["monitor_max"].reshape(45, 70)[~wall])
+    pos = (~wall) & (o["monitor_max"].reshape(45, 70) > 0.0)
+    assert pos.any() and np.array_equal(tpeak[pos], o["monitor_tmax"].reshape(45, 70)[pos])
+    assert np.isnan(tpeak[(~wall) & ~pos]).all()

@@ -106,3 +106,38 @@ def test_bench_case_terrain_is_reference_terrain():
                    math.cos((oy - (r + 0.5) * 2.0) / 17.0) for c in range(64)] for r in range(64)])
     ot = po.Terrain(z, manning=0.035, dx=2.0)
     faces_equal(base, ot)
+
+
+def test_boundary_segments_match_oracle():
+    """Stage / Inflow / transmissive segments (Boundaries::from_segments, terrain.rs:151-187):
+    codes equal to the oracle's; the per-cell parameter is stage - z_ref (Stage) or
+    1 / segment length (Inflow, cells snapped outward to whole cells), with the inflow's
+    hydrograph index."""
+    segs = [{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [200.0, 1.5]], "range_m": (21.0, 39.0)},
+            {"kind": "inflow", "side": "north", "hydrograph": [[0.0, 0.3]], "range_m": (4.0, 10.0)},
+            {"kind": "transmissive", "side": "east"},
+            {"kind": "stage", "side": "south", "stage_m": 100.3, "range_m": (5.0, 25.0)}]
+    base = sc.validation_terrain(30, 25, 2.0, lambda x, y: 100.0 + 0.01 * x, n=0.035, segments=segs)
+    ot = oracle_terrain(base)
+    g = ot.ghosts()
+    for side in ("north", "south", "west", "east"):
+        assert np.array_equal(g[side], base.ghosts[side]["code"]), side
+    codes, par, hyd = base.ghost_codes(), base.ghost_params(), base.ghost_hydro()
+    west = slice(2 * 30, 2 * 30 + 25)
+    assert list(np.nonzero(codes[west] == tr.GHOST_INFLOW)[0]) == list(range(10, 20))   # floor(21/2)..ceil(39/2)
+    assert np.all(par[west][10:20] == 1.0 / 20.0) and np.all(hyd[west][10:20] == 0)
+    assert np.all(par[0:30][2:5] == 1.0 / 6.0) and np.all(hyd[0:30][2:5] == 1)
+    south = slice(30, 60)
+    assert np.all(par[south][2:13] == 100.3 - base.z_ref)
+    assert base.has_stage
+
+
+def test_ensemble_scaling():
+    """problem.rs:56-63: member forcing = every rain rate times the rain factor, the inflow
+    scale times the inflow factor; the level's parameters are untouched."""
+    p = sc.SolverParams(600.0, 60.0, rain=sc.mm_h([[0.0, 20.0], [200.0, 70.0]]))
+    p.inflow_scale = 1.5
+    q = sc.scaled(p, 1.4, 0.5)
+    assert [r for _, r in q.rain] == [r * 1.4 for _, r in p.rain]
+    assert [t for t, _ in q.rain] == [t for t, _ in p.rain]
+    assert q.inflow_scale == 0.75 and p.inflow_scale == 1.5

@@ -26,7 +26,7 @@
 //   dzx    [cand][ny][P]        west-face bed jump of cell (col); col 0 = 0
 //   dzy    [cand][ny][P]        north-face bed jump of row (row); row 0 = 0
 //   dzx2/dzy2                   crest mode second rises, same layout
-//   wet, gn2, mon_map [ny][P]   shared by every candidate of the grid
+//   wet, gn2 [ny][P]            shared by every candidate of the grid
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,8 +92,7 @@ struct DevGrid {
     const float* dzy2;
     const float* wet;
     const float* gn2;
-    const int32_t* mon_map;
-    const uint16_t* band_mon;  // [ny] per band: bit w set if wave w updates a monitored cell
+    const int32_t* mon_row;    // [ny + 1] first position in mon_cells of each row (cells sorted row-major)
     const int32_t* ghost;
     // Stage / Inflow ghosts (codes 2 / 3); special = 0 when the grid has none.
     int32_t special, n_hydro;
@@ -537,18 +536,19 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const int wave = tid >> 6;
     const int col = wave * kColsPerWave - 1 + lane;
     const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
-    const bool band_mon = ((G.band_mon[band] >> wave) & 1u) != 0;   // this wave's columns hold monitors
     const int npar = par ^ 1;
 
     const size_t cstride = 6 * G.plane;
     float* base = G.state + (size_t)local * cstride;
     ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)npar * 3 * G.plane);
-    const ITR_GLOBAL int32_t* __restrict__ MONMAP = gp(G.mon_map);
     const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
     const ITR_GLOBAL float* __restrict__ mon_in = gp(G.mon_max + slot(G, par, local) * G.n_mon);
     const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
+    // The band's monitors: positions mon_q0 .. mon_q1-1 of the sorted monitor list.
+    const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
+    const ITR_GLOBAL uint32_t* __restrict__ MONCELL = gp(G.mon_cells);
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
     ITR_GLOBAL float* __restrict__ fixbuf = gp(G.fixbuf + (size_t)local * G.plane);
     const RowAcc* ra_in = G.rowacc + slot(G, par, local) * ny;
@@ -625,14 +625,11 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
                     bstore(0.0f, rs_ho, cb, so);
                     bstore(0.0f, rs_ho, cb, so + planeb);
                     bstore(0.0f, rs_ho, cb, so + 2 * planeb);
-                    if (band_mon) {
-                        const int mi = MONMAP[(size_t)j * P + ccol];
-                        if (mi >= 0) {
-                            mon_out[mi] = mon_in[mi];
-                            tmax_out[mi] = tmax_in[mi];
-                        }
-                    }
                 }
+            }
+            for (int q = mon_q0 + tid; q < mon_q1; q += nthr) {
+                mon_out[q] = mon_in[q];
+                tmax_out[q] = tmax_in[q];
             }
             if (a0 == 0 && owner) yb_out[col] = 0.0f;
             if (b0 == ny && owner) yb_out[nx + col] = 0.0f;
@@ -804,15 +801,6 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             smax_bits = max(smax_bits, __float_as_int(u.s));
             bad = bad || u.bad;
             if (sync) sbuf[(size_t)j * P + ccol] = u.s;
-            if (band_mon) {
-                const int mi = MONMAP[(size_t)j * P + ccol];
-                if (mi >= 0) {   // kernel.rs:311-321: strict >, time of the new state
-                    const float old = mon_in[mi];
-                    const bool up = u.h > old;
-                    mon_out[mi] = up ? u.h : old;
-                    tmax_out[mi] = up ? t_new : tmax_in[mi];
-                }
-            }
         }
         if (edge_wave) {
             if (col == 0 && lane >= 1) l_bw[r] = fw.m;
@@ -841,6 +829,19 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         l_red[kMaxWaves + wave] = bad_any ? 1 : 0;
     }
     __syncthreads();
+
+    // Monitors of the band (kernel.rs:311-321 on the new state: strict >, time of the new
+    // state).  Done after the march so that no wave of the march waits on their loads:
+    // every thread takes monitored cells of the band's rows and reads their new depth back.
+    for (int q = mon_q0 + tid; q < mon_q1; q += nthr) {
+        const uint32_t cell = MONCELL[q];
+        const uint32_t r = cell / (uint32_t)nx, c = cell - r * (uint32_t)nx;
+        const float hn = Ho[(size_t)r * P + c];
+        const float old = mon_in[q];
+        const bool up = hn > old;
+        mon_out[q] = up ? hn : old;
+        tmax_out[q] = up ? t_new : tmax_in[q];
+    }
 
     // Row FixedSums (real.rs:132-175): lane L = padded column & 15, sequential in column
     // order inside a lane, then the fixed pairwise tree.
@@ -1423,19 +1424,8 @@ struct GridHost {
     std::vector<void*> allocs;
     int waves = 1;
     std::vector<uint32_t> mon;   // monitored cells (row * nx + col)
-    uint16_t* d_bandmon = nullptr;
 };
 
-// Per band of R rows: bit w set if wave w (updated columns 62w .. 62w+61) holds a
-// monitored cell in the band.
-std::vector<uint16_t> band_masks(const std::vector<uint32_t>& mon, int nx, int ny, int R) {
-    std::vector<uint16_t> m((size_t)ny, 0);
-    for (uint32_t cell : mon) {
-        const int r = (int)(cell / (uint32_t)nx), c = (int)(cell % (uint32_t)nx);
-        m[(size_t)(r / R)] |= (uint16_t)(1u << (c / kColsPerWave));
-    }
-    return m;
-}
 
 }  // namespace
 
@@ -1898,18 +1888,17 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             dzy2.assign(dzy.size(), 0.0f);
         }
         std::vector<float> wet(plane, 0.0f), gn2(plane, 0.0f);
-        std::vector<int32_t> mon_map(plane, -1);
         GH.mon.assign(I.mon_cells, I.mon_cells + I.n_mon);
-        const std::vector<uint16_t> band_mon = band_masks(GH.mon, nx, ny, R);
+        std::vector<int32_t> mon_row((size_t)ny + 1, 0);   // monitor cells are sorted row-major
+        for (int r = 0, q = 0; r <= ny; r++) {
+            while (q < I.n_mon && I.mon_cells[q] < (uint32_t)r * (uint32_t)nx) q++;
+            mon_row[r] = q;
+        }
         for (int r = 0; r < ny; r++)
             for (int c = 0; c < nx; c++) {
                 wet[(size_t)r * D.P + c] = I.wet[(size_t)r * nx + c];
                 gn2[(size_t)r * D.P + c] = I.gn2[(size_t)r * nx + c];
             }
-        for (int q = 0; q < I.n_mon; q++) {
-            const int r = (int)(I.mon_cells[q] / (uint32_t)nx), c = (int)(I.mon_cells[q] % (uint32_t)nx);
-            mon_map[(size_t)r * D.P + c] = q;
-        }
         std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);   // parity 0 half
         std::memset(rowacc.data(), 0, rowacc.size() * sizeof(RowAcc));
         for (int k = 0; k < nc; k++) {
@@ -1972,8 +1961,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         float *d_state, *d_dzx, *d_dzy, *d_dzx2 = nullptr, *d_dzy2 = nullptr, *d_wet, *d_gn2, *d_monmax, *d_ybnd, *d_sbuf,
             *d_fixbuf;
         double *d_montmax, *d_ledger, *d_base = nullptr;
-        int32_t *d_monmap, *d_ghost, *d_gpos = nullptr;
-        uint16_t* d_bandmon;
+        int32_t *d_monrow, *d_ghost, *d_gpos = nullptr;
         uint32_t* d_moncells;
         RowAcc* d_rowacc;
         const size_t ncx = (size_t)std::max(nc, 1);
@@ -1989,8 +1977,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         }
         ALLOC(d_wet, plane);
         ALLOC(d_gn2, plane);
-        ALLOC(d_monmap, plane);
-        ALLOC(d_bandmon, (size_t)ny);
+        ALLOC(d_monrow, (size_t)ny + 1);
         ALLOC(d_ghost, (size_t)(2 * nx + 2 * ny));
         ALLOC(d_monmax, 2 * ncx * std::max(I.n_mon, 1));
         ALLOC(d_montmax, 2 * ncx * std::max(I.n_mon, 1));
@@ -2025,8 +2012,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         };
         bool ok = up(d_state, st.data(), st.size() * 4) && up(d_dzx, dzx.data(), dzx.size() * 4) &&
                   up(d_dzy, dzy.data(), dzy.size() * 4) && up(d_wet, wet.data(), plane * 4) &&
-                  up(d_gn2, gn2.data(), plane * 4) && up(d_monmap, mon_map.data(), plane * 4) &&
-                  up(d_bandmon, band_mon.data(), band_mon.size() * sizeof(uint16_t)) &&
+                  up(d_gn2, gn2.data(), plane * 4) && up(d_monrow, mon_row.data(), mon_row.size() * 4) &&
                   up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
                   up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
                   up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
@@ -2062,9 +2048,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.dzy2 = d_dzy2;
         D.wet = d_wet;
         D.gn2 = d_gn2;
-        D.mon_map = d_monmap;
-        D.band_mon = d_bandmon;
-        GH.d_bandmon = d_bandmon;
+        D.mon_row = d_monrow;
         D.ghost = d_ghost;
         D.mon_max = d_monmax;
         D.mon_tmax = d_montmax;
@@ -2699,8 +2683,6 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
         const int R = std::max(1, std::min(Rs[g], D.ny));
         D.R = R;
         D.nbands = (D.ny + R - 1) / R;
-        const std::vector<uint16_t> bm = band_masks(GH.mon, D.nx, D.ny, R);
-        HIP_TRY(hipMemcpy(GH.d_bandmon, bm.data(), bm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
         for (int k = 0; k < D.ncand; k++)
             for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4((int)g, k, band, 0));
         dg.push_back(D);

@@ -79,7 +79,7 @@ struct DevParams {
 };
 
 struct DevGrid {
-    const struct DevParams* prm;   // this grid's solver parameters (ensemble members differ)
+    DevParams prm;             // this grid's solver parameters (ensemble members differ)
     int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
     int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
     int32_t max_log;           // capacity of steplog per candidate (0: no log)
@@ -93,6 +93,7 @@ struct DevGrid {
     const float* wet;
     const float* gn2;
     const int32_t* mon_row;    // [ny + 1] first position in mon_cells of each row (cells sorted row-major)
+    const uint32_t* mon_off;   // [n_mon] offset row * P + col of each monitored cell in a state plane
     const int32_t* ghost;
     // Stage / Inflow ghosts (codes 2 / 3); special = 0 when the grid has none.
     int32_t special, n_hydro;
@@ -150,7 +151,7 @@ struct StepAcc {
 
 struct KArgs {
     const DevGrid* grids;
-    const int4* tasks;         // (grid, local cand, band, -)
+    const int4* tasks;         // (grid, local cand, band, batch cand)
     CandCtl* snap;             // [2][ncand]
     StepAcc* acc;              // [4][ncand]
     uint32_t* status;          // [ncand] flags of the latest control block, for the host
@@ -164,7 +165,7 @@ struct KArgs {
 // infiltration is staged per cell.  Grids with Stage / Inflow ghosts add the band's
 // special ghost values: (hv, qv) pairs for the west / east ghosts of rows a0-1 .. b0 and
 // the north / south ghost rows.
-struct LdsLayout { size_t infil, fixmask, bw, be, wet, red, acc, gsp, total; };
+struct LdsLayout { size_t infil, fixmask, bw, be, wet, red, acc, moff, gsp, total; };
 __host__ __device__ inline size_t gsp_pairs(int R, int nx) { return (size_t)2 * (R + 2) + (size_t)2 * nx; }
 __host__ __device__ inline LdsLayout lds_layout(int R, int nx, bool special) {
     auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
@@ -177,7 +178,8 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nx, bool special) {
     L.red = al(L.wet + R);
     L.acc = al(L.red + 2 * kMaxWaves);
     const size_t acc_n = (size_t)R * 64;
-    L.gsp = al(L.acc + acc_n);
+    L.moff = al(L.acc + acc_n);                       // a monitored cell's plane offset per thread
+    L.gsp = al(L.moff + (size_t)kMaxWaves * 64);
     L.total = al(L.gsp + (special ? 2 * gsp_pairs(R, nx) : 0));
     // the control epilogue stages 2*nx floats and rows of 8 words in the same space
     if (L.total < (size_t)2 * nx) L.total = al((size_t)2 * nx);
@@ -519,6 +521,23 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
     return !(S.flags & (F_DONE | F_PAUSED));
 }
 
+// One FixedSum lane over a row: value(c) for c = c0, c0 + 16, ... < n, added in that
+// order (real.rs:152-160), with the values of four columns fetched before they are added.
+template <class F>
+__device__ __forceinline__ float lane_chain(int c0, int n, F value) {
+    float s = 0.0f;
+    int c = c0;
+    for (; c + 3 * 16 < n; c += 4 * 16) {
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = value(c + 16 * k);
+#pragma unroll
+        for (int k = 0; k < 4; k++) s += v[k];
+    }
+    for (; c < n; c += 16) s += value(c);
+    return s;
+}
+
 // One step of one band of one candidate (kernel.rs:590-727 semantics; the sweep order
 // differs but every face and cell value is the same IEEE expression of the old state).
 // Reads state copy `par`, writes copy par^1 and the row partials / monitors of parity
@@ -546,9 +565,14 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
-    // The band's monitors: positions mon_q0 .. mon_q1-1 of the sorted monitor list.
+    // The band's monitors: positions mon_q0 .. mon_q1-1 of the sorted monitor list; this
+    // thread's first one (qm) has its plane offset staged in LDS for the epilogue.
     const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
-    const ITR_GLOBAL uint32_t* __restrict__ MONCELL = gp(G.mon_cells);
+    const ITR_GLOBAL uint32_t* __restrict__ MONOFF = gp(G.mon_off);
+    const int qm = mon_q0 + tid;
+    const bool has_m = qm < mon_q1;
+    const uint32_t m_off0 = has_m ? MONOFF[qm] : 0u;
+
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
     ITR_GLOBAL float* __restrict__ fixbuf = gp(G.fixbuf + (size_t)local * G.plane);
     const RowAcc* ra_in = G.rowacc + slot(G, par, local) * ny;
@@ -579,7 +603,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const size_t plane = G.plane;
     const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
-    const bool skip_dry = G.prm->skip_dry != 0;
+    const bool skip_dry = G.prm.skip_dry != 0;
     const bool special = SPECIAL && G.special != 0;
     const LdsLayout LL = lds_layout(R, nx, G.special != 0);
     float2* l_gsp = (float2*)(lds + LL.gsp);
@@ -603,7 +627,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             GhostSp gs;
             gs.hv = 0.0f;
             gs.qv = 0.0f;
-            if (k >= 0) gs = special_ghost(G, G.prm->inflow_scale, GHOST[k], k, local, t, inward);
+            if (k >= 0) gs = special_ghost(G, G.prm.inflow_scale, GHOST[k], k, local, t, inward);
             l_gsp[i] = make_float2(gs.hv, gs.qv);
             fw = fw || gs.hv > 0.0f;
         }
@@ -627,7 +651,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
                     bstore(0.0f, rs_ho, cb, so + 2 * planeb);
                 }
             }
-            for (int q = mon_q0 + tid; q < mon_q1; q += nthr) {
+            for (int q = qm; q < mon_q1; q += nthr) {
                 mon_out[q] = mon_in[q];
                 tmax_out[q] = tmax_in[q];
             }
@@ -741,6 +765,9 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         nxt_gn2 = row_at(rs_gn2, a0 + 1);
     }
 
+    uint32_t* l_moff = (uint32_t*)(lds + LL.moff);
+    if (has_m) l_moff[tid] = m_off0;   // read back after the march
+
     int smax_bits = 0;
     bool bad = false;
     for (int j = a0; j < b0; j++) {
@@ -832,32 +859,33 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
 
     // Monitors of the band (kernel.rs:311-321 on the new state: strict >, time of the new
     // state).  Done after the march so that no wave of the march waits on their loads:
-    // every thread takes monitored cells of the band's rows and reads their new depth back.
-    for (int q = mon_q0 + tid; q < mon_q1; q += nthr) {
-        const uint32_t cell = MONCELL[q];
-        const uint32_t r = cell / (uint32_t)nx, c = cell - r * (uint32_t)nx;
-        const float hn = Ho[(size_t)r * P + c];
-        const float old = mon_in[q];
-        const bool up = hn > old;
-        mon_out[q] = up ? hn : old;
-        tmax_out[q] = up ? t_new : tmax_in[q];
+    // each thread reads its first monitored cell's new depth and old maximum now and stores
+    // the update after the row sums, which hide the loads' latency.
+    float m_h = 0.0f, m_old = 0.0f;
+    double m_t = 0.0;
+    if (has_m) {
+        m_h = Ho[l_moff[tid]];
+        m_old = mon_in[qm];
+        m_t = tmax_in[qm];
     }
 
     // Row FixedSums (real.rs:132-175): lane L = padded column & 15, sequential in column
     // order inside a lane, then the fixed pairwise tree.
     for (int t = tid; t < nrows * 16; t += nthr) {
         const int r = t >> 4, L = t & 15;
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         const int j = a0 + r;
+        const int c0 = (L == 0 ? 16 : L) - 1;   // first column of this lane
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float* rin = l_infil + (size_t)r * nx;
+        acc[0] = lane_chain(c0, nx, [&](int c) { return rin[c]; });
         const uint32_t fm = l_fixmask[r];   // waves whose fix values are in fixbuf (others hold +0)
-        for (int pc = (L == 0 ? 16 : L); pc <= nx; pc += 16) {
-            const int c = pc - 1;
-            acc[0] += l_infil[(size_t)r * nx + c];
-            if (fm != 0u) acc[1] += ((fm >> (c / kColsPerWave)) & 1u) ? fixbuf[(size_t)j * P + c] : 0.0f;
-            if (sync) {
-                acc[2] += Ho[(size_t)j * P + c];
-                acc[3] += sbuf[(size_t)j * P + c];
-            }
+        if (fm != 0u)
+            acc[1] = lane_chain(c0, nx, [&](int c) {
+                return ((fm >> (c / kColsPerWave)) & 1u) ? fixbuf[(size_t)j * P + c] : 0.0f;
+            });
+        if (sync) {
+            acc[2] = lane_chain(c0, nx, [&](int c) { return Ho[(size_t)j * P + c]; });
+            acc[3] = lane_chain(c0, nx, [&](int c) { return sbuf[(size_t)j * P + c]; });
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) l_acc[((size_t)r * 16 + L) * 4 + q] = acc[q];
@@ -886,6 +914,18 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         q.pad = 0;
         ra_out[a0 + r] = q;
     }
+    if (has_m) {
+        const bool up = m_h > m_old;
+        mon_out[qm] = up ? m_h : m_old;
+        tmax_out[qm] = up ? t_new : m_t;
+    }
+    for (int q = qm + nthr; q < mon_q1; q += nthr) {   // bands with more monitored cells than threads
+        const float hn = Ho[MONOFF[q]];
+        const float old = mon_in[q];
+        const bool up = hn > old;
+        mon_out[q] = up ? hn : old;
+        tmax_out[q] = up ? t_new : tmax_in[q];
+    }
     if (tid == 0) {
         int m = 0, bd = 0;
         for (int w = 0; w < nwaves; w++) {
@@ -908,7 +948,7 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
                                                 CandCtl& Oc, float* stage, size_t stage_floats, double* shd) {
     const CandCtl& Si = *Sin;
     CandCtl* O = &Oc;
-    const DevParams& p = *G.prm;
+    const DevParams& p = G.prm;
     const int tid = threadIdx.x;
     const int nthr = G.waves * kWave;
     const int local = Si.local;
@@ -1040,9 +1080,9 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     if ((tid >> 6) >= G.waves) return;   // the launch is sized for the widest grid
     const int nthr = G.waves * kWave;
     const int local = tk.y;
-    const int cand = G.cand0 + local;
+    const int cand = tk.w;
     const bool designated = tk.z == 0;
-    const DevParams& p = *G.prm;
+    const DevParams& p = G.prm;
     const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
     if ((Sin->flags & F_DONE) && !designated) return;
@@ -1137,12 +1177,12 @@ __global__ void itr_init_kernel(KArgs a) {
         return;
     }
     CandCtl S = *C;
-    bool ok = prepare_step(*G.prm, G, &S, G.prm->rain_t, G.prm->rain_r);
+    bool ok = prepare_step(G.prm, G, &S, G.prm.rain_t, G.prm.rain_r);
     if (!ok) {
         S.abort_kind = 4;
         S.flags |= F_DONE;
     } else {
-        S.skipped_now = count_skippable<true>(*G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
+        S.skipped_now = count_skippable<true>(G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
     }
     if (threadIdx.x == 0) {
         *C = S;
@@ -1472,7 +1512,6 @@ struct itr_batch {
     void* d_tasks = nullptr;
     void* d_snap = nullptr;         // CandCtl [2][ncand]
     void* d_acc = nullptr;          // StepAcc [4][ncand]
-    void* d_prm = nullptr;
     void* d_status = nullptr;
     uint32_t* h_status = nullptr;   // pinned
     int ncand = 0;
@@ -1492,7 +1531,7 @@ namespace {
 void free_batch_mem(itr_batch* b) {
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
-    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_prm, b->d_status};
+    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_status};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (b->h_status) (void)hipHostFree(b->h_status);
@@ -1559,13 +1598,14 @@ itr_status set_params(const itr_solver_params* p, DevParams& P, int& max_ledger,
 // The sync-callback flag in every grid's parameters: candidates pause at sync points for
 // the host only while a callback is attached.
 itr_status set_callback_flag(itr_batch* b, int has_cb, char* err, size_t errlen) {
-    std::vector<DevParams> h(b->grids.size());
+    std::vector<DevGrid> h(b->grids.size());
     for (size_t g = 0; g < b->grids.size(); g++) {
         b->grids[g].prm.has_cb = has_cb;
-        h[g] = b->grids[g].prm;
+        b->grids[g].d.prm.has_cb = has_cb;
+        h[g] = b->grids[g].d;
     }
     b->prm.has_cb = has_cb;
-    if (!h.empty()) HIP_TRY(hipMemcpy(b->d_prm, h.data(), sizeof(DevParams) * h.size(), hipMemcpyHostToDevice));
+    if (!h.empty()) HIP_TRY(hipMemcpy(b->d_grids, h.data(), sizeof(DevGrid) * h.size(), hipMemcpyHostToDevice));
     return ITR_OK;
 }
 
@@ -1828,6 +1868,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         {
             itr_status ps = set_params(p, GH.prm, GH.max_ledger, GH.allocs, err, errlen);
             if (ps != ITR_OK) return fail(ps);
+            D.prm = GH.prm;
         }
         const int nx = I.nx, ny = I.ny, nc = I.n_cand;
         D.nx = nx;
@@ -1889,6 +1930,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         }
         std::vector<float> wet(plane, 0.0f), gn2(plane, 0.0f);
         GH.mon.assign(I.mon_cells, I.mon_cells + I.n_mon);
+        std::vector<uint32_t> mon_off((size_t)std::max(I.n_mon, 1), 0u);
+        for (int q = 0; q < I.n_mon; q++)
+            mon_off[q] = (I.mon_cells[q] / (uint32_t)nx) * (uint32_t)D.P + I.mon_cells[q] % (uint32_t)nx;
         std::vector<int32_t> mon_row((size_t)ny + 1, 0);   // monitor cells are sorted row-major
         for (int r = 0, q = 0; r <= ny; r++) {
             while (q < I.n_mon && I.mon_cells[q] < (uint32_t)r * (uint32_t)nx) q++;
@@ -1954,7 +1998,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             if (!(0.0 < dur && 0u < n_sync)) C.flags = F_DONE;
             if (p->max_steps < 0) C.flags = F_DONE;
             ctl.push_back(C);
-            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4(g, k, band, 0));
+            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4(g, k, band, total + k));
         }
         total += nc;
         // device buffers
@@ -1962,6 +2006,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             *d_fixbuf;
         double *d_montmax, *d_ledger, *d_base = nullptr;
         int32_t *d_monrow, *d_ghost, *d_gpos = nullptr;
+        uint32_t* d_monoff;
         uint32_t* d_moncells;
         RowAcc* d_rowacc;
         const size_t ncx = (size_t)std::max(nc, 1);
@@ -1978,6 +2023,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         ALLOC(d_wet, plane);
         ALLOC(d_gn2, plane);
         ALLOC(d_monrow, (size_t)ny + 1);
+        ALLOC(d_monoff, mon_off.size());
         ALLOC(d_ghost, (size_t)(2 * nx + 2 * ny));
         ALLOC(d_monmax, 2 * ncx * std::max(I.n_mon, 1));
         ALLOC(d_montmax, 2 * ncx * std::max(I.n_mon, 1));
@@ -2012,7 +2058,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         };
         bool ok = up(d_state, st.data(), st.size() * 4) && up(d_dzx, dzx.data(), dzx.size() * 4) &&
                   up(d_dzy, dzy.data(), dzy.size() * 4) && up(d_wet, wet.data(), plane * 4) &&
-                  up(d_gn2, gn2.data(), plane * 4) && up(d_monrow, mon_row.data(), mon_row.size() * 4) &&
+                  up(d_gn2, gn2.data(), plane * 4) && up(d_monrow, mon_row.data(), mon_row.size() * 4) && up(d_monoff, mon_off.data(), mon_off.size() * 4) &&
                   up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
                   up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
                   up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
@@ -2049,6 +2095,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.wet = d_wet;
         D.gn2 = d_gn2;
         D.mon_row = d_monrow;
+        D.mon_off = d_monoff;
         D.ghost = d_ghost;
         D.mon_max = d_monmax;
         D.mon_tmax = d_montmax;
@@ -2090,7 +2137,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     int4* d_tasks;
     CandCtl* d_snap;
     StepAcc* d_acc;
-    DevParams* d_prm;
     uint32_t* d_status;
     if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
     b->d_grids = d_grids;
@@ -2102,15 +2148,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->d_snap = d_snap;
     if ((s = dev_alloc(misc, &d_acc, 4 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_acc = d_acc;
-    if ((s = dev_alloc(misc, &d_prm, (size_t)n_grids, err, errlen)) != ITR_OK) return fail(s);
-    b->d_prm = d_prm;
-    std::vector<DevParams> hprm(n_grids);
-    for (int g = 0; g < n_grids; g++) {
-        hprm[g] = b->grids[g].prm;
-        dgrids[g].prm = d_prm + g;
-        b->grids[g].d.prm = d_prm + g;
-    }
-    b->prm = hprm[0];
+    b->prm = b->grids[0].prm;
     if ((s = dev_alloc(misc, &d_status, std::max(total, 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_status = d_status;
     if (hipHostMalloc((void**)&b->h_status, sizeof(uint32_t) * std::max(total, 1), hipHostMallocDefault) != hipSuccess) {
@@ -2122,8 +2160,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
               (ctl.empty() || (hipMemcpy(d_snap, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess &&
                                hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
                                          hipMemcpyHostToDevice) == hipSuccess)) &&
-              hipMemset(d_acc, 0, 4 * std::max<size_t>(ctl.size(), 1) * sizeof(StepAcc)) == hipSuccess &&
-              hipMemcpy(d_prm, hprm.data(), sizeof(DevParams) * n_grids, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemset(d_acc, 0, 4 * std::max<size_t>(ctl.size(), 1) * sizeof(StepAcc)) == hipSuccess;
     if (!ok) {
         set_err(err, errlen, "control upload failed");
         return fail(ITR_EDEVICE);
@@ -2684,7 +2721,7 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
         D.R = R;
         D.nbands = (D.ny + R - 1) / R;
         for (int k = 0; k < D.ncand; k++)
-            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4((int)g, k, band, 0));
+            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4((int)g, k, band, D.cand0 + k));
         dg.push_back(D);
     }
     HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));

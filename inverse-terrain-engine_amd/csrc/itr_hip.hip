@@ -1216,6 +1216,7 @@ __global__ void __launch_bounds__(256) itr_math_check_kernel(unsigned long long*
 struct AdjArgs {
     const float* U;        // old state [cand][3][plane] (h, qx, qy)
     float* Uo;             // replay: new state
+    float* pre;            // replay writes / sources adjoint reads: (h1, qx1, qy1) after the flux divergence
     const float* lam;      // adjoint of the new state [cand][3][plane]
     float* lam1;           // adjoint after the sources, before the flux divergence
     float* lamo;           // adjoint of the old state
@@ -1318,25 +1319,25 @@ __global__ void __launch_bounds__(256) itr_replay_kernel(AdjArgs a) {
     Uo[o] = u.h;
     Uo[a.plane + o] = u.qx;
     Uo[2 * a.plane + o] = u.qy;
+    // The state after the flux divergence (numerics.rs:148-153, as cell_update forms it),
+    // kept for the reverse of the sources.
+    const StepK& K = a.K;
+    float* Pr = a.pre + (size_t)cand * a.cstride;
+    Pr[o] = s.C.h - K.dtdx * (fe.m - fw.m) - K.dtdy * (fs.m - fn.m);
+    Pr[a.plane + o] = s.C.qx - K.dtdx * (fe.nl - fw.nr) - K.dtdy * (fs.t - fn.t);
+    Pr[2 * a.plane + o] = s.C.qy - K.dtdx * (fe.t - fw.t) - K.dtdy * (fs.nl - fn.nr);
 }
 
-// Reverse of the sources (numerics.rs:158-195): lam1 = (d new / d (h1, qx1, qy1))^T lam.
+// Reverse of the sources (numerics.rs:158-195): lam1 = (d new / d (h1, qx1, qy1))^T lam,
+// at the post-divergence state the replay kept.
 __global__ void __launch_bounds__(256) itr_adj_sources_kernel(AdjArgs a) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, cand = blockIdx.z;
     if (c >= a.nx) return;
-    const float* U = a.U + (size_t)cand * a.cstride;
-    const Stencil s = stencil(a, U, cand, r, c);
-    const Flux fw = face_signed<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, a.K);
-    const Flux fe = face_signed<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, a.K);
-    const Flux fn = face_signed<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, a.K);
-    const Flux fs = face_signed<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, a.K);
-    const StepK& K = a.K;
-    const float h1 = s.C.h - K.dtdx * (fe.m - fw.m) - K.dtdy * (fs.m - fn.m);
-    const float qx1 = s.C.qx - K.dtdx * (fe.nl - fw.nr) - K.dtdy * (fs.t - fn.t);
-    const float qy1 = s.C.qy - K.dtdx * (fe.t - fw.t) - K.dtdy * (fs.nl - fn.nr);
     const size_t o = (size_t)cand * a.cstride + (size_t)r * a.P + c;
+    const size_t oc = (size_t)r * a.P + c;
+    const float h1 = a.pre[o], qx1 = a.pre[a.plane + o], qy1 = a.pre[2 * a.plane + o];
     const adj::S3<float> b{a.lam[o], a.lam[a.plane + o], a.lam[2 * a.plane + o]};
-    const adj::S3<float> l1 = adj::sources_adj(h1, qx1, qy1, s.wet, s.gn2, adj_consts(K), b);
+    const adj::S3<float> l1 = adj::sources_adj(h1, qx1, qy1, a.wet[oc], a.gn2[oc], adj_consts(a.K), b);
     a.lam1[o] = l1.h;
     a.lam1[a.plane + o] = l1.qx;
     a.lam1[2 * a.plane + o] = l1.qy;
@@ -2495,11 +2496,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         if (e == hipSuccess) mem.push_back(*out);
         return e;
     };
-    float *ckpt, *seg, *lam, *lam1;
+    float *ckpt, *seg, *pre, *lam, *lam1;
     double *gx, *gy;
     unsigned long long* d_mis;
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(nseg + 1), (void**)&ckpt));
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(C + 1), (void**)&seg));
+    ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)C, (void**)&pre));   // post-divergence state of each step
     ADJ_TRY(dalloc(sizeof(float) * state_n * 2, (void**)&lam));
     ADJ_TRY(dalloc(sizeof(float) * state_n, (void**)&lam1));
     ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * plane, (void**)&gx));
@@ -2628,6 +2630,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         for (uint64_t n = n0 + 1; n <= n1; n++) {   // replay; the segment's last step is the check
             A.U = seg + (size_t)(n - 1 - n0) * state_n;
             A.Uo = seg + (size_t)(n - n0) * state_n;
+            A.pre = pre + (size_t)(n - 1 - n0) * state_n;
             A.K = consts(n);
             hipLaunchKernelGGL(itr_replay_kernel, grd, blk, 0, st, A);
         }
@@ -2640,6 +2643,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             for (const Seed& sd : seeds)
                 if (sd.step == n) hipLaunchKernelGGL(itr_adj_seed_kernel, dim3(1), dim3(64), 0, st, lam_n, cstride, sd.cand, sd.cell, sd.v);
             A.U = seg + (size_t)(n - 1 - n0) * state_n;
+            A.pre = pre + (size_t)(n - 1 - n0) * state_n;
             A.lam = lam_n;
             A.lamo = lam + (size_t)(li ^ 1) * state_n;
             A.K = consts(n);

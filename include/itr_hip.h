@@ -160,7 +160,9 @@ void itr_hip_destroy(itr_ctx* ctx);
 const char* itr_hip_device_info(itr_ctx* ctx);
 /* Tuning knobs: "band_rows" (rows per workgroup task), "steps_per_chunk" (step launches
  * per host round trip), "use_graph" (0/1: capture the chunk in a hipGraph), "min_lds"
- * (bytes of LDS reserved per step workgroup, which caps occupancy).  0 = default. */
+ * (bytes of LDS reserved per step workgroup, which caps occupancy), "adjoint_mode"
+ * (reverse sweep of itr_hip_adjoint: 1 band march, 2 per-cell kernels; both give the
+ * same bits).  0 = default (adjoint_mode: band march for large batches). */
 itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value);
 
 /* GpuSolver::run_batch (lib.rs:149-372) / BatchBackend::run (evaluator.rs:134):

@@ -1227,7 +1227,8 @@ struct AdjArgs {
     const float* wet;      // [ny][P]
     const float* gn2;      // [ny][P]
     const int32_t* ghost;  // [2nx + 2ny]
-    int32_t nx, ny, P, pad;
+    int32_t nx, ny, P, R;    // R: rows per band of the reverse march
+    int32_t waves, pad;      // waves per row (62 columns each)
     size_t plane, cstride; // plane = ny*P, cstride = 3*plane
     size_t dzx_stride, dzy_stride;
     StepK K;
@@ -1410,6 +1411,155 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
     a.lamo[2 * a.plane + o] = bC.qy + q.qy;
 }
 
+// Reverse of one step for a band of R rows of one candidate, marched like the forward
+// band (lanes are columns with a halo lane on each side, neighbours by DPP).  Per row:
+// the reverse of the sources, one velocity, the west face (whose
+// left-side bars reach the left lane by DPP as its east face) and the south face
+// (carried to the next row as its north face); each face is differentiated once where
+// itr_adj_faces_kernel does it twice.  Each cell adds its faces' contributions in that
+// kernel's order (west, east, north, south, ghost folds included), so lamo is bitwise
+// the same.  Each interior face's jump gradient is written by the lane that owns its
+// east / south cell.  A row's loads are issued one row ahead.
+struct NbW { float h, qx, qy, wall; };
+struct RevIn { float h, qx, qy, lh, lqx, lqy, ph, pqx, pqy, wet, gn2, dzs, dzw; int code; };
+__global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
+    const int band = blockIdx.x, cand = blockIdx.y;
+    const int nx = a.nx, ny = a.ny;
+    const int a0 = band * a.R, b0 = min(a0 + a.R, ny);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = wave * kColsPerWave - 1 + lane;
+    const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
+    const bool cin = col >= 0 && col < nx;
+    const int cc = min(max(col, 0), nx - 1), xc = min(max(col, 0), nx);
+    const float* U = a.U + (size_t)cand * a.cstride;
+    const float* LAM = a.lam + (size_t)cand * a.cstride;
+    const float* PRE = a.pre + (size_t)cand * a.cstride;
+    float* LO = a.lamo + (size_t)cand * a.cstride;
+    const float* dzx = a.dzx + (size_t)cand * a.dzx_stride;
+    const float* dzy = a.dzy + (size_t)cand * a.dzy_stride;
+    double* gx = a.gx + (size_t)cand * a.dzx_stride;
+    double* gy = a.gy + (size_t)cand * a.dzy_stride;
+    const adj::Consts<float> k = adj_consts(a.K);
+    const float dtdx = a.K.dtdx, dtdy = a.K.dtdy;
+    // Raw inputs of row k1: its old state (the interior neighbour for a ghost), the adjoint
+    // of its new state and its post-divergence state (for the reverse of the sources), and
+    // the jumps of its north face and of its west face.
+    auto load = [&](int k1) -> RevIn {
+        RevIn v;
+        const int rr = min(max(k1, 0), ny - 1);
+        const size_t o = (size_t)rr * a.P + cc;
+        v.h = U[o];
+        v.qx = U[a.plane + o];
+        v.qy = U[2 * a.plane + o];
+        const bool rin = k1 >= 0 && k1 < ny;
+        v.code = 1;
+        if (rin && !cin) v.code = a.ghost[2 * nx + (col < 0 ? 0 : ny) + k1];
+        else if (!rin && cin) v.code = a.ghost[(k1 < 0 ? 0 : nx) + col];
+        v.lh = LAM[o];
+        v.lqx = LAM[a.plane + o];
+        v.lqy = LAM[2 * a.plane + o];
+        v.ph = PRE[o];
+        v.pqx = PRE[a.plane + o];
+        v.pqy = PRE[2 * a.plane + o];
+        const size_t ow = (size_t)rr * a.P + cc;
+        v.wet = a.wet[ow];
+        v.gn2 = a.gn2[ow];
+        v.dzs = dzy[(size_t)min(max(k1, 0), ny) * a.P + cc];
+        v.dzw = dzx[(size_t)min(max(k1, 0), ny - 1) * a.P + xc];
+        return v;
+    };
+    // The ghost rule (boundary.rs:12-87) on a loaded row: wall mirrors the normal
+    // discharge, transmissive copies, corners are 0.
+    auto cell = [&](const RevIn& v, int k1) -> NbW {
+        const bool rin = k1 >= 0 && k1 < ny;
+        if (!rin && !cin) return NbW{0.0f, 0.0f, 0.0f, 0.0f};
+        const bool wall = (!rin || !cin) && v.code == 0;
+        NbW w{v.h, v.qx, v.qy, wall ? 1.0f : 0.0f};
+        if (!cin && wall) w.qx = -w.qx;
+        if (!rin && wall) w.qy = -w.qy;
+        return w;
+    };
+    auto velo = [&](const NbW& c) {
+        const Vel w = cell_velocity(c.h, c.qx, c.qy, a.K);
+        return adj::V<float>{w.u, w.v, w.c};
+    };
+    // Adjoint after the reverse of the sources (lam1); 0 outside the grid (ghosts are not
+    // updated).
+    auto lam1 = [&](const RevIn& v, int k1) -> adj::S3<float> {
+        if (!(k1 >= 0 && k1 < ny && cin)) return adj::S3<float>{0.0f, 0.0f, 0.0f};
+        return adj::sources_adj(v.ph, v.pqx, v.pqy, v.wet, v.gn2, k, adj::S3<float>{v.lh, v.lqx, v.lqy});
+    };
+    auto yface = [&](const NbW& L, const adj::V<float>& vl, const adj::S3<float>& ll, const NbW& R,
+                     const adj::V<float>& vr, const adj::S3<float>& lr, float dz) {
+        const adj::F<float> b{dtdy * (lr.h - ll.h), -dtdy * ll.qy, dtdy * lr.qy, dtdy * (lr.qx - ll.qx)};
+        return adj::face_adj<false>(L.h, vl, R.h, vr, dz, k, b);
+    };
+    const RevIn ip = load(a0 - 1), ic = load(a0);
+    RevIn nxt = load(a0 + 1);
+    NbW Pn = cell(ip, a0 - 1), Cn = cell(ic, a0);
+    adj::V<float> vC = velo(Cn);
+    adj::S3<float> lC = lam1(ic, a0);
+    float dzw = ic.dzw;   // west face jump of the current row
+    adj::FaceBar<float> fN =
+        yface(Pn, velo(Pn), lam1(ip, a0 - 1), Cn, vC, lC, ic.dzs);
+    if (upd && a0 >= 1) gy[(size_t)a0 * a.P + col] += (double)fN.dz;
+    for (int j = a0; j < b0; j++) {
+        const RevIn in = nxt;   // row j+1
+        if (j + 1 < b0) nxt = load(j + 2);
+        const NbW Sn = cell(in, j + 1);
+        const adj::V<float> vS = velo(Sn);
+        const adj::S3<float> lS = lam1(in, j + 1);
+        const adj::FaceBar<float> fS = yface(Cn, vC, lC, Sn, vS, lS, in.dzs);
+        if (upd && j + 1 <= ny - 1 && j + 1 < b0) gy[(size_t)(j + 1) * a.P + col] += (double)fS.dz;
+        // West face: L = the left lane's cell, R = this lane's.
+        const NbW Wn{from_left(Cn.h), from_left(Cn.qx), from_left(Cn.qy), from_left(Cn.wall)};
+        const adj::V<float> vW{from_left(vC.u), from_left(vC.v), from_left(vC.c)};
+        const adj::S3<float> lW{from_left(lC.h), from_left(lC.qx), from_left(lC.qy)};
+        const adj::F<float> bw{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
+        const adj::FaceBar<float> fW = adj::face_adj<true>(Wn.h, vW, Cn.h, vC, dzw, k, bw);
+        if (upd && col >= 1 && col <= nx - 1) gx[(size_t)j * a.P + col] += (double)fW.dz;
+        dzw = in.dzw;   // row j+1's west face
+        // East face: the right lane's west face.
+        adj::FaceBar<float> fE;
+        fE.hl = from_right(fW.hl); fE.ul = from_right(fW.ul); fE.vl = from_right(fW.vl); fE.cl = from_right(fW.cl);
+        fE.hr = from_right(fW.hr); fE.ur = from_right(fW.ur); fE.vr = from_right(fW.vr); fE.cr = from_right(fW.cr);
+        const NbW En{from_right(Cn.h), from_right(Cn.qx), from_right(Cn.qy), from_right(Cn.wall)};
+        adj::S3<float> bC = lC;
+        adj::V<float> bv{0.0f, 0.0f, 0.0f};
+        auto fold = [&](const NbW& g, bool xdir, float bh, const adj::V<float>& gv) {
+            const adj::S3<float> q = adj::vel_adj(g.h, g.qx, g.qy, k, gv);
+            const bool wall = g.wall != 0.0f;
+            bC.h += bh + q.h;
+            bC.qx += (xdir && wall) ? -q.qx : q.qx;
+            bC.qy += (!xdir && wall) ? -q.qy : q.qy;
+        };
+        bC.h += fW.hr;
+        bv.u += fW.ur; bv.v += fW.vr; bv.c += fW.cr;
+        if (col == 0) fold(Wn, true, fW.hl, adj::V<float>{fW.ul, fW.vl, fW.cl});
+        bC.h += fE.hl;
+        bv.u += fE.ul; bv.v += fE.vl; bv.c += fE.cl;
+        if (col == nx - 1) fold(En, true, fE.hr, adj::V<float>{fE.ur, fE.vr, fE.cr});
+        bC.h += fN.hr;
+        bv.u += fN.ur; bv.v += fN.vr; bv.c += fN.cr;
+        if (j == 0) fold(Pn, false, fN.hl, adj::V<float>{fN.ul, fN.vl, fN.cl});
+        bC.h += fS.hl;
+        bv.u += fS.ul; bv.v += fS.vl; bv.c += fS.cl;
+        if (j == ny - 1) fold(Sn, false, fS.hr, adj::V<float>{fS.ur, fS.vr, fS.cr});
+        const adj::S3<float> q = adj::vel_adj(Cn.h, Cn.qx, Cn.qy, k, bv);
+        if (upd) {
+            const size_t o = (size_t)j * a.P + col;
+            LO[o] = bC.h + q.h;
+            LO[a.plane + o] = bC.qx + q.qx;
+            LO[2 * a.plane + o] = bC.qy + q.qy;
+        }
+        Pn = Cn;
+        Cn = Sn;
+        vC = vS;
+        lC = lS;
+        fN = fS;
+    }
+}
+
 // lam[cand][plane h][cell] += v (a peak-depth seed at the step that set the peak).
 __global__ void itr_adj_seed_kernel(float* lam, size_t cstride, int cand, size_t cell, float v) {
     if (threadIdx.x == 0) lam[(size_t)cand * cstride + cell] += v;
@@ -1478,6 +1628,7 @@ struct itr_ctx {
     int steps_per_chunk = 0;
     int use_graph = 1;
     int64_t min_lds = 0;   // reserve at least this much LDS per step workgroup (caps occupancy)
+    int adjoint_mode = 0;  // reverse sweep: 0 by batch size, 1 band march, 2 per-cell kernels
     struct Tuned { int nx, ny, crest, R; };
     std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
 };
@@ -1753,6 +1904,7 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     else if (n == "steps_per_chunk") ctx->steps_per_chunk = (int)value;
     else if (n == "use_graph") ctx->use_graph = value != 0;
     else if (n == "min_lds") ctx->min_lds = value;
+    else if (n == "adjoint_mode" && value >= 0 && value <= 2) ctx->adjoint_mode = (int)value;
     else return ITR_EINVAL;
     return ITR_OK;
 }
@@ -2619,6 +2771,13 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         return K;
     };
     const dim3 grd((unsigned)((nx + 63) / 64), (unsigned)ny, (unsigned)nc), blk(64);
+    // The band march needs enough bands in flight to hide its row-to-row latency; small
+    // batches keep the per-cell faces kernel (one thread per cell).
+    A.R = std::min(16, ny);
+    A.waves = (nx + kColsPerWave - 1) / kColsPerWave;
+    const dim3 bgrd((unsigned)((ny + A.R - 1) / A.R), (unsigned)nc);
+    const bool band_mode = ctx->adjoint_mode == 1 ||
+                           (ctx->adjoint_mode == 0 && (size_t)bgrd.x * bgrd.y * (size_t)A.waves >= 2048);
     hipEvent_t er[4];
     for (auto& e : er) ADJ_TRY(hipEventCreate(&e));
     double replay_ms = 0.0, reverse_ms = 0.0;
@@ -2647,8 +2806,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             A.lam = lam_n;
             A.lamo = lam + (size_t)(li ^ 1) * state_n;
             A.K = consts(n);
-            hipLaunchKernelGGL(itr_adj_sources_kernel, grd, blk, 0, st, A);
-            hipLaunchKernelGGL(itr_adj_faces_kernel, grd, blk, 0, st, A);
+            if (band_mode) {
+                hipLaunchKernelGGL(itr_adj_band_kernel, bgrd, dim3((unsigned)(A.waves * 64)), 0, st, A);
+            } else {
+                hipLaunchKernelGGL(itr_adj_sources_kernel, grd, blk, 0, st, A);
+                hipLaunchKernelGGL(itr_adj_faces_kernel, grd, blk, 0, st, A);
+            }
             li ^= 1;
         }
         ADJ_TRY(hipGetLastError());

@@ -230,6 +230,27 @@ def adjoint_best16(solver, sc, ob, w, res, horizon):
             "cell_steps_per_s_reverse": 16 * base.nx * base.ny * r0["steps"] / (r0["reverse_ms"] * 1e-3)}
 
 
+def prep_leg(solver, sc, tr, w):
+    """Candidate preparation of the same batch two ways, each timed up to a resident batch:
+    host materialize + apply_edit (design.rs:316-385, terrain.rs:460-505) then upload, or
+    the primitives uploaded and materialized on the device.  Both give the same faces
+    (bitwise; tests/test_gpu_parity.py)."""
+    base, geo = w.base, w.base.fine.geo
+    groups_mon = w.monitors
+    t0 = time.perf_counter()
+    edits = [tr.materialize(d, geo, base.nx, base.ny, None, 1.0) for d in w.designs]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    sess = solver.session(w.params, [(base, terrs, groups_mon, None)])
+    t_host = time.perf_counter() - t0
+    sess.close()
+    t0 = time.perf_counter()
+    sess = solver.session_designs(w.params, base, w.designs, groups_mon, None, 1.0)
+    t_dev = time.perf_counter() - t0
+    sess.close()
+    return {"workload": f"{len(w.designs)} candidates x 4 primitives, {base.nx}x{base.ny}",
+            "host_prep_s": t_host, "device_prep_s": t_dev, "speedup": t_host / max(t_dev, 1e-9)}
+
+
 def ladder(solver_cls, sc, steps):
     """The fidelity-ladder batch (256 instances: 45x35 crest, 90x70 and 180x140 signed,
     4:2:1) in one launch per step, with band heights from the autotuner."""
@@ -344,6 +365,9 @@ def main():
     }
     if not a.no_ladder and not a.copy_probe:
         out["ladder"] = ladder(GpuSolver, sc, 200)
+    if not a.copy_probe:
+        from inverse_terrain_engine_amd import terrain as tr
+        out["prep"] = prep_leg(solver, sc, tr, w)
     if adj is not None:
         out["adjoint"] = adj
     if adj16 is not None:

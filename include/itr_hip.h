@@ -40,6 +40,26 @@ typedef enum {
 typedef struct itr_ctx itr_ctx;
 typedef struct itr_batch itr_batch;
 
+/* Candidate designs materialized on the device: DesignSpace::materialize
+ * (crates/itr-core/src/design.rs:316-385: 4x4 sub-sample quadrature of each primitive's
+ * profile, design.rs:108-134, summed in primitive order, clamped, masked) and
+ * PreparedTerrain::apply_edit (crates/itr-hydro/src/terrain.rs:460-505) at level 1 with
+ * signed faces.  The per-primitive scalars that need libm (Primitive::segment,
+ * design.rs:97-105) and the cell windows (design.rs:320-337) come from the host; every
+ * per-cell value is computed on the device with the same f64 operations. */
+typedef struct {
+    const int32_t* prim_start; /* [n_cand + 1] offsets into prim */
+    const double* prim;        /* [n][12]: profile (0 cut, 1 berm/swale/mound), segment x0, y0, x1,
+                                  y1, width, height, cell window c0, r0, c1, r1 (half-open), 0 */
+    const int32_t* bbox;       /* [n_cand][4]: union window c0, r0, w, h of the candidate (w or h 0: no edit) */
+    const double* z_fine;      /* [ny][nx] FineTerrain::z (re-based) */
+    const uint8_t* wall;       /* [ny][nx] 1 = wall cell (level z = z_wall) */
+    const uint8_t* editable;   /* [ny][nx] 1 = editable, or NULL: every cell */
+    double origin_x, origin_y, dx, dy; /* GeoTransform: x = origin_x + col dx, y = origin_y - row dy */
+    double max_abs_dz, z_wall;
+    double* volumes;           /* out: [n_cand][2] cut_m3, fill_m3 of each edit, or NULL */
+} itr_design;
+
 /* SolverParams (crates/itr-hydro/src/solver.rs:26-49).  Rain points are (t [s],
  * rate [m/s]) of the piecewise-linear hyetograph (crates/itr-core/src/scenario.rs:527-589).
  * locked_dt: baseline-locked dt schedule (solver.rs:41-44, 452-455) or NULL; a step
@@ -113,6 +133,11 @@ typedef struct {
      * members of a level run in one batch (the reference runs them on the CPU,
      * evaluator.rs:212-215, 494-560). */
     const itr_solver_params* params;
+    /* Candidates given as designs (itr_design) instead of prepared faces, or NULL.  With a
+     * design, dzx / dzy hold the grid's base faces (one copy: [ny][nx+1] / [ny+1][nx]); each
+     * candidate's faces are built on the device.  Level-1 signed grids without Stage
+     * boundaries or initial_stage only (else ITR_DECLINED: prepare the faces on the host). */
+    const itr_design* design;
 } itr_grid;
 
 /* Per-candidate result: GpuRun (crates/itr-gpu/src/lib.rs:49-60) + BatchRun

@@ -30,6 +30,13 @@ class SolverParamsC(C.Structure):
                 ("has_initial_stage", C.c_int32), ("skip_dry", C.c_int32), ("reserved", C.c_int32)]
 
 
+class DesignC(C.Structure):
+    _fields_ = [("prim_start", _ip), ("prim", _dp), ("bbox", _ip), ("z_fine", _dp),
+                ("wall", C.POINTER(C.c_uint8)), ("editable", C.POINTER(C.c_uint8)),
+                ("origin_x", C.c_double), ("origin_y", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("max_abs_dz", C.c_double), ("z_wall", C.c_double), ("volumes", _dp)]
+
+
 class GridC(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("dy", C.c_double), ("g", C.c_double),
                 ("z_ref", C.c_double), ("crest", C.c_int32), ("wet_count", C.c_int32), ("wet", _fp), ("gn2", _fp),
@@ -37,7 +44,8 @@ class GridC(C.Structure):
                 ("dzx_r", _fp), ("dzy_r", _fp), ("z_level", _dp), ("mon_cells", _up), ("baseline_max", _dp),
                 ("guard_pos", _ip), ("n_guard", C.c_int32), ("reserved", C.c_int32), ("guard_tol", C.c_double),
                 ("ghost_value", _dp), ("ghost_hydro", _ip), ("hydro_start", _ip), ("hydro_t", _dp), ("hydro_q", _dp),
-                ("n_hydro", C.c_int32), ("reserved2", C.c_int32), ("params", C.POINTER(SolverParamsC))]
+                ("n_hydro", C.c_int32), ("reserved2", C.c_int32), ("params", C.POINTER(SolverParamsC)),
+                ("design", C.POINTER(DesignC))]
 
 
 class RunOutC(C.Structure):

@@ -1207,6 +1207,119 @@ __global__ void __launch_bounds__(256) itr_math_check_kernel(unsigned long long*
     if (ns) atomicAdd(&bad[1], ns);
 }
 
+// ------------------------------------------------------------------ design materialization
+// Candidate edits built on the device from their primitives (itr_design): Δz over each
+// candidate's union window (DesignSpace::materialize, design.rs:316-385), the level
+// z = 0 + (z_fine + Δz) of open cells (coarse_cell / fine_z at level 1,
+// terrain.rs:353-373) and every signed face jump f32(z_R - z_L) (terrain.rs:460-505).
+struct DesignDev {
+    const double* prim;        // [n][12]
+    const int32_t* prim_start; // [ncand + 1]
+    const int32_t* bbox;       // [ncand][4] c0, r0, w, h
+    const int64_t* dz_off;     // [ncand] offset of the candidate's window in dzbuf
+    double* dzbuf;
+    const double* z;           // [ny][nx]
+    const uint8_t* wall;
+    const uint8_t* editable;   // or nullptr
+    double ox, oy, dx, dy, max_dz, z_wall, area;
+    int32_t nx, ny, P, ncand;
+    float* dzx;                // [ncand][plane]
+    float* dzy;                // [ncand][plane_y]
+    size_t plane, plane_y;
+    double* volumes;           // [ncand][2]
+};
+
+// Primitive::profile (design.rs:108-134): value in [0, 1], exactly 0 outside the support.
+__device__ double prim_profile(const double* pr, double x, double y) {
+    const double x0 = pr[1], y0 = pr[2], x1 = pr[3], y1 = pr[4];
+    const double vx = x1 - x0, vy = y1 - y0;
+    const double l2 = vx * vx + vy * vy;
+    double t = 0.0;
+    if (l2 > 0.0) {
+        t = ((x - x0) * vx + (y - y0) * vy) / l2;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    const double px = x0 + t * vx - x, py = y0 + t * vy - y;
+    const double d = __builtin_sqrt(px * px + py * py);
+    const double s = d / (0.5 * pr[5]);
+    if (s >= 1.0) return 0.0;
+    if (pr[0] == 0.0) {   // cut: flat bottom, quartic shoulder
+        if (s <= 0.5) return 1.0;
+        const double u = (s - 0.5) / 0.5;
+        const double w = 1.0 - u * u;
+        return w * w;
+    }
+    const double w = 1.0 - s * s;
+    return w * w;
+}
+
+// Δz of every cell of each candidate's window: 4x4 sub-sample mean of each primitive's
+// profile times its height, summed in primitive order (a primitive adds exactly 0 outside
+// its own cell window, so it is skipped there), clamped, 0 where not editable.
+__global__ void __launch_bounds__(256) itr_materialize_kernel(DesignDev d) {
+    const int cand = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c0 = d.bbox[4 * cand], r0 = d.bbox[4 * cand + 1], w = d.bbox[4 * cand + 2], h = d.bbox[4 * cand + 3];
+    if (i >= w * h) return;
+    const int col = c0 + i % w, row = r0 + i / w;
+    double v = 0.0;
+    if (!d.editable || d.editable[(size_t)row * d.nx + col]) {
+        double acc = 0.0;
+        for (int q = d.prim_start[cand]; q < d.prim_start[cand + 1]; q++) {
+            const double* pr = d.prim + (size_t)12 * q;
+            if (pr[6] == 0.0) continue;
+            if (col < (int)pr[7] || row < (int)pr[8] || col >= (int)pr[9] || row >= (int)pr[10]) continue;
+            double sum = 0.0;
+            for (int sy = 0; sy < 4; sy++)
+                for (int sx = 0; sx < 4; sx++) {
+                    const double x = d.ox + ((double)col + ((double)sx + 0.5) / 4.0) * d.dx;
+                    const double y = d.oy - ((double)row + ((double)sy + 0.5) / 4.0) * d.dy;
+                    sum += prim_profile(pr, x, y);
+                }
+            acc += pr[6] * sum * (1.0 / 16.0);
+        }
+        v = acc < -d.max_dz ? -d.max_dz : (acc > d.max_dz ? d.max_dz : acc);
+    }
+    d.dzbuf[d.dz_off[cand] + i] = v;
+}
+
+// Level z of cell (c, r) of a candidate with its edit.
+__device__ __forceinline__ double design_z(const DesignDev& d, int cand, int c, int r) {
+    const size_t k = (size_t)r * d.nx + c;
+    if (d.wall[k]) return d.z_wall;
+    const int c0 = d.bbox[4 * cand], r0 = d.bbox[4 * cand + 1], w = d.bbox[4 * cand + 2], h = d.bbox[4 * cand + 3];
+    double e = 0.0;
+    if (c >= c0 && c < c0 + w && r >= r0 && r < r0 + h) e = d.dzbuf[d.dz_off[cand] + (size_t)(r - r0) * w + (c - c0)];
+    return 0.0 + (d.z[k] + e);
+}
+
+// Every face jump of every candidate (boundary faces stay 0): x-face i of row r joins
+// cells i-1 and i; y-face f of column c joins rows f-1 and f.
+__global__ void __launch_bounds__(256) itr_design_faces_kernel(DesignDev d) {
+    const int cand = blockIdx.z, r = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > d.nx) return;
+    if (r < d.ny && i >= 1 && i <= d.nx - 1)
+        d.dzx[(size_t)cand * d.plane + (size_t)r * d.P + i] = (float)(design_z(d, cand, i, r) - design_z(d, cand, i - 1, r));
+    if (r >= 1 && r <= d.ny - 1 && i < d.nx)
+        d.dzy[(size_t)cand * d.plane_y + (size_t)r * d.P + i] = (float)(design_z(d, cand, i, r) - design_z(d, cand, i, r - 1));
+}
+
+// Cut and fill volumes of each edit: row-major over the window, as design.rs:373-380 sums.
+__global__ void itr_design_volumes_kernel(DesignDev d) {
+    const int cand = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cand >= d.ncand) return;
+    const int64_t n = (int64_t)d.bbox[4 * cand + 2] * d.bbox[4 * cand + 3];
+    const double* v = d.dzbuf + d.dz_off[cand];
+    double cut = 0.0, fill = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        if (v[i] > 0.0) fill += v[i] * d.area;
+        else cut -= v[i] * d.area;
+    }
+    d.volumes[2 * cand] = cut;
+    d.volumes[2 * cand + 1] = fill;
+}
+
 // ------------------------------------------------------------------ discrete adjoint
 // Per-cell kernels of the reverse sweep (one thread per cell, grid x = column blocks,
 // y = rows, z = candidates).  The forward replay recomputes a step cell by cell with
@@ -1699,6 +1812,100 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
     return ITR_OK;
 }
 
+// Builds the candidates' faces of a grid given as designs (itr_design) into d_dzx /
+// d_dzy (zero-initialised: boundary faces stay 0) and returns the edits' volumes.
+itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D, float* d_dzx, float* d_dzy, char* err,
+                              size_t errlen) {
+    const itr_design& Ds = *I.design;
+    const int nx = I.nx, ny = I.ny, nc = I.n_cand;
+    if (nc <= 0) return ITR_OK;
+    const int np = Ds.prim_start[nc];
+    std::vector<int64_t> off(nc);
+    int64_t tot = 0, maxwh = 0;
+    for (int k = 0; k < nc; k++) {
+        off[k] = tot;
+        const int64_t wh = (int64_t)Ds.bbox[4 * k + 2] * Ds.bbox[4 * k + 3];
+        tot += wh;
+        maxwh = std::max(maxwh, wh);
+    }
+    std::vector<void*> tmp;
+    auto release = [&]() {
+        for (void* q : tmp) (void)hipFree(q);
+    };
+    DesignDev dd{};
+    double *prim = nullptr, *dzbuf = nullptr, *z = nullptr, *vol = nullptr;
+    int32_t *ps = nullptr, *bb = nullptr;
+    int64_t* doff = nullptr;
+    uint8_t *wall = nullptr, *edit = nullptr;
+    itr_status s;
+#define DTRY(expr)        \
+    if ((s = (expr)) != ITR_OK) { \
+        release();        \
+        return s;         \
+    }
+    DTRY(dev_alloc(tmp, &prim, (size_t)std::max(np, 1) * 12, err, errlen));
+    DTRY(dev_alloc(tmp, &ps, (size_t)nc + 1, err, errlen));
+    DTRY(dev_alloc(tmp, &bb, (size_t)nc * 4, err, errlen));
+    DTRY(dev_alloc(tmp, &doff, (size_t)nc, err, errlen));
+    DTRY(dev_alloc(tmp, &dzbuf, (size_t)std::max<int64_t>(tot, 1), err, errlen));
+    DTRY(dev_alloc(tmp, &z, (size_t)nx * ny, err, errlen));
+    DTRY(dev_alloc(tmp, &wall, (size_t)nx * ny, err, errlen));
+    if (Ds.editable) DTRY(dev_alloc(tmp, &edit, (size_t)nx * ny, err, errlen));
+    DTRY(dev_alloc(tmp, &vol, (size_t)nc * 2, err, errlen));
+#undef DTRY
+    const hipStream_t st = ctx->stream;
+    bool ok = (np == 0 || hipMemcpy(prim, Ds.prim, sizeof(double) * 12 * np, hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemcpy(ps, Ds.prim_start, sizeof(int32_t) * (nc + 1), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(bb, Ds.bbox, sizeof(int32_t) * 4 * nc, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(doff, off.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(z, Ds.z_fine, sizeof(double) * nx * ny, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(wall, Ds.wall, (size_t)nx * ny, hipMemcpyHostToDevice) == hipSuccess &&
+              (!Ds.editable || hipMemcpy(edit, Ds.editable, (size_t)nx * ny, hipMemcpyHostToDevice) == hipSuccess);
+    if (!ok) {
+        release();
+        set_err(err, errlen, "design upload failed");
+        return ITR_EDEVICE;
+    }
+    dd.prim = prim;
+    dd.prim_start = ps;
+    dd.bbox = bb;
+    dd.dz_off = doff;
+    dd.dzbuf = dzbuf;
+    dd.z = z;
+    dd.wall = wall;
+    dd.editable = edit;
+    dd.ox = Ds.origin_x;
+    dd.oy = Ds.origin_y;
+    dd.dx = Ds.dx;
+    dd.dy = Ds.dy;
+    dd.max_dz = Ds.max_abs_dz;
+    dd.z_wall = Ds.z_wall;
+    dd.area = Ds.dx * Ds.dy;
+    dd.nx = nx;
+    dd.ny = ny;
+    dd.P = D.P;
+    dd.ncand = nc;
+    dd.dzx = d_dzx;
+    dd.dzy = d_dzy;
+    dd.plane = D.plane;
+    dd.plane_y = D.plane_y;
+    dd.volumes = vol;
+    if (maxwh > 0)
+        hipLaunchKernelGGL(itr_materialize_kernel, dim3((unsigned)((maxwh + 255) / 256), (unsigned)nc), dim3(256), 0, st, dd);
+    hipLaunchKernelGGL(itr_design_faces_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
+                       dim3(256), 0, st, dd);
+    hipLaunchKernelGGL(itr_design_volumes_kernel, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, st, dd);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && Ds.volumes) e = hipMemcpy(Ds.volumes, vol, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost);
+    release();
+    if (e != hipSuccess) {
+        set_err(err, errlen, "design materialization failed: %s", hipGetErrorString(e));
+        return ITR_EDEVICE;
+    }
+    return ITR_OK;
+}
+
 // Device parameters of one grid (SolverParams, solver.rs:26-49, with the f32 constants of
 // Solver::consts, solver.rs:269-285), the hyetograph and locked schedule uploaded into
 // `allocs`.  max_ledger: ledger rows per candidate (one per sync point plus one).
@@ -1936,7 +2143,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             set_err(err, errlen, "grid %d: invalid solver parameters", g);
             return ITR_EINVAL;
         }
-        if (G.nx <= 0 || G.ny <= 0 || G.n_cand < 0 || !G.wet || !G.gn2 || !G.ghost || !G.dzx || !G.dzy) {
+        if (G.nx <= 0 || G.ny <= 0 || G.n_cand < 0 || !G.wet || !G.gn2 || !G.ghost ||
+            (!G.design && (!G.dzx || !G.dzy))) {
             set_err(err, errlen, "grid %d: missing arrays or empty shape", g);
             return ITR_EINVAL;
         }
@@ -1989,6 +2197,28 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     set_err(err, errlen, "grid %d: guard position out of range", g);
                     return ITR_EINVAL;
                 }
+        if (G.design) {
+            const itr_design& Ds = *G.design;
+            bool stage = false;
+            for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++) stage = stage || G.ghost[i] == 2;
+            if (G.crest || q->has_initial_stage || stage) {
+                set_err(err, errlen, "grid %d: device designs need a level-1 signed grid without stage levels", g);
+                return ITR_DECLINED;
+            }
+            if (!Ds.prim_start || !Ds.bbox || !Ds.z_fine || !Ds.wall || G.n_cand > 65535 ||
+                (G.n_cand > 0 && Ds.prim_start[G.n_cand] > 0 && !Ds.prim)) {
+                set_err(err, errlen, "grid %d: incomplete design arrays", g);
+                return ITR_EINVAL;
+            }
+            for (int c = 0; c < G.n_cand; c++) {
+                const int32_t* bb = Ds.bbox + 4 * c;
+                if (bb[0] < 0 || bb[1] < 0 || bb[2] < 0 || bb[3] < 0 || bb[0] + bb[2] > G.nx || bb[1] + bb[3] > G.ny ||
+                    Ds.prim_start[c + 1] < Ds.prim_start[c]) {
+                    set_err(err, errlen, "grid %d: design window of candidate %d outside the grid", g, c);
+                    return ITR_EINVAL;
+                }
+            }
+        }
         if (q->has_initial_stage && !G.z_level) {
             set_err(err, errlen, "grid %d: initial_stage needs z_level", g);
             return ITR_EINVAL;
@@ -2072,15 +2302,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                 }
         }
         const size_t plane = D.plane;
-        // host staging of the pitched arrays
-        std::vector<float> st((size_t)std::max(nc, 1) * 6 * plane, 0.0f);
+        // Host staging of the shared planes only: faces and initial states go to the device
+        // as pitched copies from the caller's arrays (and zero fills).
         const size_t plane_y = D.plane_y;
-        std::vector<float> dzx((size_t)std::max(nc, 1) * plane, 0.0f), dzy((size_t)std::max(nc, 1) * plane_y, 0.0f);
-        std::vector<float> dzx2, dzy2;
-        if (I.crest) {
-            dzx2.assign(dzx.size(), 0.0f);
-            dzy2.assign(dzy.size(), 0.0f);
-        }
         std::vector<float> wet(plane, 0.0f), gn2(plane, 0.0f);
         GH.mon.assign(I.mon_cells, I.mon_cells + I.n_mon);
         std::vector<uint32_t> mon_off((size_t)std::max(I.n_mon, 1), 0u);
@@ -2098,24 +2322,16 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             }
         std::vector<RowAcc> rowacc((size_t)std::max(nc, 1) * ny);   // parity 0 half
         std::memset(rowacc.data(), 0, rowacc.size() * sizeof(RowAcc));
-        for (int k = 0; k < nc; k++) {
-            const float* sx = I.dzx + (size_t)k * ny * (nx + 1);
-            const float* sy = I.dzy + (size_t)k * (ny + 1) * nx;
-            for (int r = 0; r < ny; r++)
-                for (int c = 1; c < nx; c++) dzx[(size_t)k * plane + (size_t)r * D.P + c] = sx[(size_t)r * (nx + 1) + c];
-            for (int r = 1; r < ny; r++)
-                for (int c = 0; c < nx; c++) dzy[(size_t)k * plane_y + (size_t)r * D.P + c] = sy[(size_t)r * nx + c];
-            if (I.crest) {
-                const float* sx2 = I.dzx_r + (size_t)k * ny * (nx + 1);
-                const float* sy2 = I.dzy_r + (size_t)k * (ny + 1) * nx;
-                for (int r = 0; r < ny; r++)
-                    for (int c = 1; c < nx; c++)
-                        dzx2[(size_t)k * plane + (size_t)r * D.P + c] = sx2[(size_t)r * (nx + 1) + c];
-                for (int r = 1; r < ny; r++)
-                    for (int c = 0; c < nx; c++) dzy2[(size_t)k * plane_y + (size_t)r * D.P + c] = sy2[(size_t)r * nx + c];
-            }
-            // init_state (solver.rs:287-312) and the initial CFL rate (solver.rs:315-332)
-            std::vector<float> h0((size_t)nx * ny, 0.0f);
+        // init_state (solver.rs:287-312) and the initial CFL rate (solver.rs:315-332): one
+        // initial depth field per candidate with initial_stage (it depends on the levels),
+        // else one for all.
+        const int n_h0 = p->has_initial_stage ? std::max(nc, 1) : 1;
+        std::vector<std::vector<float>> h0s(n_h0);
+        std::vector<double> s0s(n_h0, 0.0), v0s(n_h0, 0.0);
+        std::vector<uint32_t> wet_rows((size_t)n_h0 * ny, 0u);
+        for (int k = 0; k < n_h0; k++) {
+            std::vector<float>& h0 = h0s[k];
+            h0.assign((size_t)nx * ny, 0.0f);
             double s0 = 0.0;
             for (int r = 0; r < ny; r++)
                 for (int c = 0; c < nx; c++) {
@@ -2126,7 +2342,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     float hv = (float)std::fmax(d, 0.0);
                     if (hv == 0.0f) hv = 0.0f;
                     h0[q] = hv;
-                    st[(size_t)k * 6 * plane + (size_t)r * D.P + c] = hv;
                     const double hd = (double)hv;
                     if (hd <= p->h_dry) continue;
                     const double cc = std::sqrt(I.g * hd);
@@ -2136,13 +2351,20 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             for (int r = 0; r < ny; r++) {
                 uint32_t w = 0;
                 for (int c = 0; c < nx; c++) w |= h0[(size_t)r * nx + c] > 0.0f ? 1u : 0u;
-                rowacc[(size_t)k * ny + r].wet = w;
+                wet_rows[(size_t)k * ny + r] = w;
             }
+            s0s[k] = s0;
+            v0s[k] = host_volume(h0, nx, ny, D.area);
+        }
+        for (int k = 0; k < nc; k++) {
+            const int hk = p->has_initial_stage ? k : 0;
+            const double s0 = s0s[hk];
+            for (int r = 0; r < ny; r++) rowacc[(size_t)k * ny + r].wet = wet_rows[(size_t)hk * ny + r];
             CandCtl C;
             std::memset(&C, 0, sizeof C);
             C.grid = g;
             C.local = k;
-            C.v0 = host_volume(h0, nx, ny, D.area);
+            C.v0 = v0s[hk];
             C.t = 0.0;
             C.s_cur = s0;
             C.dt_cfl = s0 > 0.0 ? std::fmin(p->cfl / s0, p->dt_max_s) : p->dt_max_s;
@@ -2209,13 +2431,44 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         auto up = [&](void* dst, const void* src, size_t bytes) -> bool {
             return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
         };
-        bool ok = up(d_state, st.data(), st.size() * 4) && up(d_dzx, dzx.data(), dzx.size() * 4) &&
-                  up(d_dzy, dzy.data(), dzy.size() * 4) && up(d_wet, wet.data(), plane * 4) &&
+        // Pitched planes go through one reusable host plane: rows of P floats holding
+        // columns c0 .. c1-1 of the caller's [rows][src_w] block (everything else 0).
+        std::vector<float> pit(std::max(plane, plane_y));
+        auto up_plane = [&](float* dst, const float* src, int src_w, int c0, int c1, int r0, int r1, int rows) -> bool {
+            float* q = pit.data();
+            for (int r = 0; r < rows; r++) {
+                float* row = q + (size_t)r * D.P;
+                if (r < r0 || r >= r1) {
+                    std::fill(row, row + D.P, 0.0f);
+                    continue;
+                }
+                std::fill(row, row + c0, 0.0f);
+                std::memcpy(row + c0, src + (size_t)r * src_w + c0, sizeof(float) * (size_t)std::max(c1 - c0, 0));
+                std::fill(row + std::max(c1, c0), row + D.P, 0.0f);
+            }
+            return hipMemcpy(dst, q, sizeof(float) * (size_t)rows * D.P, hipMemcpyHostToDevice) == hipSuccess;
+        };
+        bool ok = hipMemset(d_state, 0, ncx * 6 * plane * 4) == hipSuccess;
+        if (I.design)
+            ok = ok && hipMemset(d_dzx, 0, ncx * plane * 4) == hipSuccess &&
+                 hipMemset(d_dzy, 0, ncx * plane_y * 4) == hipSuccess;
+        for (int k = 0; ok && k < nc; k++) {
+            // initial depth (parity 0, h plane): uploaded once when every candidate shares it
+            if (p->has_initial_stage || k == 0) ok = up_plane(d_state + (size_t)k * 6 * plane, h0s[p->has_initial_stage ? k : 0].data(), nx, 0, nx, 0, ny, ny);
+            else ok = hipMemcpy(d_state + (size_t)k * 6 * plane, d_state, sizeof(float) * plane, hipMemcpyDeviceToDevice) == hipSuccess;
+            if (I.design) continue;   // faces built on the device
+            // interior faces only: x-faces 1 .. nx-1, y-face rows 1 .. ny-1 (boundary faces 0)
+            ok = ok && up_plane(d_dzx + (size_t)k * plane, I.dzx + (size_t)k * ny * (nx + 1), nx + 1, 1, nx, 0, ny, ny) &&
+                 up_plane(d_dzy + (size_t)k * plane_y, I.dzy + (size_t)k * (ny + 1) * nx, nx, 0, nx, 1, ny, ny + 1);
+            if (I.crest)
+                ok = ok && up_plane(d_dzx2 + (size_t)k * plane, I.dzx_r + (size_t)k * ny * (nx + 1), nx + 1, 1, nx, 0, ny, ny) &&
+                     up_plane(d_dzy2 + (size_t)k * plane_y, I.dzy_r + (size_t)k * (ny + 1) * nx, nx, 0, nx, 1, ny, ny + 1);
+        }
+        ok = ok && up(d_wet, wet.data(), plane * 4) &&
                   up(d_gn2, gn2.data(), plane * 4) && up(d_monrow, mon_row.data(), mon_row.size() * 4) && up(d_monoff, mon_off.data(), mon_off.size() * 4) &&
                   up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
                   up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
                   up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
-        if (I.crest) ok = ok && up(d_dzx2, dzx2.data(), dzx2.size() * 4) && up(d_dzy2, dzy2.data(), dzy2.size() * 4);
         if (I.baseline_max)
             ok = ok && up(d_base, I.baseline_max, sizeof(double) * I.n_mon) &&
                  up(d_gpos, I.guard_pos, sizeof(int32_t) * I.n_guard);
@@ -2239,6 +2492,10 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         if (!ok) {
             set_err(err, errlen, "upload of grid %d failed", g);
             return fail(ITR_EDEVICE);
+        }
+        if (I.design) {
+            itr_status ms = materialize_design(ctx, I, D, d_dzx, d_dzy, err, errlen);
+            if (ms != ITR_OK) return fail(ms);
         }
         D.state = d_state;
         D.dzx = d_dzx;

@@ -136,6 +136,43 @@ def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need
     return g
 
 
+def _design_grid_c(base: TerrainBase, designs, monitors, keep: _Keep, editable=None, max_abs_dz=1e9, guard=None):
+    """A grid whose candidates are primitive lists, materialized on the device
+    (itr_design): level-1 signed grids only."""
+    from .terrain import design_rows
+    if base.level != 1 or base.crest:
+        raise Declined("device designs need a level-1 signed grid")
+    nx, ny = base.nx, base.ny
+    fine = base.fine
+    rows, start, boxes = [], [0], []
+    for prims in designs:
+        r, box = design_rows(prims, fine.geo, nx, ny)
+        rows += r
+        start.append(len(rows))
+        boxes.append(box)
+    d = _abi.DesignC()
+    d.prim_start = keep.arr(start, np.int32).ctypes.data_as(_abi._ip)
+    d.prim = keep.arr(np.asarray(rows, np.float64).reshape(-1, 12) if rows else np.zeros((1, 12)),
+                      np.float64).ctypes.data_as(_abi._dp)
+    d.bbox = keep.arr(np.asarray(boxes, np.int32).reshape(-1, 4) if boxes else np.zeros((1, 4)),
+                      np.int32).ctypes.data_as(_abi._ip)
+    d.z_fine = keep.arr(fine.z, np.float64).ctypes.data_as(_abi._dp)
+    d.wall = keep.arr(fine.wall, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8))
+    if editable is not None:
+        d.editable = keep.arr(np.asarray(editable, bool), np.uint8).ctypes.data_as(C.POINTER(C.c_uint8))
+    g = fine.geo
+    d.origin_x, d.origin_y, d.dx, d.dy = g.origin_x, g.origin_y, g.dx, g.dy
+    d.max_abs_dz, d.z_wall = float(max_abs_dz), float(fine.z_wall)
+    vol = keep.arr(np.zeros((max(len(designs), 1), 2)), np.float64)
+    d.volumes = vol.ctypes.data_as(_abi._dp)
+    keep.append(d)
+    grid = _grid_c(base, [], monitors, keep, guard)
+    grid.n_cand = len(designs)
+    grid.dzx = grid.dzy = None
+    grid.design = C.pointer(d)
+    return grid, vol
+
+
 class GpuSolver:
     """GpuSolver (lib.rs:62-373) on one MI355X through the C ABI."""
 
@@ -279,6 +316,31 @@ class GpuSolver:
         n = len(terrains)
         return [flat[m * n:(m + 1) * n] for m in range(len(members))]
 
+    def run_designs(self, p: SolverParams, base: TerrainBase, designs, monitors, editable=None, max_abs_dz=1e9,
+                    guard=None, want_state=True):
+        """Candidates given as primitive lists: DesignSpace::materialize and
+        PreparedTerrain::apply_edit (design.rs:316-385, terrain.rs:460-505) run on the
+        device, then the batch.  Returns (runs, volumes[n, 2] = cut_m3, fill_m3)."""
+        if len(designs) == 0:
+            return [], np.zeros((0, 2))
+        s = self.session_designs(p, base, designs, monitors, editable, max_abs_dz, guard)
+        try:
+            s.run()
+            runs = s.results(want_state)
+        finally:
+            s.close()
+        return runs, s.volumes
+
+    def session_designs(self, p: SolverParams, base: TerrainBase, designs, monitors, editable=None, max_abs_dz=1e9,
+                        guard=None):
+        """A resident batch of candidates given as primitive lists, materialized on the
+        device; ``session.volumes`` holds each edit's (cut_m3, fill_m3)."""
+        keep = _Keep()
+        grid, vol = _design_grid_c(base, designs, monitors, keep, editable, max_abs_dz, guard)
+        s = BatchSession(self, p, [(base, [None] * len(designs), monitors, guard)], None, grids=[grid], keep=keep)
+        s.volumes = vol[:len(designs)].copy()
+        return s
+
     def full_run(self, p: SolverParams, terrain):
         """Full-domain run for the result rasters (crates/itr-cli/src/run.rs:73-93): every
         cell monitored; returns (run, peak, tpeak) with peak NaN on walls and tpeak NaN on
@@ -307,15 +369,20 @@ class BatchSession:
     """A resident batch.  ``group_params``: optional per-group SolverParams (None = ``p``),
     e.g. one group per ensemble member."""
 
-    def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None, group_params=None):
+    def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None, group_params=None, grids=None,
+                 keep=None):
         self.solver = solver
         L = solver._L
         self._L = L
-        self._keep = _Keep()
+        self._keep = keep if keep is not None else _Keep()
         self.groups = groups
         q = _params_c(p, self._keep)
+        prepared = grids
         grids = (_abi.GridC * len(groups))()
         for i, (base, terrs, mons, guard) in enumerate(groups):
+            if prepared is not None and prepared[i] is not None:
+                grids[i] = prepared[i]
+                continue
             gp = group_params[i] if group_params is not None else None
             eff = gp if gp is not None else p
             grids[i] = _grid_c(base, terrs, mons, self._keep, guard, eff.initial_stage is not None)

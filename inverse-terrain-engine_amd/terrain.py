@@ -388,6 +388,36 @@ class Primitive:
         return min(x0, x1) - r, min(y0, y1) - r, max(x0, x1) + r, max(y0, y1) + r
 
 
+def prim_window(p, geo: GeoTransform, nx, ny):
+    """Cells (c0, r0, c1, r1), half-open, that can hold the primitive's support
+    (design.rs:322-331): its world bbox snapped outward to whole cells, inside the grid."""
+    x0, y0, x1, y1 = p.bbox()
+    fc0, fr1 = geo.to_grid(x0, y0)
+    fc1, fr0 = geo.to_grid(x1, y1)
+    a = min(int(max(math.floor(fc0), 0.0)), nx)
+    b = min(int(max(math.floor(fr0), 0.0)), ny)
+    c = min(int(max(math.ceil(fc1), 0.0)), nx)
+    d = min(int(max(math.ceil(fr1), 0.0)), ny)
+    return a, b, c, d
+
+
+def design_rows(prims, geo: GeoTransform, nx, ny):
+    """One candidate's primitives as the device materializes them: rows of (profile kind,
+    segment x0 y0 x1 y1, width, height, cell window c0 r0 c1 r1, 0), and the candidate's
+    union window (c0, r0, w, h) of materialize's first loop (0 0 0 0: no edit)."""
+    rows = []
+    c0 = r0 = 1 << 62
+    c1 = r1 = 0
+    for p in prims:
+        (x0, y0), (x1, y1) = p.segment()
+        a, b, c, d = prim_window(p, geo, nx, ny)
+        rows.append([0.0 if p.kind == "cut" else 1.0, x0, y0, x1, y1, p.width, p.height, a, b, c, d, 0.0])
+        if p.height != 0.0 and a < c and b < d:
+            c0, r0, c1, r1 = min(c0, a), min(r0, b), max(c1, c), max(r1, d)
+    box = [c0, r0, c1 - c0, r1 - r0] if (c0 < c1 and r0 < r1) else [0, 0, 0, 0]
+    return rows, box
+
+
 def materialize(prims, geo: GeoTransform, nx, ny, editable=None, max_abs_dz=1e9) -> TerrainEdit:
     """DesignSpace::materialize (design.rs:318-385): 4x4 sub-sample quadrature per cell,
     summed over primitives in order, masked, clamped; cut/fill volumes."""
@@ -396,13 +426,7 @@ def materialize(prims, geo: GeoTransform, nx, ny, editable=None, max_abs_dz=1e9)
     for p in prims:
         if p.height == 0.0:
             continue
-        x0, y0, x1, y1 = p.bbox()
-        fc0, fr1 = geo.to_grid(x0, y0)
-        fc1, fr0 = geo.to_grid(x1, y1)
-        a = min(int(max(math.floor(fc0), 0.0)), nx)
-        b = min(int(max(math.floor(fr0), 0.0)), ny)
-        c = min(int(max(math.ceil(fc1), 0.0)), nx)
-        d = min(int(max(math.ceil(fr1), 0.0)), ny)
+        a, b, c, d = prim_window(p, geo, nx, ny)
         if a < c and b < d:
             c0, r0, c1, r1 = min(c0, a), min(r0, b), max(c1, c), max(r1, d)
     if c0 >= c1 or r0 >= r1:
@@ -416,13 +440,8 @@ def materialize(prims, geo: GeoTransform, nx, ny, editable=None, max_abs_dz=1e9)
             continue
         # The profile is exactly 0 outside the primitive's own bbox, and adding
         # height * 0 * inv (= +-0) leaves acc unchanged, so only that window is summed.
-        x0, y0, x1, y1 = p.bbox()
-        fc0, fr1 = geo.to_grid(x0, y0)
-        fc1, fr0 = geo.to_grid(x1, y1)
-        pa = max(min(int(max(math.floor(fc0), 0.0)), nx), c0)
-        pb = max(min(int(max(math.floor(fr0), 0.0)), ny), r0)
-        pc = min(min(int(max(math.ceil(fc1), 0.0)), nx), c1)
-        pd = min(min(int(max(math.ceil(fr1), 0.0)), ny), r1)
+        wa, wb, wc, wd = prim_window(p, geo, nx, ny)
+        pa, pb, pc, pd = max(wa, c0), max(wb, r0), min(wc, c1), min(wd, r1)
         if pa >= pc or pb >= pd:
             continue
         cols = np.arange(pa, pc, dtype=np.float64)[None, :]

@@ -7,7 +7,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import assert_run_equal, oracle_params, oracle_terrain, po
+from conftest import assert_run_equal, bits64, oracle_params, oracle_terrain, po
 from inverse_terrain_engine_amd import scenario as sc
 from inverse_terrain_engine_amd import terrain as tr
 
@@ -388,3 +388,51 @@ def test_full_domain_rasters(solver):
     pos = (~wall) & (o["monitor_max"].reshape(45, 70) > 0.0)
     assert pos.any() and np.array_equal(tpeak[pos], o["monitor_tmax"].reshape(45, 70)[pos])
     assert np.isnan(tpeak[(~wall) & ~pos]).all()
+
+
+def _runs_equal(a, b, what):
+    """Two GpuRuns bitwise equal on everything a run reports."""
+    for k in ("depth", "qx", "qy", "monitor_max", "monitor_tmax", "ledger"):
+        x, y = np.asarray(getattr(a, k), np.float64), np.asarray(getattr(b, k), np.float64)
+        assert x.shape == y.shape and np.array_equal(x.view(np.uint64), y.view(np.uint64)), f"{what}: {k}"
+    for k in ("steps", "cell_updates", "rows_skipped", "sync_count", "aborted"):
+        assert getattr(a, k) == getattr(b, k), f"{what}: {k}"
+    for k in ("t_end", "dt_min", "dt_max", "dt_mean", "v0", "rel_residual", "volume_m3"):
+        assert bits64(getattr(a, k)) == bits64(getattr(b, k)), f"{what}: {k}"
+
+
+def test_device_designs_equal_host_prepared(solver):
+    """Candidate designs materialized on the device (DesignSpace::materialize and
+    PreparedTerrain::apply_edit, design.rs:316-385, terrain.rs:460-505) run bitwise like the
+    same designs materialized on the host: every primitive kind, a zero-height primitive, an
+    empty design, a partial editable mask, an active clamp, primitives at the grid edge; the
+    edit volumes are bitwise the host's."""
+    _say("case: device designs vs host-prepared terrains, 96x64")
+    nx, ny = 96, 64
+    base = sc.validation_terrain(nx, ny, 2.0, lambda x, y: 3.0 + 0.008 * (192.0 - x) + 0.3 * math.sin(y / 11.0),
+                                 walls=lambda c, r: 40 <= c < 44 and 10 <= r < 30, n=0.035,
+                                 segments=[{"kind": "transmissive", "side": "east"}])
+    geo = base.fine.geo
+    rng = sc.SplitMix64(21)
+    kinds = ("berm", "swale", "mound", "cut")
+    designs = [sc.random_primitives(rng, k, geo, nx, ny, kinds=kinds) for k in (1, 2, 3, 4, 4, 3)]
+    designs.append([])
+    zero = sc.random_primitives(rng, 2, geo, nx, ny, kinds=kinds)
+    zero[0].height = 0.0
+    designs.append(zero)
+    edge = sc.random_primitives(rng, 1, geo, nx, ny, kinds=("berm",))
+    edge[0].cx, edge[0].cy, edge[0].length, edge[0].width = 188.0, 125.0, 30.0, 8.0
+    designs.append(edge)
+    editable = np.ones((ny, nx), bool)
+    editable[:, :12] = False
+    max_dz = 0.35
+    p = sc.SolverParams(240.0, 60.0, rain=sc.mm_h([[0, 70], [240, 30]]), infil_capacity=2e-6)
+    mon = np.arange(0, nx * ny, 7, dtype=np.uint32)
+    runs, vol = solver.run_designs(p, base, designs, mon, editable=editable, max_abs_dz=max_dz)
+    edits = [tr.materialize(d, geo, nx, ny, editable, max_dz) for d in designs]
+    host = solver.run_batch(p, [tr.PreparedTerrain(base).apply_edit(e) for e in edits], mon)
+    assert any(np.abs(e.dz).max() == max_dz for e in edits if e.dz.size), "the clamp should be active"
+    for i, (a, b, e) in enumerate(zip(runs, host, edits)):
+        _runs_equal(a, b, f"design {i}")
+        assert bits64(vol[i, 0]) == bits64(e.cut_m3) and bits64(vol[i, 1]) == bits64(e.fill_m3), f"design {i} volumes"
+    assert not np.array_equal(runs[0].depth, runs[1].depth)

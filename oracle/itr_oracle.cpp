@@ -17,6 +17,14 @@
 //   crates/itr-core/src/scenario.rs      Hyetograph
 //   crates/itr-core/src/objective.rs     Objective
 //   crates/itr-core/src/design.rs        Primitive, materialize
+//   crates/itr-hydro/src/inertial.rs     step_local_inertial (kernel 2, screening physics)
+//
+// Pins of the local-inertial step: the reference's own tests (lake at rest, rain mass,
+// tilted-plane outflow, peak depths within 15 % of HLL), the kinematic normal depth
+// (friction exponent, g n^2 and its face mean), the time step of a still lake and of a
+// steady plane.  Partially unpinned: the value of the theta weighting (0.8) and the
+// max(u, v) of the 2-D CFL rate follow the published formula; no check tells them from
+// nearby alternatives.
 
 #include "itr_oracle.h"
 
@@ -918,6 +926,170 @@ StepStats step_fused(const Prepared<T>& pt, Workspace<T>& ws, const StepConsts<T
 }
 
 // ---------------------------------------------------------------- objective.rs:70-149
+// ---------------------------------------------------------------- inertial.rs (screening physics)
+// Local-inertial step on staggered discharges: qx at (i, j) is the east face of padded
+// cell i (i = 0: west boundary face), qy at (i, j) the south face of padded row j
+// (j = 0: north boundary face).  f64 arithmetic on the stored state.
+inline void li_rises(bool crest, double f0, double f1, double& dl, double& dr) {  // inertial.rs:25-35
+    if (crest) { dl = f0; dr = f1; return; }
+    dl = std::fmax(f0, 0.0);
+    dr = std::fmax(-f0, 0.0);
+}
+inline double li_jump(bool crest, double f0, double f1) { return crest ? f0 - f1 : f0; }  // inertial.rs:37-43
+
+inline double li_face_q(double q, double hl, double hr, double dl, double dr, double d, double gn2, double g, double dt,
+                        double h_dry) {  // inertial.rs:46-58
+    const double el = hl - dl, er = hr - dr;
+    const double hf = std::fmax(std::fmax(el, er), 0.0);
+    if (hf <= h_dry) return 0.0;
+    const double slope = (er - el) / d;
+    const double r = rcbrt(hf);
+    const double r7 = r * r * r * r * r * r * r;
+    return (q - g * hf * dt * slope) / (1.0 + gn2 * dt * std::fabs(q) * r7);
+}
+
+template <class T>
+StepStats step_local_inertial(const Prepared<T>& pt, Workspace<T>& ws, const StepConsts<T>& k, const StepCtx& cx,
+                              std::vector<double>& old) {  // inertial.rs:60-219
+    const TerrainBase& b = *pt.b;
+    const int nx = b.nx, ny = b.ny;
+    const double dx = b.dx, dy = b.dy, g = b.g, dt = cx.dt;
+    const double h_dry = (double)k.h_dry;
+    const bool crest = b.crest;
+    auto at = [&](int i, int j) { return b.at(i, j); };
+    auto gn2 = [&](int i, int j) { return (double)pt.gn2[at(i, j)]; };
+    auto hv = [&](int i, int j) { return (double)ws.h[at(i, j)]; };
+    const size_t len = ws.h.size();
+    old.resize(2 * len);
+    for (size_t q = 0; q < len; q++) {
+        old[q] = (double)ws.qx[q];
+        old[len + q] = (double)ws.qy[q];
+    }
+    const double* ox = old.data();
+    const double* oy = old.data() + len;
+    const double TH = 0.8;   // de Almeida et al. (2012) weighting of the old discharge
+    auto qtx = [&](int i, int j) {
+        const double c = ox[at(i, j)];
+        return (i >= 1 && i + 1 < nx) ? TH * c + 0.5 * (1.0 - TH) * (ox[at(i - 1, j)] + ox[at(i + 1, j)]) : c;
+    };
+    auto qty = [&](int i, int j) {
+        const double c = oy[at(i, j)];
+        return (j >= 1 && j + 1 < ny) ? TH * c + 0.5 * (1.0 - TH) * (oy[at(i, j - 1)] + oy[at(i, j + 1)]) : c;
+    };
+    for (int j = 1; j <= ny; j++)   // x-faces 0 ..= nx
+        for (int i = 0; i <= nx; i++) {
+            const size_t kk = at(i, j);
+            double q;
+            if (i == 0 || i == nx) {
+                const Ghost& spec = i == 0 ? b.west[j - 1] : b.east[j - 1];
+                const double inward = i == 0 ? 1.0 : -1.0;
+                if (spec.kind == G_WALL) {
+                    q = 0.0;
+                } else if (spec.kind == G_TRANS) {   // normal-depth outflow only
+                    const int ci = i == 0 ? 1 : nx, fi = i == 0 ? 1 : nx - 1;
+                    double dz = nx > 1 ? li_jump(crest, (double)pt.fx0[at(fi, j)], (double)pt.fx1[at(fi, j)]) : 0.0;
+                    dz = i == 0 ? std::fmax(dz, 0.0) : std::fmin(dz, 0.0);
+                    const double hc = hv(ci, j);
+                    q = li_face_q((double)ws.qx[kk], hc, hc, std::fmax(dz, 0.0), std::fmax(-dz, 0.0), dx, gn2(ci, j), g, dt,
+                                  h_dry);
+                    q = i == 0 ? std::fmin(q, 0.0) : std::fmax(q, 0.0);
+                } else if (spec.kind == G_INFLOW) {
+                    q = inward * std::fabs((double)ws.qx[at(i == 0 ? 0 : nx + 1, j)]);
+                } else {   // stage
+                    q = li_face_q((double)ws.qx[kk], hv(i, j), hv(i + 1, j), 0.0, 0.0, dx, gn2(std::min(std::max(i, 1), nx), j),
+                                  g, dt, h_dry);
+                }
+            } else {
+                double dl, dr;
+                li_rises(crest, (double)pt.fx0[kk], (double)pt.fx1[kk], dl, dr);
+                const double n2 = 0.5 * (gn2(i, j) + gn2(i + 1, j));
+                q = li_face_q(qtx(i, j), hv(i, j), hv(i + 1, j), dl, dr, dx, n2, g, dt, h_dry);
+            }
+            ws.qx[kk] = canon((T)q);
+        }
+    for (int j = 0; j <= ny; j++)   // y-faces 0 ..= ny, +q southward
+        for (int i = 1; i <= nx; i++) {
+            const size_t kk = at(i, j);
+            double q;
+            if (j == 0 || j == ny) {
+                const Ghost& spec = j == 0 ? b.north[i - 1] : b.south[i - 1];
+                const double inward = j == 0 ? 1.0 : -1.0;
+                if (spec.kind == G_WALL) {
+                    q = 0.0;
+                } else if (spec.kind == G_TRANS) {
+                    const int cj = j == 0 ? 1 : ny, fj = j == 0 ? 1 : ny - 1;
+                    double dz = ny > 1 ? li_jump(crest, (double)pt.fy0[at(i, fj)], (double)pt.fy1[at(i, fj)]) : 0.0;
+                    dz = j == 0 ? std::fmax(dz, 0.0) : std::fmin(dz, 0.0);
+                    const double hc = hv(i, cj);
+                    q = li_face_q((double)ws.qy[kk], hc, hc, std::fmax(dz, 0.0), std::fmax(-dz, 0.0), dy, gn2(i, cj), g, dt,
+                                  h_dry);
+                    q = j == 0 ? std::fmin(q, 0.0) : std::fmax(q, 0.0);
+                } else if (spec.kind == G_INFLOW) {
+                    q = inward * std::fabs((double)ws.qy[at(i, j == 0 ? 0 : ny + 1)]);
+                } else {
+                    q = li_face_q((double)ws.qy[kk], hv(i, j), hv(i, j + 1), 0.0, 0.0, dy, gn2(i, std::min(std::max(j, 1), ny)),
+                                  g, dt, h_dry);
+                }
+            } else {
+                double dl, dr;
+                li_rises(crest, (double)pt.fy0[kk], (double)pt.fy1[kk], dl, dr);
+                const double n2 = 0.5 * (gn2(i, j) + gn2(i, j + 1));
+                q = li_face_q(qty(i, j), hv(i, j), hv(i, j + 1), dl, dr, dy, n2, g, dt, h_dry);
+            }
+            ws.qy[kk] = canon((T)q);
+        }
+    // continuity and sources, ledger, CFL rate, monitors
+    StepStats st;
+    const double rain = (double)k.rain, cap = (double)k.cap;
+    const double rx = dt / dx, ry = dt / dy;
+    for (int j = 1; j <= ny; j++) {
+        bool any = false;
+        for (int i = 1; i <= nx; i++) {
+            const size_t kk = at(i, j);
+            const double wet = (double)pt.wet[kk];
+            const double qw = (double)ws.qx[at(i - 1, j)], qe = (double)ws.qx[kk];
+            const double qn = (double)ws.qy[at(i, j - 1)], qs = (double)ws.qy[kk];
+            double hn = (double)ws.h[kk] + rx * (qw - qe) + ry * (qn - qs) + rain * wet;
+            const double infil = std::fmin(cap, std::fmax(hn, 0.0)) * wet;
+            hn -= infil;
+            st.infil += infil;
+            if (hn < 0.0) {
+                st.fix -= hn;
+                hn = 0.0;
+            }
+            if (!std::isfinite(hn)) st.bad = true;
+            ws.h[kk] = canon((T)(hn * wet));
+            any = any || hn > 0.0;
+            double s = 0.0;
+            if (hn > h_dry) {
+                const double u = 0.5 * (std::fabs(qw) + std::fabs(qe)) / hn;
+                const double v = 0.5 * (std::fabs(qn) + std::fabs(qs)) / hn;
+                s = (std::sqrt(g * hn) + std::fmax(u, v)) * 0.7 / std::fmin(dx, dy);
+            }
+            st.smax = std::fmax(st.smax, s);
+            if (cx.want_sum_s) st.sum_s += s;
+        }
+        ws.row_dry[j] = !any;
+        const double w = (double)ws.qx[at(0, j)] * dt * dy, e = (double)ws.qx[at(nx, j)] * dt * dy;
+        if (w > 0.0) st.bnd_in += w; else st.bnd_out -= w;
+        if (e > 0.0) st.bnd_out += e; else st.bnd_in -= e;
+        for (size_t m = ws.mon.row_start[j - 1]; m < ws.mon.row_start[j]; m++) {
+            const int col = (int)(ws.mon.cells[m] % (uint32_t)nx);
+            const double v = (double)ws.h[at(col + 1, j)];
+            if (v > ws.mon.max[m]) {
+                ws.mon.max[m] = v;
+                ws.mon.tmax[m] = cx.t_new;
+            }
+        }
+    }
+    for (int i = 1; i <= nx; i++) {
+        const double n = (double)ws.qy[at(i, 0)] * dt * dx, s = (double)ws.qy[at(i, ny)] * dt * dx;
+        if (n > 0.0) st.bnd_in += n; else st.bnd_out -= n;
+        if (s > 0.0) st.bnd_out += s; else st.bnd_in -= s;
+    }
+    return st;
+}
+
 double softplus(double x) { return x > 30.0 ? x : (x < -30.0 ? std::exp(x) : std::log1p(std::exp(x))); }
 
 double objective_j(const orc_observer* o, const double* mon) {
@@ -1011,17 +1183,24 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         std::fill(ws.mon.tmax.begin(), ws.mon.tmax.end(), 0.0);
         acc.v0 = volume(ws, area);
         t = 0.0; step = 0; sync_index = 0;
-        // initial_smax (solver.rs:315-332)
+        // initial_smax (solver.rs:315-332), initial_smax_li (solver.rs:334-344)
         double s0 = 0.0;
-        for (int j = 1; j <= ny; j++)
-            for (int i = 1; i <= nx; i++) {
-                size_t k = b.at(i, j);
-                double h = (double)ws.h[k];
-                if (h <= p->h_dry) continue;
-                double c = std::sqrt(b.g * h);
-                double r = (std::fabs((double)ws.qx[k]) / h + c) / b.dx + (std::fabs((double)ws.qy[k]) / h + c) / b.dy;
-                s0 = std::fmax(s0, r);
-            }
+        if (p->kernel == 2) {
+            double hmax = 0.0;
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) hmax = std::fmax(hmax, (double)ws.h[b.at(i, j)]);
+            s0 = std::sqrt(b.g * hmax) * 0.7 / std::fmin(b.dx, b.dy);
+        } else {
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t k = b.at(i, j);
+                    double h = (double)ws.h[k];
+                    if (h <= p->h_dry) continue;
+                    double c = std::sqrt(b.g * h);
+                    double r = (std::fabs((double)ws.qx[k]) / h + c) / b.dx + (std::fabs((double)ws.qy[k]) / h + c) / b.dy;
+                    s0 = std::fmax(s0, r);
+                }
+        }
         s_cur = s0;
         dt_cfl = s0 > 0.0 ? std::fmin(p->cfl / s0, p->dt_max_s) : p->dt_max_s;
     }
@@ -1037,6 +1216,8 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         snap.t = t; snap.step = step; snap.sync_index = sync_index; snap.dt_next = dt_cfl; snap.smax = s_cur;
     };
     if (locked) take_snap();
+    std::vector<T> li_faces;
+    std::vector<double> li_old;
     StepConsts<T> k;
     k.g = (T)b.g;
     k.g_half = (T)(0.5 * b.g);
@@ -1066,7 +1247,18 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         if (hits_sync) dt = t_sync - t;
         double rn = rain.integral(t, t + dt);
         double t_new = hits_sync ? t_sync : t + dt;
+        if (p->kernel == 2) {   // the staggered boundary faces survive the ghost fill (inertial.rs:219-242)
+            li_faces.clear();
+            for (int j = 1; j <= ny; j++) li_faces.push_back(ws.qx[b.at(0, j)]);
+            for (int i = 1; i <= nx; i++) li_faces.push_back(ws.qy[b.at(i, 0)]);
+        }
         fill_ghosts(b, pt.zlev, t, p->inflow_scale, ws.h.data(), ws.qx.data(), ws.qy.data(), ws.ghost_wet);
+        if (p->kernel == 2) {
+            for (int j = 1; j <= ny; j++)
+                if (b.west[j - 1].kind != G_INFLOW) ws.qx[b.at(0, j)] = li_faces[j - 1];
+            for (int i = 1; i <= nx; i++)
+                if (b.north[i - 1].kind != G_INFLOW) ws.qy[b.at(i, 0)] = li_faces[ny + i - 1];
+        }
         if (dts_out) dts_out->push_back(dt);
         k.dt = (T)dt;
         k.dtdx = (T)(dt / b.dx);
@@ -1074,7 +1266,13 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         k.rain = (T)rn;
         k.cap = (T)(p->infil_capacity * dt);
         StepCtx cx{t_new, dt, hits_sync, p->skip_dry != 0};
-        StepStats s = p->kernel == 0 ? step_oracle(pt, ws, k, cx) : step_fused(pt, ws, k, cx);
+        StepStats s;
+        if (p->kernel == 2) {
+            s = step_local_inertial(pt, ws, k, cx, li_old);
+            if (!hits_sync) s.sum_s = 0.0;
+        } else {
+            s = p->kernel == 0 ? step_oracle(pt, ws, k, cx) : step_fused(pt, ws, k, cx);
+        }
         step += 1;
         t = t_new;
         acc.rain += rn * area * (double)b.wet_count;

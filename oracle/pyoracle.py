@@ -255,7 +255,7 @@ def _params(p: SolverParams, precision, strip_rows=0, max_steps=0):
         keep.append(ld)
         q.locked_dt, q.n_locked = _d(ld), len(ld)
     q.max_steps = max_steps
-    q.kernel = {"oracle": 0, "fused": 1}[p.kernel]
+    q.kernel = {"oracle": 0, "fused": 1, "inertial": 2}[p.kernel]
     q.skip_dry = 1 if p.skip_dry else 0
     q.strip_rows = strip_rows
     q.precision = precision

@@ -325,3 +325,107 @@ def test_positivity_and_ledger_under_rain_and_edits(seed):
                edit={"c0": 3, "r0": 4, "dz": dz}, strip_rows=3)
     assert r["h"].min() >= -1e-10
     assert r["rel_residual"] < 1e-10
+
+
+# ---------------------------------------------------------------- local-inertial screening physics
+def test_inertial_lake_at_rest():
+    """inertial.rs (tests):9-25: a still lake over a wavy bed with a step stays at rest."""
+    z = dem(48, 36, 1.0, lambda x, y: 0.3 * math.sin(x / 7.0) * math.cos(y / 5.0) + (0.4 if x > 24.0 else 0.0))
+    T = po.Terrain(z, manning=0.03, dx=1.0)
+    r = po.run(T, P(120.0, 10.0, kernel="inertial", initial_stage=1.5), precision=64)
+    assert r["aborted"] == 0
+    assert max(np.abs(r["qx"]).max(), np.abs(r["qy"]).max()) < 1e-12
+
+
+def test_inertial_rain_mass_and_tilted_plane():
+    """inertial.rs (tests):27-53: mass closes on a closed plain under rain, and a tilted
+    plane draining east reaches outflow = rain x area within 2 %."""
+    T = po.Terrain(dem(30, 20, 2.0, lambda x, y: 0.001 * x), manning=0.03, dx=2.0)
+    r = po.run(T, P(600.0, 60.0, kernel="inertial", rain=po.hyeto_mm_h([[0.0, 60.0], [300.0, 60.0], [301.0, 0.0]])),
+               precision=64)
+    assert r["rel_residual"] < 1e-12
+    nx, ny, dx = 60, 4, 2.0
+    T = po.Terrain(dem(nx, ny, dx, lambda x, y: 0.01 * (nx * dx - x)), manning=0.03, dx=dx,
+                   segments=[{"kind": "transmissive", "side": "east"}])
+    r = po.run(T, P(3600.0, 60.0, kernel="inertial", rain=po.hyeto_mm_h([[0.0, 50.0]])), precision=64)
+    rows = r["rows"]
+    a, b = rows[-6], rows[-1]
+    q_out = (b[4] - a[4]) / (b[0] - a[0])
+    q_rain = 50.0 / 1000.0 / 3600.0 * nx * ny * dx * dx
+    assert abs(q_out / q_rain - 1.0) < 0.02, (q_out, q_rain)
+
+
+def test_inertial_peak_depths_track_hll():
+    """inertial.rs (tests):55-82: with rain and a west inflow, local-inertial peak depths at
+    the monitored cells stay within 15 % (relative L1) of HLL, in fewer steps."""
+    segs = [{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [300.0, 3.0], [900.0, 0.5]], "range_m": (30.0, 60.0)},
+            {"kind": "transmissive", "side": "east"}]
+    z = dem(64, 48, 2.0, lambda x, y: 20.0 + 0.003 * (128.0 - x) + 0.2 * math.sin(x / 13.0) * math.cos(y / 9.0))
+    T = po.Terrain(z, manning=0.04, dx=2.0, segments=segs)
+    mon = np.array([c for c in range(64 * 48) if c % 17 == 4], np.uint32)
+    rain = po.hyeto_mm_h([[0.0, 30.0], [600.0, 30.0], [601.0, 0.0]])
+    hll = po.run(T, P(1200.0, 60.0, rain=rain), precision=64, monitors=mon)
+    li = po.run(T, P(1200.0, 60.0, rain=rain, kernel="inertial"), precision=64, monitors=mon)
+    assert li["rel_residual"] < 1e-10
+    rel = np.abs(hll["monitor_max"] - li["monitor_max"]).sum() / np.abs(hll["monitor_max"]).sum()
+    assert rel < 0.15, rel
+    assert li["steps"] < hll["steps"]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_inertial_mass_closes_with_all_boundaries(prec):
+    """The staggered boundary faces survive the ghost fill (inertial.rs:219-242): with
+    inflow, transmissive and stage segments, walls, rain and infiltration the ledger
+    closes, depths stay non-negative and the boundary ledger sees both directions."""
+    T = equivalence_terrain()
+    r = po.run(T, equivalence_params("inertial", False), precision=prec)
+    assert r["aborted"] == 0
+    assert r["rel_residual"] < (1e-12 if prec == 64 else 1e-4)
+    assert r["h"].min() >= 0.0
+    rows = r["rows"]
+    assert rows[-1][3] > 0.0 and rows[-1][4] > 0.0
+
+
+@pytest.mark.parametrize("alternate", [False, True])
+def test_inertial_tilted_plane_normal_depth(alternate):
+    """Closed form: at steady state the local-inertial face update reduces to Manning's
+    law q = h_f^(5/3) sqrt(S) / n with h_f the upwind depth (inertial.rs:46-58 with
+    q_new = q_old), and q = rain * x on a plane draining east.  So each cell's depth is
+    (rain * x_e * n_f / sqrt(S))^(3/5), with n_f^2 the mean of the two cells' n^2 across
+    the east face.  Pins the friction exponent, the g n^2 factor and its face average."""
+    nx, ny, dx, S = 60, 4, 2.0, 0.01
+    rate = 50.0 / 1000.0 / 3600.0
+    n = np.where(np.arange(nx) % 2 == 0, 0.02, 0.05) if alternate else np.full(nx, 0.03)
+    T = po.Terrain(dem(nx, ny, dx, lambda x, y: S * (nx * dx - x)), manning=np.broadcast_to(n, (ny, nx)), dx=dx,
+                   segments=[{"kind": "transmissive", "side": "east"}])
+    h = po.run(T, P(3600.0, 60.0, kernel="inertial", rain=po.hyeto_mm_h([[0.0, 50.0]])), precision=64)["h"]
+    m = nx - 5   # the last cells feel the outlet's drawdown
+    i = np.arange(1, m + 1)
+    nf = np.sqrt(0.5 * (n[:m] ** 2 + n[1 : m + 1] ** 2))
+    hk = (rate * i * dx * nf / math.sqrt(S)) ** 0.6
+    ratio = h[:, :m] / hk[None, :]
+    assert np.abs(ratio - 1.0).max() < 0.01, ratio
+
+
+def test_inertial_time_step_closed_forms():
+    """The local-inertial step is dt = (cfl / 0.7) dx / (sqrt(g h) + |u|) (inertial.rs:14-16,
+    spec.md:1366).  Still lake of depth H on a flat bed: every step that does not end on a
+    sync point, the first included, has dt = cfl dx / (0.7 sqrt(g H)).  Steady tilted plane:
+    u = q / h with q = rain * x at the cell centre (the mean of the two face discharges)."""
+    H, dx = 1.0, 2.0
+    T = po.Terrain(np.zeros((10, 12)), manning=0.03, dx=dx)
+    r = po.run(T, P(60.0, 20.0, kernel="inertial", initial_stage=H), precision=64, want_dts=True)
+    dts = r["dts"]
+    want = 0.35 * dx / (0.7 * math.sqrt(9.81 * H))
+    free = dts[np.abs(dts - want) < 1e-9 * want]
+    assert abs(dts[0] - want) < 1e-12 * want
+    assert len(free) >= len(dts) - 3 and len(dts) == 3 * math.ceil(20.0 / want)
+    nx, ny, S = 60, 4, 0.01
+    rate = 50.0 / 1000.0 / 3600.0
+    T = po.Terrain(dem(nx, ny, dx, lambda x, y: S * (nx * dx - x)), manning=0.03, dx=dx,
+                   segments=[{"kind": "transmissive", "side": "east"}])
+    r = po.run(T, P(3600.0, 60.0, kernel="inertial", rain=po.hyeto_mm_h([[0.0, 50.0]])), precision=64, want_dts=True)
+    h = r["h"][1]
+    u = rate * (np.arange(nx) + 0.5) * dx / h
+    want = 0.35 / (0.7 * (np.sqrt(9.81 * h) + u) / dx).max()
+    assert abs(np.median(r["dts"][-50:]) / want - 1.0) < 2e-3

@@ -17,7 +17,11 @@
  *
  * Results are bitwise equal to the reference's CPU f32 fused kernel
  * (crates/itr-hydro/src/kernel.rs:573-727 driven by solver.rs:346-660): state, step
- * counts, t_end, monitor maxima and times, ledger rows and abort decisions.
+ * counts, t_end, monitor maxima and times, ledger rows and abort decisions.  Grids run
+ * with the local-inertial physics (crates/itr-hydro/src/inertial.rs) are bitwise equal
+ * to the reference's f32 local-inertial run in all of these except two ledger sums that
+ * the reference adds cell by cell in f64 (infiltration and the CFL-rate sum behind lts):
+ * those agree to within f64 rounding.
  */
 #ifndef ITR_HIP_H
 #define ITR_HIP_H
@@ -81,8 +85,12 @@ typedef struct {
     int32_t n_rain;
     int32_t has_initial_stage;
     int32_t skip_dry; /* dry-row accounting rule of kernel.rs:605-612 (cell_updates) */
-    int32_t reserved;
+    int32_t physics;  /* KernelKind (solver.rs:19-24): ITR_PHYSICS_HLL (Fused) or
+                         ITR_PHYSICS_LOCAL_INERTIAL (inertial.rs, the screening physics of
+                         coarse fidelity levels, crates/itr-opt/src/problem.rs:178-181) */
 } itr_solver_params;
+
+enum { ITR_PHYSICS_HLL = 0, ITR_PHYSICS_LOCAL_INERTIAL = 1 };
 
 /* One grid (a TerrainBase, crates/itr-hydro/src/terrain.rs:237-257) and the candidates
  * run on it (PreparedTerrain overlays flattened as in crates/itr-gpu/src/lib.rs:177-194).

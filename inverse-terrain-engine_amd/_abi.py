@@ -13,6 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ITR_HIP_LIB") or os.path.join(PKG, "_lib", "libitr_hip.so")
 
 ITR_OK, ITR_DECLINED, ITR_EINVAL, ITR_EDEVICE, ITR_ENOMEM = range(5)
+ITR_PHYSICS_HLL, ITR_PHYSICS_LOCAL_INERTIAL = 0, 1
 STATUS_NAMES = {0: "ok", 1: "declined", 2: "invalid argument", 3: "device error", 4: "out of memory"}
 
 _dp = C.POINTER(C.c_double)
@@ -27,7 +28,7 @@ class SolverParamsC(C.Structure):
                 ("infil_capacity", C.c_double), ("initial_depth", C.c_double), ("initial_stage", C.c_double),
                 ("cfl_hard", C.c_double), ("inflow_scale", C.c_double), ("rain_t", _dp), ("rain_r", _dp),
                 ("locked_dt", _dp), ("n_locked", C.c_int64), ("max_steps", C.c_int64), ("n_rain", C.c_int32),
-                ("has_initial_stage", C.c_int32), ("skip_dry", C.c_int32), ("reserved", C.c_int32)]
+                ("has_initial_stage", C.c_int32), ("skip_dry", C.c_int32), ("physics", C.c_int32)]
 
 
 class DesignC(C.Structure):

@@ -63,6 +63,18 @@ struct RowAcc {
     uint32_t pad;
 };
 
+// The same 32 bytes for a local-inertial grid: the band's f64 sums of infiltration and
+// CFL rate sit in its first row (0 in the others), the FixedSum of the new depth and the
+// mask of waves whose positivity-fix values are in the grid's fix64 buffer in every row.
+struct LiRowAcc {
+    double infil, sum_s;
+    float vol;
+    uint32_t fixmask;
+    uint32_t wet;          // unused (no dry-row rule in the local-inertial step)
+    uint32_t pad;
+};
+static_assert(sizeof(LiRowAcc) == sizeof(RowAcc), "row partials share one buffer");
+
 struct DevParams {
     double duration, sync, cfl, dt_max, cfl_hard, infil_cap, inflow_scale;
     const double* rain_t;
@@ -83,6 +95,8 @@ struct DevGrid {
     int32_t nx, ny, P, R, nbands, cand0, ncand, n_mon, n_guard, crest, wet_count, waves;
     int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
     int32_t max_log;           // capacity of steplog per candidate (0: no log)
+    int32_t li;                // local-inertial physics (inertial.rs) instead of HLL
+    int32_t pad_li;
     size_t plane;              // ny * P
     size_t plane_y;            // (ny + 1) * P: dzy rows 0..ny (rows 0 and ny are zero)
     float* state;              // [ncand][2][3][plane]
@@ -110,6 +124,8 @@ struct DevGrid {
     float* sbuf;               // [nslot][plane] CFL rate at sync steps (within one workgroup)
     float* fixbuf;             // [nslot][plane] positivity-fix values of rows that have one
     double* ledger;            // [nslot][max_ledger][8]
+    float* lif;                // local-inertial: [2][nslot][ny + nx] west then north boundary faces
+    double* fix64;             // local-inertial: [2][nslot][plane] positivity-fix values (f64), by state parity
     struct StepRec* steplog;   // [nslot][max_log] constants of every step run (adjoint replay)
     const double* baseline_max;
     const int32_t* guard_pos;
@@ -147,6 +163,7 @@ struct CandCtl {
 struct StepAcc {
     int32_t smax_bits;   // max CFL rate as ordered int bits (rates are >= 0)
     uint32_t bad;
+    unsigned long long smax64;   // local inertial: the f64 CFL rate max, as ordered bits
 };
 
 struct KArgs {
@@ -466,6 +483,12 @@ __device__ __forceinline__ int guard_breach(const DevGrid& G, const float* mm, i
     return have ? wq : -1;
 }
 
+// The completed step's CFL rate maximum: f32 for the HLL step (kernel.rs), f64 for the
+// local-inertial step (inertial.rs:186-196).
+__device__ __forceinline__ double step_smax(const DevGrid& G, const StepAcc& A) {
+    return G.li ? __longlong_as_double((long long)A.smax64) : (double)__int_as_float(max(A.smax_bits, 0));
+}
+
 // Completion of the candidate's step launched by the previous launch (solver.rs:529-545
 // and the sync point, 554-603, except the ledger sums), then the termination test and
 // the head of the next step (prepare_step).  Every wave of every workgroup of the
@@ -489,7 +512,7 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
             flags |= F_DONE;
             go = false;
         } else {
-            const double smax = (double)__int_as_float(max(A.smax_bits, 0));
+            const double smax = step_smax(G, A);
             S.dt_cfl = smax > 0.0 ? p.cfl / smax : p.dt_max;
             S.s_cur = smax;
             if (S.hits_sync) {
@@ -536,6 +559,37 @@ __device__ __forceinline__ float lane_chain(int c0, int n, F value) {
     }
     for (; c < n; c += 16) s += value(c);
     return s;
+}
+
+// Stage / Inflow ghosts of a band at the step start into LDS, as (hv, qv) pairs: west /
+// east of rows a0-1 .. b0 (pairs 0 .. 2R+3), then the north ghost row when a0 <= north_upto
+// and the south one when the band ends at the last row.  Returns whether any ghost of
+// the band is wet (all threads).
+__device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local, int a0, int b0, double t, float2* l_gsp,
+                                                     int north_upto) {
+    const int nx = G.nx, ny = G.ny, R = G.R;
+    const int tid = threadIdx.x;
+    const int nthr = G.waves * kWave;
+    const int nwe = 2 * (R + 2);
+    bool fw = false;
+    for (int i = tid; i < nwe + 2 * nx; i += nthr) {
+        int k = -1;
+        bool inward = true;
+        if (i < nwe) {
+            const int side = i / (R + 2), row = a0 - 1 + i % (R + 2);
+            if (row >= 0 && row < ny) { k = 2 * nx + side * ny + row; inward = side == 0; }
+        } else {
+            const int side = (i - nwe) / nx, c = (i - nwe) % nx;
+            if (side == 0 ? a0 <= north_upto : b0 == ny) { k = side * nx + c; inward = side == 0; }
+        }
+        GhostSp gs;
+        gs.hv = 0.0f;
+        gs.qv = 0.0f;
+        if (k >= 0) gs = special_ghost(G, G.prm.inflow_scale, G.ghost[k], k, local, t, inward);
+        l_gsp[i] = make_float2(gs.hv, gs.qv);
+        fw = fw || gs.hv > 0.0f;
+    }
+    return __syncthreads_or(fw ? 1 : 0) != 0;
 }
 
 // One step of one band of one candidate (kernel.rs:590-727 semantics; the sweep order
@@ -608,31 +662,8 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const LdsLayout LL = lds_layout(R, nx, G.special != 0);
     float2* l_gsp = (float2*)(lds + LL.gsp);
 
-    // Stage / Inflow ghosts of the band at the step start: west / east of rows a0-1 .. b0
-    // (pairs 0 .. 2R+3), then the north / south ghost rows when the band touches them.
     bool forced_wet = false;
-    if (special) {
-        const int nwe = 2 * (R + 2);
-        bool fw = false;
-        for (int i = tid; i < nwe + 2 * nx; i += nthr) {
-            int k = -1;
-            bool inward = true;
-            if (i < nwe) {
-                const int side = i / (R + 2), row = a0 - 1 + i % (R + 2);
-                if (row >= 0 && row < ny) { k = 2 * nx + side * ny + row; inward = side == 0; }
-            } else {
-                const int side = (i - nwe) / nx, c = (i - nwe) % nx;
-                if (side == 0 ? a0 == 0 : b0 == ny) { k = side * nx + c; inward = side == 0; }
-            }
-            GhostSp gs;
-            gs.hv = 0.0f;
-            gs.qv = 0.0f;
-            if (k >= 0) gs = special_ghost(G, G.prm.inflow_scale, GHOST[k], k, local, t, inward);
-            l_gsp[i] = make_float2(gs.hv, gs.qv);
-            fw = fw || gs.hv > 0.0f;
-        }
-        forced_wet = __syncthreads_or(fw ? 1 : 0) != 0;
-    }
+    if (special) forced_wet = stage_special_ghosts(G, local, a0, b0, t, l_gsp, 0);
 
     // Dry band (kernel.rs:605-612 rule for every row of the band: rows a0-1 .. b0 exactly
     // dry, ghost rows mirroring their neighbour or holding no wet special ghost, and no
@@ -939,6 +970,39 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     // candidate's next launch reads these results; the launch boundary orders them.
 }
 
+// The completed step's sums (shd[40..45]: infiltration, fix, boundary in, boundary out,
+// volume, CFL rate sum) into the new control block and, at a sync point, the ledger row
+// (solver.rs:529-540, 554-592).  One thread.
+__device__ __forceinline__ void ledger_finish(const DevGrid& G, const CandCtl& Si, const StepAcc& A, CandCtl& Oc, bool sync,
+                                              const double* shd) {
+    CandCtl* O = &Oc;
+    const DevParams& p = G.prm;
+    const int local = Si.local;
+    const double st_infil = shd[40], st_fix = shd[41], st_in = shd[42], st_out = shd[43];
+    const double st_vol = shd[44], st_sum_s = shd[45];
+    const double area = G.area;
+    const uint64_t cells = (uint64_t)G.nx * (uint64_t)G.ny;
+    O->acc_rain += Si.rain * area * (double)G.wet_count;
+    O->inflow += st_in;
+    O->outflow += st_out;
+    O->infil += st_infil * area;
+    O->fix += st_fix * area;
+    O->cell_updates += cells - (uint64_t)Si.skipped_now * (uint64_t)G.nx;
+    O->rows_skipped += Si.skipped_now;
+    if (sync) {
+        const double smax = step_smax(G, A);
+        if (st_sum_s > 0.0) O->lts = fmax(O->lts, smax * (double)cells / st_sum_s);
+        const double v = st_vol * area;
+        const double expected = O->v0 + O->acc_rain + O->inflow - O->outflow - O->infil + O->fix;
+        if ((int)O->n_ledger < p.max_ledger) {
+            double* row = G.ledger + ((size_t)local * p.max_ledger + O->n_ledger) * 8;
+            row[0] = O->t; row[1] = v; row[2] = O->acc_rain; row[3] = O->inflow;
+            row[4] = O->outflow; row[5] = O->infil; row[6] = O->fix; row[7] = v - expected;
+        }
+        O->n_ledger += 1;
+    }
+}
+
 // Ledger accounting of the completed step (kernel.rs:453-500 sums, solver.rs:529-540,
 // 554-592), run by the candidate's first band after its own step.  Row-ordered f64
 // sums of the row partials and the FixedSum<f64> of the boundary y-faces, then thread
@@ -1026,31 +1090,448 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
         if (tid < 6) shd[40 + tid] = s;
     }
     __syncthreads();
-    if (tid == 0) {
-        const double st_infil = shd[40], st_fix = shd[41], st_in = shd[42], st_out = shd[43];
-        const double st_vol = shd[44], st_sum_s = shd[45];
-        const double area = G.area;
-        const uint64_t cells = (uint64_t)nx * (uint64_t)ny;
-        O->acc_rain += Si.rain * area * (double)G.wet_count;
-        O->inflow += st_in;
-        O->outflow += st_out;
-        O->infil += st_infil * area;
-        O->fix += st_fix * area;
-        O->cell_updates += cells - (uint64_t)Si.skipped_now * (uint64_t)nx;
-        O->rows_skipped += Si.skipped_now;
-        if (sync) {
-            const double smax = (double)__int_as_float(max(A.smax_bits, 0));
-            if (st_sum_s > 0.0) O->lts = fmax(O->lts, smax * (double)cells / st_sum_s);
-            const double v = st_vol * area;
-            const double expected = O->v0 + O->acc_rain + O->inflow - O->outflow - O->infil + O->fix;
-            if ((int)O->n_ledger < p.max_ledger) {
-                double* row = G.ledger + ((size_t)local * p.max_ledger + O->n_ledger) * 8;
-                row[0] = O->t; row[1] = v; row[2] = O->acc_rain; row[3] = O->inflow;
-                row[4] = O->outflow; row[5] = O->infil; row[6] = O->fix; row[7] = v - expected;
-            }
-            O->n_ledger += 1;
+    if (tid == 0) ledger_finish(G, Si, A, Oc, sync, shd);
+}
+
+// ------------------------------------------------------------------ local-inertial screening step
+// crates/itr-hydro/src/inertial.rs:60-219, the physics the reference runs on coarse
+// fidelity levels (crates/itr-opt/src/problem.rs:178-181).  Staggered discharges on the
+// grid's state planes: the qx plane holds the east face of each cell (face f = col + 1)
+// and the qy plane the south face of each row (face k = row + 1), as the reference's
+// qx[at(i, j)] / qy[at(i, j)]; the west boundary faces (f = 0) and north boundary faces
+// (k = 0) live in the grid's `lif` array.  Every face and cell is the reference's f64
+// expression of the f32 state, evaluated in its order without fused operations.
+// A wave tiles 61 columns: lanes 2..62 update, lanes 0, 1 and 63 are halos (lane 1's
+// east face is lane 2's west face; the theta weighting of a face reads its neighbours'
+// old discharges, so each computed face needs one lane on either side).
+constexpr int kLiCols = 61;
+constexpr double kLiTheta = 0.8;   // de Almeida et al. (2012) weighting (inertial.rs:85-98)
+
+struct LiK { double dt, rx, ry, rain, cap, h_dry, g, dx, dy, dmin; };
+
+// face_q (inertial.rs:46-58): zero when the face depth is at most h_dry.
+__device__ __forceinline__ double li_face_q(double q, double hl, double hr, double dl, double dr, double d, double gn2,
+                                            const LiK& K) {
+    const double el = hl - dl, er = hr - dr;
+    const double hf = fmax(fmax(el, er), 0.0);
+    double out = 0.0;
+    if (hf > K.h_dry) {
+        const double slope = (er - el) / d;
+        const double r = rcbrt64(hf);
+        const double r7 = r * r * r * r * r * r * r;
+        out = (q - K.g * hf * K.dt * slope) / (1.0 + gn2 * K.dt * fabs(q) * r7);
+    }
+    return out;
+}
+// rises / jump (inertial.rs:25-43): signed faces store the jump z_R - z_L, crest faces
+// the two rises.
+template <bool CREST> __device__ __forceinline__ void li_rises(float f0, float f1, double& dl, double& dr) {
+    if (CREST) {
+        dl = (double)f0;
+        dr = (double)f1;
+    } else {
+        const double dz = (double)f0;
+        dl = fmax(dz, 0.0);
+        dr = fmax(-dz, 0.0);
+    }
+}
+template <bool CREST> __device__ __forceinline__ double li_jump(float f0, float f1) {
+    return CREST ? (double)f0 - (double)f1 : (double)f0;
+}
+// A boundary face (inertial.rs:104-124, 143-157).  lo: west / north face (the interior
+// cell is on its right), else east / south.  qold: the face's discharge before the step;
+// hc, n2c: the interior cell's depth and g n^2; hg: the stage ghost depth; qv: the inflow
+// ghost discharge; f0, f1: the adjacent interior face (transmissive bed slope).
+template <bool CREST>
+__device__ __forceinline__ double li_boundary(int code, bool lo, float qold, float hc, float hg, float qv, float f0, float f1,
+                                              bool has_face, double d, float n2c, const LiK& K) {
+    double q = 0.0;
+    if (code == 1) {   // normal-depth outflow only
+        double dz = has_face ? li_jump<CREST>(f0, f1) : 0.0;
+        dz = lo ? fmax(dz, 0.0) : fmin(dz, 0.0);
+        const double h = (double)hc;
+        q = li_face_q((double)qold, h, h, fmax(dz, 0.0), fmax(-dz, 0.0), d, (double)n2c, K);
+        q = lo ? fmin(q, 0.0) : fmax(q, 0.0);
+    } else if (code == 3) {
+        q = (lo ? 1.0 : -1.0) * fabs((double)qv);
+    } else if (code == 2) {
+        q = lo ? li_face_q((double)qold, (double)hg, (double)hc, 0.0, 0.0, d, (double)n2c, K)
+               : li_face_q((double)qold, (double)hc, (double)hg, 0.0, 0.0, d, (double)n2c, K);
+    }
+    return q;
+}
+
+// Dynamic LDS of the local-inertial step, in floats.
+struct LiLds { size_t fixmask, red, smx, dsum, acc, moff, gsp, total; };
+__host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
+    auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
+    LiLds L;
+    L.fixmask = 0;
+    L.red = al(L.fixmask + (size_t)R);
+    L.smx = al(L.red + kMaxWaves);                  // kMaxWaves u64
+    L.dsum = al(L.smx + 2 * kMaxWaves);             // kMaxWaves x 2 f64
+    L.acc = al(L.dsum + 4 * kMaxWaves);             // R x 16 FixedSum lanes
+    L.moff = al(L.acc + (size_t)R * 16);
+    L.gsp = al(L.moff + (size_t)kMaxWaves * 64);
+    L.total = al(L.gsp + (special ? 2 * gsp_pairs(R, nx) : 0));
+    if (L.total < 64) L.total = 64;
+    return L;
+}
+
+// One local-inertial step of one band (rows a0 .. b0-1) of one candidate: reads state
+// copy `par` (and the boundary faces of that parity), writes copy par^1.  Per row: the
+// row's x-faces, its south y-face (the north one is carried from the row above, or
+// computed in the prologue), then continuity and sources (inertial.rs:160-196).
+template <bool CREST, bool SPECIAL>
+__device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, int local, int band, const LiK& K, double t,
+                                             double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
+    const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
+    const int a0 = band * R;
+    const int b0 = min(a0 + R, ny);
+    const int nrows = b0 - a0;
+    const int tid = threadIdx.x;
+    const int nthr = G.waves * kWave;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int col = wave * kLiCols - 2 + lane;
+    const bool upd = lane >= 2 && lane <= kLiCols + 1 && col < nx;
+    const bool west_lane = col == -1;     // computes face 0 of each row
+    const bool east_lane = col == nx - 1; // computes face nx
+    const int npar = par ^ 1;
+    const size_t plane = G.plane;
+    float* base = G.state + (size_t)local * 6 * plane;
+    ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)npar * 3 * plane);
+    const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
+    const ITR_GLOBAL float* __restrict__ lif_in = gp(G.lif + slot(G, par, local) * (size_t)(ny + nx));
+    ITR_GLOBAL float* __restrict__ lif_out = gp(G.lif + slot(G, npar, local) * (size_t)(ny + nx));
+    const ITR_GLOBAL float* __restrict__ mon_in = gp(G.mon_max + slot(G, par, local) * G.n_mon);
+    const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
+    ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
+    ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
+    const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
+    const ITR_GLOBAL uint32_t* __restrict__ MONOFF = gp(G.mon_off);
+    const int qm = mon_q0 + tid;
+    const bool has_m = qm < mon_q1;
+    const uint32_t m_off0 = has_m ? MONOFF[qm] : 0u;
+    ITR_GLOBAL double* __restrict__ fix64 = gp(G.fix64 + slot(G, npar, local) * plane);
+    LiRowAcc* ra_out = (LiRowAcc*)(G.rowacc + slot(G, npar, local) * ny);
+    const uint32_t ccol = (uint32_t)min(max(col, 0), nx - 1);
+    const uint32_t fcol = (uint32_t)max(min(max(col + 1, 1), nx - 1), 1);   // the lane's x-face jump column
+    const uint32_t cb = ccol * 4u, xb = fcol * 4u;
+    const uint32_t rowb = (uint32_t)P * 4u;
+    const uint32_t planeb = (uint32_t)(plane * 4u);
+    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(base + (size_t)par * 3 * plane, 3 * plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)npar * 3 * plane, 3 * plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzx = make_rsrc(G.dzx + (size_t)local * plane, plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzy = make_rsrc(G.dzy + (size_t)local * G.plane_y, G.plane_y * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzx2 = make_rsrc(CREST ? G.dzx2 + (size_t)local * plane : G.dzx, plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_dzy2 = make_rsrc(CREST ? G.dzy2 + (size_t)local * G.plane_y : G.dzy, G.plane_y * 4);
+    const __amdgpu_buffer_rsrc_t rs_wet = make_rsrc(G.wet, plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_gn2 = make_rsrc(G.gn2, plane * 4);
+    const int code_n = GHOST[ccol];
+    const int code_s = GHOST[nx + ccol];
+    const int we_base = 2 * nx + (west_lane ? 0 : ny);   // this lane's west / east ghost codes
+    const bool special = SPECIAL && G.special != 0;
+    const LiLds LL = li_lds_layout(R, nx, G.special != 0);
+    float2* l_gsp = (float2*)(lds + LL.gsp);
+    uint32_t* l_fixmask = (uint32_t*)(lds + LL.fixmask);
+    int* l_red = (int*)(lds + LL.red);
+    unsigned long long* l_smx = (unsigned long long*)(lds + LL.smx);
+    double* l_dsum = (double*)(lds + LL.dsum);
+    float* l_acc = lds + LL.acc;
+    uint32_t* l_moff = (uint32_t*)(lds + LL.moff);
+    // (the band starting at row 1 reads the north inflow discharge for the theta
+    // weighting of face 1)
+    if (special) (void)stage_special_ghosts(G, local, a0, b0, t, l_gsp, 1);
+    for (int r = tid; r < R; r += nthr) l_fixmask[r] = 0u;
+    if (has_m) l_moff[tid] = m_off0;
+    __syncthreads();
+    // Special ghost pairs of this lane: west / east of row r, north / south of its column.
+    auto gs_we = [&](int r) -> float2 { return l_gsp[(west_lane ? 0 : R + 2) + (r - (a0 - 1))]; };
+    auto gs_ns = [&](bool north) -> float2 { return l_gsp[2 * (R + 2) + (north ? 0 : nx) + (int)ccol]; };
+
+    struct Row { float h, qx, qy, gn2, wet, dzx, dzx2, dzy, dzy2; };
+    // Row r of the old state (clamped rows are never used); the west lane's qx is the
+    // old west boundary face (the inflow ghost discharge at an inflow cell, as the
+    // reference's ghost fill leaves it there).
+    auto fetch = [&](int r) -> Row {
+        const int rr = min(max(r, 0), ny - 1);
+        const uint32_t so = (uint32_t)rr * rowb;
+        const uint32_t sy = (uint32_t)min(max(r, 0), ny) * rowb;
+        Row v;
+        v.h = bload(rs_h, cb, so);
+        v.qx = bload(rs_h, cb, so + planeb);
+        v.qy = bload(rs_h, cb, so + 2 * planeb);
+        v.gn2 = bload(rs_gn2, cb, so);
+        v.wet = bload(rs_wet, cb, so);
+        v.dzx = bload(rs_dzx, xb, so);
+        v.dzy = bload(rs_dzy, cb, sy);
+        v.dzx2 = CREST ? bload(rs_dzx2, xb, so) : 0.0f;
+        v.dzy2 = CREST ? bload(rs_dzy2, cb, sy) : 0.0f;
+        if (west_lane) {
+            float w = lif_in[rr];
+            if (special && GHOST[we_base + rr] == 3) w = gs_we(rr).y;
+            v.qx = w;
+        }
+        return v;
+    };
+    // Old discharge of y-face k (0 ..= ny) in this lane's column.
+    auto qy_old = [&](int k) -> float {
+        if (k > 0) return bload(rs_h, cb, (uint32_t)(k - 1) * rowb + 2 * planeb);
+        float v = lif_in[ny + ccol];
+        if (special && code_n == 3) v = gs_ns(true).y;
+        return v;
+    };
+    // New x-face f = col + 1 of row r (c: this lane's cell; right-neighbour values by DPP).
+    auto xface = [&](const Row& c, int r) -> float {
+        const float h_r = from_right(c.h);
+        const float g2_r = from_right(c.gn2);
+        const float q_l = from_left(c.qx);
+        const float q_r = from_right(c.qx);
+        double q;
+        if (west_lane || east_lane) {
+            const int code = GHOST[we_base + r];
+            float2 gs = make_float2(0.0f, 0.0f);
+            if (special && code >= 2) gs = gs_we(r);
+            q = west_lane ? li_boundary<CREST>(code, true, c.qx, h_r, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, g2_r, K)
+                          : li_boundary<CREST>(code, false, c.qx, c.h, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, c.gn2, K);
+        } else {
+            double dl, dr;
+            li_rises<CREST>(c.dzx, c.dzx2, dl, dr);
+            const double n2 = 0.5 * ((double)c.gn2 + (double)g2_r);
+            double qt = (double)c.qx;
+            if (col + 2 < nx) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)q_l + (double)q_r);
+            q = li_face_q(qt, (double)c.h, (double)h_r, dl, dr, K.dx, n2, K);
+        }
+        return canon((float)q);
+    };
+    // New y-face k between rows k-1 (u) and k (d), 1 <= k <= ny-1: u.qy is its old
+    // discharge, qp and d.qy those of faces k-1 and k+1.
+    auto yface = [&](const Row& u, const Row& d, float qp, int k) -> float {
+        double dl, dr;
+        li_rises<CREST>(d.dzy, d.dzy2, dl, dr);
+        const double n2 = 0.5 * ((double)u.gn2 + (double)d.gn2);
+        double qt = (double)u.qy;
+        if (k + 1 < ny) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)qp + (double)d.qy);
+        return canon((float)li_face_q(qt, (double)u.h, (double)d.h, dl, dr, K.dy, n2, K));
+    };
+    auto yface_south = [&](const Row& u) -> float {   // k = ny, u = row ny-1
+        float2 gs = make_float2(0.0f, 0.0f);
+        if (special && code_s >= 2) gs = gs_ns(false);
+        return canon((float)li_boundary<CREST>(code_s, false, u.qy, u.h, gs.x, gs.y, u.dzy, u.dzy2, ny > 1, K.dy, u.gn2, K));
+    };
+    auto yface_north = [&](const Row& d, const Row& d1) -> float {   // k = 0, d = row 0, d1 = row 1
+        float2 gs = make_float2(0.0f, 0.0f);
+        if (special && code_n >= 2) gs = gs_ns(true);
+        return canon((float)li_boundary<CREST>(code_n, true, lif_in[ny + ccol], d.h, gs.x, gs.y, d1.dzy, d1.dzy2, ny > 1, K.dy,
+                                               d.gn2, K));
+    };
+
+    Row cur = fetch(a0);
+    Row nxt = fetch(a0 + 1);
+    float qn_new, qprev;   // new north face of the row; old discharge of that face
+    if (a0 == 0) {
+        qn_new = yface_north(cur, nxt);
+        qprev = qy_old(0);
+        if (upd) lif_out[ny + col] = qn_new;
+    } else {
+        const Row up = fetch(a0 - 1);
+        qn_new = yface(up, cur, qy_old(a0 - 1), a0);
+        qprev = up.qy;
+    }
+    double smax_l = 0.0, inf_l = 0.0, ss_l = 0.0;
+    bool bad = false;
+    for (int j = a0; j < b0; j++) {
+        const Row pre = fetch(j + 2);
+        const float qe = xface(cur, j);
+        const float qw = from_left(qe);
+        const float qs = j + 1 < ny ? yface(cur, nxt, qprev, j + 1) : yface_south(cur);
+        // Continuity and sources (inertial.rs:166-196).
+        const double wet = (double)cur.wet;
+        const double dw = (double)qw, de = (double)qe, dn = (double)qn_new, ds = (double)qs;
+        double hn = (double)cur.h + K.rx * (dw - de) + K.ry * (dn - ds) + K.rain * wet;
+        const double infil = fmin(K.cap, fmax(hn, 0.0)) * wet;
+        hn -= infil;
+        double fixv = 0.0;
+        if (hn < 0.0) {
+            fixv = -hn;
+            hn = 0.0;
+        }
+        const float hnew = canon((float)(hn * wet));
+        double s = 0.0;
+        if (hn > K.h_dry) {
+            const double u = 0.5 * (fabs(dw) + fabs(de)) / hn;
+            const double v = 0.5 * (fabs(dn) + fabs(ds)) / hn;
+            s = (sqrt(K.g * hn) + fmax(u, v)) * 0.7 / K.dmin;
+        }
+        const int r = j - a0;
+        if (upd) {
+            const uint32_t so = (uint32_t)j * rowb;
+            bstore(hnew, rs_ho, cb, so);
+            bstore(qe, rs_ho, cb, so + planeb);
+            bstore(qs, rs_ho, cb, so + 2 * planeb);
+            bad = bad || !__builtin_isfinite(hn);
+            smax_l = fmax(smax_l, s);
+            inf_l += infil;
+            if (sync) ss_l += s;
+        }
+        if (west_lane) lif_out[j] = qe;
+        if (__builtin_expect(__ballot(upd && fixv != 0.0) != 0ull, 0)) {   // rare: a cell went negative
+            if (upd) fix64[(size_t)j * P + col] = fixv;
+            if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
+        }
+        qprev = cur.qy;
+        qn_new = qs;
+        cur = nxt;
+        nxt = pre;
+    }
+
+    // Band reductions: f64 CFL max (ordered bits; rates are >= 0 and never NaN here),
+    // health flag, and the band's infiltration and CFL-rate sums (fixed lane tree, then
+    // waves in order).
+    unsigned long long sm = (unsigned long long)__double_as_longlong(smax_l);
+    double si = inf_l, ss = ss_l;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long om = __shfl_xor(sm, o, 64);
+        sm = om > sm ? om : sm;
+        si += __shfl_xor(si, o, 64);
+        ss += __shfl_xor(ss, o, 64);
+    }
+    const bool bad_any = __any(bad);
+    if (lane == 0) {
+        l_smx[wave] = sm;
+        l_red[wave] = bad_any ? 1 : 0;
+        l_dsum[2 * wave] = si;
+        l_dsum[2 * wave + 1] = ss;
+    }
+    __syncthreads();
+    // Monitors (inertial.rs:202-209 on the new state), loads issued before the row sums.
+    float m_h = 0.0f, m_old = 0.0f;
+    double m_t = 0.0;
+    if (has_m) {
+        m_h = Ho[l_moff[tid]];
+        m_old = mon_in[qm];
+        m_t = tmax_in[qm];
+    }
+    if (sync) {   // FixedSum lanes of the new depth per row (volume, solver.rs:250-262)
+        for (int q = tid; q < nrows * 16; q += nthr) {
+            const int r = q >> 4, L = q & 15;
+            const int j = a0 + r;
+            const int c0 = (L == 0 ? 16 : L) - 1;
+            l_acc[r * 16 + L] = lane_chain(c0, nx, [&](int c) { return Ho[(size_t)j * P + c]; });
         }
     }
+    __syncthreads();
+    for (int r = tid; r < nrows; r += nthr) {
+        LiRowAcc q;
+        q.infil = 0.0;
+        q.sum_s = 0.0;
+        if (r == 0) {
+            for (int w = 0; w < G.waves; w++) {
+                q.infil += l_dsum[2 * w];
+                q.sum_s += l_dsum[2 * w + 1];
+            }
+            if (!sync) q.sum_s = 0.0;
+        }
+        q.vol = 0.0f;
+        if (sync) {
+            const float* A = l_acc + (size_t)r * 16;
+            float l8[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) l8[k] = A[2 * k] + A[2 * k + 1];
+            const float l4[4] = {l8[0] + l8[1], l8[2] + l8[3], l8[4] + l8[5], l8[6] + l8[7]};
+            q.vol = (l4[0] + l4[1]) + (l4[2] + l4[3]);
+        }
+        q.fixmask = l_fixmask[r];
+        q.wet = 0u;
+        q.pad = 0u;
+        ra_out[a0 + r] = q;
+    }
+    if (has_m) {
+        const bool up = m_h > m_old;
+        mon_out[qm] = up ? m_h : m_old;
+        tmax_out[qm] = up ? t_new : m_t;
+    }
+    for (int q = qm + nthr; q < mon_q1; q += nthr) {
+        const float hn = Ho[MONOFF[q]];
+        const float old = mon_in[q];
+        const bool up = hn > old;
+        mon_out[q] = up ? hn : old;
+        tmax_out[q] = up ? t_new : tmax_in[q];
+    }
+    if (tid == 0) {
+        unsigned long long m = 0ull;
+        int bd = 0;
+        for (int w = 0; w < G.waves; w++) {
+            m = l_smx[w] > m ? l_smx[w] : m;
+            bd |= l_red[w];
+        }
+        if (m > 0ull) atomicMax(&acc_cur->smax64, m);
+        if (bd) atomicOr(&acc_cur->bad, 1u);
+    }
+}
+
+// Ledger sums of a completed local-inertial step (inertial.rs:171-216), by the first
+// band of the candidate: the positivity fixes and the boundary discharges are the
+// reference's sequential f64 sums in its order (rows, then the north / south faces by
+// column); infiltration and CFL rate are f64 sums of the band sums in band order (the
+// reference adds cell by cell: equal to within rounding); the volume is the row FixedSums
+// (solver.rs:250-262).
+__device__ __forceinline__ void li_ledger_epilogue(const DevGrid& G, const CandCtl* Sin, const StepAcc& A, CandCtl& Oc,
+                                                   double* shd) {
+    const CandCtl& Si = *Sin;
+    const int tid = threadIdx.x;
+    const int local = Si.local;
+    const int par = Si.parity ^ 1;   // parity the completed step wrote
+    const int nx = G.nx, ny = G.ny;
+    const size_t P = G.P, plane = G.plane;
+    const LiRowAcc* ra = (const LiRowAcc*)(G.rowacc + slot(G, par, local) * ny);
+    const float* lf = G.lif + slot(G, par, local) * (size_t)(ny + nx);
+    const float* S = G.state + (size_t)local * 6 * plane + (size_t)par * 3 * plane;
+    const double* fx = G.fix64 + slot(G, par, local) * plane;
+    const double dt = Si.dt;
+    const bool sync = Si.hits_sync != 0 && !A.bad;
+    __syncthreads();   // the band step is done with the LDS
+    if (tid < 4) {
+        // lane 0: infiltration, lane 1: fixes, lane 2: boundary in and out, lane 3:
+        // volume and CFL-rate sum (sync points).
+        double s0 = 0.0, s1 = 0.0;
+        for (int r = 0; r < ny; r++) {
+            const LiRowAcc q = ra[r];
+            if (tid == 0) {
+                s0 += q.infil;
+            } else if (tid == 1) {
+                uint32_t m = q.fixmask;
+                while (m != 0u) {
+                    const int w = __builtin_ctz(m);
+                    m &= m - 1u;
+                    const int c1 = min(w * kLiCols + kLiCols, nx);
+                    for (int c = w * kLiCols; c < c1; c++) s0 += fx[(size_t)r * P + c];
+                }
+            } else if (tid == 2) {
+                const double w = (double)lf[r] * dt * G.dy;
+                const double e = (double)S[plane + (size_t)r * P + (nx - 1)] * dt * G.dy;
+                if (w > 0.0) s0 += w; else s1 -= w;
+                if (e > 0.0) s1 += e; else s0 -= e;
+            } else if (sync) {
+                s0 += (double)q.vol;
+                s1 += q.sum_s;
+            }
+        }
+        if (tid == 2)
+            for (int c = 0; c < nx; c++) {
+                const double n = (double)lf[ny + c] * dt * G.dx;
+                const double s = (double)S[2 * plane + (size_t)(ny - 1) * P + c] * dt * G.dx;
+                if (n > 0.0) s0 += n; else s1 -= n;
+                if (s > 0.0) s1 += s; else s0 -= s;
+            }
+        if (tid == 0) shd[40] = s0;
+        if (tid == 1) shd[41] = s0;
+        if (tid == 2) { shd[42] = s0; shd[43] = s1; }
+        if (tid == 3) { shd[44] = s0; shd[45] = s1; }
+    }
+    __syncthreads();
+    if (tid == 0) ledger_finish(G, Si, A, Oc, sync, shd);
 }
 
 // One launch = one step of every active candidate.  A workgroup first completes the
@@ -1063,7 +1544,7 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
 // mode gets a kernel without the other mode's code and its register budget.  SPECIAL:
 // the batch has Stage / Inflow ghosts (their code stays out of the other kernels).
 constexpr int kCtlWords = (int)(sizeof(CandCtl) / 4);
-template <int MODES, bool SPECIAL>
+template <int MODES, bool SPECIAL, bool LI>
 __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     extern __shared__ float lds[];
     __shared__ double shd[48];
@@ -1094,7 +1575,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const bool go = complete_step(p, G, sS, A, tid);
         // dry-row count of the prepared step (rows of the state just completed)
         if (designated && completed && !(sS.flags & F_DONE))
-            sS.skipped_now = count_skippable<SPECIAL>(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
+            sS.skipped_now = LI ? 0u : count_skippable<SPECIAL>(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
         if (tid == 0) {
             sS.in_flight = go ? 1u : 0u;
             s_go = go ? 1 : 0;
@@ -1113,7 +1594,30 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         }
     }
     __syncthreads();
-    if (s_go) {
+    if (LI && s_go) {
+        LiK K;
+        K.dt = uni(sS.dt);
+        K.rx = K.dt / G.dx;
+        K.ry = K.dt / G.dy;
+        K.rain = (double)uni(sS.rain32);
+        K.cap = (double)uni(sS.cap32);
+        K.h_dry = (double)p.h_dry;
+        K.g = G.g;
+        K.dx = G.dx;
+        K.dy = G.dy;
+        K.dmin = fmin(G.dx, G.dy);
+        const double t_new = uni(sS.t_new);
+        const double t_now = uni(sS.t);
+        const bool sync = uni(sS.hits_sync) != 0;
+        const int par = uni(sS.parity);
+        StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
+        if (MODES == 3) {
+            if (G.crest) li_band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+            else li_band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+        } else {
+            li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+        }
+    } else if (!LI && s_go) {
         // The step's constants are wave-uniform: keep them in scalar registers.
         StepK K;
         K.g = G.g32;
@@ -1145,8 +1649,12 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     }
     if (!designated) return;
     if (completed) {
-        const LdsLayout LL = lds_layout(G.R, G.nx, G.special != 0);
-        ledger_epilogue(a, G, Sin, A, sS, lds, LL.total, shd);
+        if (LI) {
+            li_ledger_epilogue(G, Sin, A, sS, shd);
+        } else {
+            const LdsLayout LL = lds_layout(G.R, G.nx, G.special != 0);
+            ledger_epilogue(a, G, Sin, A, sS, lds, LL.total, shd);
+        }
     }
     __syncthreads();
     for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)Sout)[w] = ((const uint32_t*)&sS)[w];
@@ -1154,6 +1662,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         StepAcc z;
         z.smax_bits = 0;
         z.bad = 0u;
+        z.smax64 = 0ull;
         a.acc[(size_t)((lk + 2) & 3) * a.ncand + cand] = z;
         a.status[cand] = sS.flags;
     }
@@ -1162,10 +1671,14 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
 // The launch entry points.  The signed-face kernel (the common case) is held to 80 VGPRs
 // (6 waves per SIMD); the crest variants keep their natural budget.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
-itr_step_signed(KArgs a, uint32_t lk) { step_body<1, false>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2, false>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3, false>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_special(KArgs a, uint32_t lk) { step_body<3, true>(a, lk); }
+itr_step_signed(KArgs a, uint32_t lk) { step_body<1, false, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2, false, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3, false, false>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_special(KArgs a, uint32_t lk) { step_body<3, true, false>(a, lk); }
+// Local-inertial grids (a separate task list, launched after the HLL kernel each step).
+__global__ void __launch_bounds__(1024) itr_step_li_signed(KArgs a, uint32_t lk) { step_body<1, false, true>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_li_crest(KArgs a, uint32_t lk) { step_body<2, false, true>(a, lk); }
+__global__ void __launch_bounds__(1024) itr_step_li_general(KArgs a, uint32_t lk) { step_body<3, true, true>(a, lk); }
 
 // Head of the first step of every candidate: control copy 1 (read by launch 0).
 __global__ void itr_init_kernel(KArgs a) {
@@ -1182,7 +1695,8 @@ __global__ void itr_init_kernel(KArgs a) {
         S.abort_kind = 4;
         S.flags |= F_DONE;
     } else {
-        S.skipped_now = count_skippable<true>(G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
+        S.skipped_now = G.li ? 0u
+                             : count_skippable<true>(G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
     }
     if (threadIdx.x == 0) {
         *C = S;
@@ -1773,8 +2287,10 @@ struct itr_batch {
     std::vector<GridHost> grids;
     DevParams prm{};                // the first grid's parameters (single-grid users: adjoint, schedules)
     KArgs args{};
+    KArgs args_li{};                // local-inertial grids: their own task list and kernel
     void* d_grids = nullptr;
     void* d_tasks = nullptr;
+    void* d_tasks_li = nullptr;
     void* d_snap = nullptr;         // CandCtl [2][ncand]
     void* d_acc = nullptr;          // StepAcc [4][ncand]
     void* d_status = nullptr;
@@ -1784,6 +2300,10 @@ struct itr_batch {
     int block = 64;
     size_t lds_bytes = 0;
     int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts
+    int ntasks_li = 0;
+    int block_li = 64;
+    size_t lds_li = 0;
+    int modes_li = 0;
     uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -1796,7 +2316,7 @@ namespace {
 void free_batch_mem(itr_batch* b) {
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
-    void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_status};
+    void* ps[] = {b->d_grids, b->d_tasks, b->d_tasks_li, b->d_snap, b->d_acc, b->d_status};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (b->h_status) (void)hipHostFree(b->h_status);
@@ -1971,8 +2491,20 @@ itr_status set_callback_flag(itr_batch* b, int has_cb, char* err, size_t errlen)
 // Enqueue n step launches whose launch indices start at lk0 (mod 4).
 itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
     const int nblk = ((b->ntasks + 7) / 8) * 8;
+    const int nblk_li = ((b->ntasks_li + 7) / 8) * 8;
     for (int s = 0; s < n; s++) {
         const uint32_t lk = (uint32_t)((lk0 + (uint64_t)s) & 3u);
+        if (b->ntasks_li > 0) {
+            if (b->modes_li & 4)
+                hipLaunchKernelGGL(itr_step_li_general, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
+            else if (b->modes_li == 1)
+                hipLaunchKernelGGL(itr_step_li_signed, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
+            else if (b->modes_li == 2)
+                hipLaunchKernelGGL(itr_step_li_crest, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
+            else
+                hipLaunchKernelGGL(itr_step_li_general, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
+        }
+        if (b->ntasks == 0) continue;
         if (b->modes & 4)
             hipLaunchKernelGGL(itr_step_special, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
         else if (b->modes == 1)
@@ -2028,7 +2560,7 @@ itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
         b->graph = nullptr;
     }
     b->graph_steps = 0;
-    if (K == 0 || b->ntasks == 0) return ITR_OK;
+    if (K == 0 || b->ntasks + b->ntasks_li == 0) return ITR_OK;
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
     itr_status ls = enqueue_steps(b, K, 0, err, errlen);
@@ -2148,8 +2680,13 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             set_err(err, errlen, "grid %d: missing arrays or empty shape", g);
             return ITR_EINVAL;
         }
-        if (G.nx > kMaxWaves * kColsPerWave) {
-            set_err(err, errlen, "grid %d: nx=%d exceeds %d columns", g, G.nx, kMaxWaves * kColsPerWave);
+        if (q->physics != ITR_PHYSICS_HLL && q->physics != ITR_PHYSICS_LOCAL_INERTIAL) {
+            set_err(err, errlen, "grid %d: unknown physics %d", g, q->physics);
+            return ITR_EINVAL;
+        }
+        const int max_cols = kMaxWaves * (q->physics == ITR_PHYSICS_LOCAL_INERTIAL ? kLiCols : kColsPerWave);
+        if (G.nx > max_cols) {
+            set_err(err, errlen, "grid %d: nx=%d exceeds %d columns", g, G.nx, max_cols);
             return ITR_DECLINED;
         }
         if (G.crest && (!G.dzx_r || !G.dzy_r)) {
@@ -2233,9 +2770,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     };
     // ---- grids
     int total = 0;
-    int max_waves = 1;
-    size_t max_lds = 0;
-    std::vector<int4> tasks;
+    int max_waves = 1, max_waves_li = 1;
+    size_t max_lds = 0, max_lds_li = 0;
+    std::vector<int4> tasks, tasks_li;
     std::vector<CandCtl> ctl;
     std::vector<DevGrid> dgrids;
     b->grids.resize(n_grids);
@@ -2257,8 +2794,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.nx = nx;
         D.ny = ny;
         D.P = (nx + 1 + 63) / 64 * 64;   // >= nx + 1: column nx holds the east boundary face
-        D.waves = (nx + kColsPerWave - 1) / kColsPerWave;
-        max_waves = std::max(max_waves, D.waves);
+        D.li = p->physics == ITR_PHYSICS_LOCAL_INERTIAL ? 1 : 0;
+        D.waves = D.li ? (nx + kLiCols - 1) / kLiCols : (nx + kColsPerWave - 1) / kColsPerWave;
+        (D.li ? max_waves_li : max_waves) = std::max(D.li ? max_waves_li : max_waves, D.waves);
         const int R = choose_band_rows(ctx, grids, n_grids, g);
         D.R = R;
         D.nbands = (ny + R - 1) / R;
@@ -2343,6 +2881,10 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     if (hv == 0.0f) hv = 0.0f;
                     h0[q] = hv;
                     const double hd = (double)hv;
+                    if (D.li) {   // initial_smax_li (solver.rs:334-344): the deepest cell
+                        s0 = std::fmax(s0, hd);
+                        continue;
+                    }
                     if (hd <= p->h_dry) continue;
                     const double cc = std::sqrt(I.g * hd);
                     const double rate = (0.0 / hd + cc) / I.dx + (0.0 / hd + cc) / I.dy;
@@ -2353,6 +2895,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                 for (int c = 0; c < nx; c++) w |= h0[(size_t)r * nx + c] > 0.0f ? 1u : 0u;
                 wet_rows[(size_t)k * ny + r] = w;
             }
+            if (D.li) s0 = std::sqrt(I.g * s0) * 0.7 / std::fmin(I.dx, I.dy);
             s0s[k] = s0;
             v0s[k] = host_volume(h0, nx, ny, D.area);
         }
@@ -2373,7 +2916,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             if (!(0.0 < dur && 0u < n_sync)) C.flags = F_DONE;
             if (p->max_steps < 0) C.flags = F_DONE;
             ctl.push_back(C);
-            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4(g, k, band, total + k));
+            for (int band = 0; band < D.nbands; band++) (D.li ? tasks_li : tasks).push_back(make_int4(g, k, band, total + k));
         }
         total += nc;
         // device buffers
@@ -2404,8 +2947,10 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         ALLOC(d_montmax, 2 * ncx * std::max(I.n_mon, 1));
         ALLOC(d_rowacc, 2 * ncx * ny);
         ALLOC(d_ybnd, 2 * ncx * 2 * nx);
-        ALLOC(d_sbuf, ncx * plane);
-        ALLOC(d_fixbuf, ncx * plane);
+        ALLOC(d_sbuf, D.li ? 1 : ncx * plane);
+        ALLOC(d_fixbuf, ncx * plane * (D.li ? 4 : 1));   // f64 fix values by parity on local-inertial grids
+        float* d_lif = nullptr;
+        if (D.li) ALLOC(d_lif, 2 * ncx * (size_t)(ny + nx));
         ALLOC(d_ledger, ncx * GH.max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
         StepRec* d_steplog = nullptr;
@@ -2488,7 +3033,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         ok = ok && hipMemset(d_monmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
              hipMemset(d_montmax, 0, 2 * ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
              hipMemset(d_ybnd, 0, 2 * ncx * 2 * nx * 4) == hipSuccess &&
-             hipMemset(d_rowacc + ncx * ny, 0, ncx * ny * sizeof(RowAcc)) == hipSuccess;
+             hipMemset(d_rowacc + ncx * ny, 0, ncx * ny * sizeof(RowAcc)) == hipSuccess &&
+             (!d_lif || hipMemset(d_lif, 0, 2 * ncx * (size_t)(ny + nx) * 4) == hipSuccess);
         if (!ok) {
             set_err(err, errlen, "upload of grid %d failed", g);
             return fail(ITR_EDEVICE);
@@ -2513,6 +3059,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.ybnd = d_ybnd;
         D.sbuf = d_sbuf;
         D.fixbuf = d_fixbuf;
+        D.fix64 = D.li ? (double*)d_fixbuf : nullptr;
+        D.lif = d_lif;
         D.ledger = d_ledger;
         D.baseline_max = d_base;
         D.guard_pos = d_gpos;
@@ -2527,18 +3075,22 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.hyd_q = d_hyd_q;
         D.max_log = log_steps > 0 ? (int32_t)std::min<int64_t>(log_steps, 1 << 30) : 0;
         dgrids.push_back(D);
-        const size_t lds = lds_layout(R, nx, D.special != 0).total * sizeof(float);
-        max_lds = std::max(max_lds, lds);
+        if (D.li) max_lds_li = std::max(max_lds_li, li_lds_layout(R, nx, D.special != 0).total * sizeof(float));
+        else max_lds = std::max(max_lds, lds_layout(R, nx, D.special != 0).total * sizeof(float));
     }
     b->ncand = total;
     b->ntasks = (int)tasks.size();
+    b->ntasks_li = (int)tasks_li.size();
     b->modes = 0;
-    for (auto& D : dgrids) b->modes |= (D.crest ? 2 : 1) | (D.special ? 4 : 0);
+    b->modes_li = 0;
+    for (auto& D : dgrids) (D.li ? b->modes_li : b->modes) |= (D.crest ? 2 : 1) | (D.special ? 4 : 0);
     if (b->modes == 0) b->modes = 1;
     b->block = max_waves * kWave;
+    b->block_li = max_waves_li * kWave;
     b->lds_bytes = std::max(max_lds, (size_t)std::max<int64_t>(ctx->min_lds, 0));
-    if (b->lds_bytes > 160 * 1024 - 1024) {   // static shared memory of the kernel: < 1 KiB
-        set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", b->lds_bytes);
+    b->lds_li = std::max(max_lds_li, (size_t)std::max<int64_t>(ctx->min_lds, 0));
+    if (std::max(b->lds_bytes, b->lds_li) > 160 * 1024 - 1024) {   // static shared memory of the kernel: < 1 KiB
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", std::max(b->lds_bytes, b->lds_li));
         return fail(ITR_DECLINED);
     }
     std::vector<void*> misc;
@@ -2554,6 +3106,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)D.ny;
     if ((s = dev_alloc(misc, &d_tasks, std::max(task_cap, tasks.size()), err, errlen)) != ITR_OK) return fail(s);
     b->d_tasks = d_tasks;
+    int4* d_tasks_li;
+    if ((s = dev_alloc(misc, &d_tasks_li, std::max(task_cap, tasks_li.size()), err, errlen)) != ITR_OK) return fail(s);
+    b->d_tasks_li = d_tasks_li;
     if ((s = dev_alloc(misc, &d_snap, 2 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_snap = d_snap;
     if ((s = dev_alloc(misc, &d_acc, 4 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
@@ -2567,6 +3122,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     }
     bool ok = hipMemcpy(d_grids, dgrids.data(), dgrids.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
               (tasks.empty() || hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
+              (tasks_li.empty() ||
+               hipMemcpy(d_tasks_li, tasks_li.data(), tasks_li.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
               (ctl.empty() || (hipMemcpy(d_snap, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess &&
                                hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
                                          hipMemcpyHostToDevice) == hipSuccess)) &&
@@ -2582,6 +3139,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->args.status = d_status;
     b->args.ntasks = b->ntasks;
     b->args.ncand = total;
+    b->args_li = b->args;
+    b->args_li.tasks = d_tasks_li;
+    b->args_li.ntasks = b->ntasks_li;
     if (total > 0) {
         hipLaunchKernelGGL(itr_init_kernel, dim3(total), dim3(64), 0, ctx->stream, b->args);
         HIP_TRY(hipGetLastError());
@@ -2611,7 +3171,7 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     b->stats.use_graph = graph ? 1 : 0;
     b->stats.launches = 0;
     b->stats.syncs = 0;
-    if (b->ncand == 0 || b->ntasks == 0) return ITR_OK;
+    if (b->ncand == 0 || b->ntasks + b->ntasks_li == 0) return ITR_OK;
     itr_status s = prepare_graph(b, graph ? K : 0, err, errlen);
     if (s != ITR_OK) return s;
     std::vector<CandCtl> c0;
@@ -2697,7 +3257,7 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     HIP_TRY(hipEventCreate(&e1));
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(e0, ctx->stream));
-    if (b->ntasks > 0 && n_steps > 0)
+    if (b->ntasks + b->ntasks_li > 0 && n_steps > 0)
         if ((s = launch_steps(b, n_steps, err, errlen)) != ITR_OK) return s;
     HIP_TRY(hipEventRecord(e1, ctx->stream));
     HIP_TRY(hipEventSynchronize(e1));
@@ -2823,6 +3383,10 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     }
     if (grid->crest) {
         set_err(err, errlen, "the adjoint supports signed-face levels only");
+        return ITR_DECLINED;
+    }
+    if (p->physics != ITR_PHYSICS_HLL) {
+        set_err(err, errlen, "the adjoint differentiates the HLL step only");
         return ITR_DECLINED;
     }
     if (grid->ghost)
@@ -3127,16 +3691,18 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
 // Re-tile a resident batch: new band heights per grid (tasks, band monitor masks and
 // LDS size follow).  State and control blocks are untouched.
 itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t errlen) {
-    size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0);
+    size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0), lds_li = lds;
     for (size_t g = 0; g < b->grids.size(); g++) {
         const DevGrid& D = b->grids[g].d;
-        lds = std::max(lds, lds_layout(std::max(1, std::min(Rs[g], D.ny)), D.nx, D.special != 0).total * sizeof(float));
+        const int R = std::max(1, std::min(Rs[g], D.ny));
+        if (D.li) lds_li = std::max(lds_li, li_lds_layout(R, D.nx, D.special != 0).total * sizeof(float));
+        else lds = std::max(lds, lds_layout(R, D.nx, D.special != 0).total * sizeof(float));
     }
-    if (lds > 160 * 1024 - 1024) {
-        set_err(err, errlen, "band of %zu bytes of LDS does not fit", lds);
+    if (std::max(lds, lds_li) > 160 * 1024 - 1024) {
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit", std::max(lds, lds_li));
         return ITR_DECLINED;
     }
-    std::vector<int4> tasks;
+    std::vector<int4> tasks, tasks_li;
     std::vector<DevGrid> dg;
     for (size_t g = 0; g < b->grids.size(); g++) {
         GridHost& GH = b->grids[g];
@@ -3145,13 +3711,19 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
         D.R = R;
         D.nbands = (D.ny + R - 1) / R;
         for (int k = 0; k < D.ncand; k++)
-            for (int band = 0; band < D.nbands; band++) tasks.push_back(make_int4((int)g, k, band, D.cand0 + k));
+            for (int band = 0; band < D.nbands; band++)
+                (D.li ? tasks_li : tasks).push_back(make_int4((int)g, k, band, D.cand0 + k));
         dg.push_back(D);
     }
     HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (!tasks_li.empty())
+        HIP_TRY(hipMemcpy(b->d_tasks_li, tasks_li.data(), tasks_li.size() * sizeof(int4), hipMemcpyHostToDevice));
     b->ntasks = (int)tasks.size();
     b->args.ntasks = b->ntasks;
+    b->ntasks_li = (int)tasks_li.size();
+    b->args_li.ntasks = b->ntasks_li;
+    b->lds_li = lds_li;
     b->lds_bytes = lds;
     b->stats.band_rows = dg.empty() ? 0 : dg[0].R;
     if (b->graph) {

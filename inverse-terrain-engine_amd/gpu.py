@@ -86,6 +86,13 @@ def _params_c(p: SolverParams, keep: _Keep) -> _abi.SolverParamsC:
         q.locked_dt, q.n_locked = ld.ctypes.data_as(_abi._dp), len(ld)
     q.max_steps = int(p.max_steps)
     q.skip_dry = 1 if p.skip_dry else 0
+    # KernelKind: the oracle kernel is bitwise equal to the fused one (equivalence.rs),
+    # local inertial is the coarse-level screening physics (inertial.rs).
+    phys = {"fused": _abi.ITR_PHYSICS_HLL, "oracle": _abi.ITR_PHYSICS_HLL,
+            "inertial": _abi.ITR_PHYSICS_LOCAL_INERTIAL}.get(p.kernel)
+    if phys is None:
+        raise ValueError(f"unknown kernel {p.kernel!r}")
+    q.physics = phys
     return q
 
 

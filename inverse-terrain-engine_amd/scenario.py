@@ -39,6 +39,7 @@ class SolverParams:
     cfl_hard: float = 0.5
     inflow_scale: float = 1.0
     max_steps: int = 0                            # test control: stop after this many steps
+    kernel: str = "fused"                         # KernelKind (solver.rs:19-24): "fused" (HLL) | "inertial"
 
 
 def scaled(p: SolverParams, rain: float, inflow: float) -> SolverParams:

@@ -267,6 +267,58 @@ def ladder(solver_cls, sc, steps):
             "band_rows": [r for r, _ in tuned], "steps": steps}
 
 
+def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_cands=16, cpu_steps=300):
+    """The same candidates on the face-crest level-2 grid (2x coarser), run with the
+    local-inertial screening physics (inertial.rs; problem.rs:178-181) and with HLL.
+    Reports device rates and simulated seconds per step (the screening physics takes
+    larger steps), and the oracle's local-inertial f32 rate on the host cores."""
+    import copy
+    import numpy as np
+    base2 = tr.TerrainBase.build(w.base.fine, 2, w.base.segments)
+    terrs = [tr.PreparedTerrain(base2).apply_edit(e) for e in w.edits]
+    nx, ny = w.base.nx, w.base.ny
+    mon = np.unique((w.monitors // nx // 2) * base2.nx + (w.monitors % nx) // 2).astype(np.uint32)
+    out = {"workload": f"{len(terrs)} candidates x {base2.nx}x{base2.ny} face-crest cells at "
+                       f"{base2.dx:g} m (level 2 of the forward workload)", "steps": steps}
+    for kernel in ("inertial", "fused"):
+        p = copy.copy(w.params)
+        p.kernel = kernel
+        s = solver_cls(0)
+        sess = s.session(p, [(base2, terrs, mon, None)])
+        sess.advance(warmup)
+        t_a = np.array([r.t_end for r in sess.results(want_state=False)])
+        t0 = time.perf_counter()
+        st = sess.advance(steps)
+        wall = time.perf_counter() - t0
+        t_b = np.array([r.t_end for r in sess.results(want_state=False)])
+        sess.close()
+        s.close()
+        kern_s = st["device_ms"] / 1e3 / steps
+        cells = st["cell_steps"] / steps
+        out["local_inertial" if kernel == "inertial" else "hll"] = {
+            "cell_steps_per_s": st["cell_steps"] / wall, "us_per_step": kern_s * 1e6,
+            "sim_s_per_step": float((t_b - t_a).mean()) / steps,
+            "sim_s_per_wall_s": float((t_b - t_a).mean()) / wall,
+            "hbm_frac_40B": 40.0 * cells / kern_s / HBM_PEAK}
+    li, hl = out["local_inertial"], out["hll"]
+    out["sim_time_speedup_vs_hll"] = li["sim_s_per_wall_s"] / hl["sim_s_per_wall_s"]
+    if cpu_threads:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po  # the checker, timed here as the CPU baseline only
+        f = w.base.fine
+        ot = po.Terrain(f.dem, f.wall, f.n, dx=f.geo.dx, dy=f.geo.dy, origin=(f.geo.origin_x, f.geo.origin_y),
+                        level=2, segments=list(w.base.segments))
+        p = w.params
+        op = po.SolverParams(p.duration_s, p.sync_interval_s, cfl=p.cfl, rain=list(p.rain),
+                             infil_capacity=p.infil_capacity, kernel="inertial")
+        params, _keep = po._params(op, 32, 0, cpu_steps)
+        cw, _, cu = _run_cands(po, ot, params, [e.as_dict() for e in w.edits[:cpu_cands]], cpu_threads, mon)
+        out["cpu_local_inertial"] = {"value": float(cu.sum()) / cw, "unit": "cell-steps/s", "cores": cpu_threads,
+                                     "sample": f"{cpu_cands} candidates x {cpu_steps} steps, one per thread"}
+        out["gpu_over_cpu"] = li["cell_steps_per_s"] / out["cpu_local_inertial"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +333,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
     ap.add_argument("--adjoint-horizon", type=int, default=120)
     ap.add_argument("--no-ladder", action="store_true", help="skip the fidelity-ladder leg")
+    ap.add_argument("--no-screening", action="store_true", help="skip the local-inertial screening leg")
+    ap.add_argument("--screening-only", action="store_true",
+                    help="run only the local-inertial screening leg (profiling) and print it")
     ap.add_argument("--no-autotune", action="store_true", help="use the default band height")
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -300,6 +355,10 @@ def main():
     t0 = time.time()
     w = sc.config5(n_cand=a.cands, n=a.n)
     t_prep = time.time() - t0
+    if a.screening_only:
+        from inverse_terrain_engine_amd import terrain as tr
+        print(json.dumps(screening_leg(GpuSolver, sc, tr, w, a.steps, warmup=a.warmup)), flush=True)
+        return
     solver = GpuSolver(0, band_rows=a.band_rows)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
@@ -368,6 +427,9 @@ def main():
     if not a.copy_probe:
         from inverse_terrain_engine_amd import terrain as tr
         out["prep"] = prep_leg(solver, sc, tr, w)
+        if not a.no_screening:
+            out["screening"] = screening_leg(GpuSolver, sc, tr, w, 200,
+                                             cpu_threads=0 if a.no_cpu else (a.cpu_threads or min(16, os.cpu_count() or 1)))
     if adj is not None:
         out["adjoint"] = adj
     if adj16 is not None:

@@ -1478,9 +1478,10 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
 // reference adds cell by cell: equal to within rounding); the volume is the row FixedSums
 // (solver.rs:250-262).
 __device__ __forceinline__ void li_ledger_epilogue(const DevGrid& G, const CandCtl* Sin, const StepAcc& A, CandCtl& Oc,
-                                                   double* shd) {
+                                                   float* stage, size_t stage_floats, double* shd) {
     const CandCtl& Si = *Sin;
     const int tid = threadIdx.x;
+    const int nthr = G.waves * kWave;
     const int local = Si.local;
     const int par = Si.parity ^ 1;   // parity the completed step wrote
     const int nx = G.nx, ny = G.ny;
@@ -1491,44 +1492,85 @@ __device__ __forceinline__ void li_ledger_epilogue(const DevGrid& G, const CandC
     const double* fx = G.fix64 + slot(G, par, local) * plane;
     const double dt = Si.dt;
     const bool sync = Si.hits_sync != 0 && !A.bad;
+    // Four sequential chains, each on its own wave when the workgroup has four (else on
+    // lanes 0..3 of wave 0): 0 infiltration, 1 fixes, 2 boundary in / out, 3 volume and
+    // CFL-rate sum.  Their terms are staged through LDS in chunks of rows, then columns.
+    const int ch = nthr >= 4 * kWave ? ((tid & 63) == 0 ? (tid >> 6) : -1) : (tid < 4 ? tid : -1);
+    const bool chain = ch >= 0 && ch < 4;
+    double s0 = 0.0, s1 = 0.0;
+    double* T = (double*)stage;
+    const int chunk = max(1, (int)(stage_floats / 12));   // rows of 6 f64 terms
     __syncthreads();   // the band step is done with the LDS
-    if (tid < 4) {
-        // lane 0: infiltration, lane 1: fixes, lane 2: boundary in and out, lane 3:
-        // volume and CFL-rate sum (sync points).
-        double s0 = 0.0, s1 = 0.0;
-        for (int r = 0; r < ny; r++) {
+    for (int r0 = 0; r0 < ny; r0 += chunk) {
+        const int nr = min(chunk, ny - r0);
+        for (int i = tid; i < nr; i += nthr) {
+            const int r = r0 + i;
             const LiRowAcc q = ra[r];
-            if (tid == 0) {
-                s0 += q.infil;
-            } else if (tid == 1) {
-                uint32_t m = q.fixmask;
-                while (m != 0u) {
-                    const int w = __builtin_ctz(m);
-                    m &= m - 1u;
-                    const int c1 = min(w * kLiCols + kLiCols, nx);
-                    for (int c = w * kLiCols; c < c1; c++) s0 += fx[(size_t)r * P + c];
+            double* t = T + 6 * i;
+            t[0] = q.infil;
+            t[1] = (double)lf[r] * dt * G.dy;                                  // west face (inertial.rs:198-201)
+            t[2] = (double)S[plane + (size_t)r * P + (nx - 1)] * dt * G.dy;    // east face
+            t[3] = (double)q.vol;
+            t[4] = q.sum_s;
+            t[5] = __longlong_as_double((long long)q.fixmask);
+        }
+        __syncthreads();
+        if (chain) {
+            if (ch == 0) {
+#pragma unroll 8
+                for (int i = 0; i < nr; i++) s0 += T[6 * i];
+            } else if (ch == 1) {
+                for (int i = 0; i < nr; i++) {
+                    uint32_t m = (uint32_t)__double_as_longlong(T[6 * i + 5]);
+                    const size_t rb = (size_t)(r0 + i) * P;
+                    while (m != 0u) {   // waves of the row that had a positivity fix
+                        const int w = __builtin_ctz(m);
+                        m &= m - 1u;
+                        const int c1 = min(w * kLiCols + kLiCols, nx);
+                        for (int c = w * kLiCols; c < c1; c++) s0 += fx[rb + c];
+                    }
                 }
-            } else if (tid == 2) {
-                const double w = (double)lf[r] * dt * G.dy;
-                const double e = (double)S[plane + (size_t)r * P + (nx - 1)] * dt * G.dy;
-                if (w > 0.0) s0 += w; else s1 -= w;
-                if (e > 0.0) s1 += e; else s0 -= e;
+            } else if (ch == 2) {
+#pragma unroll 4
+                for (int i = 0; i < nr; i++) {
+                    const double w = T[6 * i + 1], e = T[6 * i + 2];
+                    if (w > 0.0) s0 += w; else s1 -= w;
+                    if (e > 0.0) s1 += e; else s0 -= e;
+                }
             } else if (sync) {
-                s0 += (double)q.vol;
-                s1 += q.sum_s;
+#pragma unroll 8
+                for (int i = 0; i < nr; i++) {
+                    s0 += T[6 * i + 3];
+                    s1 += T[6 * i + 4];
+                }
             }
         }
-        if (tid == 2)
-            for (int c = 0; c < nx; c++) {
-                const double n = (double)lf[ny + c] * dt * G.dx;
-                const double s = (double)S[2 * plane + (size_t)(ny - 1) * P + c] * dt * G.dx;
+        __syncthreads();
+    }
+    const int cchunk = max(1, (int)(stage_floats / 4));   // columns of 2 f64 terms
+    for (int c0 = 0; c0 < nx; c0 += cchunk) {
+        const int nc = min(cchunk, nx - c0);
+        for (int i = tid; i < nc; i += nthr) {
+            const int c = c0 + i;
+            T[2 * i] = (double)lf[ny + c] * dt * G.dx;                                  // north face
+            T[2 * i + 1] = (double)S[2 * plane + (size_t)(ny - 1) * P + c] * dt * G.dx;  // south face
+        }
+        __syncthreads();
+        if (chain && ch == 2) {
+#pragma unroll 4
+            for (int i = 0; i < nc; i++) {
+                const double n = T[2 * i], sv = T[2 * i + 1];
                 if (n > 0.0) s0 += n; else s1 -= n;
-                if (s > 0.0) s1 += s; else s0 -= s;
+                if (sv > 0.0) s1 += sv; else s0 -= sv;
             }
-        if (tid == 0) shd[40] = s0;
-        if (tid == 1) shd[41] = s0;
-        if (tid == 2) { shd[42] = s0; shd[43] = s1; }
-        if (tid == 3) { shd[44] = s0; shd[45] = s1; }
+        }
+        __syncthreads();
+    }
+    if (chain) {
+        if (ch == 0) shd[40] = s0;
+        if (ch == 1) shd[41] = s0;
+        if (ch == 2) { shd[42] = s0; shd[43] = s1; }
+        if (ch == 3) { shd[44] = s0; shd[45] = s1; }
     }
     __syncthreads();
     if (tid == 0) ledger_finish(G, Si, A, Oc, sync, shd);
@@ -1650,7 +1692,8 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     if (!designated) return;
     if (completed) {
         if (LI) {
-            li_ledger_epilogue(G, Sin, A, sS, shd);
+            const LiLds LL = li_lds_layout(G.R, G.nx, G.special != 0);
+            li_ledger_epilogue(G, Sin, A, sS, lds, LL.total, shd);
         } else {
             const LdsLayout LL = lds_layout(G.R, G.nx, G.special != 0);
             ledger_epilogue(a, G, Sin, A, sS, lds, LL.total, shd);

@@ -1492,7 +1492,7 @@ double orc_run_candidates(const orc_terrain* base, const orc_edit* edits, int32_
             orc_summary s;
             std::memset(&s, 0, sizeof(s));
             orc_params q = *p;
-            q.kernel = 1;
+            if (q.kernel != 2) q.kernel = 1;   // the fused HLL kernel, or the local-inertial one
             q.precision = 32;
             run_impl(pts[c], &q, ws, nullptr, nullptr, &s, nullptr, nullptr);
             steps_out[c] = s.steps;

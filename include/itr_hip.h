@@ -66,9 +66,9 @@ typedef struct {
 
 /* SolverParams (crates/itr-hydro/src/solver.rs:26-49).  Rain points are (t [s],
  * rate [m/s]) of the piecewise-linear hyetograph (crates/itr-core/src/scenario.rs:527-589).
- * locked_dt: baseline-locked dt schedule (solver.rs:41-44, 452-455) or NULL; a step
- * whose locked dt fails the hard-CFL check ends the candidate with aborted = 4
- * (the reference restarts it adaptively from the last sync point instead).
+ * locked_dt: baseline-locked dt schedule (solver.rs:41-44, 452-455) or NULL; when a
+ * step's locked dt fails the hard-CFL check the candidate restarts adaptively from its
+ * last sync point, as the reference does (solver.rs:456-480; itr_run_out.dt_fallback_t).
  * max_steps: stop each candidate after this many steps (0 = no limit; test control). */
 typedef struct {
     double duration_s, sync_interval_s, cfl, dt_max_s, h_dry, h_eps;
@@ -154,7 +154,7 @@ typedef struct {
     double t_end, volume_m3, dt_min, dt_max, dt_mean, lts, v0, rel_residual, abort_value;
     uint64_t steps, cell_updates, rows_skipped;
     int32_t bad;        /* Unhealthy (non-finite or beyond-roundoff negative depth) */
-    int32_t aborted;    /* 0 none, 1 guard, 2 sync callback, 3 unhealthy, 4 locked-dt check failed */
+    int32_t aborted;    /* 0 none, 1 guard, 2 sync callback, 3 unhealthy */
     int32_t abort_cell; /* guard: offending cell (row*nx+col) */
     int32_t abort_sync; /* sync index at the abort */
     int32_t sync_count;
@@ -167,6 +167,9 @@ typedef struct {
     double* ledger;       /* [max_ledger][8] rows (t, volume, rain, inflow, outflow, infil, fix, residual) */
     int32_t max_ledger;
     int32_t reserved;
+    double dt_fallback_t; /* RunSummary::dt_fallback_t: the time a locked dt schedule failed its
+                             hard-CFL check and the run restarted adaptively from its last sync
+                             point (solver.rs:456-480), or -1 */
 } itr_run_out;
 
 /* Observer::on_sync (crates/itr-core/src/model.rs:59-61) for one candidate, called on

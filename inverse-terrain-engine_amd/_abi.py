@@ -56,7 +56,7 @@ class RunOutC(C.Structure):
                 ("rows_skipped", C.c_uint64), ("bad", C.c_int32), ("aborted", C.c_int32), ("abort_cell", C.c_int32),
                 ("abort_sync", C.c_int32), ("sync_count", C.c_int32), ("n_ledger", C.c_int32),
                 ("monitor_max", _dp), ("monitor_tmax", _dp), ("depth", _fp), ("qx", _fp), ("qy", _fp),
-                ("ledger", _dp), ("max_ledger", C.c_int32), ("reserved", C.c_int32)]
+                ("ledger", _dp), ("max_ledger", C.c_int32), ("reserved", C.c_int32), ("dt_fallback_t", C.c_double)]
 
 
 class StatsC(C.Structure):

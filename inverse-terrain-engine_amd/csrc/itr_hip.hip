@@ -125,6 +125,14 @@ struct DevGrid {
     float* fixbuf;             // [nslot][plane] positivity-fix values of rows that have one
     double* ledger;            // [nslot][max_ledger][8]
     float* lif;                // local-inertial: [2][nslot][ny + nx] west then north boundary faces
+    // Locked dt schedule: the state at the last sync point, for the adaptive restart
+    // (solver.rs:446-480).  NULL when the grid's parameters have no schedule.
+    float* snap_state;         // [nslot][3][plane]
+    float* snap_mon;           // [nslot][n_mon]
+    double* snap_tmax;         // [nslot][n_mon]
+    RowAcc* snap_rows;         // [nslot][ny]
+    float* snap_lif;           // local-inertial: [nslot][ny + nx]
+    struct CandCtl* snap_ctl;  // [nslot]
     double* fix64;             // local-inertial: [2][nslot][plane] positivity-fix values (f64), by state parity
     struct StepRec* steplog;   // [nslot][max_log] constants of every step run (adjoint replay)
     const double* baseline_max;
@@ -155,7 +163,10 @@ struct CandCtl {
     int32_t abort_kind, abort_cell, abort_sync, local;
     double abort_value;
     uint32_t in_flight;   // a step was launched whose results are not yet accounted
-    uint32_t pad[3];
+    uint32_t locked_off;  // the locked dt schedule was abandoned (adaptive from dt_fallback_t on)
+    uint32_t restore;     // the prepared step reads its old state from the sync snapshot
+    uint32_t pad;
+    double dt_fallback_t; // RunSummary::dt_fallback_t, -1 when the schedule held
 };
 
 // Cross-workgroup results of one step of one candidate (atomics).  Four copies,
@@ -378,7 +389,7 @@ __device__ __forceinline__ bool prepare_step(const DevParams& p, const DevGrid& 
     const double t = C->t;
     const double t_sync = fmin((double)(C->sync_index + 1) * p.sync, p.duration);
     double dt = fmin(C->dt_cfl, p.dt_max);
-    if (p.locked) {
+    if (p.locked && !C->locked_off) {
         if (C->step < (uint64_t)p.n_locked && p.locked[C->step] * C->s_cur <= p.cfl_hard) dt = p.locked[C->step];
         else return false;
     }
@@ -498,7 +509,9 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
                                               int lane) {
     if (S.flags & F_DONE) return false;
     if (S.in_flight) {
+        const bool was_sync = S.hits_sync != 0;
         S.in_flight = 0;
+        S.restore = 0;
         S.step += 1;
         S.t = S.t_new;
         S.dt_min = fmin(S.dt_min, S.dt);
@@ -534,9 +547,20 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
             if (go && !(S.t < p.duration && S.sync_index < p.n_sync)) { flags |= F_DONE; go = false; }
             if (go && p.max_steps > 0 && S.step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = false; }
             if (go && !prepare_step(p, G, &S, p.rain_t, p.rain_r)) {
-                S.abort_kind = 4;
-                flags |= F_DONE;
-                go = false;
+                // The locked step fails the hard-CFL check: restart adaptively from the
+                // last sync snapshot (solver.rs:456-480).  After a sync step the snapshot
+                // is the state just completed, so only the schedule is dropped.
+                if (!was_sync) {
+                    CandCtl K = G.snap_ctl[S.local];
+                    K.parity = S.parity;
+                    K.flags = flags;
+                    K.restore = 1;
+                    S = K;
+                }
+                S.locked_off = 1;
+                S.dt_fallback_t = S.t;
+                if (p.max_steps > 0 && S.step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = false; }
+                else (void)prepare_step(p, G, &S, p.rain_t, p.rain_r);   // adaptive: cannot fail
             }
         }
         S.flags = flags;
@@ -598,7 +622,8 @@ __device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local
 // par^1; the band's CFL max and health flag go to the candidate's StepAcc.
 template <bool CREST, int DEPTH, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
-                                          double t, double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
+                                          double t, double t_new, bool sync, int par, bool from_snap, StepAcc* acc_cur,
+                                          float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -615,8 +640,12 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     float* base = G.state + (size_t)local * cstride;
     ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)npar * 3 * G.plane);
     const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
-    const ITR_GLOBAL float* __restrict__ mon_in = gp(G.mon_max + slot(G, par, local) * G.n_mon);
-    const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
+    // The old state: copy `par`, or the sync snapshot after a locked-dt restart.
+    const float* old_state = from_snap ? G.snap_state + (size_t)local * 3 * G.plane : base + (size_t)par * 3 * G.plane;
+    const ITR_GLOBAL float* __restrict__ mon_in =
+        gp(from_snap ? G.snap_mon + (size_t)local * G.n_mon : G.mon_max + slot(G, par, local) * G.n_mon);
+    const ITR_GLOBAL double* __restrict__ tmax_in =
+        gp(from_snap ? G.snap_tmax + (size_t)local * G.n_mon : G.mon_tmax + slot(G, par, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
     // The band's monitors: positions mon_q0 .. mon_q1-1 of the sorted monitor list; this
@@ -629,7 +658,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
 
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
     ITR_GLOBAL float* __restrict__ fixbuf = gp(G.fixbuf + (size_t)local * G.plane);
-    const RowAcc* ra_in = G.rowacc + slot(G, par, local) * ny;
+    const RowAcc* ra_in = from_snap ? G.snap_rows + (size_t)local * ny : G.rowacc + slot(G, par, local) * ny;
     RowAcc* ra_out = G.rowacc + slot(G, npar, local) * ny;
     float* yb_out = G.ybnd + slot(G, npar, local) * 2 * nx;
     // Lane columns for addressing (unsigned, so loads use a scalar row base + 32-bit
@@ -645,7 +674,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     // Everything read from the grid descriptor is read here, before the first store:
     // later loads of it could not be proven unclobbered and would become per-lane
     // loads (and per-lane buffer descriptors).
-    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(base + (size_t)par * 3 * G.plane, 3 * G.plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(old_state, 3 * G.plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzx = make_rsrc(G.dzx + (size_t)local * G.plane, G.plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzy = make_rsrc(G.dzy + (size_t)local * G.plane_y, G.plane_y * 4);
     const __amdgpu_buffer_rsrc_t rs_dzx2 = make_rsrc(CREST ? G.dzx2 + (size_t)local * G.plane : G.dzx, G.plane * 4);
@@ -1184,7 +1213,8 @@ __host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
 // computed in the prologue), then continuity and sources (inertial.rs:160-196).
 template <bool CREST, bool SPECIAL>
 __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, int local, int band, const LiK& K, double t,
-                                             double t_new, bool sync, int par, StepAcc* acc_cur, float* lds) {
+                                             double t_new, bool sync, int par, bool from_snap, StepAcc* acc_cur,
+                                             float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -1202,10 +1232,15 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     float* base = G.state + (size_t)local * 6 * plane;
     ITR_GLOBAL float* __restrict__ Ho = gp(base + (size_t)npar * 3 * plane);
     const ITR_GLOBAL int32_t* __restrict__ GHOST = gp(G.ghost);
-    const ITR_GLOBAL float* __restrict__ lif_in = gp(G.lif + slot(G, par, local) * (size_t)(ny + nx));
+    // The old state: copy `par`, or the sync snapshot after a locked-dt restart.
+    const float* old_state = from_snap ? G.snap_state + (size_t)local * 3 * plane : base + (size_t)par * 3 * plane;
+    const ITR_GLOBAL float* __restrict__ lif_in =
+        gp(from_snap ? G.snap_lif + (size_t)local * (ny + nx) : G.lif + slot(G, par, local) * (size_t)(ny + nx));
     ITR_GLOBAL float* __restrict__ lif_out = gp(G.lif + slot(G, npar, local) * (size_t)(ny + nx));
-    const ITR_GLOBAL float* __restrict__ mon_in = gp(G.mon_max + slot(G, par, local) * G.n_mon);
-    const ITR_GLOBAL double* __restrict__ tmax_in = gp(G.mon_tmax + slot(G, par, local) * G.n_mon);
+    const ITR_GLOBAL float* __restrict__ mon_in =
+        gp(from_snap ? G.snap_mon + (size_t)local * G.n_mon : G.mon_max + slot(G, par, local) * G.n_mon);
+    const ITR_GLOBAL double* __restrict__ tmax_in =
+        gp(from_snap ? G.snap_tmax + (size_t)local * G.n_mon : G.mon_tmax + slot(G, par, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
     const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
@@ -1220,7 +1255,7 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     const uint32_t cb = ccol * 4u, xb = fcol * 4u;
     const uint32_t rowb = (uint32_t)P * 4u;
     const uint32_t planeb = (uint32_t)(plane * 4u);
-    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(base + (size_t)par * 3 * plane, 3 * plane * 4);
+    const __amdgpu_buffer_rsrc_t rs_h = make_rsrc(old_state, 3 * plane * 4);
     const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)npar * 3 * plane, 3 * plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzx = make_rsrc(G.dzx + (size_t)local * plane, plane * 4);
     const __amdgpu_buffer_rsrc_t rs_dzy = make_rsrc(G.dzy + (size_t)local * G.plane_y, G.plane_y * 4);
@@ -1576,6 +1611,40 @@ __device__ __forceinline__ void li_ledger_epilogue(const DevGrid& G, const CandC
     if (tid == 0) ledger_finish(G, Si, A, Oc, sync, shd);
 }
 
+// The band's part of the sync snapshot of a locked-dt run (solver.rs:446-448, 609-612):
+// the new state rows, row partials, monitors and (local inertial) boundary faces the band
+// just wrote as copy `npar`.
+__device__ __forceinline__ void snapshot_band(const DevGrid& G, int local, int band, int npar) {
+    const int nx = G.nx, ny = G.ny, R = G.R, P = G.P;
+    const int a0 = band * R, b0 = min(a0 + R, ny), nrows = b0 - a0;
+    const int tid = threadIdx.x, nthr = G.waves * kWave;
+    const size_t plane = G.plane;
+    __syncthreads();   // the band's stores are done
+    const float* src = G.state + (size_t)local * 6 * plane + (size_t)npar * 3 * plane;
+    float* dst = G.snap_state + (size_t)local * 3 * plane;
+    const int per = nrows * nx;
+    for (int i = tid; i < 3 * per; i += nthr) {
+        const int q = i / per, r = (i % per) / nx, c = i % nx;
+        const size_t o = (size_t)q * plane + (size_t)(a0 + r) * P + c;
+        dst[o] = src[o];
+    }
+    const RowAcc* ra = G.rowacc + slot(G, npar, local) * ny;
+    for (int r = tid; r < nrows; r += nthr) G.snap_rows[(size_t)local * ny + a0 + r] = ra[a0 + r];
+    const float* mm = G.mon_max + slot(G, npar, local) * G.n_mon;
+    const double* mt = G.mon_tmax + slot(G, npar, local) * G.n_mon;
+    for (int q = G.mon_row[a0] + tid; q < G.mon_row[b0]; q += nthr) {
+        G.snap_mon[(size_t)local * G.n_mon + q] = mm[q];
+        G.snap_tmax[(size_t)local * G.n_mon + q] = mt[q];
+    }
+    if (G.li) {
+        const float* lf = G.lif + slot(G, npar, local) * (size_t)(ny + nx);
+        float* sl = G.snap_lif + (size_t)local * (ny + nx);
+        for (int r = tid; r < nrows; r += nthr) sl[a0 + r] = lf[a0 + r];
+        if (a0 == 0)
+            for (int c = tid; c < nx; c += nthr) sl[ny + c] = lf[ny + c];
+    }
+}
+
 // One launch = one step of every active candidate.  A workgroup first completes the
 // candidate's previous step from its results (identically in every workgroup of the
 // candidate), then, unless the candidate finished or waits for the host, runs its band
@@ -1617,7 +1686,11 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const bool go = complete_step(p, G, sS, A, tid);
         // dry-row count of the prepared step (rows of the state just completed)
         if (designated && completed && !(sS.flags & F_DONE))
-            sS.skipped_now = LI ? 0u : count_skippable<SPECIAL>(p, G, &sS, G.rowacc + slot(G, sS.parity, local) * G.ny, tid);
+            sS.skipped_now = LI ? 0u
+                                : count_skippable<SPECIAL>(p, G, &sS,
+                                                           sS.restore ? G.snap_rows + (size_t)local * G.ny
+                                                                      : G.rowacc + slot(G, sS.parity, local) * G.ny,
+                                                           tid);
         if (tid == 0) {
             sS.in_flight = go ? 1u : 0u;
             s_go = go ? 1 : 0;
@@ -1652,13 +1725,15 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const double t_now = uni(sS.t);
         const bool sync = uni(sS.hits_sync) != 0;
         const int par = uni(sS.parity);
+        const bool from_snap = uni((int)sS.restore) != 0;
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
         if (MODES == 3) {
-            if (G.crest) li_band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
-            else li_band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+            if (G.crest) li_band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
+            else li_band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
         } else {
-            li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+            li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
         }
+        if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!LI && s_go) {
         // The step's constants are wave-uniform: keep them in scalar registers.
         StepK K;
@@ -1678,19 +1753,21 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const double t_now = uni(sS.t);
         const bool sync = uni(sS.hits_sync) != 0;
         const int par = uni(sS.parity);
+        const bool from_snap = uni((int)sS.restore) != 0;
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
         if (MODES == 3) {
-            if (G.crest) band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
-            else band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur, lds);
+            if (G.crest) band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
+            else band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
         } else {
-            band_step<MODES == 2, MODES == 1 ? 2 : 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, acc_cur,
-                                                              lds);
+            band_step<MODES == 2, MODES == 1 ? 2 : 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
+                                                              acc_cur, lds);
         }
+        if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!designated) {
         return;
     }
     if (!designated) return;
-    if (completed) {
+    if (completed && !sS.restore) {   // (a restart discards the completed step)
         if (LI) {
             const LiLds LL = li_lds_layout(G.R, G.nx, G.special != 0);
             li_ledger_epilogue(G, Sin, A, sS, lds, LL.total, shd);
@@ -1701,6 +1778,9 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     }
     __syncthreads();
     for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)Sout)[w] = ((const uint32_t*)&sS)[w];
+    // The control part of the sync snapshot, once the sync step is accounted.
+    if (completed && Sin->hits_sync && G.snap_ctl && !sS.locked_off && !(sS.flags & F_DONE))
+        for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)(G.snap_ctl + local))[w] = ((const uint32_t*)&sS)[w];
     if (tid == 0) {
         StepAcc z;
         z.smax_bits = 0;
@@ -1734,10 +1814,12 @@ __global__ void itr_init_kernel(KArgs a) {
     }
     CandCtl S = *C;
     bool ok = prepare_step(G.prm, G, &S, G.prm.rain_t, G.prm.rain_r);
-    if (!ok) {
-        S.abort_kind = 4;
-        S.flags |= F_DONE;
-    } else {
+    if (!ok) {   // the first locked step fails its check: adaptive from t = 0 (solver.rs:456-480)
+        S.locked_off = 1;
+        S.dt_fallback_t = S.t;
+        ok = prepare_step(G.prm, G, &S, G.prm.rain_t, G.prm.rain_r);
+    }
+    {
         S.skipped_now = G.li ? 0u
                              : count_skippable<true>(G.prm, G, &S, G.rowacc + slot(G, S.parity, S.local) * G.ny, threadIdx.x & 63);
     }
@@ -2955,6 +3037,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             C.s_cur = s0;
             C.dt_cfl = s0 > 0.0 ? std::fmin(p->cfl / s0, p->dt_max_s) : p->dt_max_s;
             C.dt_min = 1.79769313486231570815e+308;
+            C.dt_fallback_t = -1.0;
             C.flags = 0;
             if (!(0.0 < dur && 0u < n_sync)) C.flags = F_DONE;
             if (p->max_steps < 0) C.flags = F_DONE;
@@ -2994,6 +3077,19 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         ALLOC(d_fixbuf, ncx * plane * (D.li ? 4 : 1));   // f64 fix values by parity on local-inertial grids
         float* d_lif = nullptr;
         if (D.li) ALLOC(d_lif, 2 * ncx * (size_t)(ny + nx));
+        // Sync snapshot of a locked-dt grid (the adaptive restart, solver.rs:446-480).
+        float *d_snap_state = nullptr, *d_snap_mon = nullptr, *d_snap_lif = nullptr;
+        double* d_snap_tmax = nullptr;
+        RowAcc* d_snap_rows = nullptr;
+        CandCtl* d_snap_ctl = nullptr;
+        if (GH.prm.locked) {
+            ALLOC(d_snap_state, ncx * 3 * plane);
+            ALLOC(d_snap_mon, ncx * std::max(I.n_mon, 1));
+            ALLOC(d_snap_tmax, ncx * std::max(I.n_mon, 1));
+            ALLOC(d_snap_rows, ncx * ny);
+            ALLOC(d_snap_ctl, ncx);
+            if (D.li) ALLOC(d_snap_lif, ncx * (size_t)(ny + nx));
+        }
         ALLOC(d_ledger, ncx * GH.max_ledger * 8);
         ALLOC(d_moncells, (size_t)std::max(I.n_mon, 1));
         StepRec* d_steplog = nullptr;
@@ -3078,6 +3174,16 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
              hipMemset(d_ybnd, 0, 2 * ncx * 2 * nx * 4) == hipSuccess &&
              hipMemset(d_rowacc + ncx * ny, 0, ncx * ny * sizeof(RowAcc)) == hipSuccess &&
              (!d_lif || hipMemset(d_lif, 0, 2 * ncx * (size_t)(ny + nx) * 4) == hipSuccess);
+        if (d_snap_state) {   // the first snapshot: the initial state and control
+            for (int k = 0; ok && k < nc; k++)
+                ok = hipMemcpy(d_snap_state + (size_t)k * 3 * plane, d_state + (size_t)k * 6 * plane, sizeof(float) * 3 * plane,
+                               hipMemcpyDeviceToDevice) == hipSuccess;
+            ok = ok && hipMemset(d_snap_mon, 0, ncx * std::max(I.n_mon, 1) * 4) == hipSuccess &&
+                 hipMemset(d_snap_tmax, 0, ncx * std::max(I.n_mon, 1) * 8) == hipSuccess &&
+                 up(d_snap_rows, rowacc.data(), ncx * ny * sizeof(RowAcc)) &&
+                 (nc == 0 || up(d_snap_ctl, ctl.data() + (ctl.size() - (size_t)nc), (size_t)nc * sizeof(CandCtl))) &&
+                 (!d_snap_lif || hipMemset(d_snap_lif, 0, ncx * (size_t)(ny + nx) * 4) == hipSuccess);
+        }
         if (!ok) {
             set_err(err, errlen, "upload of grid %d failed", g);
             return fail(ITR_EDEVICE);
@@ -3104,6 +3210,12 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.fixbuf = d_fixbuf;
         D.fix64 = D.li ? (double*)d_fixbuf : nullptr;
         D.lif = d_lif;
+        D.snap_state = d_snap_state;
+        D.snap_mon = d_snap_mon;
+        D.snap_tmax = d_snap_tmax;
+        D.snap_rows = d_snap_rows;
+        D.snap_lif = d_snap_lif;
+        D.snap_ctl = d_snap_ctl;
         D.ledger = d_ledger;
         D.baseline_max = d_base;
         D.guard_pos = d_gpos;
@@ -3347,6 +3459,7 @@ itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, siz
         o.abort_cell = C.abort_cell;
         o.abort_sync = C.abort_sync;
         o.abort_value = C.abort_value;
+        o.dt_fallback_t = C.dt_fallback_t;
         o.sync_count = (int32_t)C.sync_index;
         const int nl = (int)std::min<uint32_t>(C.n_ledger, (uint32_t)GH.max_ledger);
         std::vector<double> led((size_t)std::max(nl, 1) * 8);
@@ -3468,6 +3581,11 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         for (int c = 0; c < nc; c++) {
             if (cc[c].abort_kind != 0) {
                 set_err(err, errlen, "candidate %d: forward aborted (kind %d) on the locked schedule", c, cc[c].abort_kind);
+                return ITR_EINVAL;
+            }
+            if (cc[c].locked_off) {
+                set_err(err, errlen, "candidate %d: the locked schedule fails the hard-CFL check at t = %.17g", c,
+                        cc[c].dt_fallback_t);
                 return ITR_EINVAL;
             }
             if (c > 0 && cc[c].step != N) {

@@ -62,6 +62,7 @@ class GpuRun:
     abort_value: float
     sync_count: int
     ledger: np.ndarray = field(repr=False, default=None)
+    dt_fallback_t: float = -1.0
 
 
 class _Keep(list):
@@ -466,7 +467,8 @@ class BatchSession:
             res.append(GpuRun(b["mm"][:nm].copy(), b["mt"][:nm].copy(), o.steps, o.t_end, o.volume_m3, b.get("h"),
                               b.get("qx"), b.get("qy"), bool(o.bad), o.cell_updates, o.rows_skipped, o.dt_min,
                               o.dt_max, o.dt_mean, o.lts, o.v0, o.rel_residual, o.aborted, o.abort_cell,
-                              o.abort_sync, o.abort_value, o.sync_count, b["led"][:min(o.n_ledger, 4096)].copy()))
+                              o.abort_sync, o.abort_value, o.sync_count, b["led"][:min(o.n_ledger, 4096)].copy(),
+                              o.dt_fallback_t))
         return res
 
     def close(self):

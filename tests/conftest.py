@@ -99,6 +99,7 @@ def assert_run_equal(g, o, what="", li=False):
             continue
         assert bits64(getattr(g, k)) == bits64(o[k]), f"{what}: {k} {getattr(g, k)!r} vs {o[k]!r}"
     assert g.cell_updates == o["cell_updates"], f"{what}: cell_updates {g.cell_updates} vs {o['cell_updates']}"
+    assert bits64(g.dt_fallback_t) == bits64(o["dt_fallback_t"]), f"{what}: dt_fallback_t {g.dt_fallback_t} vs {o['dt_fallback_t']}"
     assert g.rows_skipped == o["rows_skipped"], f"{what}: rows_skipped"
     kind = {0: 0, 1: 1, 2: 2, 3: 3}[o["aborted"]]
     assert g.aborted == kind, f"{what}: aborted {g.aborted} vs {o['aborted']}"

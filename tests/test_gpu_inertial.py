@@ -144,3 +144,46 @@ def test_li_launch_modes_and_guard(use_graph, chunk):
         obs = {"baseline_max": base_run.monitor_max, "guard_pos": gpos, "tol": 0.002, "require": True}
         o = oracle_run(base, p, edit=e.as_dict(), monitors=mon, observer=obs)
         assert_run_equal(r, o, f"li guard cand {i}", li=True)
+
+
+def test_li_locked_dt_and_restart(solver):
+    """A locked local-inertial schedule with the adaptive restart from the last sync
+    snapshot (solver.rs:456-480), which for local inertial includes the boundary faces:
+    natural failures on edited terrain and a failure forced mid-interval."""
+    _say("case: local inertial, locked dt with restarts, 2 groups of 5 candidates on 60x48")
+    from inverse_terrain_engine_amd.gpu import BatchSession
+    nx, ny, dx = 60, 48, 5.0
+    rng = sc.SplitMix64(31)
+    dem = sc.hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
+    geo = tr.GeoTransform(0.0, ny * dx, dx, dx)
+    segs = [{"kind": "transmissive", "side": "east"},
+            {"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [300.0, 2.0], [900.0, 0.3]],
+             "range_m": (60.0, 120.0)}]
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.035, geo), 1, segs)
+    mk = lambda: li(sc.SolverParams(900.0, 30.0, rain=sc.mm_h([[0, 0], [300, 80], [900, 0]]), infil_capacity=1.4e-6))
+    dts = po.run(oracle_terrain(base), oracle_params(mk()), precision=32, want_dts=True)["dts"]
+    t = np.cumsum(dts)
+    syncs = np.nonzero(np.abs(t / 30.0 - np.round(t / 30.0)) < 1e-9)[0]
+    pa, pb = mk(), mk()
+    pa.locked_dt, pa.cfl_hard = dts, 0.37
+    pb.locked_dt = dts.copy()
+    pb.locked_dt[(syncs[7] + syncs[8]) // 2] = 50.0
+    edits = [tr.materialize(sc.random_primitives(rng, 3, geo, nx, ny), geo, nx, ny, None, 1.0) for _ in range(5)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    mon = np.arange(0, nx * ny, 11, dtype=np.uint32)
+    sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 2, None, group_params=[pa, pb])
+    try:
+        sess.run()
+        res = sess.results()
+    finally:
+        sess.close()
+    falls = []
+    for gi, q in enumerate((pa, pb)):
+        for ci, e in enumerate(edits):
+            r = res[gi * len(edits) + ci]
+            assert_run_equal(r, oracle_run(base, q, edit=e.as_dict(), monitors=mon), f"li locked {gi}/{ci}", li=True)
+            falls.append((gi, r.dt_fallback_t))
+    _say("fallback times:", falls)
+    # group 1 falls back at the forced step unless its edits make the check fail earlier
+    assert all(0.0 <= f <= t[syncs[7]] for g, f in falls if g == 1)
+    assert any(f == t[syncs[7]] for g, f in falls if g == 1) and any(f < 0 for g, f in falls if g == 0)

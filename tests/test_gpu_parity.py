@@ -436,3 +436,52 @@ def test_device_designs_equal_host_prepared(solver):
         _runs_equal(a, b, f"design {i}")
         assert bits64(vol[i, 0]) == bits64(e.cut_m3) and bits64(vol[i, 1]) == bits64(e.fill_m3), f"design {i} volumes"
     assert not np.array_equal(runs[0].depth, runs[1].depth)
+
+
+def test_locked_dt_schedule_and_adaptive_restart(solver):
+    """Baseline-locked dt (solver.rs:452-455) and its fallback: a step whose locked dt
+    fails the hard-CFL check restarts adaptively from the last sync snapshot
+    (solver.rs:456-480).  Failures from edited terrain (cfl_hard 0.37), forced right after
+    a sync step (the snapshot is the state just completed), forced mid-interval (state,
+    monitors, ledger and control roll back), and at the first step; each group has its
+    own parameters in one resident batch."""
+    _say("case: locked dt schedules with restarts, 4 groups of 6 candidates on 60x48")
+    from inverse_terrain_engine_amd.gpu import BatchSession
+    nx, ny, dx = 60, 48, 5.0
+    rng = sc.SplitMix64(31)
+    dem = sc.hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
+    geo = tr.GeoTransform(0.0, ny * dx, dx, dx)
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.035, geo), 1, [{"kind": "transmissive", "side": "east"}])
+    mk = lambda: sc.SolverParams(900.0, 30.0, rain=sc.mm_h([[0, 0], [300, 80], [900, 0]]), infil_capacity=1.4e-6)
+    dts = po.run(oracle_terrain(base), oracle_params(mk()), precision=32, want_dts=True)["dts"]
+    t = np.cumsum(dts)
+    syncs = np.nonzero(np.abs(t / 30.0 - np.round(t / 30.0)) < 1e-9)[0]
+    pa, pb, pc, pd = mk(), mk(), mk(), mk()
+    pa.locked_dt, pa.cfl_hard = dts, 0.37
+    pb.locked_dt = dts.copy()
+    pb.locked_dt[syncs[5] + 1] = 50.0
+    pc.locked_dt = dts.copy()
+    pc.locked_dt[(syncs[7] + syncs[8]) // 2] = 50.0
+    pd.locked_dt, pd.cfl_hard = dts, 0.0
+    edits = [tr.materialize(sc.random_primitives(rng, 3, geo, nx, ny), geo, nx, ny, None, 1.0) for _ in range(6)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    mon = np.arange(0, nx * ny, 11, dtype=np.uint32)
+    params = [pa, pb, pc, pd]
+    sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 4, None, group_params=params)
+    try:
+        sess.run()
+        res = sess.results()
+    finally:
+        sess.close()
+    fallbacks = []
+    for gi, q in enumerate(params):
+        for ci, e in enumerate(edits):
+            r = res[gi * len(edits) + ci]
+            o = oracle_run(base, q, edit=e.as_dict(), monitors=mon)
+            assert_run_equal(r, o, f"locked group {gi} cand {ci}")
+            fallbacks.append((gi, r.dt_fallback_t))
+    by = {g: [f for gg, f in fallbacks if gg == g] for g in range(4)}
+    _say("fallback times:", by)
+    assert any(f >= 0 for f in by[0]) and any(f < 0 for f in by[0])
+    assert all(f == t[syncs[5]] for f in by[1]) and all(f == t[syncs[7]] for f in by[2])
+    assert all(f == 0.0 for f in by[3])

@@ -1133,9 +1133,42 @@ template <class T> struct Snap {
     uint32_t sync_index;
 };
 
+// ---------------------------------------------------------------- replay.rs:17-181
+// Region of interest in level-grid cells (half-open), its ghost ring and inner band in
+// the ROI's padded layout, in the reference's fixed orders.
+struct Roi { int c0, r0, w, h; };
+std::vector<std::pair<int, int>> roi_ring(const Roi& r) {  // replay.rs:27-41
+    std::vector<std::pair<int, int>> v;
+    for (int i = 0; i < r.w + 2; i++) v.push_back({i, 0});
+    for (int i = 0; i < r.w + 2; i++) v.push_back({i, r.h + 1});
+    for (int j = 1; j <= r.h; j++) {
+        v.push_back({0, j});
+        v.push_back({r.w + 1, j});
+    }
+    return v;
+}
+std::vector<std::pair<int, int>> roi_band(const Roi& r) {  // replay.rs:43-54
+    std::vector<std::pair<int, int>> v;
+    for (int j = 1; j <= r.h; j++)
+        for (int i = 1; i <= r.w; i++)
+            if (i == 1 || j == 1 || i == r.w || j == r.h) v.push_back({i, j});
+    return v;
+}
+
+// Tape recording during a full-domain run, or a replay on the ROI grid (solver.rs:484-493,
+// 573-591).  Values are stored as f64 of the run's precision.
+struct ReplayCtx {
+    Roi roi{0, 0, 0, 0};
+    bool record = false, replay = false;
+    std::vector<size_t> ring_idx, band_idx;   // padded indices in the run's own layout
+    std::vector<double> ring, band;           // record: appended; replay: the tape
+    int64_t n_ring_steps = 0, n_band_syncs = 0;
+    double tol_h = 0.0, tol_q = 0.0;
+};
+
 template <class T>
 int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const orc_observer* obs, const orc_start* st,
-             orc_summary* sum, std::vector<double>* rows_out, std::vector<double>* dts_out) {
+             orc_summary* sum, std::vector<double>* rows_out, std::vector<double>* dts_out, ReplayCtx* rp = nullptr) {
     const TerrainBase& b = *pt.b;
     const int nx = b.nx, ny = b.ny;
     const double area = b.dx * b.dy;
@@ -1233,6 +1266,10 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         if (locked) {   // solver.rs:452-482
             if (step < (uint64_t)p->n_locked && p->locked_dt[step] * s_cur <= p->cfl_hard) {
                 dt = p->locked_dt[step];
+            } else if (rp && rp->replay) {   // replay-invalid: locked-dt verification failed
+                aborted = 5;
+                abort_sync = (int)sync_index;
+                break;
             } else {
                 ws.h = snap.h; ws.qx = snap.qx; ws.qy = snap.qy; ws.row_dry = snap.row_dry;
                 ws.mon.max = snap.mon_max; ws.mon.tmax = snap.mon_tmax; acc = snap.acc; rows = snap.rows;
@@ -1247,6 +1284,28 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
         if (hits_sync) dt = t_sync - t;
         double rn = rain.integral(t, t + dt);
         double t_new = hits_sync ? t_sync : t + dt;
+        if (rp && rp->replay) {   // Tape::fill (replay.rs:136-159): the recorded ring is the ghost ring
+            const size_t n = rp->ring_idx.size();
+            if ((int64_t)step >= rp->n_ring_steps) {   // tape exhausted
+                aborted = 5;
+                abort_sync = (int)sync_index;
+                break;
+            }
+            std::fill(ws.ghost_wet.begin(), ws.ghost_wet.end(), 0);
+            const double* src = rp->ring.data() + (size_t)step * 3 * n;
+            const int w = rp->roi.w;
+            for (size_t k = 0; k < n; k++) {
+                const size_t q = rp->ring_idx[k];
+                ws.h[q] = (T)src[3 * k];
+                ws.qx[q] = (T)src[3 * k + 1];
+                ws.qy[q] = (T)src[3 * k + 2];
+                if (ws.h[q] > T(0)) {
+                    const int kk = (int)k;
+                    const int j = kk < w + 2 ? 0 : kk < 2 * (w + 2) ? rp->roi.h + 1 : 1 + (kk - 2 * (w + 2)) / 2;
+                    ws.ghost_wet[j] = 1;
+                }
+            }
+        } else {
         if (p->kernel == 2) {   // the staggered boundary faces survive the ghost fill (inertial.rs:219-242)
             li_faces.clear();
             for (int j = 1; j <= ny; j++) li_faces.push_back(ws.qx[b.at(0, j)]);
@@ -1259,6 +1318,13 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
             for (int i = 1; i <= nx; i++)
                 if (b.north[i - 1].kind != G_INFLOW) ws.qy[b.at(i, 0)] = li_faces[ny + i - 1];
         }
+        }
+        if (rp && rp->record)   // Tape::record_ring (replay.rs:112-125), after the ghost fill
+            for (size_t q : rp->ring_idx) {
+                rp->ring.push_back((double)ws.h[q]);
+                rp->ring.push_back((double)ws.qx[q]);
+                rp->ring.push_back((double)ws.qy[q]);
+            }
         if (dts_out) dts_out->push_back(dt);
         k.dt = (T)dt;
         k.dtdx = (T)(dt / b.dx);
@@ -1295,6 +1361,32 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
             double expected = acc.v0 + acc.rain + acc.inflow - acc.outflow - acc.infil + acc.fix;
             double row[8] = {t, v, acc.rain, acc.inflow, acc.outflow, acc.infil, acc.fix, v - expected};
             rows.insert(rows.end(), row, row + 8);
+            if (rp && rp->record)   // Tape::record_band (replay.rs:126-133)
+                for (size_t q : rp->band_idx) {
+                    rp->band.push_back((double)ws.h[q]);
+                    rp->band.push_back((double)ws.qx[q]);
+                    rp->band.push_back((double)ws.qy[q]);
+                }
+            if (rp && rp->replay) {   // Tape::band_diff (replay.rs:161-176) against the tolerances
+                const size_t n = rp->band_idx.size();
+                bool ok = (int64_t)sync_index <= rp->n_band_syncs;
+                if (ok) {
+                    const double* s0 = rp->band.data() + (size_t)(sync_index - 1) * 3 * n;
+                    double dh = 0.0, dq = 0.0;
+                    for (size_t k = 0; k < n; k++) {
+                        const size_t q = rp->band_idx[k];
+                        dh = std::fmax(dh, std::fabs((double)ws.h[q] - (double)(T)s0[3 * k]));
+                        dq = std::fmax(std::fmax(dq, std::fabs((double)ws.qx[q] - (double)(T)s0[3 * k + 1])),
+                                       std::fabs((double)ws.qy[q] - (double)(T)s0[3 * k + 2]));
+                    }
+                    ok = dh <= rp->tol_h && dq <= rp->tol_q;
+                }
+                if (!ok) {
+                    aborted = 5;
+                    abort_sync = (int)sync_index;
+                    break;
+                }
+            }
             if (obs) {   // GuardObserver::on_sync (evaluator.rs:95-107)
                 if (obs->require_no_offsite_worsening) {
                     bool have = false; double wv = 0; int wc = 0;
@@ -1336,6 +1428,114 @@ int run_impl(const Prepared<T>& pt, const orc_params* p, Workspace<T>& ws, const
     sum->n_rows = (int32_t)(rows.size() / 8);
     if (rows_out) *rows_out = rows;
     return 0;
+}
+
+// sub_terrain (replay.rs:181-241): the ROI grid with every face of the candidate's full
+// prepared terrain on and around it, wall boundaries (the tape supplies the ghosts).
+template <class T> struct SubTerrain {
+    TerrainBase base;
+    Prepared<T> pt;
+};
+template <class T> void sub_terrain(const TerrainBase& b, const Prepared<T>& full, const Roi& r, SubTerrain<T>& out) {
+    TerrainBase& s = out.base;
+    s.nx = r.w; s.ny = r.h; s.level = b.level; s.crest = b.crest;
+    s.dx = b.dx; s.dy = b.dy; s.g = b.g; s.fine = b.fine;
+    s.z.assign((size_t)r.w * r.h, 0.0);
+    s.wall.assign((size_t)r.w * r.h, 0);
+    s.nval.assign((size_t)r.w * r.h, 0.0);
+    s.wet_count = 0;
+    for (int rr = 0; rr < r.h; rr++)
+        for (int c = 0; c < r.w; c++) {
+            const size_t k = (size_t)(rr + r.r0) * b.nx + (size_t)(c + r.c0), q = (size_t)rr * r.w + c;
+            s.z[q] = full.zlev[k];
+            s.wall[q] = b.wall[k];
+            s.nval[q] = b.nval[k];
+            s.wet_count += b.wall[k] ? 0 : 1;
+        }
+    s.north.assign(r.w, Ghost()); s.south.assign(r.w, Ghost());
+    s.west.assign(r.h, Ghost()); s.east.assign(r.h, Ghost());
+    Prepared<T>& q = out.pt;
+    q.b = &s;
+    const size_t len = (size_t)(r.h + 2) * s.P();
+    q.fx0.assign(len, T(0)); q.fx1.assign(len, T(0)); q.fy0.assign(len, T(0)); q.fy1.assign(len, T(0));
+    q.wet.assign(len, T(0)); q.gn2.assign(len, T(0));
+    for (int js = 0; js < r.h + 2; js++)
+        for (int is = 0; is < r.w + 2; is++) {
+            const size_t ks = s.at(is, js), kf = b.at(is + r.c0, js + r.r0);
+            q.fx0[ks] = full.fx0[kf]; q.fx1[ks] = full.fx1[kf];
+            q.fy0[ks] = full.fy0[kf]; q.fy1[ks] = full.fy1[kf];
+            q.wet[ks] = full.wet[kf]; q.gn2[ks] = full.gn2[kf];
+        }
+    q.zlev = s.z;
+}
+
+template <class T>
+void copy_outputs(const TerrainBase& b, Workspace<T>& ws, const std::vector<double>& rows, int n_mon, double* ledger_rows,
+                  int max_rows, double* mon_max, double* mon_tmax, double* out_h, double* out_qx, double* out_qy) {
+    if (ledger_rows) {
+        int n = std::min((int)(rows.size() / 8), max_rows);
+        std::memcpy(ledger_rows, rows.data(), sizeof(double) * 8 * (size_t)n);
+    }
+    if (mon_max) for (int q = 0; q < n_mon; q++) mon_max[q] = ws.mon.max[q];
+    if (mon_tmax) for (int q = 0; q < n_mon; q++) mon_tmax[q] = ws.mon.tmax[q];
+    for (int r = 0; r < b.ny; r++)
+        for (int c = 0; c < b.nx; c++) {
+            size_t k = (size_t)r * b.nx + c, q = b.at(c + 1, r + 1);
+            if (out_h) out_h[k] = (double)ws.h[q];
+            if (out_qx) out_qx[k] = (double)ws.qx[q];
+            if (out_qy) out_qy[k] = (double)ws.qy[q];
+        }
+}
+
+// A baseline run recording the ROI tape (problem.rs:306-319, record_ring / record_band).
+template <class T>
+int record_typed(const TerrainBase& b, const orc_params* p, const Roi& roi, const uint32_t* mon_cells, int n_mon,
+                 orc_summary* sum, std::vector<double>& ring, std::vector<double>& band) {
+    Prepared<T> pt = prepare<T>(b, nullptr);
+    Workspace<T> ws;
+    ws.init(b.nx, b.ny, p->strip_rows > 0 ? p->strip_rows : b.ny);
+    ws.mon.init(mon_cells, n_mon, b.nx, b.ny);
+    ReplayCtx rp;
+    rp.roi = roi;
+    rp.record = true;
+    for (auto& ij : roi_ring(roi)) rp.ring_idx.push_back(b.at(ij.first + roi.c0, ij.second + roi.r0));
+    for (auto& ij : roi_band(roi)) rp.band_idx.push_back(b.at(ij.first + roi.c0, ij.second + roi.r0));
+    std::vector<double> rows;
+    int rc = run_impl(pt, p, ws, nullptr, nullptr, sum, &rows, nullptr, &rp);
+    ring.swap(rp.ring);
+    band.swap(rp.band);
+    return rc;
+}
+
+// A candidate replayed on the ROI (evaluator.rs:427-440): cold start, the tape's ring as
+// ghosts, the band checked at every sync point.
+template <class T>
+int replay_typed(const TerrainBase& b, const orc_edit* edit, const orc_params* p, const Roi& roi, const double* ring,
+                 int64_t n_steps, const double* band, int64_t n_syncs, double tol_h, double tol_q, const uint32_t* mon_sub,
+                 int n_mon, const orc_observer* obs, orc_summary* sum, double* ledger_rows, int max_rows, double* mon_max,
+                 double* mon_tmax, double* out_h, double* out_qx, double* out_qy) {
+    Prepared<T> full = prepare<T>(b, edit);
+    SubTerrain<T> st;
+    sub_terrain(b, full, roi, st);
+    const TerrainBase& sb = st.base;
+    Workspace<T> ws;
+    ws.init(sb.nx, sb.ny, p->strip_rows > 0 ? p->strip_rows : sb.ny);
+    ws.mon.init(mon_sub, n_mon, sb.nx, sb.ny);
+    ReplayCtx rp;
+    rp.roi = roi;
+    rp.replay = true;
+    for (auto& ij : roi_ring(roi)) rp.ring_idx.push_back(sb.at(ij.first, ij.second));
+    for (auto& ij : roi_band(roi)) rp.band_idx.push_back(sb.at(ij.first, ij.second));
+    rp.ring.assign(ring, ring + (size_t)n_steps * 3 * rp.ring_idx.size());
+    rp.band.assign(band, band + (size_t)n_syncs * 3 * rp.band_idx.size());
+    rp.n_ring_steps = n_steps;
+    rp.n_band_syncs = n_syncs;
+    rp.tol_h = tol_h;
+    rp.tol_q = tol_q;
+    std::vector<double> rows;
+    int rc = run_impl(st.pt, p, ws, obs, nullptr, sum, &rows, nullptr, &rp);
+    copy_outputs(sb, ws, rows, n_mon, ledger_rows, max_rows, mon_max, mon_tmax, out_h, out_qx, out_qy);
+    return rc;
 }
 
 template <class T>
@@ -1469,6 +1669,35 @@ int32_t orc_run(const orc_terrain* base, const orc_edit* edit, const orc_params*
                                  mon_tmax, out_h, out_qx, out_qy, out_dts, max_dts);
     return run_typed<float>(*base->b, edit, p, mon_cells, n_mon, obs, start, sum, ledger_rows, max_rows, mon_max,
                             mon_tmax, out_h, out_qx, out_qy, out_dts, max_dts);
+}
+
+int32_t orc_record_tape(const orc_terrain* base, const orc_params* p, const int32_t* roi4, const uint32_t* mon_cells,
+                        int32_t n_mon, orc_summary* sum, double* ring, int64_t ring_cap, double* band, int64_t band_cap,
+                        int64_t* n_ring, int64_t* n_band) {
+    const Roi roi{roi4[0], roi4[1], roi4[2], roi4[3]};
+    std::vector<double> r, bd;
+    std::memset(sum, 0, sizeof(*sum));
+    int rc = p->precision == 64 ? record_typed<double>(*base->b, p, roi, mon_cells, n_mon, sum, r, bd)
+                                : record_typed<float>(*base->b, p, roi, mon_cells, n_mon, sum, r, bd);
+    *n_ring = (int64_t)r.size();
+    *n_band = (int64_t)bd.size();
+    if (ring) std::memcpy(ring, r.data(), sizeof(double) * (size_t)std::min<int64_t>(ring_cap, *n_ring));
+    if (band) std::memcpy(band, bd.data(), sizeof(double) * (size_t)std::min<int64_t>(band_cap, *n_band));
+    return rc;
+}
+
+int32_t orc_run_replay(const orc_terrain* base, const orc_edit* edit, const orc_params* p, const int32_t* roi4,
+                       const double* ring, int64_t n_steps, const double* band, int64_t n_syncs, double tol_h, double tol_q,
+                       const uint32_t* mon_sub, int32_t n_mon, const orc_observer* obs, orc_summary* sum,
+                       double* ledger_rows, int32_t max_rows, double* mon_max, double* mon_tmax, double* out_h,
+                       double* out_qx, double* out_qy) {
+    const Roi roi{roi4[0], roi4[1], roi4[2], roi4[3]};
+    std::memset(sum, 0, sizeof(*sum));
+    if (p->precision == 64)
+        return replay_typed<double>(*base->b, edit, p, roi, ring, n_steps, band, n_syncs, tol_h, tol_q, mon_sub, n_mon,
+                                    obs, sum, ledger_rows, max_rows, mon_max, mon_tmax, out_h, out_qx, out_qy);
+    return replay_typed<float>(*base->b, edit, p, roi, ring, n_steps, band, n_syncs, tol_h, tol_q, mon_sub, n_mon, obs,
+                               sum, ledger_rows, max_rows, mon_max, mon_tmax, out_h, out_qx, out_qy);
 }
 
 double orc_run_candidates(const orc_terrain* base, const orc_edit* edits, int32_t n_cand, const orc_params* p,

@@ -95,7 +95,7 @@ typedef struct {
     int64_t steps, cell_updates, rows_skipped;
     int32_t sync_count;
     int32_t n_rows;              /* ledger rows written */
-    int32_t aborted;             /* 0 none, 1 guard, 2 incumbent, 3 unhealthy */
+    int32_t aborted;             /* 0 none, 1 guard, 2 incumbent, 3 unhealthy, 5 replay invalid */
     int32_t abort_cell;
     int32_t abort_sync;
     int32_t n_dts;               /* per-step dt values written */
@@ -122,6 +122,21 @@ int32_t orc_run(const orc_terrain* base, const orc_edit* edit, const orc_params*
                 int32_t n_mon, const orc_observer* obs, const orc_start* start, orc_summary* sum,
                 double* ledger_rows /* max_rows*8 */, int32_t max_rows, double* mon_max, double* mon_tmax,
                 double* out_h, double* out_qx, double* out_qy, double* out_dts, int64_t max_dts);
+
+/* Subdomain replay (crates/itr-hydro/src/replay.rs).  roi4 = c0, r0, w, h in level-grid
+ * cells.  orc_record_tape runs the baseline (no edit) and returns the ghost ring
+ * (3 values per ring cell per step, replay.rs ring order) and the inner band (3 values per
+ * band cell per sync point).  orc_run_replay runs a candidate on the ROI grid with the
+ * tape as its ghosts; aborted = 5 is ReplayInvalid (band beyond tol_h / tol_q, tape
+ * exhausted, or a failed locked-dt check). */
+int32_t orc_record_tape(const orc_terrain* base, const orc_params* p, const int32_t* roi4, const uint32_t* mon_cells,
+                        int32_t n_mon, orc_summary* sum, double* ring, int64_t ring_cap, double* band, int64_t band_cap,
+                        int64_t* n_ring, int64_t* n_band);
+int32_t orc_run_replay(const orc_terrain* base, const orc_edit* edit, const orc_params* p, const int32_t* roi4,
+                       const double* ring, int64_t n_steps, const double* band, int64_t n_syncs, double tol_h, double tol_q,
+                       const uint32_t* mon_sub, int32_t n_mon, const orc_observer* obs, orc_summary* sum,
+                       double* ledger_rows, int32_t max_rows, double* mon_max, double* mon_tmax, double* out_h,
+                       double* out_qx, double* out_qy);
 
 /* Candidate-parallel CPU evaluation: one candidate per worker thread, each worker running
  * the fused kernel with a single-threaded workspace (evaluator.rs:337-352).  Fills the

@@ -96,6 +96,15 @@ def lib():
         L.orc_run.argtypes = [C.c_void_p, C.POINTER(Edit), C.POINTER(Params), _up, C.c_int32, C.POINTER(Observer),
                               C.POINTER(Start), C.POINTER(Summary), _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp,
                               C.c_int64]
+        _ip4 = C.POINTER(C.c_int32)
+        _lp1 = C.POINTER(C.c_int64)
+        L.orc_record_tape.restype = C.c_int32
+        L.orc_record_tape.argtypes = [C.c_void_p, C.POINTER(Params), _ip4, _up, C.c_int32, C.POINTER(Summary), _dp,
+                                      C.c_int64, _dp, C.c_int64, _lp1, _lp1]
+        L.orc_run_replay.restype = C.c_int32
+        L.orc_run_replay.argtypes = [C.c_void_p, C.POINTER(Edit), C.POINTER(Params), _ip4, _dp, C.c_int64, _dp,
+                                     C.c_int64, C.c_double, C.c_double, _up, C.c_int32, C.POINTER(Observer),
+                                     C.POINTER(Summary), _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp]
         L.orc_run_candidates.restype = C.c_double
         L.orc_run_candidates.argtypes = [C.c_void_p, C.POINTER(Edit), C.c_int32, C.POINTER(Params), _up, C.c_int32,
                                          C.c_int32, _lp, _lp]
@@ -339,6 +348,74 @@ def run(terrain: Terrain, p: SolverParams, precision=32, edit=None, monitors=(),
     out["h"], out["qx"], out["qy"] = h, qx, qy
     if want_dts:
         out["dts"] = dts[: s.n_dts].copy()
+    return out
+
+
+def roi_ring_len(roi):
+    c0, r0, w, h = roi
+    return 2 * (w + 2) + 2 * h
+
+
+def roi_band_len(roi):
+    c0, r0, w, h = roi
+    return sum(1 for j in range(1, h + 1) for i in range(1, w + 1) if i == 1 or j == 1 or i == w or j == h)
+
+
+def record_tape(terrain: Terrain, p: SolverParams, roi, precision=32, monitors=()):
+    """The baseline run recording the subdomain-replay tape (replay.rs:71-133): ring
+    [steps][ring cells][3] and band [syncs][band cells][3] (h, qx, qy)."""
+    q, keep = _params(p, precision)
+    mon = np.ascontiguousarray(monitors, dtype=np.uint32)
+    r4 = (C.c_int32 * 4)(*[int(v) for v in roi])
+    s = Summary()
+    nr, nb = C.c_int64(0), C.c_int64(0)
+    lib().orc_record_tape(terrain.h, C.byref(q), r4, mon.ctypes.data_as(_up) if len(mon) else None, len(mon),
+                          C.byref(s), None, 0, None, 0, C.byref(nr), C.byref(nb))
+    ring = np.zeros(max(nr.value, 1))
+    band = np.zeros(max(nb.value, 1))
+    rc = lib().orc_record_tape(terrain.h, C.byref(q), r4, mon.ctypes.data_as(_up) if len(mon) else None, len(mon),
+                               C.byref(s), _d(ring), nr.value, _d(band), nb.value, C.byref(nr), C.byref(nb))
+    if rc != 0:
+        raise ValueError(f"orc_record_tape failed ({rc})")
+    out = {k: getattr(s, k) for k, _ in Summary._fields_}
+    out["ring"] = ring[: nr.value].reshape(-1, roi_ring_len(roi), 3)
+    out["band"] = band[: nb.value].reshape(-1, roi_band_len(roi), 3)
+    out["roi"] = tuple(int(v) for v in roi)
+    return out
+
+
+def run_replay(terrain: Terrain, p: SolverParams, tape, precision=32, edit=None, monitors_sub=(), tol_h=2e-3,
+               tol_q=1e-3, observer=None, max_rows=100000):
+    """A candidate replayed on the ROI grid with the tape's ring as ghosts
+    (evaluator.rs:427-440); aborted = 5 is ReplayInvalid.  State arrays are ROI-sized."""
+    q, keep = _params(p, precision)
+    roi = tape["roi"]
+    r4 = (C.c_int32 * 4)(*roi)
+    mon = np.ascontiguousarray(monitors_sub, dtype=np.uint32)
+    n_mon = len(mon)
+    e = make_edit(edit)
+    ob = make_observer(observer)
+    ring = np.ascontiguousarray(tape["ring"], dtype=np.float64)
+    band = np.ascontiguousarray(tape["band"], dtype=np.float64)
+    s = Summary()
+    rows = np.zeros((max_rows, 8))
+    mmax = np.zeros(max(n_mon, 1))
+    mtmax = np.zeros(max(n_mon, 1))
+    w, h_ = roi[2], roi[3]
+    h = np.zeros((h_, w))
+    qx = np.zeros((h_, w))
+    qy = np.zeros((h_, w))
+    rc = lib().orc_run_replay(terrain.h, C.byref(e[0]) if e else None, C.byref(q), r4, _d(ring), ring.shape[0],
+                              _d(band), band.shape[0], tol_h, tol_q, mon.ctypes.data_as(_up) if n_mon else None, n_mon,
+                              C.byref(ob[0]) if ob else None, C.byref(s), _d(rows), max_rows, _d(mmax), _d(mtmax),
+                              _d(h), _d(qx), _d(qy))
+    if rc != 0:
+        raise ValueError(f"orc_run_replay failed ({rc})")
+    out = {k: getattr(s, k) for k, _ in Summary._fields_}
+    out["rows"] = rows[: s.n_rows].copy()
+    out["monitor_max"] = mmax[:n_mon].copy()
+    out["monitor_tmax"] = mtmax[:n_mon].copy()
+    out["h"], out["qx"], out["qy"] = h, qx, qy
     return out
 
 

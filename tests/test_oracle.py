@@ -429,3 +429,60 @@ def test_inertial_time_step_closed_forms():
     u = rate * (np.arange(nx) + 0.5) * dx / h
     want = 0.35 / (0.7 * (np.sqrt(9.81 * h) + u) / dx).max()
     assert abs(np.median(r["dts"][-50:]) / want - 1.0) < 2e-3
+
+
+# ---------------------------------------------------------------- locked dt and subdomain replay
+RNX, RNY = 72, 56
+RROI = (20, 14, 30, 26)
+
+
+def _replay_base():
+    """crates/itr-hydro/tests/replay.rs:15-21: west inflow segment, transmissive east."""
+    segs = [{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [200.0, 2.0], [600.0, 0.3]], "range_m": (40.0, 70.0)},
+            {"kind": "transmissive", "side": "east"}]
+    z = dem(RNX, RNY, 2.0, lambda x, y: 50.0 + 0.004 * (150.0 - x) + 0.3 * math.sin(x / 11.0) * math.cos(y / 9.0))
+    return po.Terrain(z, manning=0.035, dx=2.0, segments=segs)
+
+
+def _replay_params(**k):
+    return P(800.0, 40.0, rain=po.hyeto_mm_h([[0.0, 0.0], [100.0, 0.0], [200.0, 60.0], [500.0, 0.0]]), **k)
+
+
+def _in_roi(c, roi=RROI):
+    col, row = c % RNX, c // RNX
+    return roi[0] <= col < roi[0] + roi[2] and roi[1] <= row < roi[1] + roi[3]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_locked_dt_and_subdomain_replay(prec):
+    """crates/itr-hydro/tests/replay.rs:47-120, f64 and f32: the locked zero-edit run is
+    bitwise the baseline; an edit that speeds flow past cfl_hard = 0.36 falls back to
+    adaptive dt and keeps the ledger closed; the zero-edit replay on the ROI is bitwise
+    the baseline there (monitors and depths); a small interior edit stays valid; a berm
+    across the ROI's downstream edge is flagged replay-invalid."""
+    T = _replay_base()
+    mon = np.array([c for c in range(RNX * RNY) if _in_roi(c) and c % 5 == 1], np.uint32)
+    sub_mon = np.array([((c // RNX) - RROI[1]) * RROI[2] + (c % RNX) - RROI[0] for c in mon], np.uint32)
+    b0 = po.run(T, _replay_params(), precision=prec, monitors=mon, want_dts=True)
+    assert b0["aborted"] == 0 and len(b0["dts"]) == b0["steps"]
+    tape = po.record_tape(T, _replay_params(), RROI, precision=prec, monitors=mon)
+    assert tape["ring"].shape[0] == b0["steps"] and tape["band"].shape[0] == b0["sync_count"]
+    pl = lambda **k: _replay_params(locked_dt=b0["dts"], **k)
+    s1 = po.run(T, pl(), precision=prec, monitors=mon)
+    assert np.array_equal(s1["h"].view(np.uint64), b0["h"].view(np.uint64)) and s1["steps"] == b0["steps"]
+    assert s1["dt_fallback_t"] < 0
+    w = 30
+    stripe = {"c0": 0, "r0": 0, "dz": np.array([-0.6 * ((k % w) / w) for k in range(w * RNY)]).reshape(RNY, w)}
+    sf = po.run(T, pl(cfl_hard=0.36), precision=prec, monitors=mon, edit=stripe)
+    assert sf["aborted"] == 0 and sf["dt_fallback_t"] >= 0.0
+    assert sf["rel_residual"] < (1e-10 if prec == 64 else 1e-4)
+    sr = po.run_replay(T, pl(), tape, precision=prec, monitors_sub=sub_mon)
+    assert sr["aborted"] == 0
+    assert np.array_equal(sr["monitor_max"].view(np.uint64), b0["monitor_max"].view(np.uint64))
+    assert np.array_equal(sr["monitor_tmax"].view(np.uint64), b0["monitor_tmax"].view(np.uint64))
+    c0, r0, rw, rh = RROI
+    assert np.array_equal(sr["h"].view(np.uint64), b0["h"][r0:r0 + rh, c0:c0 + rw].view(np.uint64))
+    small = {"c0": 34, "r0": 26, "dz": np.full((3, 3), 0.02)}
+    assert po.run_replay(T, pl(), tape, precision=prec, edit=small, monitors_sub=sub_mon)["aborted"] == 0
+    berm = {"c0": c0 + rw - 3, "r0": r0, "dz": np.full((rh, 2), 1.5)}
+    assert po.run_replay(T, pl(), tape, precision=prec, edit=berm, monitors_sub=sub_mon)["aborted"] == 5

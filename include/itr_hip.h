@@ -92,6 +92,32 @@ typedef struct {
 
 enum { ITR_PHYSICS_HLL = 0, ITR_PHYSICS_LOCAL_INERTIAL = 1 };
 
+/* Subdomain replay (crates/itr-hydro/src/replay.rs, spec §7.11.3).
+ * mode 1 (record, Tape::record_ring / record_band): the grid's run records, for its
+ *   candidate 0, the 1-cell ghost ring around the region of interest (c0, r0, w, h, in the
+ *   grid's cells) at every step start, after the ghost fill, and the ROI's 1-cell inner
+ *   band after every sync step: ring [ring_steps][2(w+2)+2h][3] and band
+ *   [band_syncs][n_band][3] (h, qx, qy; replay.rs ring and band orders), capacities in
+ *   steps / sync points.  *n_steps_out / *n_syncs_out receive the counts the run reached
+ *   (a count above the capacity means the tape overflowed).
+ * mode 2 (replay, evaluator.rs:427-440): the grid IS the w x h region (faces from
+ *   sub_terrain, replay.rs:181-241: dzx / dzy include the faces between the region and its
+ *   ring), its ghosts replay the ring of step k at step k, and after every sync step the
+ *   band must stay within tol_h / tol_q of the tape (problem.rs:318: 2e-3 m, 1e-3 m^2/s),
+ *   else aborted = 5 (ReplayInvalid; also when the tape is exhausted or a locked dt fails
+ *   its check).  HLL physics only (the reference does not replay local-inertial levels). */
+typedef struct {
+    int32_t mode;
+    int32_t c0, r0, w, h;
+    int32_t reserved;
+    float* ring;
+    float* band;
+    int64_t ring_steps, band_syncs;
+    double tol_h, tol_q;
+    int64_t* n_steps_out;
+    int64_t* n_syncs_out;
+} itr_replay;
+
 /* One grid (a TerrainBase, crates/itr-hydro/src/terrain.rs:237-257) and the candidates
  * run on it (PreparedTerrain overlays flattened as in crates/itr-gpu/src/lib.rs:177-194).
  * Several grids of different shapes may be passed to one batch (fidelity ladder). */
@@ -146,6 +172,8 @@ typedef struct {
      * candidate's faces are built on the device.  Level-1 signed grids without Stage
      * boundaries or initial_stage only (else ITR_DECLINED: prepare the faces on the host). */
     const itr_design* design;
+    /* Subdomain replay recording or replay for this grid (itr_replay), or NULL. */
+    const itr_replay* replay;
 } itr_grid;
 
 /* Per-candidate result: GpuRun (crates/itr-gpu/src/lib.rs:49-60) + BatchRun
@@ -154,7 +182,7 @@ typedef struct {
     double t_end, volume_m3, dt_min, dt_max, dt_mean, lts, v0, rel_residual, abort_value;
     uint64_t steps, cell_updates, rows_skipped;
     int32_t bad;        /* Unhealthy (non-finite or beyond-roundoff negative depth) */
-    int32_t aborted;    /* 0 none, 1 guard, 2 sync callback, 3 unhealthy */
+    int32_t aborted;    /* 0 none, 1 guard, 2 sync callback, 3 unhealthy, 5 replay invalid */
     int32_t abort_cell; /* guard: offending cell (row*nx+col) */
     int32_t abort_sync; /* sync index at the abort */
     int32_t sync_count;

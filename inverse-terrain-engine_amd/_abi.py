@@ -38,6 +38,13 @@ class DesignC(C.Structure):
                 ("max_abs_dz", C.c_double), ("z_wall", C.c_double), ("volumes", _dp)]
 
 
+class ReplayC(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("c0", C.c_int32), ("r0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32),
+                ("reserved", C.c_int32), ("ring", _fp), ("band", _fp), ("ring_steps", C.c_int64),
+                ("band_syncs", C.c_int64), ("tol_h", C.c_double), ("tol_q", C.c_double),
+                ("n_steps_out", C.POINTER(C.c_int64)), ("n_syncs_out", C.POINTER(C.c_int64))]
+
+
 class GridC(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("dx", C.c_double), ("dy", C.c_double), ("g", C.c_double),
                 ("z_ref", C.c_double), ("crest", C.c_int32), ("wet_count", C.c_int32), ("wet", _fp), ("gn2", _fp),
@@ -46,7 +53,7 @@ class GridC(C.Structure):
                 ("guard_pos", _ip), ("n_guard", C.c_int32), ("reserved", C.c_int32), ("guard_tol", C.c_double),
                 ("ghost_value", _dp), ("ghost_hydro", _ip), ("hydro_start", _ip), ("hydro_t", _dp), ("hydro_q", _dp),
                 ("n_hydro", C.c_int32), ("reserved2", C.c_int32), ("params", C.POINTER(SolverParamsC)),
-                ("design", C.POINTER(DesignC))]
+                ("design", C.POINTER(DesignC)), ("replay", C.POINTER(ReplayC))]
 
 
 class RunOutC(C.Structure):

@@ -133,6 +133,14 @@ struct DevGrid {
     RowAcc* snap_rows;         // [nslot][ny]
     float* snap_lif;           // local-inertial: [nslot][ny + nx]
     struct CandCtl* snap_ctl;  // [nslot]
+    // Subdomain replay (replay.rs).  tape_mode 1: this run records the tape of the ROI
+    // (roi_*) for its candidate 0; 2: this grid is the ROI and its ghosts (code 4) replay
+    // the tape, with the band checked at every sync point.
+    int32_t tape_mode, roi_c0, roi_r0, roi_w, roi_h, n_ring, n_band;
+    float* tape_ring;          // [tape_steps][n_ring][3] h, qx, qy
+    float* tape_band;          // [tape_syncs][n_band][3]
+    int64_t tape_steps, tape_syncs;
+    double tol_h, tol_q;
     double* fix64;             // local-inertial: [2][nslot][plane] positivity-fix values (f64), by state parity
     struct StepRec* steplog;   // [nslot][max_log] constants of every step run (adjoint replay)
     const double* baseline_max;
@@ -175,6 +183,7 @@ struct StepAcc {
     int32_t smax_bits;   // max CFL rate as ordered int bits (rates are >= 0)
     uint32_t bad;
     unsigned long long smax64;   // local inertial: the f64 CFL rate max, as ordered bits
+    unsigned long long dh64, dq64;   // replay: band deviation maxima (f64 bits, >= 0)
 };
 
 struct KArgs {
@@ -208,7 +217,7 @@ __host__ __device__ inline LdsLayout lds_layout(int R, int nx, bool special) {
     const size_t acc_n = (size_t)R * 64;
     L.moff = al(L.acc + acc_n);                       // a monitored cell's plane offset per thread
     L.gsp = al(L.moff + (size_t)kMaxWaves * 64);
-    L.total = al(L.gsp + (special ? 2 * gsp_pairs(R, nx) : 0));
+    L.total = al(L.gsp + (special ? 4 * gsp_pairs(R, nx) : 0));
     // the control epilogue stages 2*nx floats and rows of 8 words in the same space
     if (L.total < (size_t)2 * nx) L.total = al((size_t)2 * nx);
     if (L.total < 64) L.total = 64;
@@ -357,13 +366,29 @@ __device__ __forceinline__ double rcbrt64(double x) {
 // is max(h_interior, critical)), qv = f32(inward * q) is the normal discharge and the
 // tangential one is 0.  Both kinds make the ghost wet exactly when hv > 0 (or the
 // interior is wet).
-struct GhostSp { float hv, qv; };
+struct GhostSp { float hv, qv, qt; };
+// Ring position (replay.rs:27-41 order) of ghost cell k of a w x h replay grid:
+// north / south columns after the ring row's corner, west / east alternating per row.
+__host__ __device__ inline int ring_of_ghost(int k, int w, int h) {
+    if (k < w) return 1 + k;
+    if (k < 2 * w) return (w + 2) + 1 + (k - w);
+    if (k < 2 * w + h) return 2 * (w + 2) + 2 * (k - 2 * w);
+    return 2 * (w + 2) + 2 * (k - 2 * w - h) + 1;
+}
 __device__ __forceinline__ GhostSp special_ghost(const DevGrid& G, double inflow_scale, int code, int k, int local,
-                                                 double t, bool inward_pos) {
+                                                 double t, bool inward_pos, uint64_t step) {
     GhostSp s;
     s.hv = 0.0f;
     s.qv = 0.0f;
-    if (code == 2) {
+    s.qt = 0.0f;
+    if (code == 4) {   // Tape::fill (replay.rs:136-159): the recorded ring cell, as it was
+        if (step < (uint64_t)G.tape_steps) {
+            const float* v = G.tape_ring + ((size_t)step * G.n_ring + ring_of_ghost(k, G.nx, G.ny)) * 3;
+            s.hv = v[0];
+            s.qv = v[1];
+            s.qt = v[2];
+        }
+    } else if (code == 2) {
         s.hv = G.gstage[(size_t)local * (2 * G.nx + 2 * G.ny) + k];
     } else if (code == 3) {
         const int hi = G.ghydro[k];
@@ -413,15 +438,23 @@ __device__ __forceinline__ bool prepare_step(const DevParams& p, const DevGrid& 
 // wall and transmissive boundaries; a wet Stage / Inflow ghost makes its row (west /
 // east) or the ghost row (north / south) wet (boundary.rs:50-52, 73-75).  Computed by one
 // wave at the step's start time C->t.
-__device__ __forceinline__ bool we_ghost_wet(const DevParams& p, const DevGrid& G, int local, double t, int r) {
+__device__ __forceinline__ bool we_ghost_wet(const DevParams& p, const DevGrid& G, int local, double t, int r, uint64_t step) {
     const int nx = G.nx, ny = G.ny;
     bool w = false;
     for (int side = 0; side < 2; side++) {
         const int k = 2 * nx + side * ny + r;
         const int code = G.ghost[k];
-        if (code >= 2) w = w || special_ghost(G, p.inflow_scale, code, k, local, t, side == 0).hv > 0.0f;
+        if (code >= 2) w = w || special_ghost(G, p.inflow_scale, code, k, local, t, side == 0, step).hv > 0.0f;
     }
     return w;
+}
+// A replay grid's ghost-row corners (ring cells 0, w+1 / w+2, 2w+3) also make the ghost
+// row wet (replay.rs:148-153).
+__device__ __forceinline__ bool tape_corner_wet(const DevGrid& G, bool north, uint64_t step) {
+    if (G.tape_mode != 2 || step >= (uint64_t)G.tape_steps) return false;
+    const float* v = G.tape_ring + (size_t)step * G.n_ring * 3;
+    const int a = north ? 0 : G.nx + 2, b = north ? G.nx + 1 : 2 * G.nx + 3;
+    return v[3 * a] > 0.0f || v[3 * b] > 0.0f;
 }
 template <bool SPECIAL>
 __device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const DevGrid& G, const CandCtl* C, const RowAcc* ra,
@@ -442,11 +475,12 @@ __device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const De
         // cell of theirs is wet (a wall / transmissive ghost is wet only with its row).
         const int nx = G.nx, local = C->local;
         const double t = C->t;
-        bool top = false, bot = false;
+        const uint64_t step = C->step;
+        bool top = tape_corner_wet(G, true, step), bot = tape_corner_wet(G, false, step);
         for (int c = lane; c < nx; c += kWave) {
             const int cn = G.ghost[c], cs = G.ghost[nx + c];
-            if (cn >= 2) top = top || special_ghost(G, p.inflow_scale, cn, c, local, t, true).hv > 0.0f;
-            if (cs >= 2) bot = bot || special_ghost(G, p.inflow_scale, cs, nx + c, local, t, false).hv > 0.0f;
+            if (cn >= 2) top = top || special_ghost(G, p.inflow_scale, cn, c, local, t, true, step).hv > 0.0f;
+            if (cs >= 2) bot = bot || special_ghost(G, p.inflow_scale, cs, nx + c, local, t, false, step).hv > 0.0f;
         }
         top = __any(top);
         bot = __any(bot);
@@ -456,7 +490,7 @@ __device__ __forceinline__ uint32_t count_skippable(const DevParams& p, const De
             auto dry = [&](int r) -> bool {
                 if (r < 0) return !top;
                 if (r >= ny) return !bot;
-                return !ra[r].wet && !we_ghost_wet(p, G, local, t, r);
+                return !ra[r].wet && !we_ghost_wet(p, G, local, t, r, step);
             };
             bool d0 = dry(r0 - 1), d1 = dry(r0);
             for (int r = r0; r < r1; r++) {
@@ -530,7 +564,16 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
             S.s_cur = smax;
             if (S.hits_sync) {
                 S.sync_index += 1;
-                if (G.baseline_max && G.n_guard > 0) {
+                if (G.tape_mode == 2) {   // Tape::band_diff within tolerance, else replay-invalid (solver.rs:578-590)
+                    const double dh = __longlong_as_double((long long)A.dh64), dq = __longlong_as_double((long long)A.dq64);
+                    if ((int64_t)S.sync_index > G.tape_syncs || !(dh <= G.tol_h && dq <= G.tol_q)) {
+                        S.abort_kind = 5;
+                        S.abort_sync = (int)S.sync_index;
+                        flags |= F_DONE;
+                        go = false;
+                    }
+                }
+                if (go && G.baseline_max && G.n_guard > 0) {
                     double worst = 0.0;
                     const int q = guard_breach(G, G.mon_max + slot(G, S.parity, S.local) * G.n_mon, lane, &worst);
                     if (q >= 0) {
@@ -546,7 +589,13 @@ __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid&
             }
             if (go && !(S.t < p.duration && S.sync_index < p.n_sync)) { flags |= F_DONE; go = false; }
             if (go && p.max_steps > 0 && S.step >= (uint64_t)p.max_steps) { flags |= F_DONE; go = false; }
-            if (go && !prepare_step(p, G, &S, p.rain_t, p.rain_r)) {
+            if (go && G.tape_mode == 2 && (S.step >= (uint64_t)G.tape_steps || !prepare_step(p, G, &S, p.rain_t, p.rain_r))) {
+                S.abort_kind = 5;   // replay-invalid: tape exhausted or locked-dt check failed (solver.rs:456-460, 484-489)
+                S.abort_sync = (int)S.sync_index;
+                flags |= F_DONE;
+                go = false;
+            }
+            if (go && G.tape_mode != 2 && !prepare_step(p, G, &S, p.rain_t, p.rain_r)) {
                 // The locked step fails the hard-CFL check: restart adaptively from the
                 // last sync snapshot (solver.rs:456-480).  After a sync step the snapshot
                 // is the state just completed, so only the schedule is dropped.
@@ -589,8 +638,8 @@ __device__ __forceinline__ float lane_chain(int c0, int n, F value) {
 // east of rows a0-1 .. b0 (pairs 0 .. 2R+3), then the north ghost row when a0 <= north_upto
 // and the south one when the band ends at the last row.  Returns whether any ghost of
 // the band is wet (all threads).
-__device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local, int a0, int b0, double t, float2* l_gsp,
-                                                     int north_upto) {
+__device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local, int a0, int b0, double t, float4* l_gsp,
+                                                     int north_upto, uint64_t step) {
     const int nx = G.nx, ny = G.ny, R = G.R;
     const int tid = threadIdx.x;
     const int nthr = G.waves * kWave;
@@ -609,10 +658,12 @@ __device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local
         GhostSp gs;
         gs.hv = 0.0f;
         gs.qv = 0.0f;
-        if (k >= 0) gs = special_ghost(G, G.prm.inflow_scale, G.ghost[k], k, local, t, inward);
-        l_gsp[i] = make_float2(gs.hv, gs.qv);
+        gs.qt = 0.0f;
+        if (k >= 0) gs = special_ghost(G, G.prm.inflow_scale, G.ghost[k], k, local, t, inward, step);
+        l_gsp[i] = make_float4(gs.hv, gs.qv, gs.qt, 0.0f);
         fw = fw || gs.hv > 0.0f;
     }
+    if (tid == 0 && ((a0 == 0 && tape_corner_wet(G, true, step)) || (b0 == ny && tape_corner_wet(G, false, step)))) fw = true;
     return __syncthreads_or(fw ? 1 : 0) != 0;
 }
 
@@ -620,10 +671,19 @@ __device__ __forceinline__ bool stage_special_ghosts(const DevGrid& G, int local
 // differs but every face and cell value is the same IEEE expression of the old state).
 // Reads state copy `par`, writes copy par^1 and the row partials / monitors of parity
 // par^1; the band's CFL max and health flag go to the candidate's StepAcc.
+// Position of cell (i, j) (1-based in the ROI, on its inner band) in the band's
+// row-major order (replay.rs:43-54).
+__device__ __forceinline__ int band_index(int i, int j, int w, int h) {
+    if (j == 1) return i - 1;
+    const int mid = w == 1 ? 1 : 2;   // band cells of a middle row
+    if (j < h) return w + mid * (j - 2) + (i == 1 ? 0 : 1);
+    return w + mid * (h - 2) + (i - 1);
+}
+
 template <bool CREST, int DEPTH, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
-                                          double t, double t_new, bool sync, int par, bool from_snap, StepAcc* acc_cur,
-                                          float* lds) {
+                                          double t, double t_new, bool sync, int par, bool from_snap, uint64_t step,
+                                          uint32_t sync_idx, StepAcc* acc_cur, float* lds) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -689,17 +749,18 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const bool skip_dry = G.prm.skip_dry != 0;
     const bool special = SPECIAL && G.special != 0;
     const LdsLayout LL = lds_layout(R, nx, G.special != 0);
-    float2* l_gsp = (float2*)(lds + LL.gsp);
+    float4* l_gsp = (float4*)(lds + LL.gsp);
 
     bool forced_wet = false;
-    if (special) forced_wet = stage_special_ghosts(G, local, a0, b0, t, l_gsp, 0);
+    if (special) forced_wet = stage_special_ghosts(G, local, a0, b0, t, l_gsp, 0, step);
 
     // Dry band (kernel.rs:605-612 rule for every row of the band: rows a0-1 .. b0 exactly
     // dry, ghost rows mirroring their neighbour or holding no wet special ghost, and no
     // rain).  Every face of such a row is dry-dry (exact zero flux) and every cell stays
     // (0, 0, 0), so the band's new state, row partials and boundary masses are zeros and
     // its monitors carry over.
-    if (skip_dry && !(K.rain > 0.0f) && !forced_wet) {
+    // (not while recording a tape, nor at a replay's sync steps: those read the band's cells)
+    if (skip_dry && !(K.rain > 0.0f) && !forced_wet && !(SPECIAL && (G.tape_mode == 1 || (G.tape_mode == 2 && sync)))) {
         bool dry = true;
         for (int r = max(a0 - 1, 0); r <= min(b0, ny - 1); r++) dry = dry && ra_in[r].wet == 0u;
         if (dry) {
@@ -758,9 +819,13 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         const bool cin = col >= 0 && col < nx;
         if (rin && !cin) {
             if (v.code_we >= 2) {
-                const float2 g = l_gsp[(col < 0 ? 0 : R + 2) + (row - (a0 - 1))];
+                const float4 g = l_gsp[(col < 0 ? 0 : R + 2) + (row - (a0 - 1))];
                 if (v.code_we == 2) {
                     w.h = g.x;
+                } else if (v.code_we == 4) {
+                    w.h = g.x;
+                    w.qx = g.y;
+                    w.qy = g.z;
                 } else {
                     w.h = inflow_depth(v.h, g.x);
                     w.qx = g.y;
@@ -770,9 +835,13 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         } else if (!rin && cin) {
             const int code = row < 0 ? code_n : code_s;
             if (code >= 2) {
-                const float2 g = l_gsp[2 * (R + 2) + (row < 0 ? 0 : nx) + col];
+                const float4 g = l_gsp[2 * (R + 2) + (row < 0 ? 0 : nx) + col];
                 if (code == 2) {
                     w.h = g.x;
+                } else if (code == 4) {
+                    w.h = g.x;
+                    w.qx = g.y;
+                    w.qy = g.z;
                 } else {
                     w.h = inflow_depth(v.h, g.x);
                     w.qx = 0.0f;
@@ -800,6 +869,38 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const Cell cp = fix(fetch(a0 - 1), a0 - 1);
     const Cell c0 = fix(fetch(a0), a0);
     float ch = c0.h, cqx = c0.qx, cqy = c0.qy;
+
+    // Subdomain replay tape (replay.rs:112-176; special kernel only).  Record mode: the
+    // ROI's ghost ring from the old state with ghosts applied (Tape::record_ring, after the
+    // ghost fill) and, at sync steps, its inner band from the new state; replay mode: the
+    // band measured against the tape at sync steps (Tape::band_diff).
+    const bool rec = SPECIAL && G.tape_mode == 1 && local == 0 && step < (uint64_t)G.tape_steps;
+    const bool rec_band = rec && sync && (int64_t)sync_idx < G.tape_syncs;
+    const bool chk = SPECIAL && G.tape_mode == 2 && sync && (int64_t)sync_idx < G.tape_syncs;
+    const int rc0 = G.roi_c0, rr0 = G.roi_r0, rw = G.roi_w, rh = G.roi_h;
+    const bool ring_owner = upd || (col == -1 && wave == 0 && lane == 0) || (col == nx && lane >= 1);
+    auto ring_put = [&](int row, float h, float qx, float qy) {
+        int k = -1;
+        if (row == rr0 - 1 || row == rr0 + rh) {
+            if (col >= rc0 - 1 && col <= rc0 + rw) k = (row == rr0 - 1 ? 0 : rw + 2) + col - (rc0 - 1);
+        } else if (row >= rr0 && row < rr0 + rh) {
+            if (col == rc0 - 1) k = 2 * (rw + 2) + 2 * (row - rr0);
+            else if (col == rc0 + rw) k = 2 * (rw + 2) + 2 * (row - rr0) + 1;
+        }
+        if (k >= 0) {
+            float* d = G.tape_ring + ((size_t)step * G.n_ring + k) * 3;
+            d[0] = h;
+            d[1] = qx;
+            d[2] = qy;
+        }
+    };
+    auto band_at = [&](int row) -> int {   // this lane's band cell in `row`, or -1
+        if (!upd || row < rr0 || row >= rr0 + rh || col < rc0 || col >= rc0 + rw) return -1;
+        const int i = col - rc0 + 1, jj = row - rr0 + 1;
+        return (i == 1 || jj == 1 || i == rw || jj == rh) ? band_index(i, jj, rw, rh) : -1;
+    };
+    double dh_l = 0.0, dq_l = 0.0;
+    if (rec && ring_owner && a0 == 0) ring_put(-1, cp.h, cp.qx, cp.qy);
     const Vel vp = cell_velocity(cp.h, cp.qx, cp.qy, K);
     Vel vc = cell_velocity(ch, cqx, cqy, K);
     Flux fn;
@@ -879,6 +980,21 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         const CellNew u = cell_update(ch, cqx, cqy, fw.m, fw.nr, fw.t, em, enl, et, fn.m, fn.nr, fn.t, fs.m, fs.nl,
                                       fs.t, wet, gn2, K);
         const int r = j - a0;
+        if (SPECIAL && (rec || chk)) {
+            if (rec && ring_owner) ring_put(j, ch, cqx, cqy);
+            const int bi = (rec_band || chk) ? band_at(j) : -1;
+            if (bi >= 0) {
+                float* bt = G.tape_band + ((size_t)sync_idx * G.n_band + bi) * 3;
+                if (rec_band) {
+                    bt[0] = u.h;
+                    bt[1] = u.qx;
+                    bt[2] = u.qy;
+                } else {
+                    dh_l = fmax(dh_l, fabs((double)u.h - (double)bt[0]));
+                    dq_l = fmax(fmax(dq_l, fabs((double)u.qx - (double)bt[1])), fabs((double)u.qy - (double)bt[2]));
+                }
+            }
+        }
         if (upd) {
             const uint32_t so = (uint32_t)j * rowb;
             bstore(u.h, rs_ho, cb, so);
@@ -906,6 +1022,20 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         fn = fs;
     }
     if (b0 == ny && owner) yb_out[nx + col] = fn.m;
+    if (SPECIAL && rec && ring_owner && b0 == ny) ring_put(ny, ch, cqx, cqy);   // the south ghost row
+    if (SPECIAL && chk) {   // band deviation maxima (f64 >= 0, never NaN: max ignores NaN)
+        unsigned long long bh = (unsigned long long)__double_as_longlong(dh_l);
+        unsigned long long bq = (unsigned long long)__double_as_longlong(dq_l);
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned long long oh = __shfl_xor(bh, o, 64), oq = __shfl_xor(bq, o, 64);
+            bh = oh > bh ? oh : bh;
+            bq = oq > bq ? oq : bq;
+        }
+        if (lane == 0) {
+            if (bh > 0ull) atomicMax(&acc_cur->dh64, bh);
+            if (bq > 0ull) atomicMax(&acc_cur->dq64, bq);
+        }
+    }
 
     // Reductions of the band: CFL rate max (as ordered int bits; -0.0 never wins over
     // +0.0) and the health flag.
@@ -1202,7 +1332,7 @@ __host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
     L.acc = al(L.dsum + 4 * kMaxWaves);             // R x 16 FixedSum lanes
     L.moff = al(L.acc + (size_t)R * 16);
     L.gsp = al(L.moff + (size_t)kMaxWaves * 64);
-    L.total = al(L.gsp + (special ? 2 * gsp_pairs(R, nx) : 0));
+    L.total = al(L.gsp + (special ? 4 * gsp_pairs(R, nx) : 0));
     if (L.total < 64) L.total = 64;
     return L;
 }
@@ -1268,7 +1398,7 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     const int we_base = 2 * nx + (west_lane ? 0 : ny);   // this lane's west / east ghost codes
     const bool special = SPECIAL && G.special != 0;
     const LiLds LL = li_lds_layout(R, nx, G.special != 0);
-    float2* l_gsp = (float2*)(lds + LL.gsp);
+    float4* l_gsp = (float4*)(lds + LL.gsp);
     uint32_t* l_fixmask = (uint32_t*)(lds + LL.fixmask);
     int* l_red = (int*)(lds + LL.red);
     unsigned long long* l_smx = (unsigned long long*)(lds + LL.smx);
@@ -1277,13 +1407,13 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     uint32_t* l_moff = (uint32_t*)(lds + LL.moff);
     // (the band starting at row 1 reads the north inflow discharge for the theta
     // weighting of face 1)
-    if (special) (void)stage_special_ghosts(G, local, a0, b0, t, l_gsp, 1);
+    if (special) (void)stage_special_ghosts(G, local, a0, b0, t, l_gsp, 1, 0);
     for (int r = tid; r < R; r += nthr) l_fixmask[r] = 0u;
     if (has_m) l_moff[tid] = m_off0;
     __syncthreads();
     // Special ghost pairs of this lane: west / east of row r, north / south of its column.
-    auto gs_we = [&](int r) -> float2 { return l_gsp[(west_lane ? 0 : R + 2) + (r - (a0 - 1))]; };
-    auto gs_ns = [&](bool north) -> float2 { return l_gsp[2 * (R + 2) + (north ? 0 : nx) + (int)ccol]; };
+    auto gs_we = [&](int r) -> float4 { return l_gsp[(west_lane ? 0 : R + 2) + (r - (a0 - 1))]; };
+    auto gs_ns = [&](bool north) -> float4 { return l_gsp[2 * (R + 2) + (north ? 0 : nx) + (int)ccol]; };
 
     struct Row { float h, qx, qy, gn2, wet, dzx, dzx2, dzy, dzy2; };
     // Row r of the old state (clamped rows are never used); the west lane's qx is the
@@ -1326,7 +1456,7 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         double q;
         if (west_lane || east_lane) {
             const int code = GHOST[we_base + r];
-            float2 gs = make_float2(0.0f, 0.0f);
+            float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (special && code >= 2) gs = gs_we(r);
             q = west_lane ? li_boundary<CREST>(code, true, c.qx, h_r, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, g2_r, K)
                           : li_boundary<CREST>(code, false, c.qx, c.h, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, c.gn2, K);
@@ -1351,12 +1481,12 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         return canon((float)li_face_q(qt, (double)u.h, (double)d.h, dl, dr, K.dy, n2, K));
     };
     auto yface_south = [&](const Row& u) -> float {   // k = ny, u = row ny-1
-        float2 gs = make_float2(0.0f, 0.0f);
+        float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (special && code_s >= 2) gs = gs_ns(false);
         return canon((float)li_boundary<CREST>(code_s, false, u.qy, u.h, gs.x, gs.y, u.dzy, u.dzy2, ny > 1, K.dy, u.gn2, K));
     };
     auto yface_north = [&](const Row& d, const Row& d1) -> float {   // k = 0, d = row 0, d1 = row 1
-        float2 gs = make_float2(0.0f, 0.0f);
+        float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (special && code_n >= 2) gs = gs_ns(true);
         return canon((float)li_boundary<CREST>(code_n, true, lif_in[ny + ccol], d.h, gs.x, gs.y, d1.dzy, d1.dzy2, ny > 1, K.dy,
                                                d.gn2, K));
@@ -1755,12 +1885,18 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const int par = uni(sS.parity);
         const bool from_snap = uni((int)sS.restore) != 0;
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
+        const uint64_t step = sS.step;
+        const uint32_t sync_idx = sS.sync_index;
         if (MODES == 3) {
-            if (G.crest) band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
-            else band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
+            if (G.crest)
+                band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx, acc_cur,
+                                            lds);
+            else
+                band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx,
+                                             acc_cur, lds);
         } else {
             band_step<MODES == 2, MODES == 1 ? 2 : 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
-                                                              acc_cur, lds);
+                                                              step, sync_idx, acc_cur, lds);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!designated) {
@@ -1786,6 +1922,8 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         z.smax_bits = 0;
         z.bad = 0u;
         z.smax64 = 0ull;
+        z.dh64 = 0ull;
+        z.dq64 = 0ull;
         a.acc[(size_t)((lk + 2) & 3) * a.ncand + cand] = z;
         a.status[cand] = sS.flags;
     }
@@ -1814,6 +1952,12 @@ __global__ void itr_init_kernel(KArgs a) {
     }
     CandCtl S = *C;
     bool ok = prepare_step(G.prm, G, &S, G.prm.rain_t, G.prm.rain_r);
+    if (G.tape_mode == 2 && (!ok || G.tape_steps == 0)) {   // replay-invalid at the first step
+        S.abort_kind = 5;
+        S.abort_sync = 0;
+        S.flags |= F_DONE;
+        ok = true;
+    }
     if (!ok) {   // the first locked step fails its check: adaptive from t = 0 (solver.rs:456-480)
         S.locked_off = 1;
         S.dt_fallback_t = S.t;
@@ -2859,6 +3003,30 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     set_err(err, errlen, "grid %d: guard position out of range", g);
                     return ITR_EINVAL;
                 }
+        if (G.replay) {
+            const itr_replay& Rp = *G.replay;
+            if (q->physics != ITR_PHYSICS_HLL || G.design) {
+                set_err(err, errlen, "grid %d: subdomain replay runs HLL levels with prepared faces only", g);
+                return ITR_DECLINED;
+            }
+            if (Rp.mode == 1) {
+                if (Rp.c0 < 0 || Rp.r0 < 0 || Rp.w <= 0 || Rp.h <= 0 || Rp.c0 + Rp.w > G.nx || Rp.r0 + Rp.h > G.ny ||
+                    Rp.ring_steps < 0 || Rp.band_syncs < 0 || (Rp.ring_steps > 0 && !Rp.ring) ||
+                    (Rp.band_syncs > 0 && !Rp.band) || G.n_cand < 1) {
+                    set_err(err, errlen, "grid %d: invalid replay recording (region, buffers)", g);
+                    return ITR_EINVAL;
+                }
+            } else if (Rp.mode == 2) {
+                if (Rp.w != G.nx || Rp.h != G.ny || Rp.ring_steps < 0 || Rp.band_syncs < 0 ||
+                    (Rp.ring_steps > 0 && !Rp.ring) || (Rp.band_syncs > 0 && !Rp.band)) {
+                    set_err(err, errlen, "grid %d: a replay grid is the w x h region with its tape", g);
+                    return ITR_EINVAL;
+                }
+            } else {
+                set_err(err, errlen, "grid %d: replay mode must be 1 (record) or 2 (replay)", g);
+                return ITR_EINVAL;
+            }
+        }
         if (G.design) {
             const itr_design& Ds = *G.design;
             bool stage = false;
@@ -2949,6 +3117,30 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         const int nb = 2 * nx + 2 * ny;
         D.special = 0;
         for (int i = 0; i < nb; i++) D.special |= I.ghost[i] >= 2 ? 1 : 0;
+        // Subdomain replay: a replay grid's ghosts are all the tape's (code 4).
+        std::vector<int32_t> ghost_codes(I.ghost, I.ghost + nb);
+        D.tape_mode = I.replay ? I.replay->mode : 0;
+        if (D.tape_mode == 2) {
+            std::fill(ghost_codes.begin(), ghost_codes.end(), 4);
+            D.special = 1;
+        }
+        {
+            const int rw = D.tape_mode == 2 ? nx : (I.replay ? I.replay->w : 0);
+            const int rh = D.tape_mode == 2 ? ny : (I.replay ? I.replay->h : 0);
+            D.roi_c0 = D.tape_mode == 1 ? I.replay->c0 : 0;
+            D.roi_r0 = D.tape_mode == 1 ? I.replay->r0 : 0;
+            D.roi_w = rw;
+            D.roi_h = rh;
+            D.n_ring = I.replay ? 2 * (rw + 2) + 2 * rh : 0;
+            int nband = 0;
+            for (int j = 1; j <= rh; j++)
+                for (int i = 1; i <= rw; i++) nband += (i == 1 || j == 1 || i == rw || j == rh) ? 1 : 0;
+            D.n_band = nband;
+            D.tape_steps = I.replay ? I.replay->ring_steps : 0;
+            D.tape_syncs = I.replay ? I.replay->band_syncs : 0;
+            D.tol_h = I.replay ? I.replay->tol_h : 0.0;
+            D.tol_q = I.replay ? I.replay->tol_q : 0.0;
+        }
         std::vector<float> gstage;
         if (D.special) {
             gstage.assign((size_t)std::max(nc, 1) * nb, 0.0f);
@@ -3077,6 +3269,11 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         ALLOC(d_fixbuf, ncx * plane * (D.li ? 4 : 1));   // f64 fix values by parity on local-inertial grids
         float* d_lif = nullptr;
         if (D.li) ALLOC(d_lif, 2 * ncx * (size_t)(ny + nx));
+        float *d_tring = nullptr, *d_tband = nullptr;
+        if (D.tape_mode) {
+            ALLOC(d_tring, std::max<size_t>((size_t)D.tape_steps * D.n_ring * 3, 1));
+            ALLOC(d_tband, std::max<size_t>((size_t)D.tape_syncs * D.n_band * 3, 1));
+        }
         // Sync snapshot of a locked-dt grid (the adaptive restart, solver.rs:446-480).
         float *d_snap_state = nullptr, *d_snap_mon = nullptr, *d_snap_lif = nullptr;
         double* d_snap_tmax = nullptr;
@@ -3141,16 +3338,18 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             if (p->has_initial_stage || k == 0) ok = up_plane(d_state + (size_t)k * 6 * plane, h0s[p->has_initial_stage ? k : 0].data(), nx, 0, nx, 0, ny, ny);
             else ok = hipMemcpy(d_state + (size_t)k * 6 * plane, d_state, sizeof(float) * plane, hipMemcpyDeviceToDevice) == hipSuccess;
             if (I.design) continue;   // faces built on the device
-            // interior faces only: x-faces 1 .. nx-1, y-face rows 1 .. ny-1 (boundary faces 0)
-            ok = ok && up_plane(d_dzx + (size_t)k * plane, I.dzx + (size_t)k * ny * (nx + 1), nx + 1, 1, nx, 0, ny, ny) &&
-                 up_plane(d_dzy + (size_t)k * plane_y, I.dzy + (size_t)k * (ny + 1) * nx, nx, 0, nx, 1, ny, ny + 1);
+            // interior faces only: x-faces 1 .. nx-1, y-face rows 1 .. ny-1 (boundary faces 0);
+            // a replay grid keeps its boundary faces (they join the region to its ring)
+            const int f0 = D.tape_mode == 2 ? 0 : 1, fx1 = D.tape_mode == 2 ? nx + 1 : nx, fy1 = D.tape_mode == 2 ? ny + 1 : ny;
+            ok = ok && up_plane(d_dzx + (size_t)k * plane, I.dzx + (size_t)k * ny * (nx + 1), nx + 1, f0, fx1, 0, ny, ny) &&
+                 up_plane(d_dzy + (size_t)k * plane_y, I.dzy + (size_t)k * (ny + 1) * nx, nx, 0, nx, f0, fy1, ny + 1);
             if (I.crest)
-                ok = ok && up_plane(d_dzx2 + (size_t)k * plane, I.dzx_r + (size_t)k * ny * (nx + 1), nx + 1, 1, nx, 0, ny, ny) &&
-                     up_plane(d_dzy2 + (size_t)k * plane_y, I.dzy_r + (size_t)k * (ny + 1) * nx, nx, 0, nx, 1, ny, ny + 1);
+                ok = ok && up_plane(d_dzx2 + (size_t)k * plane, I.dzx_r + (size_t)k * ny * (nx + 1), nx + 1, f0, fx1, 0, ny, ny) &&
+                     up_plane(d_dzy2 + (size_t)k * plane_y, I.dzy_r + (size_t)k * (ny + 1) * nx, nx, 0, nx, f0, fy1, ny + 1);
         }
         ok = ok && up(d_wet, wet.data(), plane * 4) &&
                   up(d_gn2, gn2.data(), plane * 4) && up(d_monrow, mon_row.data(), mon_row.size() * 4) && up(d_monoff, mon_off.data(), mon_off.size() * 4) &&
-                  up(d_ghost, I.ghost, sizeof(int32_t) * (2 * nx + 2 * ny)) &&
+                  up(d_ghost, ghost_codes.data(), sizeof(int32_t) * (2 * nx + 2 * ny)) &&
                   up(d_rowacc, rowacc.data(), rowacc.size() * sizeof(RowAcc)) &&
                   up(d_moncells, I.mon_cells, sizeof(uint32_t) * I.n_mon);
         if (I.baseline_max)
@@ -3164,7 +3363,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     ginv[i] = I.ghost_value[i];
                     ghydro[i] = I.ghost_hydro[i];
                 }
-            for (int h = 0; h <= n_hydro; h++) hstart[h] = I.hydro_start[h];
+            if (n_hydro > 0)   // (grids with stage or replayed ghosts only have no hydrographs)
+                for (int h = 0; h <= n_hydro; h++) hstart[h] = I.hydro_start[h];
             ok = ok && up(d_gstage, gstage.data(), gstage.size() * 4) && up(d_ginv, ginv.data(), nb * 8) &&
                  up(d_ghydro, ghydro.data(), nb * 4) && up(d_hstart, hstart.data(), hstart.size() * 4) &&
                  up(d_hyd_t, I.hydro_t, n_pts * 8) && up(d_hyd_q, I.hydro_q, n_pts * 8);
@@ -3174,6 +3374,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
              hipMemset(d_ybnd, 0, 2 * ncx * 2 * nx * 4) == hipSuccess &&
              hipMemset(d_rowacc + ncx * ny, 0, ncx * ny * sizeof(RowAcc)) == hipSuccess &&
              (!d_lif || hipMemset(d_lif, 0, 2 * ncx * (size_t)(ny + nx) * 4) == hipSuccess);
+        if (D.tape_mode == 2)
+            ok = ok && up(d_tring, I.replay->ring, sizeof(float) * (size_t)D.tape_steps * D.n_ring * 3) &&
+                 up(d_tband, I.replay->band, sizeof(float) * (size_t)D.tape_syncs * D.n_band * 3);
         if (d_snap_state) {   // the first snapshot: the initial state and control
             for (int k = 0; ok && k < nc; k++)
                 ok = hipMemcpy(d_snap_state + (size_t)k * 3 * plane, d_state + (size_t)k * 6 * plane, sizeof(float) * 3 * plane,
@@ -3216,6 +3419,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.snap_rows = d_snap_rows;
         D.snap_lif = d_snap_lif;
         D.snap_ctl = d_snap_ctl;
+        D.tape_ring = d_tring;
+        D.tape_band = d_tband;
         D.ledger = d_ledger;
         D.baseline_max = d_base;
         D.guard_pos = d_gpos;
@@ -3238,7 +3443,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->ntasks_li = (int)tasks_li.size();
     b->modes = 0;
     b->modes_li = 0;
-    for (auto& D : dgrids) (D.li ? b->modes_li : b->modes) |= (D.crest ? 2 : 1) | (D.special ? 4 : 0);
+    for (auto& D : dgrids) (D.li ? b->modes_li : b->modes) |= (D.crest ? 2 : 1) | (D.special || D.tape_mode ? 4 : 0);
     if (b->modes == 0) b->modes = 1;
     b->block = max_waves * kWave;
     b->block_li = max_waves_li * kWave;
@@ -3460,6 +3665,15 @@ itr_status itr_hip_batch_results(itr_batch* b, itr_run_out* outs, char* err, siz
         o.abort_sync = C.abort_sync;
         o.abort_value = C.abort_value;
         o.dt_fallback_t = C.dt_fallback_t;
+        if (D.tape_mode == 1 && C.local == 0) {   // the recorded tape
+            const itr_replay& Rp = *GH.in.replay;
+            const int64_t ns = std::min<int64_t>((int64_t)C.step, D.tape_steps);
+            const int64_t nsy = std::min<int64_t>((int64_t)C.sync_index, D.tape_syncs);
+            if (ns > 0) HIP_TRY(hipMemcpy(Rp.ring, D.tape_ring, sizeof(float) * (size_t)ns * D.n_ring * 3, hipMemcpyDeviceToHost));
+            if (nsy > 0) HIP_TRY(hipMemcpy(Rp.band, D.tape_band, sizeof(float) * (size_t)nsy * D.n_band * 3, hipMemcpyDeviceToHost));
+            if (Rp.n_steps_out) *Rp.n_steps_out = (int64_t)C.step;
+            if (Rp.n_syncs_out) *Rp.n_syncs_out = (int64_t)C.sync_index;
+        }
         o.sync_count = (int32_t)C.sync_index;
         const int nl = (int)std::min<uint32_t>(C.n_ledger, (uint32_t)GH.max_ledger);
         std::vector<double> led((size_t)std::max(nl, 1) * 8);

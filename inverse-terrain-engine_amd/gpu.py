@@ -181,6 +181,71 @@ def _design_grid_c(base: TerrainBase, designs, monitors, keep: _Keep, editable=N
     return grid, vol
 
 
+@dataclass
+class Tape:
+    """A subdomain-replay tape (replay.rs:71-90): the region (c0, r0, w, h), the ghost ring
+    of every step [steps][2(w+2)+2h][3] and the inner band of every sync point
+    [syncs][n_band][3], as f32 (h, qx, qy)."""
+    roi: tuple
+    ring: np.ndarray
+    band: np.ndarray
+    steps: int
+    syncs: int
+
+
+def ring_len(roi):
+    return 2 * (roi[2] + 2) + 2 * roi[3]
+
+
+def band_len(roi):
+    w, h = roi[2], roi[3]
+    return sum(1 for j in range(1, h + 1) for i in range(1, w + 1) if i in (1, w) or j in (1, h))
+
+
+def _replay_grid_c(tape: Tape, sub_terrains, sub_monitors, keep: _Keep, tol_h, tol_q, guard=None):
+    """The region as a grid of its own (sub_terrain, replay.rs:181-241): cells, every face
+    on and around it from each candidate's full prepared terrain, the tape as its ghosts."""
+    c0, r0, w, h = tape.roi
+    base = sub_terrains[0].base
+    g = _abi.GridC()
+    g.nx, g.ny, g.dx, g.dy, g.g, g.z_ref = w, h, base.dx, base.dy, base.g, base.z_ref
+    g.crest = 1 if base.crest else 0
+    wet = keep.arr(base.wet[r0:r0 + h, c0:c0 + w], np.float32)
+    g.wet_count = int((wet != 0).sum())
+    f = lambda a: a.ctypes.data_as(_abi._fp)
+    g.wet = f(wet)
+    g.gn2 = f(keep.arr(base.gn2[r0:r0 + h, c0:c0 + w], np.float32))
+    g.ghost = keep.arr(np.zeros(2 * w + 2 * h), np.int32).ctypes.data_as(_abi._ip)
+    for t in sub_terrains:
+        if t.base is not base:
+            raise Declined("all candidates must share the same TerrainBase")
+    g.n_cand = len(sub_terrains)
+    g.dzx = f(keep.arr(np.stack([t.fx0[r0:r0 + h, c0:c0 + w + 1] for t in sub_terrains]), np.float32))
+    g.dzy = f(keep.arr(np.stack([t.fy0[r0:r0 + h + 1, c0:c0 + w] for t in sub_terrains]), np.float32))
+    if base.crest:
+        g.dzx_r = f(keep.arr(np.stack([t.fx1[r0:r0 + h, c0:c0 + w + 1] for t in sub_terrains]), np.float32))
+        g.dzy_r = f(keep.arr(np.stack([t.fy1[r0:r0 + h + 1, c0:c0 + w] for t in sub_terrains]), np.float32))
+    g.z_level = keep.arr(np.stack([t.z_level[r0:r0 + h, c0:c0 + w] for t in sub_terrains]),
+                         np.float64).ctypes.data_as(_abi._dp)
+    mon = keep.arr(np.asarray(sub_monitors, np.uint32) if len(sub_monitors) else np.zeros(1, np.uint32), np.uint32)
+    g.mon_cells = mon.ctypes.data_as(_abi._up)
+    g.n_mon = len(sub_monitors)
+    if guard is not None:
+        g.baseline_max = keep.arr(guard["baseline_max"], np.float64).ctypes.data_as(_abi._dp)
+        g.guard_pos = keep.arr(guard["guard_pos"], np.int32).ctypes.data_as(_abi._ip)
+        g.n_guard = len(guard["guard_pos"])
+        g.guard_tol = float(guard["tol"])
+    rp = _abi.ReplayC()
+    rp.mode, rp.c0, rp.r0, rp.w, rp.h = 2, c0, r0, w, h
+    rp.ring = f(keep.arr(tape.ring, np.float32))
+    rp.band = f(keep.arr(tape.band, np.float32))
+    rp.ring_steps, rp.band_syncs = tape.steps, tape.syncs
+    rp.tol_h, rp.tol_q = float(tol_h), float(tol_q)
+    keep.append(rp)
+    g.replay = C.pointer(rp)
+    return g
+
+
 class GpuSolver:
     """GpuSolver (lib.rs:62-373) on one MI355X through the C ABI."""
 
@@ -260,6 +325,48 @@ class GpuSolver:
                         "replay_mismatches": int(o.replay_mismatches), "plain_forward_ms": o.plain_forward_ms,
                         "forward_ms": o.forward_ms, "replay_ms": o.replay_ms, "reverse_ms": o.reverse_ms})
         return res
+
+    def record_tape(self, p: SolverParams, base: TerrainBase, roi, monitors=(), max_steps: int = 100000,
+                    max_syncs: int = 10000):
+        """The baseline run on the device recording the subdomain-replay tape of `roi`
+        (c0, r0, w, h) (problem.rs:306-319).  Returns (Tape, GpuRun of the baseline)."""
+        keep = _Keep()
+        q = _params_c(p, keep)
+        g = _grid_c(base, [PreparedTerrain(base)], monitors, keep, None, p.initial_stage is not None)
+        c0, r0, w, h = (int(v) for v in roi)
+        ring = np.zeros((max_steps, ring_len(roi), 3), np.float32)
+        band = np.zeros((max_syncs, band_len(roi), 3), np.float32)
+        ns, nsy = C.c_int64(0), C.c_int64(0)
+        rp = _abi.ReplayC()
+        rp.mode, rp.c0, rp.r0, rp.w, rp.h = 1, c0, r0, w, h
+        rp.ring = ring.ctypes.data_as(_abi._fp)
+        rp.band = band.ctypes.data_as(_abi._fp)
+        rp.ring_steps, rp.band_syncs = max_steps, max_syncs
+        rp.n_steps_out, rp.n_syncs_out = C.pointer(ns), C.pointer(nsy)
+        g.replay = C.pointer(rp)
+        s = BatchSession(self, p, [(base, [PreparedTerrain(base)], monitors, None)], None, grids=[g], keep=keep)
+        try:
+            s.run()
+            run = s.results()[0]
+        finally:
+            s.close()
+        if ns.value > max_steps or nsy.value > max_syncs:
+            raise ValueError("tape capacity exceeded")
+        return Tape((c0, r0, w, h), ring[:ns.value].copy(), band[:nsy.value].copy(), ns.value, nsy.value), run
+
+    def run_replay(self, p: SolverParams, tape: Tape, terrains, sub_monitors, tol_h=2e-3, tol_q=1e-3, guard=None,
+                   want_state=True):
+        """Candidates replayed on the tape's region (evaluator.rs:427-440): aborted = 5 is
+        ReplayInvalid (escalate to a full-domain run).  State arrays are region-sized."""
+        keep = _Keep()
+        g = _replay_grid_c(tape, terrains, sub_monitors, keep, tol_h, tol_q, guard)
+        s = BatchSession(self, p, [(None, terrains, sub_monitors, None)], None, grids=[g], keep=keep,
+                         shapes=[(tape.roi[2], tape.roi[3])])
+        try:
+            s.run()
+            return s.results(want_state)
+        finally:
+            s.close()
 
     def dt_schedule(self, p: SolverParams, base: TerrainBase, terrain, monitors=(), max_steps: int = 0,
                     capacity: int = 1 << 20) -> np.ndarray:
@@ -378,8 +485,10 @@ class BatchSession:
     e.g. one group per ensemble member."""
 
     def __init__(self, solver: GpuSolver, p: SolverParams, groups, on_sync=None, group_params=None, grids=None,
-                 keep=None):
+                 keep=None, shapes=None):
         self.solver = solver
+        # (nx, ny) of each group's grid (a replay region's grid is not its TerrainBase's)
+        self._shapes = shapes if shapes is not None else [(g[0].nx, g[0].ny) for g in groups]
         L = solver._L
         self._L = L
         self._keep = keep if keep is not None else _Keep()
@@ -439,14 +548,14 @@ class BatchSession:
         outs = (_abi.RunOutC * max(n, 1))()
         bufs = []
         k = 0
-        for base, terrs, mons, _ in self.groups:
+        for (base, terrs, mons, _), (gnx, gny) in zip(self.groups, self._shapes):
             for _ in terrs:
                 nm = max(len(mons), 1)
                 b = {"mm": np.zeros(nm), "mt": np.zeros(nm), "led": np.zeros((4096, 8))}
                 if want_state:
-                    b["h"] = np.zeros((base.ny, base.nx), np.float32)
-                    b["qx"] = np.zeros((base.ny, base.nx), np.float32)
-                    b["qy"] = np.zeros((base.ny, base.nx), np.float32)
+                    b["h"] = np.zeros((gny, gnx), np.float32)
+                    b["qx"] = np.zeros((gny, gnx), np.float32)
+                    b["qy"] = np.zeros((gny, gnx), np.float32)
                 o = outs[k]
                 o.monitor_max = b["mm"].ctypes.data_as(_abi._dp)
                 o.monitor_tmax = b["mt"].ctypes.data_as(_abi._dp)

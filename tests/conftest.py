@@ -101,9 +101,9 @@ def assert_run_equal(g, o, what="", li=False):
     assert g.cell_updates == o["cell_updates"], f"{what}: cell_updates {g.cell_updates} vs {o['cell_updates']}"
     assert bits64(g.dt_fallback_t) == bits64(o["dt_fallback_t"]), f"{what}: dt_fallback_t {g.dt_fallback_t} vs {o['dt_fallback_t']}"
     assert g.rows_skipped == o["rows_skipped"], f"{what}: rows_skipped"
-    kind = {0: 0, 1: 1, 2: 2, 3: 3}[o["aborted"]]
+    kind = {0: 0, 1: 1, 2: 2, 3: 3, 5: 5}[o["aborted"]]
     assert g.aborted == kind, f"{what}: aborted {g.aborted} vs {o['aborted']}"
-    if kind in (1, 2):
+    if kind in (1, 2, 5):
         assert g.abort_sync == o["abort_sync"], f"{what}: abort sync"
         assert bits64(g.abort_value) == bits64(o["abort_value"]), f"{what}: abort value"
     if kind == 1:

@@ -1,0 +1,91 @@
+"""Subdomain replay on the device (crates/itr-hydro/src/replay.rs, spec §7.11.3): the
+tape recorded by the baseline run, and candidates replayed on the region of interest,
+bitwise against the oracle's restatement (itself pinned by the reference's replay suite,
+crates/itr-hydro/tests/replay.rs)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import assert_run_equal, oracle_params, oracle_terrain, po
+from inverse_terrain_engine_amd import scenario as sc
+from inverse_terrain_engine_amd import terrain as tr
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 72, 56
+ROI = (20, 14, 30, 26)
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _base():
+    """crates/itr-hydro/tests/replay.rs:15-21: west inflow segment, transmissive east."""
+    segs = [{"kind": "inflow", "side": "west", "hydrograph": [[0.0, 0.0], [200.0, 2.0], [600.0, 0.3]],
+             "range_m": (40.0, 70.0)},
+            {"kind": "transmissive", "side": "east"}]
+    return sc.validation_terrain(NX, NY, 2.0, lambda x, y: 50.0 + 0.004 * (150.0 - x) + 0.3 * math.sin(x / 11.0)
+                                 * math.cos(y / 9.0), n=0.035, segments=segs)
+
+
+def _params():
+    return sc.SolverParams(800.0, 40.0, rain=sc.mm_h([[0.0, 0.0], [100.0, 0.0], [200.0, 60.0], [500.0, 0.0]]))
+
+
+def _in_roi(c, roi=ROI):
+    col, row = c % NX, c // NX
+    return roi[0] <= col < roi[0] + roi[2] and roi[1] <= row < roi[1] + roi[3]
+
+
+def _sub(cells, roi=ROI):
+    return np.array([((c // NX) - roi[1]) * roi[2] + (c % NX) - roi[0] for c in cells], np.uint32)
+
+
+@pytest.mark.parametrize("roi", [ROI, (0, 0, 24, 20), (50, 30, 22, 26)])
+def test_tape_recorded_on_device(solver, roi):
+    """The baseline run records the ring (every step, after the ghost fill) and the band
+    (every sync point) exactly as the oracle's Tape does, also for regions on the grid's
+    edges, whose ring holds ghost cells (wall, transmissive, inflow)."""
+    _say(f"case: record tape, roi {roi}")
+    base = _base()
+    mon = np.array([c for c in range(NX * NY) if c % 7 == 2], np.uint32)
+    tape, run = solver.record_tape(_params(), base, roi, mon)
+    o = po.record_tape(oracle_terrain(base), oracle_params(_params()), roi, precision=32, monitors=mon)
+    assert tape.steps == o["steps"] and tape.syncs == o["sync_count"]
+    assert tape.ring.shape == o["ring"].shape and tape.band.shape == o["band"].shape
+    assert np.array_equal(tape.ring.astype(np.float64).view(np.uint64), o["ring"].view(np.uint64)), "ring differs"
+    assert np.array_equal(tape.band.astype(np.float64).view(np.uint64), o["band"].view(np.uint64)), "band differs"
+    full = po.run(oracle_terrain(base), oracle_params(_params()), precision=32, monitors=mon)
+    assert_run_equal(run, full, "recording baseline")
+
+
+def test_replay_on_region(solver):
+    """crates/itr-hydro/tests/replay.rs:85-119 on the device: with the baseline's locked
+    schedule, the zero-edit replay equals the baseline on the region bitwise; a small
+    interior edit stays valid; a berm across the downstream edge is replay-invalid.  Every
+    replay equals the oracle's replay bitwise (state, monitors, ledger, abort sync)."""
+    _say("case: replay zero edit / small edit / berm on the 30x26 region")
+    base = _base()
+    mon = np.array([c for c in range(NX * NY) if _in_roi(c) and c % 5 == 1], np.uint32)
+    sub_mon = _sub(mon)
+    dts = po.run(oracle_terrain(base), oracle_params(_params()), precision=32, want_dts=True)["dts"]
+    tape, run0 = solver.record_tape(_params(), base, ROI, mon)
+    pl = _params()
+    pl.locked_dt = dts
+    c0, r0, w, h = ROI
+    edits = [None, tr.TerrainEdit(c0=34, r0=26, dz=np.full((3, 3), 0.02)),
+             tr.TerrainEdit(c0=c0 + w - 3, r0=r0, dz=np.full((h, 2), 1.5))]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    runs = solver.run_replay(pl, tape, terrs, sub_mon)
+    otape = {"roi": ROI, "ring": tape.ring.astype(np.float64), "band": tape.band.astype(np.float64)}
+    for i, (r, e) in enumerate(zip(runs, edits)):
+        o = po.run_replay(oracle_terrain(base), oracle_params(pl), otape, precision=32,
+                          edit=None if e is None else e.as_dict(), monitors_sub=sub_mon)
+        assert_run_equal(r, o, f"replay candidate {i}")
+    assert runs[0].aborted == 0 and runs[1].aborted == 0 and runs[2].aborted == 5
+    assert np.array_equal(runs[0].depth.view(np.uint32), run0.depth[r0:r0 + h, c0:c0 + w].view(np.uint32))
+    assert np.array_equal(runs[0].monitor_max.view(np.uint64), run0.monitor_max.view(np.uint64))

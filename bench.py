@@ -319,6 +319,61 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
     return out
 
 
+def replay_leg(solver_cls, sc, tr, n_cand=128, duration=1800.0):
+    """Subdomain replay (replay.rs) on the forward workload's terrain and storm: the
+    candidates' primitives, the assets and the guard strip lie in a 64x64-cell window, and
+    the region of interest is that window with a 48-cell margin.  The baseline records the
+    tape; the candidates then run with the baseline's locked schedule on the full domain
+    and replayed on the region (replay-invalid ones are the evaluator's escalations)."""
+    import numpy as np
+    nx = ny = 512
+    dx = 2.0
+    rng = sc.SplitMix64(5)
+    dem = sc.hills_dem(nx, ny, dx, rng, 40, n_gullies=8, sigma=(15.0, 60.0), noise=0.0)
+    geo = tr.GeoTransform(0.0, ny * dx, dx, dx)
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.035, geo), 1, [{"kind": "transmissive", "side": "east"}])
+    p = sc.SolverParams(duration, 60.0, rain=sc.mm_h([[0, 0], [1800, 50], [3600, 10], [5400, 90], [7200, 0]]),
+                        infil_capacity=5.0 / 1000.0 / 3600.0)
+    w0, wn, margin = 224, 64, 48
+    mon, _, _ = sc.asset_and_guard_cells(nx, ny, [(248, 248, 8, 8)], [])
+    guard = np.array([r * nx + c for r in range(w0, w0 + wn) for c in range(w0 + wn + 12, w0 + wn + 28)], np.uint32)
+    mon = np.unique(np.concatenate([mon, guard])).astype(np.uint32)
+    roi = (w0 - margin, w0 - margin, wn + 2 * margin, wn + 2 * margin)
+    geo_w = tr.GeoTransform(w0 * dx, (ny - w0) * dx, dx, dx)
+    edits = [tr.materialize(sc.random_primitives(rng, 4, geo_w, wn, wn), geo, nx, ny, None, 1.0) for _ in range(n_cand)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    c0, r0, rw, rh = roi
+    sub_mon = np.array([((c // nx) - r0) * rw + (c % nx) - c0 for c in mon], np.uint32)
+    s = solver_cls(0)
+    t0 = time.perf_counter()
+    dts = s.dt_schedule(p, base, tr.PreparedTerrain(base), mon)
+    tape, b0 = s.record_tape(p, base, roi, mon)
+    t_rec = time.perf_counter() - t0
+    pl = sc.SolverParams(duration, 60.0, rain=p.rain, infil_capacity=p.infil_capacity, locked_dt=dts)
+    t0 = time.perf_counter()
+    sess = s.session(pl, [(base, terrs, mon, None)])
+    sess.run()
+    full = sess.results(want_state=False)
+    sess.close()
+    t_full = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rep = s.run_replay(pl, tape, terrs, sub_mon, want_state=False)
+    t_rep = time.perf_counter() - t0
+    s.close()
+    valid = [r for r in rep if r.aborted == 0]
+    same = sum(1 for r, f in zip(rep, full) if r.aborted == 0 and np.array_equal(r.monitor_max, f.monitor_max))
+    dmax = max([float(np.abs(r.monitor_max - f.monitor_max).max()) for r, f in zip(rep, full) if r.aborted == 0] or [0.0])
+    cs_full = sum(r.cell_updates for r in full)
+    cs_rep = sum(r.cell_updates for r in rep)
+    return {"workload": f"{n_cand} candidates x 4 primitives in a 64x64 window of the 512x512 terrain, "
+                        f"{duration:g} s of the storm, region {rw}x{rh} cells ({100.0 * rw * rh / (nx * ny):.1f}% of the grid)",
+            "steps": int(b0.steps), "record_s": t_rec, "full_domain_s": t_full, "replay_s": t_rep,
+            "full_cell_steps_per_s": cs_full / t_full, "replay_cell_steps_per_s": cs_rep / t_rep,
+            "valid": len(valid), "invalid": len(rep) - len(valid), "valid_monitors_equal_full": same,
+            "valid_max_peak_depth_diff_m": dmax,
+            "speedup_vs_full_domain": t_full / t_rep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +389,8 @@ def main():
     ap.add_argument("--adjoint-horizon", type=int, default=120)
     ap.add_argument("--no-ladder", action="store_true", help="skip the fidelity-ladder leg")
     ap.add_argument("--no-screening", action="store_true", help="skip the local-inertial screening leg")
+    ap.add_argument("--no-replay", action="store_true", help="skip the subdomain replay leg")
+    ap.add_argument("--replay-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--screening-only", action="store_true",
                     help="run only the local-inertial screening leg (profiling) and print it")
     ap.add_argument("--no-autotune", action="store_true", help="use the default band height")
@@ -355,6 +412,10 @@ def main():
     t0 = time.time()
     w = sc.config5(n_cand=a.cands, n=a.n)
     t_prep = time.time() - t0
+    if a.replay_only:
+        from inverse_terrain_engine_amd import terrain as tr
+        print(json.dumps(replay_leg(GpuSolver, sc, tr)), flush=True)
+        return
     if a.screening_only:
         from inverse_terrain_engine_amd import terrain as tr
         print(json.dumps(screening_leg(GpuSolver, sc, tr, w, a.steps, warmup=a.warmup)), flush=True)
@@ -427,6 +488,8 @@ def main():
     if not a.copy_probe:
         from inverse_terrain_engine_amd import terrain as tr
         out["prep"] = prep_leg(solver, sc, tr, w)
+        if not a.no_replay:
+            out["replay"] = replay_leg(GpuSolver, sc, tr)
         if not a.no_screening:
             out["screening"] = screening_leg(GpuSolver, sc, tr, w, 200,
                                              cpu_threads=0 if a.no_cpu else (a.cpu_threads or min(16, os.cpu_count() or 1)))

@@ -743,7 +743,6 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const __amdgpu_buffer_rsrc_t rs_gn2 = make_rsrc(G.gn2, G.plane * 4);
     const int code_n = GHOST[ccol];
     const int code_s = GHOST[nx + ccol];
-    const size_t plane = G.plane;
     const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
     const bool skip_dry = G.prm.skip_dry != 0;
@@ -1170,8 +1169,6 @@ __device__ __forceinline__ void ledger_finish(const DevGrid& G, const CandCtl& S
 __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G, const CandCtl* Sin, const StepAcc& A,
                                                 CandCtl& Oc, float* stage, size_t stage_floats, double* shd) {
     const CandCtl& Si = *Sin;
-    CandCtl* O = &Oc;
-    const DevParams& p = G.prm;
     const int tid = threadIdx.x;
     const int nthr = G.waves * kWave;
     const int local = Si.local;

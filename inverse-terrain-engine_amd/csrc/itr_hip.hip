@@ -2086,18 +2086,35 @@ __global__ void __launch_bounds__(256) itr_design_faces_kernel(DesignDev d) {
 }
 
 // Cut and fill volumes of each edit: row-major over the window, as design.rs:373-380 sums.
-__global__ void itr_design_volumes_kernel(DesignDev d) {
-    const int cand = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cand >= d.ncand) return;
+// One workgroup per candidate: the window's dz values are staged through LDS with
+// coalesced loads, then one thread adds them in the reference's (row-major) order.
+constexpr int kVolChunk = 4096;
+__global__ void __launch_bounds__(256) itr_design_volumes_kernel(DesignDev d) {
+    __shared__ double buf[kVolChunk];
+    const int cand = blockIdx.x;
+    const int tid = threadIdx.x;
     const int64_t n = (int64_t)d.bbox[4 * cand + 2] * d.bbox[4 * cand + 3];
     const double* v = d.dzbuf + d.dz_off[cand];
+    const double area = d.area;
     double cut = 0.0, fill = 0.0;
-    for (int64_t i = 0; i < n; i++) {
-        if (v[i] > 0.0) fill += v[i] * d.area;
-        else cut -= v[i] * d.area;
+    for (int64_t b = 0; b < n; b += kVolChunk) {
+        const int cnt = (int)min((int64_t)kVolChunk, n - b);
+        for (int i = tid; i < cnt; i += blockDim.x) buf[i] = v[b + i];
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (int i = 0; i < cnt; i++) {
+                const double x = buf[i];
+                if (x > 0.0) fill += x * area;
+                else cut -= x * area;
+            }
+        }
+        __syncthreads();
     }
-    d.volumes[2 * cand] = cut;
-    d.volumes[2 * cand + 1] = fill;
+    if (tid == 0) {
+        d.volumes[2 * cand] = cut;
+        d.volumes[2 * cand + 1] = fill;
+    }
 }
 
 // ------------------------------------------------------------------ discrete adjoint
@@ -2680,7 +2697,7 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
         hipLaunchKernelGGL(itr_materialize_kernel, dim3((unsigned)((maxwh + 255) / 256), (unsigned)nc), dim3(256), 0, st, dd);
     hipLaunchKernelGGL(itr_design_faces_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
                        dim3(256), 0, st, dd);
-    hipLaunchKernelGGL(itr_design_volumes_kernel, dim3((unsigned)((nc + 63) / 64)), dim3(64), 0, st, dd);
+    hipLaunchKernelGGL(itr_design_volumes_kernel, dim3((unsigned)nc), dim3(256), 0, st, dd);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess && Ds.volumes) e = hipMemcpy(Ds.volumes, vol, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost);

@@ -212,7 +212,8 @@ def adjoint_best16(solver, sc, ob, w, res, horizon):
     objv = ob.Objective([ob.Asset(np.asarray(a), 0.05, 1.0) for a in w.asset_pos], 0.01, 1e-3)
     js = [objv.j(r.monitor_max, e.cut_m3, e.fill_m3) for r, e in zip(res, w.edits)]
     best = [int(i) for i in np.argsort(js)[:16]]
-    p = w.params
+    import copy
+    p = copy.copy(w.params)   # (the workload's own parameters serve the later legs)
     p.cfl_hard = 1.0
     sched = solver.dt_schedule(p, base, w.candidates[best[0]], w.monitors, max_steps=horizon)
     p.locked_dt = sched

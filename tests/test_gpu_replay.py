@@ -89,3 +89,44 @@ def test_replay_on_region(solver):
     assert runs[0].aborted == 0 and runs[1].aborted == 0 and runs[2].aborted == 5
     assert np.array_equal(runs[0].depth.view(np.uint32), run0.depth[r0:r0 + h, c0:c0 + w].view(np.uint32))
     assert np.array_equal(runs[0].monitor_max.view(np.uint64), run0.monitor_max.view(np.uint64))
+
+
+def test_replay_crest_level_with_guard(solver):
+    """Replay on a face-crest coarse level (level 2: two rises per face, sub-terrain faces
+    copied from each candidate's crest faces), with the device guard abort running inside
+    the replay as the reference's GuardObserver does (evaluator.rs:427-431)."""
+    _say("case: replay on the level-2 crest grid with a guard")
+    fine = _base().fine
+    base = tr.TerrainBase.build(fine, 2, _base().segments)
+    nx, ny = base.nx, base.ny
+    roi = (10, 7, 15, 13)
+    inside = lambda c: roi[0] <= c % nx < roi[0] + roi[2] and roi[1] <= c // nx < roi[1] + roi[3]
+    mon = np.array([c for c in range(nx * ny) if inside(c) and c % 3 == 0], np.uint32)
+    sub_mon = np.array([((c // nx) - roi[1]) * roi[2] + (c % nx) - roi[0] for c in mon], np.uint32)
+    p = _params()
+    dts = po.run(oracle_terrain(base), oracle_params(p), precision=32, want_dts=True)["dts"]
+    tape, run0 = solver.record_tape(p, base, roi, mon)
+    o = po.record_tape(oracle_terrain(base), oracle_params(p), roi, precision=32, monitors=mon)
+    assert np.array_equal(tape.ring.astype(np.float64).view(np.uint64), o["ring"].view(np.uint64))
+    assert np.array_equal(tape.band.astype(np.float64).view(np.uint64), o["band"].view(np.uint64))
+    pl = _params()
+    pl.locked_dt = dts
+    rng = sc.SplitMix64(17)
+    geo = fine.geo
+    c0f, r0f = 2 * roi[0] + 6, 2 * roi[1] + 6
+    geo_w = tr.GeoTransform(geo.origin_x + c0f * geo.dx, geo.origin_y - r0f * geo.dy, geo.dx, geo.dy)
+    edits = [tr.materialize(sc.random_primitives(rng, 2, geo_w, 16, 12), geo, fine.nx, fine.ny, None, 1.0)
+             for _ in range(6)]
+    terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
+    gpos = np.arange(0, len(mon), 4, dtype=np.int32)
+    guard = {"baseline_max": run0.monitor_max, "guard_pos": gpos, "tol": 0.0005}
+    runs = solver.run_replay(pl, tape, terrs, sub_mon, guard=guard)
+    otape = {"roi": roi, "ring": tape.ring.astype(np.float64), "band": tape.band.astype(np.float64)}
+    kinds = []
+    for i, (r, e) in enumerate(zip(runs, edits)):
+        obs = {"baseline_max": run0.monitor_max, "guard_pos": gpos, "tol": 0.0005, "require": True}
+        oo = po.run_replay(oracle_terrain(base), oracle_params(pl), otape, precision=32, edit=e.as_dict(),
+                           monitors_sub=sub_mon, observer=obs)
+        assert_run_equal(r, oo, f"crest replay candidate {i}")
+        kinds.append(r.aborted)
+    _say("abort kinds:", kinds)

@@ -285,6 +285,7 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
         p = copy.copy(w.params)
         p.kernel = kernel
         s = solver_cls(0)
+        tuned = s.autotune(p, [(base2, terrs, mon, None)], trial_steps=16)
         sess = s.session(p, [(base2, terrs, mon, None)])
         sess.advance(warmup)
         t_a = np.array([r.t_end for r in sess.results(want_state=False)])
@@ -300,7 +301,7 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
             "cell_steps_per_s": st["cell_steps"] / wall, "us_per_step": kern_s * 1e6,
             "sim_s_per_step": float((t_b - t_a).mean()) / steps,
             "sim_s_per_wall_s": float((t_b - t_a).mean()) / wall,
-            "hbm_frac_40B": 40.0 * cells / kern_s / HBM_PEAK}
+            "hbm_frac_40B": 40.0 * cells / kern_s / HBM_PEAK, "band_rows": tuned[0][0]}
     li, hl = out["local_inertial"], out["hll"]
     out["sim_time_speedup_vs_hll"] = li["sim_s_per_wall_s"] / hl["sim_s_per_wall_s"]
     if cpu_threads:

@@ -1318,7 +1318,7 @@ __device__ __forceinline__ double li_boundary(int code, bool lo, float qold, flo
 }
 
 // Dynamic LDS of the local-inertial step, in floats.
-struct LiLds { size_t fixmask, red, smx, dsum, acc, moff, gsp, total; };
+struct LiLds { size_t fixmask, red, smx, dsum, acc, moff, bnd, gsp, total; };
 __host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
     auto al = [](size_t v) { return (v + 3) & ~(size_t)3; };
     LiLds L;
@@ -1328,7 +1328,8 @@ __host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
     L.dsum = al(L.smx + 2 * kMaxWaves);             // kMaxWaves x 2 f64
     L.acc = al(L.dsum + 4 * kMaxWaves);             // R x 16 FixedSum lanes
     L.moff = al(L.acc + (size_t)R * 16);
-    L.gsp = al(L.moff + (size_t)kMaxWaves * 64);
+    L.bnd = al(L.moff + (size_t)kMaxWaves * 64);      // the band's west / east boundary faces
+    L.gsp = al(L.bnd + (size_t)2 * R);
     L.total = al(L.gsp + (special ? 4 * gsp_pairs(R, nx) : 0));
     if (L.total < 64) L.total = 64;
     return L;
@@ -1408,6 +1409,35 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     for (int r = tid; r < R; r += nthr) l_fixmask[r] = 0u;
     if (has_m) l_moff[tid] = m_off0;
     __syncthreads();
+    // The band's west and east boundary faces (inertial.rs:104-124), one thread each, so
+    // that the march's edge waves carry no boundary branch.
+    float* l_bnd = lds + LL.bnd;
+    {
+        const float* S0 = old_state;
+        for (int i = tid; i < 2 * nrows; i += nthr) {
+            const bool west = i < nrows;
+            const int r = a0 + (west ? i : i - nrows);
+            const size_t ro = (size_t)r * P;
+            const int code = GHOST[2 * nx + (west ? 0 : ny) + r];
+            float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (special && code >= 2) gs = l_gsp[(west ? 0 : R + 2) + (r - (a0 - 1))];
+            const int fc = west ? min(1, nx - 1) : nx - 1;   // the adjacent interior face (nx > 1)
+            const int cc = west ? 0 : nx - 1;                // the interior cell
+            float qold;
+            if (west) {
+                qold = lif_in[r];
+                if (special && code == 3) qold = gs.y;
+            } else {
+                qold = S0[plane + ro + cc];
+            }
+            const float f0 = G.dzx[(size_t)local * plane + ro + max(fc, 1)];
+            const float f1 = CREST ? G.dzx2[(size_t)local * plane + ro + max(fc, 1)] : 0.0f;
+            const double q = li_boundary<CREST>(code, west, qold, S0[ro + cc], gs.x, gs.y, f0, f1, nx > 1, K.dx,
+                                                G.gn2[ro + cc], K);
+            l_bnd[i] = canon((float)q);
+        }
+    }
+    __syncthreads();
     // Special ghost pairs of this lane: west / east of row r, north / south of its column.
     auto gs_we = [&](int r) -> float4 { return l_gsp[(west_lane ? 0 : R + 2) + (r - (a0 - 1))]; };
     auto gs_ns = [&](bool north) -> float4 { return l_gsp[2 * (R + 2) + (north ? 0 : nx) + (int)ccol]; };
@@ -1450,22 +1480,14 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         const float g2_r = from_right(c.gn2);
         const float q_l = from_left(c.qx);
         const float q_r = from_right(c.qx);
-        double q;
-        if (west_lane || east_lane) {
-            const int code = GHOST[we_base + r];
-            float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (special && code >= 2) gs = gs_we(r);
-            q = west_lane ? li_boundary<CREST>(code, true, c.qx, h_r, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, g2_r, K)
-                          : li_boundary<CREST>(code, false, c.qx, c.h, gs.x, gs.y, c.dzx, c.dzx2, nx > 1, K.dx, c.gn2, K);
-        } else {
-            double dl, dr;
-            li_rises<CREST>(c.dzx, c.dzx2, dl, dr);
-            const double n2 = 0.5 * ((double)c.gn2 + (double)g2_r);
-            double qt = (double)c.qx;
-            if (col + 2 < nx) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)q_l + (double)q_r);
-            q = li_face_q(qt, (double)c.h, (double)h_r, dl, dr, K.dx, n2, K);
-        }
-        return canon((float)q);
+        double dl, dr;
+        li_rises<CREST>(c.dzx, c.dzx2, dl, dr);
+        const double n2 = 0.5 * ((double)c.gn2 + (double)g2_r);
+        double qt = (double)c.qx;
+        if (col + 2 < nx) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)q_l + (double)q_r);
+        const float qi = canon((float)li_face_q(qt, (double)c.h, (double)h_r, dl, dr, K.dx, n2, K));
+        // boundary faces come from the band's prologue
+        return west_lane ? l_bnd[r - a0] : east_lane ? l_bnd[nrows + r - a0] : qi;
     };
     // New y-face k between rows k-1 (u) and k (d), 1 <= k <= ny-1: u.qy is its old
     // discharge, qp and d.qy those of faces k-1 and k+1.
@@ -1501,13 +1523,23 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         qn_new = yface(up, cur, qy_old(a0 - 1), a0);
         qprev = up.qy;
     }
+    // The faces of a row depend on the old state only, so each iteration computes the next
+    // row's faces beside this row's continuity (two independent f64 chains per lane).
+    // Row loads are issued two rows ahead of the row they complete (rows j+2 and j+3 are
+    // in flight while row j is finished).
+    float qe = xface(cur, a0);
+    float qs = a0 + 1 < ny ? yface(cur, nxt, qprev, a0 + 1) : yface_south(cur);
     double smax_l = 0.0, inf_l = 0.0, ss_l = 0.0;
     bool bad = false;
+    Row pre = fetch(a0 + 2);
     for (int j = a0; j < b0; j++) {
-        const Row pre = fetch(j + 2);
-        const float qe = xface(cur, j);
+        const Row pre2 = fetch(j + 3);
+        float qe_n = 0.0f, qs_n = 0.0f;   // faces of row j + 1
+        if (j + 1 < b0) {
+            qe_n = xface(nxt, j + 1);
+            qs_n = j + 2 < ny ? yface(nxt, pre, cur.qy, j + 2) : yface_south(nxt);
+        }
         const float qw = from_left(qe);
-        const float qs = j + 1 < ny ? yface(cur, nxt, qprev, j + 1) : yface_south(cur);
         // Continuity and sources (inertial.rs:166-196).
         const double wet = (double)cur.wet;
         const double dw = (double)qw, de = (double)qe, dn = (double)qn_new, ds = (double)qs;
@@ -1542,10 +1574,12 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
             if (upd) fix64[(size_t)j * P + col] = fixv;
             if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
         }
-        qprev = cur.qy;
         qn_new = qs;
+        qe = qe_n;
+        qs = qs_n;
         cur = nxt;
         nxt = pre;
+        pre = pre2;
     }
 
     // Band reductions: f64 CFL max (ordered bits; rates are >= 0 and never NaN here),

@@ -46,14 +46,17 @@ def test_struct_layouts_match_header():
 #include "itr_hip.h"
 #define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
 int main(void) {
-  printf("itr_solver_params %zu\nitr_grid %zu\nitr_run_out %zu\nitr_stats %zu\nitr_objective %zu\nitr_adjoint_out %zu\nitr_tune_result %zu\nitr_design %zu\n",
+  printf("itr_solver_params %zu\nitr_grid %zu\nitr_run_out %zu\nitr_stats %zu\nitr_objective %zu\nitr_adjoint_out %zu\nitr_tune_result %zu\nitr_design %zu\nitr_replay %zu\n",
          sizeof(itr_solver_params), sizeof(itr_grid), sizeof(itr_run_out), sizeof(itr_stats), sizeof(itr_objective),
-         sizeof(itr_adjoint_out), sizeof(itr_tune_result), sizeof(itr_design));
+         sizeof(itr_adjoint_out), sizeof(itr_tune_result), sizeof(itr_design), sizeof(itr_replay));
   F(itr_solver_params, rain_t) F(itr_solver_params, max_steps) F(itr_solver_params, skip_dry)
+  F(itr_solver_params, physics)
   F(itr_grid, wet) F(itr_grid, n_cand) F(itr_grid, dzx) F(itr_grid, mon_cells) F(itr_grid, guard_tol)
   F(itr_grid, ghost_value) F(itr_grid, hydro_q) F(itr_grid, n_hydro) F(itr_grid, params) F(itr_grid, design)
+  F(itr_grid, replay)
   F(itr_design, bbox) F(itr_design, origin_x) F(itr_design, volumes)
-  F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger)
+  F(itr_replay, ring) F(itr_replay, ring_steps) F(itr_replay, tol_q) F(itr_replay, n_syncs_out)
+  F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger) F(itr_run_out, dt_fallback_t)
   F(itr_stats, launches) F(itr_stats, use_graph)
   F(itr_objective, asset_start) F(itr_objective, tau_m) F(itr_adjoint_out, dJ_dz_cell) F(itr_adjoint_out, steps)
   F(itr_adjoint_out, replay_mismatches) F(itr_adjoint_out, reverse_ms) F(itr_tune_result, us_per_step)
@@ -69,7 +72,7 @@ int main(void) {
         out = dict(line.rsplit(" ", 1) for line in subprocess.run([exe], capture_output=True, text=True).stdout.split("\n") if line)
     ct = {"itr_solver_params": _abi.SolverParamsC, "itr_grid": _abi.GridC, "itr_run_out": _abi.RunOutC,
           "itr_stats": _abi.StatsC, "itr_objective": _abi.ObjectiveC, "itr_adjoint_out": _abi.AdjointOutC,
-          "itr_tune_result": _abi.TuneResultC, "itr_design": _abi.DesignC}
+          "itr_tune_result": _abi.TuneResultC, "itr_design": _abi.DesignC, "itr_replay": _abi.ReplayC}
     for k, T in ct.items():
         assert int(out[k]) == C.sizeof(T), k
     for key, v in out.items():

@@ -49,6 +49,13 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kColsPerWave = 62;   // updated columns per wave (lanes 1..62)
+// Row groups the last wave of an HLL grid marches side by side: a lane group holds the
+// last wave's t columns plus a halo lane each side, so 64 / (t + 2) groups fit (at most 8).
+constexpr int tail_group_count(int nx, int waves) {
+    const int t = nx - (waves - 1) * kColsPerWave;
+    const int s = 64 / (t + 2);
+    return s < 2 ? 1 : (s > 8 ? 8 : s);
+}
 constexpr int kMaxWaves = 16;      // 1024 threads: rows up to 992 columns
 
 constexpr uint32_t F_DONE = 1u;
@@ -96,7 +103,9 @@ struct DevGrid {
     int32_t nslot;             // max(ncand, 1): stride of the double-buffered per-candidate arrays
     int32_t max_log;           // capacity of steplog per candidate (0: no log)
     int32_t li;                // local-inertial physics (inertial.rs) instead of HLL
-    int32_t pad_li;
+    // HLL grids whose last wave updates few columns: that wave marches tail_groups row
+    // groups of each band side by side, in lane groups of 64 / tail_groups lanes (1: off).
+    int32_t tail_groups;
     size_t plane;              // ny * P
     size_t plane_y;            // (ny + 1) * P: dzy rows 0..ny (rows 0 and ny are zero)
     float* state;              // [ncand][2][3][plane]
@@ -232,6 +241,8 @@ __device__ __forceinline__ float from_left(float x) {   // lane i <- lane i-1 (D
 __device__ __forceinline__ float from_right(float x) {  // lane i <- lane i+1 (DPP wave_shl:1)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
 }
+
+template <bool B> struct BoolC { static constexpr bool value = B; };
 
 // Wave-uniform values into scalar registers.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -692,8 +703,19 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const int nthr = G.waves * kWave;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int col = wave * kColsPerWave - 1 + lane;
-    const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
+    const int nwaves = G.waves;
+    // The tail wave (the last wave, when it updates few columns) splits into `tail_s` lane
+    // groups of `tgw` lanes; group g marches rows a0 + g*t_rows .. of the band.  Every
+    // other wave is one group of 64 lanes.  A group's first and last lanes are halos.
+    const int tail_s = SPECIAL ? 1 : G.tail_groups;
+    const bool tail = tail_s > 1 && uni(wave) == nwaves - 1;
+    const int tgw = tail ? 64 / tail_s : 64;
+    const int tg = tail ? lane / tgw : 0;
+    const int gl = lane - tg * tgw;   // lane within its group
+    const int t_rows = tail ? (nrows + tail_s - 1) / tail_s : nrows;
+    const int lrow = tg * t_rows;     // the group's first row, relative to a0
+    const int col = wave * kColsPerWave - 1 + gl;
+    const bool upd = gl >= 1 && gl <= tgw - 2 && tg < tail_s && col < nx;
     const int npar = par ^ 1;
 
     const size_t cstride = 6 * G.plane;
@@ -730,7 +752,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const uint32_t planeb = (uint32_t)(G.plane * 4u);
     const __amdgpu_buffer_rsrc_t rs_ho = make_rsrc(base + (size_t)npar * 3 * G.plane, 3 * G.plane * 4);
     const bool in_y = col >= 0 && col < nx;   // columns that own a y-face
-    const bool owner = in_y && lane >= 1 && lane <= kColsPerWave;
+    const bool owner = in_y && gl >= 1 && gl <= tgw - 2 && tg < tail_s;
     // Everything read from the grid descriptor is read here, before the first store:
     // later loads of it could not be proven unclobbered and would become per-lane
     // loads (and per-lane buffer descriptors).
@@ -743,7 +765,6 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const __amdgpu_buffer_rsrc_t rs_gn2 = make_rsrc(G.gn2, G.plane * 4);
     const int code_n = GHOST[ccol];
     const int code_s = GHOST[nx + ccol];
-    const int nwaves = G.waves;
     const bool edge_wave = wave == 0 || wave == nwaves - 1;   // only they hold ghost columns
     const bool skip_dry = G.prm.skip_dry != 0;
     const bool special = SPECIAL && G.special != 0;
@@ -863,11 +884,27 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     auto dzy_at = [&](__amdgpu_buffer_rsrc_t D, int row) -> float { return bload(D, cb, (uint32_t)min(row, ny) * rowb); };
     auto dzx_at = [&](__amdgpu_buffer_rsrc_t D, int row) -> float { return bload(D, xb, (uint32_t)row * rowb); };
     auto row_at = [&](__amdgpu_buffer_rsrc_t A, int row) -> float { return bload(A, cb, (uint32_t)min(row, ny - 1) * rowb); };
-
-    // Prologue: rows a0-1 and a0, and the y-face between them.
-    const Cell cp = fix(fetch(a0 - 1), a0 - 1);
-    const Cell c0 = fix(fetch(a0), a0);
-    float ch = c0.h, cqx = c0.qx, cqy = c0.qy;
+    // The same reads for the tail wave, whose lanes read row `row + lrow` (their group's
+    // row): per-lane row offsets, clamped per lane.
+    auto fetch_t = [&](int row) -> Cell {
+        const int rr = min(max(row + lrow, 0), ny - 1);
+        const uint32_t vo = cb + (uint32_t)rr * rowb;
+        Cell v;
+        v.h = bload(rs_h, vo, 0u);
+        v.qx = bload(rs_h, vo, planeb);
+        v.qy = bload(rs_h, vo, 2 * planeb);
+        v.code_we = GHOST[2 * nx + (col < 0 ? 0 : ny) + rr];
+        return v;
+    };
+    auto dzy_t = [&](__amdgpu_buffer_rsrc_t D, int row) -> float {
+        return bload(D, cb + (uint32_t)min(row + lrow, ny) * rowb, 0u);
+    };
+    auto dzx_t = [&](__amdgpu_buffer_rsrc_t D, int row) -> float {
+        return bload(D, xb + (uint32_t)min(row + lrow, ny - 1) * rowb, 0u);
+    };
+    auto row_t = [&](__amdgpu_buffer_rsrc_t A, int row) -> float {
+        return bload(A, cb + (uint32_t)min(row + lrow, ny - 1) * rowb, 0u);
+    };
 
     // Subdomain replay tape (replay.rs:112-176; special kernel only).  Record mode: the
     // ROI's ghost ring from the old state with ghosts applied (Tape::record_ring, after the
@@ -899,129 +936,180 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         return (i == 1 || jj == 1 || i == rw || jj == rh) ? band_index(i, jj, rw, rh) : -1;
     };
     double dh_l = 0.0, dq_l = 0.0;
-    if (rec && ring_owner && a0 == 0) ring_put(-1, cp.h, cp.qx, cp.qy);
-    const Vel vp = cell_velocity(cp.h, cp.qx, cp.qy, K);
-    Vel vc = cell_velocity(ch, cqx, cqy, K);
-    Flux fn;
-    if (CREST) fn = face_crest<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), dzy_at(rs_dzy2, a0), K);
-    else fn = face_signed<false>(cp.h, vp, ch, vc, dzy_at(rs_dzy, a0), K);
-    if (a0 == 0 && owner) yb_out[col] = fn.m;
-
-    // Row inputs are loaded DEPTH rows ahead of their use (software pipelining; two rows
-    // cover HBM latency at two resident workgroups per CU, one row keeps the register
-    // budget of the crest variants).
-    Cell pre_c = fetch(a0 + 1), pre_c2{};
-    float pre_dzs = dzy_at(rs_dzy, a0 + 1), pre_dzs2 = CREST ? dzy_at(rs_dzy2, a0 + 1) : 0.0f;
-    float pre_dzw = dzx_at(rs_dzx, a0), pre_dzw2 = CREST ? dzx_at(rs_dzx2, a0) : 0.0f;
-    float pre_wet = row_at(rs_wet, a0), pre_gn2 = row_at(rs_gn2, a0);
-    float nxt_dzs = 0.0f, nxt_dzs2 = 0.0f, nxt_dzw = 0.0f, nxt_dzw2 = 0.0f, nxt_wet = 0.0f, nxt_gn2 = 0.0f;
-    if (DEPTH == 2) {
-        pre_c2 = fetch(a0 + 2);
-        nxt_dzs = dzy_at(rs_dzy, a0 + 2);
-        if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, a0 + 2);
-        nxt_dzw = dzx_at(rs_dzx, min(a0 + 1, ny - 1));
-        if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(a0 + 1, ny - 1));
-        nxt_wet = row_at(rs_wet, a0 + 1);
-        nxt_gn2 = row_at(rs_gn2, a0 + 1);
-    }
-
     uint32_t* l_moff = (uint32_t*)(lds + LL.moff);
     if (has_m) l_moff[tid] = m_off0;   // read back after the march
 
     int smax_bits = 0;
     bool bad = false;
-    for (int j = a0; j < b0; j++) {
-        const Cell cn = fix(pre_c, j + 1);
-        const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
-        const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
-        const float wet = pre_wet, gn2 = pre_gn2;
+    // The march down the band.  TL: the tail wave's form, whose lanes are at row j + lrow
+    // (their group's row) when the march is at row j; everything else is the same code.
+    auto march = [&](auto tag) {
+        constexpr bool TL = decltype(tag)::value;
+        auto FETCH = [&](int row) -> Cell {
+            if constexpr (TL) return fetch_t(row);
+            else return fetch(row);
+        };
+        auto DZY = [&](__amdgpu_buffer_rsrc_t D, int row) -> float {
+            if constexpr (TL) return dzy_t(D, row);
+            else return dzy_at(D, row);
+        };
+        auto DZX = [&](__amdgpu_buffer_rsrc_t D, int row) -> float {
+            if constexpr (TL) return dzx_t(D, row);
+            else return dzx_at(D, row);
+        };
+        auto ROW = [&](__amdgpu_buffer_rsrc_t A, int row) -> float {
+            if constexpr (TL) return row_t(A, row);
+            else return row_at(A, row);
+        };
+        const int lr = TL ? lrow : 0;
+
+        // Prologue: rows a0-1 and a0, and the y-face between them.
+        const Cell cp = fix(FETCH(a0 - 1), a0 - 1 + lr);
+        const Cell c0 = fix(FETCH(a0), a0 + lr);
+        float ch = c0.h, cqx = c0.qx, cqy = c0.qy;
+        if (!TL && rec && ring_owner && a0 == 0) ring_put(-1, cp.h, cp.qx, cp.qy);
+        const Vel vp = cell_velocity(cp.h, cp.qx, cp.qy, K);
+        Vel vc = cell_velocity(ch, cqx, cqy, K);
+        Flux fn;
+        if (CREST) fn = face_crest<false>(cp.h, vp, ch, vc, DZY(rs_dzy, a0), DZY(rs_dzy2, a0), K);
+        else fn = face_signed<false>(cp.h, vp, ch, vc, DZY(rs_dzy, a0), K);
+        if (a0 == 0 && owner && lr == 0) yb_out[col] = fn.m;
+
+        // Row inputs are loaded DEPTH rows ahead of their use (software pipelining; two rows
+        // cover HBM latency at two resident workgroups per CU, one row keeps the register
+        // budget of the crest variants).
+        Cell pre_c = FETCH(a0 + 1), pre_c2{};
+        float pre_dzs = DZY(rs_dzy, a0 + 1), pre_dzs2 = CREST ? DZY(rs_dzy2, a0 + 1) : 0.0f;
+        float pre_dzw = DZX(rs_dzx, a0), pre_dzw2 = CREST ? DZX(rs_dzx2, a0) : 0.0f;
+        float pre_wet = ROW(rs_wet, a0), pre_gn2 = ROW(rs_gn2, a0);
+        float nxt_dzs = 0.0f, nxt_dzs2 = 0.0f, nxt_dzw = 0.0f, nxt_dzw2 = 0.0f, nxt_wet = 0.0f, nxt_gn2 = 0.0f;
         if (DEPTH == 2) {
-            pre_c = pre_c2;
-            pre_dzs = nxt_dzs;
-            pre_dzs2 = nxt_dzs2;
-            pre_dzw = nxt_dzw;
-            pre_dzw2 = nxt_dzw2;
-            pre_wet = nxt_wet;
-            pre_gn2 = nxt_gn2;
-            pre_c2 = fetch(j + 3);
-            nxt_dzs = dzy_at(rs_dzy, j + 3);
-            if (CREST) nxt_dzs2 = dzy_at(rs_dzy2, j + 3);
-            nxt_dzw = dzx_at(rs_dzx, min(j + 2, ny - 1));
-            if (CREST) nxt_dzw2 = dzx_at(rs_dzx2, min(j + 2, ny - 1));
-            nxt_wet = row_at(rs_wet, j + 2);
-            nxt_gn2 = row_at(rs_gn2, j + 2);
-        } else {
-            pre_c = fetch(j + 2);
-            pre_dzs = dzy_at(rs_dzy, j + 2);
-            if (CREST) pre_dzs2 = dzy_at(rs_dzy2, j + 2);
-            pre_dzw = dzx_at(rs_dzx, min(j + 1, ny - 1));
-            if (CREST) pre_dzw2 = dzx_at(rs_dzx2, min(j + 1, ny - 1));
-            pre_wet = row_at(rs_wet, j + 1);
-            pre_gn2 = row_at(rs_gn2, j + 1);
+            pre_c2 = FETCH(a0 + 2);
+            nxt_dzs = DZY(rs_dzy, a0 + 2);
+            if (CREST) nxt_dzs2 = DZY(rs_dzy2, a0 + 2);
+            nxt_dzw = DZX(rs_dzx, min(a0 + 1, ny - 1));
+            if (CREST) nxt_dzw2 = DZX(rs_dzx2, min(a0 + 1, ny - 1));
+            nxt_wet = ROW(rs_wet, a0 + 1);
+            nxt_gn2 = ROW(rs_gn2, a0 + 1);
         }
 
-        const Vel vn = cell_velocity(nh, nqx, nqy, K);
-        Flux fs;
-        if (CREST) fs = face_crest<false>(ch, vc, nh, vn, dzs, dzs2, K);
-        else fs = face_signed<false>(ch, vc, nh, vn, dzs, K);
-        // West face of this lane's column, from the left neighbour's old state.
-        Vel vw;
-        const float hw = from_left(ch);
-        vw.u = from_left(vc.u);
-        vw.v = from_left(vc.v);
-        vw.c = from_left(vc.c);
-        Flux fw;
-        if (CREST) fw = face_crest<true>(hw, vw, ch, vc, dzw, dzw2, K);
-        else fw = face_signed<true>(hw, vw, ch, vc, dzw, K);
-        const float em = from_right(fw.m);
-        const float enl = from_right(fw.nl);
-        const float et = from_right(fw.t);
-        const CellNew u = cell_update(ch, cqx, cqy, fw.m, fw.nr, fw.t, em, enl, et, fn.m, fn.nr, fn.t, fs.m, fs.nl,
-                                      fs.t, wet, gn2, K);
-        const int r = j - a0;
-        if (SPECIAL && (rec || chk)) {
-            if (rec && ring_owner) ring_put(j, ch, cqx, cqy);
-            const int bi = (rec_band || chk) ? band_at(j) : -1;
-            if (bi >= 0) {
-                float* bt = G.tape_band + ((size_t)sync_idx * G.n_band + bi) * 3;
-                if (rec_band) {
-                    bt[0] = u.h;
-                    bt[1] = u.qx;
-                    bt[2] = u.qy;
-                } else {
-                    dh_l = fmax(dh_l, fabs((double)u.h - (double)bt[0]));
-                    dq_l = fmax(fmax(dq_l, fabs((double)u.qx - (double)bt[1])), fabs((double)u.qy - (double)bt[2]));
+        const int j_end = TL ? a0 + t_rows : b0;
+        for (int j = a0; j < j_end; j++) {
+            const int jl = j + lr;          // this lane's row
+            const bool rv = !TL || jl < b0;   // a row of the band (a tail group's last rows may not be)
+            const Cell cn = fix(pre_c, jl + 1);
+            const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
+            const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
+            const float wet = pre_wet, gn2 = pre_gn2;
+            if (DEPTH == 2) {
+                pre_c = pre_c2;
+                pre_dzs = nxt_dzs;
+                pre_dzs2 = nxt_dzs2;
+                pre_dzw = nxt_dzw;
+                pre_dzw2 = nxt_dzw2;
+                pre_wet = nxt_wet;
+                pre_gn2 = nxt_gn2;
+                pre_c2 = FETCH(j + 3);
+                nxt_dzs = DZY(rs_dzy, j + 3);
+                if (CREST) nxt_dzs2 = DZY(rs_dzy2, j + 3);
+                nxt_dzw = DZX(rs_dzx, min(j + 2, ny - 1));
+                if (CREST) nxt_dzw2 = DZX(rs_dzx2, min(j + 2, ny - 1));
+                nxt_wet = ROW(rs_wet, j + 2);
+                nxt_gn2 = ROW(rs_gn2, j + 2);
+            } else {
+                pre_c = FETCH(j + 2);
+                pre_dzs = DZY(rs_dzy, j + 2);
+                if (CREST) pre_dzs2 = DZY(rs_dzy2, j + 2);
+                pre_dzw = DZX(rs_dzx, min(j + 1, ny - 1));
+                if (CREST) pre_dzw2 = DZX(rs_dzx2, min(j + 1, ny - 1));
+                pre_wet = ROW(rs_wet, j + 1);
+                pre_gn2 = ROW(rs_gn2, j + 1);
+            }
+
+            const Vel vn = cell_velocity(nh, nqx, nqy, K);
+            Flux fs;
+            if (CREST) fs = face_crest<false>(ch, vc, nh, vn, dzs, dzs2, K);
+            else fs = face_signed<false>(ch, vc, nh, vn, dzs, K);
+            // West face of this lane's column, from the left neighbour's old state.
+            Vel vw;
+            const float hw = from_left(ch);
+            vw.u = from_left(vc.u);
+            vw.v = from_left(vc.v);
+            vw.c = from_left(vc.c);
+            Flux fw;
+            if (CREST) fw = face_crest<true>(hw, vw, ch, vc, dzw, dzw2, K);
+            else fw = face_signed<true>(hw, vw, ch, vc, dzw, K);
+            const float em = from_right(fw.m);
+            const float enl = from_right(fw.nl);
+            const float et = from_right(fw.t);
+            const CellNew u = cell_update(ch, cqx, cqy, fw.m, fw.nr, fw.t, em, enl, et, fn.m, fn.nr, fn.t, fs.m, fs.nl,
+                                          fs.t, wet, gn2, K);
+            const int r = jl - a0;
+            if (!TL && SPECIAL && (rec || chk)) {
+                if (rec && ring_owner) ring_put(j, ch, cqx, cqy);
+                const int bi = (rec_band || chk) ? band_at(j) : -1;
+                if (bi >= 0) {
+                    float* bt = G.tape_band + ((size_t)sync_idx * G.n_band + bi) * 3;
+                    if (rec_band) {
+                        bt[0] = u.h;
+                        bt[1] = u.qx;
+                        bt[2] = u.qy;
+                    } else {
+                        dh_l = fmax(dh_l, fabs((double)u.h - (double)bt[0]));
+                        dq_l = fmax(fmax(dq_l, fabs((double)u.qx - (double)bt[1])), fabs((double)u.qy - (double)bt[2]));
+                    }
                 }
             }
+            if (upd && rv) {
+                if constexpr (TL) {
+                    const uint32_t vo = cb + (uint32_t)jl * rowb;
+                    bstore(u.h, rs_ho, vo, 0u);
+                    bstore(u.qx, rs_ho, vo, planeb);
+                    bstore(u.qy, rs_ho, vo, 2 * planeb);
+                } else {
+                    const uint32_t so = (uint32_t)j * rowb;
+                    bstore(u.h, rs_ho, cb, so);
+                    bstore(u.qx, rs_ho, cb, so + planeb);
+                    bstore(u.qy, rs_ho, cb, so + 2 * planeb);
+                }
+                l_infil[r * nx + col] = u.infil;
+                smax_bits = max(smax_bits, __float_as_int(u.s));
+                bad = bad || u.bad;
+                if (sync) sbuf[(size_t)jl * P + ccol] = u.s;
+            }
+            if (edge_wave) {
+                if (col == 0 && gl >= 1 && rv) l_bw[r] = fw.m;
+                if (col == nx && gl >= 1 && rv) l_be[r] = fw.m;
+            }
+            if constexpr (TL) {   // the groups' rows differ: per-lane row flags
+                if (upd && rv && u.h > 0.0f) atomicOr(&l_wet[r], 1u);
+                if (__builtin_expect(__ballot(upd && rv && u.fix != 0.0f) != 0ull, 0)) {
+                    if (upd && rv) fixbuf[(size_t)jl * P + ccol] = u.fix;
+                    if (upd && rv && u.fix != 0.0f) atomicOr(&l_fixmask[r], 1u << wave);
+                }
+                if (b0 == ny && owner && jl == ny - 1) yb_out[nx + col] = fs.m;
+            } else {
+                const unsigned long long wb = __ballot(upd && u.h > 0.0f);
+                if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
+                if (__builtin_expect(__ballot(upd && u.fix != 0.0f) != 0ull, 0)) {   // rare: a cell went negative
+                    if (upd) fixbuf[(size_t)j * P + ccol] = u.fix;
+                    if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
+                }
+            }
+            ch = nh;
+            cqx = nqx;
+            cqy = nqy;
+            vc = vn;
+            fn = fs;
         }
-        if (upd) {
-            const uint32_t so = (uint32_t)j * rowb;
-            bstore(u.h, rs_ho, cb, so);
-            bstore(u.qx, rs_ho, cb, so + planeb);
-            bstore(u.qy, rs_ho, cb, so + 2 * planeb);
-            l_infil[r * nx + col] = u.infil;
-            smax_bits = max(smax_bits, __float_as_int(u.s));
-            bad = bad || u.bad;
-            if (sync) sbuf[(size_t)j * P + ccol] = u.s;
+        if constexpr (!TL) {
+            if (b0 == ny && owner) yb_out[nx + col] = fn.m;
+            if (SPECIAL && rec && ring_owner && b0 == ny) ring_put(ny, ch, cqx, cqy);   // the south ghost row
         }
-        if (edge_wave) {
-            if (col == 0 && lane >= 1) l_bw[r] = fw.m;
-            if (col == nx && lane >= 1) l_be[r] = fw.m;
-        }
-        const unsigned long long wb = __ballot(upd && u.h > 0.0f);
-        if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
-        if (__builtin_expect(__ballot(upd && u.fix != 0.0f) != 0ull, 0)) {   // rare: a cell went negative
-            if (upd) fixbuf[(size_t)j * P + ccol] = u.fix;
-            if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
-        }
-        ch = nh;
-        cqx = nqx;
-        cqy = nqy;
-        vc = vn;
-        fn = fs;
-    }
-    if (b0 == ny && owner) yb_out[nx + col] = fn.m;
-    if (SPECIAL && rec && ring_owner && b0 == ny) ring_put(ny, ch, cqx, cqy);   // the south ghost row
+    };
+    if constexpr (SPECIAL) march(BoolC<false>{});
+    else if (tail) march(BoolC<true>{});
+    else march(BoolC<false>{});
     if (SPECIAL && chk) {   // band deviation maxima (f64 >= 0, never NaN: max ignores NaN)
         unsigned long long bh = (unsigned long long)__double_as_longlong(dh_l);
         unsigned long long bq = (unsigned long long)__double_as_longlong(dq_l);
@@ -3137,6 +3225,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.P = (nx + 1 + 63) / 64 * 64;   // >= nx + 1: column nx holds the east boundary face
         D.li = p->physics == ITR_PHYSICS_LOCAL_INERTIAL ? 1 : 0;
         D.waves = D.li ? (nx + kLiCols - 1) / kLiCols : (nx + kColsPerWave - 1) / kColsPerWave;
+        D.tail_groups = D.li ? 1 : tail_group_count(nx, D.waves);
         (D.li ? max_waves_li : max_waves) = std::max(D.li ? max_waves_li : max_waves, D.waves);
         const int R = choose_band_rows(ctx, grids, n_grids, g);
         D.R = R;

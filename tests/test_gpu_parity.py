@@ -179,6 +179,30 @@ def test_walls_and_wide_grid(solver):
     assert_run_equal(g, oracle_run(base, p, monitors=mon), "wide")
 
 
+@pytest.mark.parametrize("nx,ny,band_rows,level", [(80, 37, 5, 1), (130, 29, 3, 1), (200, 23, 4, 1),
+                                                   (70, 41, 7, 2), (126, 33, 6, 2)])
+def test_tail_wave_row_groups(nx, ny, band_rows, level):
+    """Grids whose last wave updates few columns (8..18 of 62): that wave marches 3 to 8
+    row groups of each band side by side.  Band heights not divisible by the group count,
+    a short last band, walls on three sides and inside, a lake draining down a slope, both
+    face modes, monitors and ledger rows, bitwise against the oracle."""
+    _say(f"case: tail-wave row groups {nx}x{ny}, band_rows {band_rows}, level {level}")
+    from inverse_terrain_engine_amd.gpu import GpuSolver
+    s = GpuSolver(0, band_rows=band_rows, steps_per_chunk=6)
+    fx, fy = nx * level, ny * level
+    base = sc.validation_terrain(fx, fy, 2.0, lambda x, y: 4.0 + 0.02 * y + 0.3 * math.sin(x / 11.0) * math.cos(y / 5.0),
+                                 walls=lambda c, r: fx - 12 <= c < fx - 9 and fy // 3 <= r < fy // 2, n=0.03,
+                                 segments=[{"kind": "transmissive", "side": "south"}], level=level)
+    assert base.nx == nx and base.ny == ny
+    p = sc.SolverParams(150.0, 30.0, rain=sc.mm_h([[0, 100], [90, 100], [120, 0]]), initial_stage=4.3,
+                        infil_capacity=2e-6)
+    mon = np.arange(1, nx * ny, 7, dtype=np.uint32)
+    g = s.run_batch(p, [tr.PreparedTerrain(base)], mon)[0]
+    s.close()
+    o = oracle_run(base, p, monitors=mon)
+    assert_run_equal(g, o, f"tail groups {nx}x{ny} R={band_rows} level {level}")
+
+
 def _equivalence_base():
     """The boundary fixture of crates/itr-hydro/tests/equivalence.rs:38-53: a west inflow
     segment (hydrograph rising from 0), a transmissive east side, a south stage segment

@@ -46,11 +46,13 @@ typedef struct itr_batch itr_batch;
 
 /* Candidate designs materialized on the device: DesignSpace::materialize
  * (crates/itr-core/src/design.rs:316-385: 4x4 sub-sample quadrature of each primitive's
- * profile, design.rs:108-134, summed in primitive order, clamped, masked) and
- * PreparedTerrain::apply_edit (crates/itr-hydro/src/terrain.rs:460-505) at level 1 with
- * signed faces.  The per-primitive scalars that need libm (Primitive::segment,
- * design.rs:97-105) and the cell windows (design.rs:320-337) come from the host; every
- * per-cell value is computed on the device with the same f64 operations. */
+ * profile, design.rs:108-134, summed in primitive order, clamped, masked) on the fine grid,
+ * and PreparedTerrain::apply_edit (crates/itr-hydro/src/terrain.rs:460-505): signed faces at
+ * level 1, or at level >= 2 the coarse cells (coarse_cell, terrain.rs:357-373) and the
+ * face-crest rises (terrain.rs:378-429) of the candidate's fine terrain.  The per-primitive
+ * scalars that need libm (Primitive::segment, design.rs:97-105) and the cell windows
+ * (design.rs:320-337) come from the host; every per-cell value is computed on the device
+ * with the same f64 operations.  Arrays, windows and volumes are on the fine grid. */
 typedef struct {
     const int32_t* prim_start; /* [n_cand + 1] offsets into prim */
     const double* prim;        /* [n][12]: profile (0 cut, 1 berm/swale/mound), segment x0, y0, x1,
@@ -62,6 +64,10 @@ typedef struct {
     double origin_x, origin_y, dx, dy; /* GeoTransform: x = origin_x + col dx, y = origin_y - row dy */
     double max_abs_dz, z_wall;
     double* volumes;           /* out: [n_cand][2] cut_m3, fill_m3 of each edit, or NULL */
+    int32_t level;             /* fidelity level lv: 0 or 1, the grid is the fine grid (signed faces);
+                                  >= 2, the grid is the face-crest level lv of the fine grid */
+    int32_t fine_nx, fine_ny;  /* the fine grid at level >= 2: grid nx = ceil(fine_nx / lv), ny likewise */
+    int32_t reserved;
 } itr_design;
 
 /* SolverParams (crates/itr-hydro/src/solver.rs:26-49).  Rain points are (t [s],
@@ -169,8 +175,9 @@ typedef struct {
     const itr_solver_params* params;
     /* Candidates given as designs (itr_design) instead of prepared faces, or NULL.  With a
      * design, dzx / dzy hold the grid's base faces (one copy: [ny][nx+1] / [ny+1][nx]); each
-     * candidate's faces are built on the device.  Level-1 signed grids without Stage
-     * boundaries or initial_stage only (else ITR_DECLINED: prepare the faces on the host). */
+     * candidate's faces are built on the device.  Level-1 signed or level >= 2 face-crest
+     * grids without Stage boundaries or initial_stage (else ITR_DECLINED: prepare the faces
+     * on the host). */
     const itr_design* design;
     /* Subdomain replay recording or replay for this grid (itr_replay), or NULL. */
     const itr_replay* replay;

@@ -35,7 +35,8 @@ class DesignC(C.Structure):
     _fields_ = [("prim_start", _ip), ("prim", _dp), ("bbox", _ip), ("z_fine", _dp),
                 ("wall", C.POINTER(C.c_uint8)), ("editable", C.POINTER(C.c_uint8)),
                 ("origin_x", C.c_double), ("origin_y", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
-                ("max_abs_dz", C.c_double), ("z_wall", C.c_double), ("volumes", _dp)]
+                ("max_abs_dz", C.c_double), ("z_wall", C.c_double), ("volumes", _dp),
+                ("level", C.c_int32), ("fine_nx", C.c_int32), ("fine_ny", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ReplayC(C.Structure):

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -2120,13 +2121,17 @@ struct DesignDev {
     const int32_t* bbox;       // [ncand][4] c0, r0, w, h
     const int64_t* dz_off;     // [ncand] offset of the candidate's window in dzbuf
     double* dzbuf;
-    const double* z;           // [ny][nx]
-    const uint8_t* wall;
-    const uint8_t* editable;   // or nullptr
+    const double* z;           // [fny][fnx] fine-grid z
+    const uint8_t* wall;       // [fny][fnx]
+    const uint8_t* editable;   // [fny][fnx] or nullptr
     double ox, oy, dx, dy, max_dz, z_wall, area;
-    int32_t nx, ny, P, ncand;
+    int32_t nx, ny, P, ncand;  // the grid (a coarse level when lv >= 2)
+    int32_t fnx, fny, lv, pad; // the fine grid the designs are drawn on (the grid itself when lv == 1)
     float* dzx;                // [ncand][plane]
     float* dzy;                // [ncand][plane_y]
+    float* dzx2;               // face-crest second rises (lv >= 2)
+    float* dzy2;
+    double* zc;                // [ncand][ny][nx] coarse level z with the edit (lv >= 2)
     size_t plane, plane_y;
     double* volumes;           // [ncand][2]
 };
@@ -2165,7 +2170,7 @@ __global__ void __launch_bounds__(256) itr_materialize_kernel(DesignDev d) {
     if (i >= w * h) return;
     const int col = c0 + i % w, row = r0 + i / w;
     double v = 0.0;
-    if (!d.editable || d.editable[(size_t)row * d.nx + col]) {
+    if (!d.editable || d.editable[(size_t)row * d.fnx + col]) {
         double acc = 0.0;
         for (int q = d.prim_start[cand]; q < d.prim_start[cand + 1]; q++) {
             const double* pr = d.prim + (size_t)12 * q;
@@ -2185,14 +2190,17 @@ __global__ void __launch_bounds__(256) itr_materialize_kernel(DesignDev d) {
     d.dzbuf[d.dz_off[cand] + i] = v;
 }
 
-// Level z of cell (c, r) of a candidate with its edit.
-__device__ __forceinline__ double design_z(const DesignDev& d, int cand, int c, int r) {
-    const size_t k = (size_t)r * d.nx + c;
-    if (d.wall[k]) return d.z_wall;
+// Fine cell (c, r) of a candidate: z + its edit (terrain.rs:353-355, fine_z; 0 outside the window).
+__device__ __forceinline__ double design_fine_z(const DesignDev& d, int cand, int c, int r) {
     const int c0 = d.bbox[4 * cand], r0 = d.bbox[4 * cand + 1], w = d.bbox[4 * cand + 2], h = d.bbox[4 * cand + 3];
     double e = 0.0;
     if (c >= c0 && c < c0 + w && r >= r0 && r < r0 + h) e = d.dzbuf[d.dz_off[cand] + (size_t)(r - r0) * w + (c - c0)];
-    return 0.0 + (d.z[k] + e);
+    return d.z[(size_t)r * d.fnx + c] + e;
+}
+// Level z of cell (c, r) of a candidate with its edit (level 1).
+__device__ __forceinline__ double design_z(const DesignDev& d, int cand, int c, int r) {
+    if (d.wall[(size_t)r * d.fnx + c]) return d.z_wall;
+    return 0.0 + design_fine_z(d, cand, c, r);
 }
 
 // Every face jump of every candidate (boundary faces stay 0): x-face i of row r joins
@@ -2205,6 +2213,81 @@ __global__ void __launch_bounds__(256) itr_design_faces_kernel(DesignDev d) {
         d.dzx[(size_t)cand * d.plane + (size_t)r * d.P + i] = (float)(design_z(d, cand, i, r) - design_z(d, cand, i - 1, r));
     if (r >= 1 && r <= d.ny - 1 && i < d.nx)
         d.dzy[(size_t)cand * d.plane_y + (size_t)r * d.P + i] = (float)(design_z(d, cand, i, r) - design_z(d, cand, i, r - 1));
+}
+
+// Coarse level z of every cell of each candidate (lv >= 2): coarse_cell
+// (terrain.rs:357-373) over the candidate's fine z, the non-wall cells of the lv x lv block
+// added row-major from 0.0 and divided by their count; z_wall when every cell is a wall.
+// Outside the edit window this is the base level's z bit for bit (z + 0.0 == z).
+__global__ void __launch_bounds__(256) itr_design_coarse_kernel(DesignDev d) {
+    const int cand = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nx * d.ny) return;
+    const int c = i % d.nx, r = i / d.nx, lv = d.lv;
+    double s = 0.0;
+    int cnt = 0;
+    for (int rr = r * lv; rr < min((r + 1) * lv, d.fny); rr++)
+        for (int cc = c * lv; cc < min((c + 1) * lv, d.fnx); cc++)
+            if (!d.wall[(size_t)rr * d.fnx + cc]) {
+                s += design_fine_z(d, cand, cc, rr);
+                cnt++;
+            }
+    d.zc[(size_t)cand * d.nx * d.ny + i] = cnt == 0 ? d.z_wall : s / (double)cnt;
+}
+
+// Face-crest rises of every face of each candidate (lv >= 2; terrain.rs:378-429): the
+// crest is the highest non-wall fine cell on either side of the face (z_wall when all are
+// walls), raised to both coarse levels; the rises crest - z_left and crest - z_right are
+// rounded to f32 once.  x-face i of row r joins coarse cells i-1 and i; y-face f of column
+// c joins rows f-1 and f.  Boundary faces stay 0.
+__global__ void __launch_bounds__(256) itr_design_crest_kernel(DesignDev d) {
+    const int cand = blockIdx.z, r = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = d.nx, ny = d.ny, lv = d.lv;
+    const double* zc = d.zc + (size_t)cand * nx * ny;
+    if (r < ny && i >= 1 && i <= nx - 1) {
+        const double zl = zc[(size_t)r * nx + i - 1], zr = zc[(size_t)r * nx + i];
+        const int cl = i * lv - 1, cr = min(i * lv, d.fnx - 1);
+        double cst = -DBL_MAX;
+        bool all_wall = true;
+        for (int rr = r * lv; rr < min((r + 1) * lv, d.fny); rr++) {
+            if (!d.wall[(size_t)rr * d.fnx + cl]) {
+                all_wall = false;
+                cst = fmax(cst, design_fine_z(d, cand, cl, rr));
+            }
+            if (!d.wall[(size_t)rr * d.fnx + cr]) {
+                all_wall = false;
+                cst = fmax(cst, design_fine_z(d, cand, cr, rr));
+            }
+        }
+        if (all_wall) cst = d.z_wall;
+        const double zf = fmax(fmax(cst, zl), zr);
+        const size_t o = (size_t)cand * d.plane + (size_t)r * d.P + i;
+        d.dzx[o] = (float)(zf - zl);
+        d.dzx2[o] = (float)(zf - zr);
+    }
+    if (r >= 1 && r <= ny - 1 && i < nx) {
+        const int c = i;
+        const double zt = zc[(size_t)(r - 1) * nx + c], zb = zc[(size_t)r * nx + c];
+        const int rt = r * lv - 1, rb = min(r * lv, d.fny - 1);
+        double cst = -DBL_MAX;
+        bool all_wall = true;
+        for (int cc = c * lv; cc < min((c + 1) * lv, d.fnx); cc++) {
+            if (!d.wall[(size_t)rt * d.fnx + cc]) {
+                all_wall = false;
+                cst = fmax(cst, design_fine_z(d, cand, cc, rt));
+            }
+            if (!d.wall[(size_t)rb * d.fnx + cc]) {
+                all_wall = false;
+                cst = fmax(cst, design_fine_z(d, cand, cc, rb));
+            }
+        }
+        if (all_wall) cst = d.z_wall;
+        const double zf = fmax(fmax(cst, zt), zb);
+        const size_t o = (size_t)cand * d.plane_y + (size_t)r * d.P + c;
+        d.dzy[o] = (float)(zf - zt);
+        d.dzy2[o] = (float)(zf - zb);
+    }
 }
 
 // Cut and fill volumes of each edit: row-major over the window, as design.rs:373-380 sums.
@@ -2739,10 +2822,13 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
 
 // Builds the candidates' faces of a grid given as designs (itr_design) into d_dzx /
 // d_dzy (zero-initialised: boundary faces stay 0) and returns the edits' volumes.
-itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D, float* d_dzx, float* d_dzy, char* err,
-                              size_t errlen) {
+itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D, float* d_dzx, float* d_dzy,
+                              float* d_dzx2, float* d_dzy2, char* err, size_t errlen) {
     const itr_design& Ds = *I.design;
     const int nx = I.nx, ny = I.ny, nc = I.n_cand;
+    const int lv = std::max(Ds.level, 1);
+    const int fnx = lv == 1 ? nx : Ds.fine_nx, fny = lv == 1 ? ny : Ds.fine_ny;
+    const size_t fn = (size_t)fnx * fny;
     if (nc <= 0) return ITR_OK;
     const int np = Ds.prim_start[nc];
     std::vector<int64_t> off(nc);
@@ -2758,7 +2844,7 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
         for (void* q : tmp) (void)hipFree(q);
     };
     DesignDev dd{};
-    double *prim = nullptr, *dzbuf = nullptr, *z = nullptr, *vol = nullptr;
+    double *prim = nullptr, *dzbuf = nullptr, *z = nullptr, *vol = nullptr, *zc = nullptr;
     int32_t *ps = nullptr, *bb = nullptr;
     int64_t* doff = nullptr;
     uint8_t *wall = nullptr, *edit = nullptr;
@@ -2773,19 +2859,20 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
     DTRY(dev_alloc(tmp, &bb, (size_t)nc * 4, err, errlen));
     DTRY(dev_alloc(tmp, &doff, (size_t)nc, err, errlen));
     DTRY(dev_alloc(tmp, &dzbuf, (size_t)std::max<int64_t>(tot, 1), err, errlen));
-    DTRY(dev_alloc(tmp, &z, (size_t)nx * ny, err, errlen));
-    DTRY(dev_alloc(tmp, &wall, (size_t)nx * ny, err, errlen));
-    if (Ds.editable) DTRY(dev_alloc(tmp, &edit, (size_t)nx * ny, err, errlen));
+    DTRY(dev_alloc(tmp, &z, fn, err, errlen));
+    DTRY(dev_alloc(tmp, &wall, fn, err, errlen));
+    if (Ds.editable) DTRY(dev_alloc(tmp, &edit, fn, err, errlen));
     DTRY(dev_alloc(tmp, &vol, (size_t)nc * 2, err, errlen));
+    if (lv >= 2) DTRY(dev_alloc(tmp, &zc, (size_t)nc * nx * ny, err, errlen));
 #undef DTRY
     const hipStream_t st = ctx->stream;
     bool ok = (np == 0 || hipMemcpy(prim, Ds.prim, sizeof(double) * 12 * np, hipMemcpyHostToDevice) == hipSuccess) &&
               hipMemcpy(ps, Ds.prim_start, sizeof(int32_t) * (nc + 1), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(bb, Ds.bbox, sizeof(int32_t) * 4 * nc, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(doff, off.data(), sizeof(int64_t) * nc, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(z, Ds.z_fine, sizeof(double) * nx * ny, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(wall, Ds.wall, (size_t)nx * ny, hipMemcpyHostToDevice) == hipSuccess &&
-              (!Ds.editable || hipMemcpy(edit, Ds.editable, (size_t)nx * ny, hipMemcpyHostToDevice) == hipSuccess);
+              hipMemcpy(z, Ds.z_fine, sizeof(double) * fn, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(wall, Ds.wall, fn, hipMemcpyHostToDevice) == hipSuccess &&
+              (!Ds.editable || hipMemcpy(edit, Ds.editable, fn, hipMemcpyHostToDevice) == hipSuccess);
     if (!ok) {
         release();
         set_err(err, errlen, "design upload failed");
@@ -2810,15 +2897,28 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
     dd.ny = ny;
     dd.P = D.P;
     dd.ncand = nc;
+    dd.fnx = fnx;
+    dd.fny = fny;
+    dd.lv = lv;
     dd.dzx = d_dzx;
     dd.dzy = d_dzy;
+    dd.dzx2 = d_dzx2;
+    dd.dzy2 = d_dzy2;
+    dd.zc = zc;
     dd.plane = D.plane;
     dd.plane_y = D.plane_y;
     dd.volumes = vol;
     if (maxwh > 0)
         hipLaunchKernelGGL(itr_materialize_kernel, dim3((unsigned)((maxwh + 255) / 256), (unsigned)nc), dim3(256), 0, st, dd);
-    hipLaunchKernelGGL(itr_design_faces_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
-                       dim3(256), 0, st, dd);
+    if (lv == 1) {
+        hipLaunchKernelGGL(itr_design_faces_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
+                           dim3(256), 0, st, dd);
+    } else {
+        hipLaunchKernelGGL(itr_design_coarse_kernel, dim3((unsigned)(((size_t)nx * ny + 255) / 256), (unsigned)nc),
+                           dim3(256), 0, st, dd);
+        hipLaunchKernelGGL(itr_design_crest_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
+                           dim3(256), 0, st, dd);
+    }
     hipLaunchKernelGGL(itr_design_volumes_kernel, dim3((unsigned)nc), dim3(256), 0, st, dd);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -3094,7 +3194,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             set_err(err, errlen, "grid %d: nx=%d exceeds %d columns", g, G.nx, max_cols);
             return ITR_DECLINED;
         }
-        if (G.crest && (!G.dzx_r || !G.dzy_r)) {
+        if (G.crest && !G.design && (!G.dzx_r || !G.dzy_r)) {
             set_err(err, errlen, "grid %d: crest mode needs dzx_r and dzy_r", g);
             return ITR_EINVAL;
         }
@@ -3167,10 +3267,19 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             const itr_design& Ds = *G.design;
             bool stage = false;
             for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++) stage = stage || G.ghost[i] == 2;
-            if (G.crest || q->has_initial_stage || stage) {
-                set_err(err, errlen, "grid %d: device designs need a level-1 signed grid without stage levels", g);
+            const int lv = std::max(Ds.level, 1);
+            if ((lv >= 2) != (G.crest != 0) || q->has_initial_stage || stage) {
+                set_err(err, errlen, "grid %d: device designs need a level-1 signed or level >= 2 face-crest grid "
+                                     "without stage levels", g);
                 return ITR_DECLINED;
             }
+            if (lv >= 2 && (Ds.fine_nx <= 0 || Ds.fine_ny <= 0 || (Ds.fine_nx + lv - 1) / lv != G.nx ||
+                            (Ds.fine_ny + lv - 1) / lv != G.ny)) {
+                set_err(err, errlen, "grid %d: the design's fine grid %dx%d is not level %d of this %dx%d grid", g,
+                        Ds.fine_nx, Ds.fine_ny, lv, G.nx, G.ny);
+                return ITR_EINVAL;
+            }
+            const int fnx = lv >= 2 ? Ds.fine_nx : G.nx, fny = lv >= 2 ? Ds.fine_ny : G.ny;
             if (!Ds.prim_start || !Ds.bbox || !Ds.z_fine || !Ds.wall || G.n_cand > 65535 ||
                 (G.n_cand > 0 && Ds.prim_start[G.n_cand] > 0 && !Ds.prim)) {
                 set_err(err, errlen, "grid %d: incomplete design arrays", g);
@@ -3178,7 +3287,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             }
             for (int c = 0; c < G.n_cand; c++) {
                 const int32_t* bb = Ds.bbox + 4 * c;
-                if (bb[0] < 0 || bb[1] < 0 || bb[2] < 0 || bb[3] < 0 || bb[0] + bb[2] > G.nx || bb[1] + bb[3] > G.ny ||
+                if (bb[0] < 0 || bb[1] < 0 || bb[2] < 0 || bb[3] < 0 || bb[0] + bb[2] > fnx || bb[1] + bb[3] > fny ||
                     Ds.prim_start[c + 1] < Ds.prim_start[c]) {
                     set_err(err, errlen, "grid %d: design window of candidate %d outside the grid", g, c);
                     return ITR_EINVAL;
@@ -3469,7 +3578,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         bool ok = hipMemset(d_state, 0, ncx * 6 * plane * 4) == hipSuccess;
         if (I.design)
             ok = ok && hipMemset(d_dzx, 0, ncx * plane * 4) == hipSuccess &&
-                 hipMemset(d_dzy, 0, ncx * plane_y * 4) == hipSuccess;
+                 hipMemset(d_dzy, 0, ncx * plane_y * 4) == hipSuccess &&
+                 (!I.crest || (hipMemset(d_dzx2, 0, ncx * plane * 4) == hipSuccess &&
+                               hipMemset(d_dzy2, 0, ncx * plane_y * 4) == hipSuccess));
         for (int k = 0; ok && k < nc; k++) {
             // initial depth (parity 0, h plane): uploaded once when every candidate shares it
             if (p->has_initial_stage || k == 0) ok = up_plane(d_state + (size_t)k * 6 * plane, h0s[p->has_initial_stage ? k : 0].data(), nx, 0, nx, 0, ny, ny);
@@ -3529,7 +3640,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             return fail(ITR_EDEVICE);
         }
         if (I.design) {
-            itr_status ms = materialize_design(ctx, I, D, d_dzx, d_dzy, err, errlen);
+            itr_status ms = materialize_design(ctx, I, D, d_dzx, d_dzy, d_dzx2, d_dzy2, err, errlen);
             if (ms != ITR_OK) return fail(ms);
         }
         D.state = d_state;

@@ -128,7 +128,7 @@ def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need
     g.n_cand = len(terrains)
     g.dzx = f(keep.arr(np.stack([t.fx0 for t in terrains]) if terrains else np.zeros(1), np.float32))
     g.dzy = f(keep.arr(np.stack([t.fy0 for t in terrains]) if terrains else np.zeros(1), np.float32))
-    if base.crest:
+    if base.crest and terrains:
         g.dzx_r = f(keep.arr(np.stack([t.fx1 for t in terrains]), np.float32))
         g.dzy_r = f(keep.arr(np.stack([t.fy1 for t in terrains]), np.float32))
     if need_z:
@@ -146,15 +146,14 @@ def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need
 
 def _design_grid_c(base: TerrainBase, designs, monitors, keep: _Keep, editable=None, max_abs_dz=1e9, guard=None):
     """A grid whose candidates are primitive lists, materialized on the device
-    (itr_design): level-1 signed grids only."""
+    (itr_design) on the fine grid: level-1 signed grids, or face-crest levels >= 2."""
     from .terrain import design_rows
-    if base.level != 1 or base.crest:
-        raise Declined("device designs need a level-1 signed grid")
-    nx, ny = base.nx, base.ny
+    if (base.level != 1) != bool(base.crest):
+        raise Declined("device designs need a level-1 signed or a face-crest coarse grid")
     fine = base.fine
     rows, start, boxes = [], [0], []
     for prims in designs:
-        r, box = design_rows(prims, fine.geo, nx, ny)
+        r, box = design_rows(prims, fine.geo, fine.nx, fine.ny)
         rows += r
         start.append(len(rows))
         boxes.append(box)
@@ -171,12 +170,13 @@ def _design_grid_c(base: TerrainBase, designs, monitors, keep: _Keep, editable=N
     g = fine.geo
     d.origin_x, d.origin_y, d.dx, d.dy = g.origin_x, g.origin_y, g.dx, g.dy
     d.max_abs_dz, d.z_wall = float(max_abs_dz), float(fine.z_wall)
+    d.level, d.fine_nx, d.fine_ny = int(base.level), int(fine.nx), int(fine.ny)
     vol = keep.arr(np.zeros((max(len(designs), 1), 2)), np.float64)
     d.volumes = vol.ctypes.data_as(_abi._dp)
     keep.append(d)
     grid = _grid_c(base, [], monitors, keep, guard)
     grid.n_cand = len(designs)
-    grid.dzx = grid.dzy = None
+    grid.dzx = grid.dzy = grid.dzx_r = grid.dzy_r = None
     grid.design = C.pointer(d)
     return grid, vol
 

@@ -103,9 +103,12 @@ class TerrainEdit:
         return out
 
 
-def _coarse(fine: FineTerrain, lv: int, zfine: np.ndarray):
-    """coarse_cell over the whole grid (terrain.rs:357-373), sequential block sums."""
-    nx, ny = -(-fine.nx // lv), -(-fine.ny // lv)
+def _coarse(fine: FineTerrain, lv: int, zfine: np.ndarray, c0=0, r0=0, c1=None, r1=None):
+    """coarse_cell (terrain.rs:357-373) over the coarse cells [r0, r1) x [c0, c1) (default:
+    the whole level), sequential block sums."""
+    c1 = -(-fine.nx // lv) if c1 is None else c1
+    r1 = -(-fine.ny // lv) if r1 is None else r1
+    nx, ny = c1 - c0, r1 - r0
     s = np.zeros((ny, nx))
     sn = np.zeros((ny, nx))
     sn_all = np.zeros((ny, nx))
@@ -113,8 +116,8 @@ def _coarse(fine: FineTerrain, lv: int, zfine: np.ndarray):
     tot = np.zeros((ny, nx), np.int64)
     for dr in range(lv):
         for dc in range(lv):
-            rr = np.arange(ny) * lv + dr
-            cc = np.arange(nx) * lv + dc
+            rr = np.arange(r0, r1) * lv + dr
+            cc = np.arange(c0, c1) * lv + dc
             vr = rr < fine.ny
             vc = cc < fine.nx
             valid = vr[:, None] & vc[None, :]
@@ -133,38 +136,6 @@ def _coarse(fine: FineTerrain, lv: int, zfine: np.ndarray):
         z = np.where(cnt == 0, fine.z_wall, s / np.maximum(cnt, 1))
         n = np.where(cnt == 0, sn_all / np.maximum(tot, 1), sn / np.maximum(cnt, 1))
     return z, cnt == 0, n
-
-
-def _crest_x(fine: FineTerrain, lv, zfine, zl, zr, i, r):
-    """face_x crest (terrain.rs:378-402) for one face."""
-    cl = i * lv - 1
-    cr = min(i * lv, fine.nx - 1)
-    crest, all_wall = -math.inf, True
-    for rr in range(r * lv, min((r + 1) * lv, fine.ny)):
-        for cc in (cl, cr):
-            if not fine.wall[rr, cc]:
-                all_wall = False
-                crest = max(crest, zfine[rr, cc])
-    if all_wall:
-        crest = fine.z_wall
-    zf = max(max(crest, zl), zr)
-    return zf - zl, zf - zr
-
-
-def _crest_y(fine: FineTerrain, lv, zfine, zt, zb, c, rb):
-    """face_y crest (terrain.rs:406-429) for one face."""
-    rt_f = rb * lv - 1
-    rb_f = min(rb * lv, fine.ny - 1)
-    crest, all_wall = -math.inf, True
-    for cc in range(c * lv, min((c + 1) * lv, fine.nx)):
-        for rr in (rt_f, rb_f):
-            if not fine.wall[rr, cc]:
-                all_wall = False
-                crest = max(crest, zfine[rr, cc])
-    if all_wall:
-        crest = fine.z_wall
-    zf = max(max(crest, zt), zb)
-    return zf - zt, zf - zb
 
 
 @dataclass
@@ -224,6 +195,45 @@ class TerrainBase:
         return bool((self.ghost_codes() == GHOST_STAGE).any())
 
 
+def _crest_x_block(fine: FineTerrain, lv, zfine, z, r0, r1, ia, ib):
+    """face_x crest (terrain.rs:378-402) for the x-faces i in [ia, ib) of coarse rows
+    [r0, r1), as arrays: the max over the non-wall fine cells either side of each face
+    (fine columns i*lv-1 and min(i*lv, nx-1); walls and rows past the grid read -inf),
+    z_wall where all of them are walls, then max with both coarse levels; the rises to the
+    crest from the left and right levels, rounded to f32.  Maxima of finite values do not
+    depend on their order, so this equals the reference's loop."""
+    rr = np.arange(r0, r1)[:, None] * lv + np.arange(lv)[None, :]          # [nr][lv] fine rows
+    out = rr >= fine.ny
+    rri = np.minimum(rr, fine.ny - 1)
+    ii = np.arange(ia, ib)
+    cols = np.stack([ii * lv - 1, np.minimum(ii * lv, fine.nx - 1)], axis=-1)  # [nf][2] fine columns
+    R, Cc = rri[:, :, None, None], cols[None, None, :, :]
+    wall = fine.wall[R, Cc] | out[:, :, None, None]
+    crest = np.where(wall, -np.inf, zfine[R, Cc]).max(axis=(1, 3))
+    crest = np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
+    zl, zr = z[r0:r1, ia - 1:ib - 1], z[r0:r1, ia:ib]
+    zf = np.maximum(np.maximum(crest, zl), zr)
+    return (zf - zl).astype(np.float32), (zf - zr).astype(np.float32)
+
+
+def _crest_y_block(fine: FineTerrain, lv, zfine, z, fa, fb, c0, c1):
+    """face_y crest (terrain.rs:406-429) for the y-faces fa .. fb-1 (face f joins coarse
+    rows f-1 and f; fine rows f*lv-1 and min(f*lv, ny-1)) of columns [c0, c1), as arrays,
+    the same way as _crest_x_block."""
+    cc = np.arange(c0, c1)[:, None] * lv + np.arange(lv)[None, :]          # [nc][lv] fine columns
+    out = cc >= fine.nx
+    cci = np.minimum(cc, fine.nx - 1)
+    ff = np.arange(fa, fb)
+    rows = np.stack([ff * lv - 1, np.minimum(ff * lv, fine.ny - 1)], axis=-1)  # [nf][2] fine rows
+    R, Cc = rows[:, :, None, None], cci[None, None, :, :]                     # [nf][2][nc][lv]
+    wall = fine.wall[R, Cc] | out[None, None, :, :]
+    crest = np.where(wall, -np.inf, zfine[R, Cc]).max(axis=(1, 3))
+    crest = np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
+    zt, zb = z[fa - 1:fb - 1, c0:c1], z[fa:fb, c0:c1]
+    zf = np.maximum(np.maximum(crest, zt), zb)
+    return (zf - zt).astype(np.float32), (zf - zb).astype(np.float32)
+
+
 def _faces(fine, lv, crest, z, zfine, nx, ny):
     fx0 = np.zeros((ny, nx + 1), np.float32)
     fx1 = np.zeros((ny, nx + 1), np.float32)
@@ -233,14 +243,10 @@ def _faces(fine, lv, crest, z, zfine, nx, ny):
         fx0[:, 1:nx] = (z[:, 1:] - z[:, :-1]).astype(np.float32)
         fy0[1:ny, :] = (z[1:, :] - z[:-1, :]).astype(np.float32)
         return fx0, fx1, fy0, fy1
-    for r in range(ny):
-        for i in range(1, nx):
-            a, b = _crest_x(fine, lv, zfine, z[r, i - 1], z[r, i], i, r)
-            fx0[r, i], fx1[r, i] = np.float32(a), np.float32(b)
-    for j in range(1, ny):
-        for c in range(nx):
-            a, b = _crest_y(fine, lv, zfine, z[j - 1, c], z[j, c], c, j)
-            fy0[j, c], fy1[j, c] = np.float32(a), np.float32(b)
+    if nx > 1:
+        fx0[:, 1:nx], fx1[:, 1:nx] = _crest_x_block(fine, lv, zfine, z, 0, ny, 1, nx)
+    if ny > 1:
+        fy0[1:ny, :], fy1[1:ny, :] = _crest_y_block(fine, lv, zfine, z, 1, ny, 0, nx)
     return fx0, fx1, fy0, fy1
 
 
@@ -305,8 +311,8 @@ class PreparedTerrain:
             zfine = None
         else:
             zfine = fine.z + edit.dense(fine.nx, fine.ny)
-            zc, _, _ = _coarse(fine, lv, zfine)
-            z[r0:r1, c0:c1] = zc[r0:r1, c0:c1]
+            zc, _, _ = _coarse(fine, lv, zfine, c0, r0, c1, r1)
+            z[r0:r1, c0:c1] = zc
         fx0, fx1, fy0, fy1 = b.fx0.copy(), b.fx1.copy(), b.fy0.copy(), b.fy1.copy()
         if not b.crest:
             ia, ib = max(c0 - 1, 0) + 1, min(c1, nx - 1) + 1
@@ -318,21 +324,14 @@ class PreparedTerrain:
             self.fx0, self.fx1, self.fy0, self.fy1 = fx0, fx1, fy0, fy1
             self.z_level = z
             return self
-        for r in range(r0, r1):
-            for cl in range(max(c0 - 1, 0), min(c1, nx - 1)):
-                i = cl + 1
-                if b.crest:
-                    a, bb = _crest_x(fine, lv, zfine, z[r, cl], z[r, cl + 1], i, r)
-                    fx0[r, i], fx1[r, i] = np.float32(a), np.float32(bb)
-                else:
-                    fx0[r, i] = np.float32(z[r, cl + 1] - z[r, cl])
-        for rt in range(max(r0 - 1, 0), min(r1, ny - 1)):
-            for c in range(c0, c1):
-                if b.crest:
-                    a, bb = _crest_y(fine, lv, zfine, z[rt, c], z[rt + 1, c], c, rt + 1)
-                    fy0[rt + 1, c], fy1[rt + 1, c] = np.float32(a), np.float32(bb)
-                else:
-                    fy0[rt + 1, c] = np.float32(z[rt + 1, c] - z[rt, c])
+        # the faces of the edited coarse window (terrain.rs:480-500): x-faces
+        # max(c0-1, 0)+1 ..= min(c1, nx-1) of rows r0..r1, y-faces max(r0-1, 0)+1 ..= min(r1, ny-1)
+        ia, ib = max(c0 - 1, 0) + 1, min(c1, nx - 1) + 1
+        if ia < ib and r0 < r1:
+            fx0[r0:r1, ia:ib], fx1[r0:r1, ia:ib] = _crest_x_block(fine, lv, zfine, z, r0, r1, ia, ib)
+        fa, fb = max(r0 - 1, 0) + 1, min(r1, ny - 1) + 1
+        if fa < fb and c0 < c1:
+            fy0[fa:fb, c0:c1], fy1[fa:fb, c0:c1] = _crest_y_block(fine, lv, zfine, z, fa, fb, c0, c1)
         self.fx0, self.fx1, self.fy0, self.fy1 = fx0, fx1, fy0, fy1
         self.z_level = z
         return self

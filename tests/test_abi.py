@@ -54,7 +54,7 @@ int main(void) {
   F(itr_grid, wet) F(itr_grid, n_cand) F(itr_grid, dzx) F(itr_grid, mon_cells) F(itr_grid, guard_tol)
   F(itr_grid, ghost_value) F(itr_grid, hydro_q) F(itr_grid, n_hydro) F(itr_grid, params) F(itr_grid, design)
   F(itr_grid, replay)
-  F(itr_design, bbox) F(itr_design, origin_x) F(itr_design, volumes)
+  F(itr_design, bbox) F(itr_design, origin_x) F(itr_design, volumes) F(itr_design, level) F(itr_design, fine_ny)
   F(itr_replay, ring) F(itr_replay, ring_steps) F(itr_replay, tol_q) F(itr_replay, n_syncs_out)
   F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger) F(itr_run_out, dt_fallback_t)
   F(itr_stats, launches) F(itr_stats, use_graph)

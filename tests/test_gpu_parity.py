@@ -462,6 +462,45 @@ def test_device_designs_equal_host_prepared(solver):
     assert not np.array_equal(runs[0].depth, runs[1].depth)
 
 
+@pytest.mark.parametrize("level", [2, 3])
+def test_device_designs_on_crest_levels(solver, level):
+    """Designs materialized on the device for a face-crest coarse level: the edit on the
+    fine grid, the coarse cells (coarse_cell, terrain.rs:357-373) and both crest rises of
+    every face (terrain.rs:378-429) built per candidate, bitwise like the host's
+    PreparedTerrain::apply_edit; fine grids not divisible by the level, fine walls (one
+    coarse cell all walls), an editable mask, an active clamp, primitives at the edge."""
+    fnx, fny = 100, 64
+    _say(f"case: device designs on level {level} of {fnx}x{fny}")
+    walls = lambda c, r: (40 <= c < 46 and 10 <= r < 30) or (c < 3 and r < 3)
+    fine_base = sc.validation_terrain(fnx, fny, 2.0, lambda x, y: 3.0 + 0.008 * (200.0 - x) + 0.3 * math.sin(y / 11.0)
+                                      + 0.2 * math.cos(x / 5.0), walls=walls, n=0.035,
+                                      segments=[{"kind": "transmissive", "side": "east"}])
+    fine = fine_base.fine
+    base = tr.TerrainBase.build(fine, level, [{"kind": "transmissive", "side": "east"}])
+    assert base.crest and base.nx == -(-fnx // level) and base.ny == -(-fny // level)
+    geo = fine.geo
+    rng = sc.SplitMix64(33 + level)
+    kinds = ("berm", "swale", "mound", "cut")
+    designs = [sc.random_primitives(rng, k, geo, fnx, fny, kinds=kinds) for k in (1, 2, 3, 4, 3)]
+    designs.append([])
+    edge = sc.random_primitives(rng, 1, geo, fnx, fny, kinds=("berm",))
+    edge[0].cx, edge[0].cy, edge[0].length, edge[0].width = 196.0, 125.0, 30.0, 8.0
+    designs.append(edge)
+    editable = np.ones((fny, fnx), bool)
+    editable[:, :10] = False
+    max_dz = 0.35
+    p = sc.SolverParams(240.0, 60.0, rain=sc.mm_h([[0, 70], [240, 30]]), infil_capacity=2e-6)
+    mon = np.arange(0, base.nx * base.ny, 5, dtype=np.uint32)
+    runs, vol = solver.run_designs(p, base, designs, mon, editable=editable, max_abs_dz=max_dz)
+    edits = [tr.materialize(d, geo, fnx, fny, editable, max_dz) for d in designs]
+    host = solver.run_batch(p, [tr.PreparedTerrain(base).apply_edit(e) for e in edits], mon)
+    assert any(np.abs(e.dz).max() == max_dz for e in edits if e.dz.size), "the clamp should be active"
+    for i, (a, b, e) in enumerate(zip(runs, host, edits)):
+        _runs_equal(a, b, f"level {level} design {i}")
+        assert bits64(vol[i, 0]) == bits64(e.cut_m3) and bits64(vol[i, 1]) == bits64(e.fill_m3), f"design {i} volumes"
+    assert not np.array_equal(runs[0].depth, runs[1].depth)
+
+
 def test_locked_dt_schedule_and_adaptive_restart(solver):
     """Baseline-locked dt (solver.rs:452-455) and its fallback: a step whose locked dt
     fails the hard-CFL check restarts adaptively from the last sync snapshot

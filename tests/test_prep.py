@@ -49,13 +49,17 @@ def test_terrain_build_matches_oracle(level):
         assert np.array_equal(g[side], base.ghosts[side]["code"])
 
 
-@pytest.mark.parametrize("level", [1, 2])
-def test_edits_match_oracle(level):
-    nx, ny, dx = 60, 48, 2.0
+@pytest.mark.parametrize("level,nx,ny", [(1, 60, 48), (2, 60, 48), (2, 61, 47), (3, 61, 47)])
+def test_edits_match_oracle(level, nx, ny):
+    """Edits on signed and face-crest levels, fine grids divisible by the level or not, fine
+    walls (a block that makes whole coarse cells walls: crest faces from z_wall)."""
+    dx = 2.0
     rng = sc.SplitMix64(17)
     dem = sc.hills_dem(nx, ny, dx, rng, 6)
     geo = tr.GeoTransform(0.0, ny * dx, dx, dx)
-    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, None, 0.035, geo), level, [])
+    walls = np.zeros((ny, nx), bool)
+    walls[12:20, 24:30] = True
+    base = tr.TerrainBase.build(tr.FineTerrain.new(dem, walls, 0.035, geo), level, [])
     ot = oracle_terrain(base)
     editable = np.ones((ny, nx), bool)
     editable[:, :8] = False

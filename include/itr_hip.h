@@ -176,8 +176,8 @@ typedef struct {
     /* Candidates given as designs (itr_design) instead of prepared faces, or NULL.  With a
      * design, dzx / dzy hold the grid's base faces (one copy: [ny][nx+1] / [ny+1][nx]); each
      * candidate's faces are built on the device.  Level-1 signed or level >= 2 face-crest
-     * grids without Stage boundaries or initial_stage (else ITR_DECLINED: prepare the faces
-     * on the host). */
+     * grids; z_level may be NULL (Stage ghosts and initial_stage read the levels built on
+     * the device).  Not with replay (ITR_DECLINED: prepare the faces on the host). */
     const itr_design* design;
     /* Subdomain replay recording or replay for this grid (itr_replay), or NULL. */
     const itr_replay* replay;

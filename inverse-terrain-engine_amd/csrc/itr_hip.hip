@@ -2820,50 +2820,56 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
     return ITR_OK;
 }
 
-// Builds the candidates' faces of a grid given as designs (itr_design) into d_dzx /
-// d_dzy (zero-initialised: boundary faces stay 0) and returns the edits' volumes.
-itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D, float* d_dzx, float* d_dzy,
-                              float* d_dzx2, float* d_dzy2, char* err, size_t errlen) {
+// A grid given as designs (itr_design), built on the device in three parts: design_begin
+// uploads the designs and computes every candidate's fine dz (and at level >= 2 its coarse
+// level z); design_levels hands the candidates' level z to the host (Stage ghosts and
+// initial_stage depend on it); design_finish builds the faces into the grid's zero-filled
+// face planes (boundary faces stay 0) and returns the edits' volumes.  The scratch buffers
+// live as long as the DesignWork.
+struct DesignWork {
+    std::vector<void*> tmp;
+    DesignDev dd{};
+    int nc = 0;
+    int64_t maxwh = 0;
+    ~DesignWork() {
+        for (void* q : tmp) (void)hipFree(q);
+    }
+};
+
+itr_status design_begin(itr_ctx* ctx, const itr_grid& I, const DevGrid& D, DesignWork& W, char* err, size_t errlen) {
     const itr_design& Ds = *I.design;
     const int nx = I.nx, ny = I.ny, nc = I.n_cand;
+    W.nc = nc;
+    if (nc <= 0) return ITR_OK;
     const int lv = std::max(Ds.level, 1);
     const int fnx = lv == 1 ? nx : Ds.fine_nx, fny = lv == 1 ? ny : Ds.fine_ny;
     const size_t fn = (size_t)fnx * fny;
-    if (nc <= 0) return ITR_OK;
     const int np = Ds.prim_start[nc];
     std::vector<int64_t> off(nc);
-    int64_t tot = 0, maxwh = 0;
+    int64_t tot = 0;
     for (int k = 0; k < nc; k++) {
         off[k] = tot;
         const int64_t wh = (int64_t)Ds.bbox[4 * k + 2] * Ds.bbox[4 * k + 3];
         tot += wh;
-        maxwh = std::max(maxwh, wh);
+        W.maxwh = std::max(W.maxwh, wh);
     }
-    std::vector<void*> tmp;
-    auto release = [&]() {
-        for (void* q : tmp) (void)hipFree(q);
-    };
-    DesignDev dd{};
     double *prim = nullptr, *dzbuf = nullptr, *z = nullptr, *vol = nullptr, *zc = nullptr;
     int32_t *ps = nullptr, *bb = nullptr;
     int64_t* doff = nullptr;
     uint8_t *wall = nullptr, *edit = nullptr;
     itr_status s;
-#define DTRY(expr)        \
-    if ((s = (expr)) != ITR_OK) { \
-        release();        \
-        return s;         \
-    }
-    DTRY(dev_alloc(tmp, &prim, (size_t)std::max(np, 1) * 12, err, errlen));
-    DTRY(dev_alloc(tmp, &ps, (size_t)nc + 1, err, errlen));
-    DTRY(dev_alloc(tmp, &bb, (size_t)nc * 4, err, errlen));
-    DTRY(dev_alloc(tmp, &doff, (size_t)nc, err, errlen));
-    DTRY(dev_alloc(tmp, &dzbuf, (size_t)std::max<int64_t>(tot, 1), err, errlen));
-    DTRY(dev_alloc(tmp, &z, fn, err, errlen));
-    DTRY(dev_alloc(tmp, &wall, fn, err, errlen));
-    if (Ds.editable) DTRY(dev_alloc(tmp, &edit, fn, err, errlen));
-    DTRY(dev_alloc(tmp, &vol, (size_t)nc * 2, err, errlen));
-    if (lv >= 2) DTRY(dev_alloc(tmp, &zc, (size_t)nc * nx * ny, err, errlen));
+#define DTRY(expr)                  \
+    if ((s = (expr)) != ITR_OK) return s;
+    DTRY(dev_alloc(W.tmp, &prim, (size_t)std::max(np, 1) * 12, err, errlen));
+    DTRY(dev_alloc(W.tmp, &ps, (size_t)nc + 1, err, errlen));
+    DTRY(dev_alloc(W.tmp, &bb, (size_t)nc * 4, err, errlen));
+    DTRY(dev_alloc(W.tmp, &doff, (size_t)nc, err, errlen));
+    DTRY(dev_alloc(W.tmp, &dzbuf, (size_t)std::max<int64_t>(tot, 1), err, errlen));
+    DTRY(dev_alloc(W.tmp, &z, fn, err, errlen));
+    DTRY(dev_alloc(W.tmp, &wall, fn, err, errlen));
+    if (Ds.editable) DTRY(dev_alloc(W.tmp, &edit, fn, err, errlen));
+    DTRY(dev_alloc(W.tmp, &vol, (size_t)nc * 2, err, errlen));
+    DTRY(dev_alloc(W.tmp, &zc, (size_t)nc * nx * ny, err, errlen));
 #undef DTRY
     const hipStream_t st = ctx->stream;
     bool ok = (np == 0 || hipMemcpy(prim, Ds.prim, sizeof(double) * 12 * np, hipMemcpyHostToDevice) == hipSuccess) &&
@@ -2874,10 +2880,10 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
               hipMemcpy(wall, Ds.wall, fn, hipMemcpyHostToDevice) == hipSuccess &&
               (!Ds.editable || hipMemcpy(edit, Ds.editable, fn, hipMemcpyHostToDevice) == hipSuccess);
     if (!ok) {
-        release();
         set_err(err, errlen, "design upload failed");
         return ITR_EDEVICE;
     }
+    DesignDev& dd = W.dd;
     dd.prim = prim;
     dd.prim_start = ps;
     dd.bbox = bb;
@@ -2900,30 +2906,58 @@ itr_status materialize_design(itr_ctx* ctx, const itr_grid& I, const DevGrid& D,
     dd.fnx = fnx;
     dd.fny = fny;
     dd.lv = lv;
-    dd.dzx = d_dzx;
-    dd.dzy = d_dzy;
-    dd.dzx2 = d_dzx2;
-    dd.dzy2 = d_dzy2;
     dd.zc = zc;
     dd.plane = D.plane;
     dd.plane_y = D.plane_y;
     dd.volumes = vol;
-    if (maxwh > 0)
-        hipLaunchKernelGGL(itr_materialize_kernel, dim3((unsigned)((maxwh + 255) / 256), (unsigned)nc), dim3(256), 0, st, dd);
-    if (lv == 1) {
+    if (W.maxwh > 0)
+        hipLaunchKernelGGL(itr_materialize_kernel, dim3((unsigned)((W.maxwh + 255) / 256), (unsigned)nc), dim3(256), 0, st,
+                           dd);
+    hipLaunchKernelGGL(itr_design_coarse_kernel, dim3((unsigned)(((size_t)nx * ny + 255) / 256), (unsigned)nc), dim3(256),
+                       0, st, dd);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_err(err, errlen, "design materialization failed: %s", hipGetErrorString(e));
+        return ITR_EDEVICE;
+    }
+    return ITR_OK;
+}
+
+itr_status design_levels(itr_ctx* ctx, const DesignWork& W, std::vector<double>& zlev, char* err, size_t errlen) {
+    const size_t n = (size_t)W.nc * W.dd.nx * W.dd.ny;
+    zlev.assign(std::max<size_t>(n, 1), 0.0);
+    if (W.nc <= 0) return ITR_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(zlev.data(), W.dd.zc, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_err(err, errlen, "design levels failed: %s", hipGetErrorString(e));
+        return ITR_EDEVICE;
+    }
+    return ITR_OK;
+}
+
+itr_status design_finish(itr_ctx* ctx, const itr_grid& I, DesignWork& W, float* d_dzx, float* d_dzy, float* d_dzx2,
+                         float* d_dzy2, char* err, size_t errlen) {
+    const int nc = W.nc;
+    if (nc <= 0) return ITR_OK;
+    DesignDev& dd = W.dd;
+    dd.dzx = d_dzx;
+    dd.dzy = d_dzy;
+    dd.dzx2 = d_dzx2;
+    dd.dzy2 = d_dzy2;
+    const hipStream_t st = ctx->stream;
+    const int nx = dd.nx, ny = dd.ny;
+    if (dd.lv == 1)
         hipLaunchKernelGGL(itr_design_faces_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
                            dim3(256), 0, st, dd);
-    } else {
-        hipLaunchKernelGGL(itr_design_coarse_kernel, dim3((unsigned)(((size_t)nx * ny + 255) / 256), (unsigned)nc),
-                           dim3(256), 0, st, dd);
+    else
         hipLaunchKernelGGL(itr_design_crest_kernel, dim3((unsigned)((nx + 1 + 255) / 256), (unsigned)ny, (unsigned)nc),
                            dim3(256), 0, st, dd);
-    }
     hipLaunchKernelGGL(itr_design_volumes_kernel, dim3((unsigned)nc), dim3(256), 0, st, dd);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess && Ds.volumes) e = hipMemcpy(Ds.volumes, vol, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost);
-    release();
+    if (e == hipSuccess && I.design->volumes)
+        e = hipMemcpy(I.design->volumes, dd.volumes, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         set_err(err, errlen, "design materialization failed: %s", hipGetErrorString(e));
         return ITR_EDEVICE;
@@ -3204,7 +3238,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                 set_err(err, errlen, "grid %d: unknown boundary code %d", g, code);
                 return ITR_EINVAL;
             }
-            if (code == 2 && (!G.ghost_value || !G.z_level)) {
+            if (code == 2 && (!G.ghost_value || (!G.z_level && !G.design))) {
                 set_err(err, errlen, "grid %d: stage boundary cells need ghost_value and z_level", g);
                 return ITR_EINVAL;
             }
@@ -3265,12 +3299,9 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         }
         if (G.design) {
             const itr_design& Ds = *G.design;
-            bool stage = false;
-            for (int i = 0; i < 2 * G.nx + 2 * G.ny; i++) stage = stage || G.ghost[i] == 2;
             const int lv = std::max(Ds.level, 1);
-            if ((lv >= 2) != (G.crest != 0) || q->has_initial_stage || stage) {
-                set_err(err, errlen, "grid %d: device designs need a level-1 signed or level >= 2 face-crest grid "
-                                     "without stage levels", g);
+            if ((lv >= 2) != (G.crest != 0)) {
+                set_err(err, errlen, "grid %d: device designs need a level-1 signed or level >= 2 face-crest grid", g);
                 return ITR_DECLINED;
             }
             if (lv >= 2 && (Ds.fine_nx <= 0 || Ds.fine_ny <= 0 || (Ds.fine_nx + lv - 1) / lv != G.nx ||
@@ -3294,7 +3325,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                 }
             }
         }
-        if (q->has_initial_stage && !G.z_level) {
+        if (q->has_initial_stage && !G.z_level && !G.design) {
             set_err(err, errlen, "grid %d: initial_stage needs z_level", g);
             return ITR_EINVAL;
         }
@@ -3387,6 +3418,19 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             D.tol_h = I.replay ? I.replay->tol_h : 0.0;
             D.tol_q = I.replay ? I.replay->tol_q : 0.0;
         }
+        // Candidates given as designs: their dz and level z on the device now; the Stage
+        // ghosts and initial_stage below read the level z.
+        DesignWork dwork;
+        std::vector<double> dz_levels;
+        const double* z_level = I.z_level;
+        if (I.design) {
+            itr_status ds = design_begin(ctx, I, D, dwork, err, errlen);
+            if (ds == ITR_OK && (D.special || p->has_initial_stage)) {
+                ds = design_levels(ctx, dwork, dz_levels, err, errlen);
+                z_level = dz_levels.data();
+            }
+            if (ds != ITR_OK) return fail(ds);
+        }
         std::vector<float> gstage;
         if (D.special) {
             gstage.assign((size_t)std::max(nc, 1) * nb, 0.0f);
@@ -3398,7 +3442,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     else if (i < 2 * nx) { r = ny - 1; c = i - nx; }
                     else if (i < 2 * nx + ny) { r = i - 2 * nx; c = 0; }
                     else { r = i - 2 * nx - ny; c = nx - 1; }
-                    const double zl = I.z_level[(size_t)k * nx * ny + (size_t)r * nx + c];
+                    const double zl = z_level[(size_t)k * nx * ny + (size_t)r * nx + c];
                     gstage[(size_t)k * nb + i] = (float)std::fmax(I.ghost_value[i] - zl, 0.0);
                 }
         }
@@ -3439,7 +3483,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
                     const size_t q = (size_t)r * nx + c;
                     if (I.wet[q] == 0.0f) continue;
                     double d = p->initial_depth;
-                    if (p->has_initial_stage) d = std::fmax(d, p->initial_stage - I.z_ref - I.z_level[(size_t)k * nx * ny + q]);
+                    if (p->has_initial_stage) d = std::fmax(d, p->initial_stage - I.z_ref - z_level[(size_t)k * nx * ny + q]);
                     float hv = (float)std::fmax(d, 0.0);
                     if (hv == 0.0f) hv = 0.0f;
                     h0[q] = hv;
@@ -3640,7 +3684,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             return fail(ITR_EDEVICE);
         }
         if (I.design) {
-            itr_status ms = materialize_design(ctx, I, D, d_dzx, d_dzy, d_dzx2, d_dzy2, err, errlen);
+            itr_status ms = design_finish(ctx, I, dwork, d_dzx, d_dzy, d_dzx2, d_dzy2, err, errlen);
             if (ms != ITR_OK) return fail(ms);
         }
         D.state = d_state;

@@ -131,7 +131,7 @@ def _grid_c(base: TerrainBase, terrains, monitors, keep: _Keep, guard=None, need
     if base.crest and terrains:
         g.dzx_r = f(keep.arr(np.stack([t.fx1 for t in terrains]), np.float32))
         g.dzy_r = f(keep.arr(np.stack([t.fy1 for t in terrains]), np.float32))
-    if need_z:
+    if need_z and terrains:
         g.z_level = keep.arr(np.stack([t.z_level for t in terrains]), np.float64).ctypes.data_as(_abi._dp)
     mon = keep.arr(np.asarray(monitors, np.uint32) if len(monitors) else np.zeros(1, np.uint32), np.uint32)
     g.mon_cells = mon.ctypes.data_as(_abi._up)

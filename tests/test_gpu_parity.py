@@ -501,6 +501,37 @@ def test_device_designs_on_crest_levels(solver, level):
     assert not np.array_equal(runs[0].depth, runs[1].depth)
 
 
+@pytest.mark.parametrize("level", [1, 2])
+def test_device_designs_with_stage_levels(solver, level):
+    """Device designs on levels whose state depends on each candidate's own levels: a Stage
+    boundary segment (ghost depth f32(max(stage - z_level, 0)) of the edited boundary cell,
+    boundary.rs:43, 66) and initial_stage (init_state, solver.rs:287-312), with edits that
+    reach the stage cells; bitwise like host-prepared terrains."""
+    fnx, fny = 80, 56
+    _say(f"case: device designs with a stage boundary and initial stage, level {level}")
+    segs = [{"kind": "stage", "side": "south", "stage_m": 3.35, "range_m": (20.0, 130.0)},
+            {"kind": "transmissive", "side": "east"}]
+    fb = sc.validation_terrain(fnx, fny, 2.0, lambda x, y: 3.0 + 0.01 * y + 0.3 * math.sin(x / 9.0), n=0.035,
+                               walls=lambda c, r: 30 <= c < 34 and 20 <= r < 30, segments=segs)
+    base = fb if level == 1 else tr.TerrainBase.build(fb.fine, level, segs)
+    geo = fb.fine.geo
+    rng = sc.SplitMix64(41 + level)
+    designs = [sc.random_primitives(rng, k, geo, fnx, fny, kinds=("berm", "swale", "mound", "cut")) for k in (2, 3, 4)]
+    south = sc.random_primitives(rng, 1, geo, fnx, fny, kinds=("mound",))
+    south[0].cx, south[0].cy, south[0].length, south[0].width, south[0].height = 60.0, 3.0, 40.0, 10.0, 0.4
+    designs.append(south)
+    designs.append([])
+    p = sc.SolverParams(200.0, 50.0, rain=sc.mm_h([[0, 60], [200, 20]]), initial_stage=3.15, infil_capacity=1e-6)
+    mon = np.arange(0, base.nx * base.ny, 6, dtype=np.uint32)
+    runs, vol = solver.run_designs(p, base, designs, mon, max_abs_dz=0.5)
+    edits = [tr.materialize(d, geo, fnx, fny, None, 0.5) for d in designs]
+    host = solver.run_batch(p, [tr.PreparedTerrain(base).apply_edit(e) for e in edits], mon)
+    for i, (a, b, e) in enumerate(zip(runs, host, edits)):
+        _runs_equal(a, b, f"level {level} stage design {i}")
+        assert bits64(vol[i, 0]) == bits64(e.cut_m3) and bits64(vol[i, 1]) == bits64(e.fill_m3), f"design {i} volumes"
+    assert not np.array_equal(runs[3].depth, runs[4].depth), "the south edit should change the run"
+
+
 def test_locked_dt_schedule_and_adaptive_restart(solver):
     """Baseline-locked dt (solver.rs:452-455) and its fallback: a step whose locked dt
     fails the hard-CFL check restarts adaptively from the last sync snapshot

@@ -142,7 +142,8 @@ typedef struct {
     const float* dzy;    /* [n_cand][ny+1][nx] fy[0] */
     const float* dzx_r;  /* crest: [n_cand][ny][nx+1] fx[1] (d_r); NULL for signed */
     const float* dzy_r;  /* crest: [n_cand][ny+1][nx] fy[1] */
-    const double* z_level; /* [n_cand][ny*nx] level z incl. edits (initial_stage), or NULL */
+    const double* z_level; /* [n_cand][ny*nx] level z incl. edits (initial_stage), or NULL; not read
+                              for a grid given as designs (the device builds the levels) */
     const uint32_t* mon_cells; /* MonitorSet: sorted unique interior cells row*nx+col */
     /* Exact guard early abort (Objective::guard_breach, crates/itr-core/src/objective.rs:132-144),
      * evaluated on the device at every sync point.  baseline_max NULL: disabled. */
@@ -155,7 +156,8 @@ typedef struct {
      * Boundaries::from_segments, crates/itr-hydro/src/terrain.rs:122-187, and filled as
      * boundary.rs:43-47, 66-70).  May be NULL when no ghost code is 2 or 3.
      * ghost_value: [2*nx + 2*ny] in the order of `ghost`: stage level minus z_ref [m] for
-     *              code 2 (the ghost depth uses z_level, which must then be given), or
+     *              code 2 (the ghost depth uses z_level, which must then be given unless the
+     *              grid is given as designs), or
      *              1 / segment length [1/m] for code 3.
      * ghost_hydro: [2*nx + 2*ny] hydrograph index of each code-3 cell.
      * hydro_start: [n_hydro + 1] offsets into hydro_t / hydro_q, the piecewise-linear

@@ -58,9 +58,9 @@ typedef struct {
     const double* prim;        /* [n][12]: profile (0 cut, 1 berm/swale/mound), segment x0, y0, x1,
                                   y1, width, height, cell window c0, r0, c1, r1 (half-open), 0 */
     const int32_t* bbox;       /* [n_cand][4]: union window c0, r0, w, h of the candidate (w or h 0: no edit) */
-    const double* z_fine;      /* [ny][nx] FineTerrain::z (re-based) */
-    const uint8_t* wall;       /* [ny][nx] 1 = wall cell (level z = z_wall) */
-    const uint8_t* editable;   /* [ny][nx] 1 = editable, or NULL: every cell */
+    const double* z_fine;      /* [ny][nx] of the fine grid: FineTerrain::z (re-based) */
+    const uint8_t* wall;       /* [ny][nx] of the fine grid: 1 = wall cell */
+    const uint8_t* editable;   /* [ny][nx] of the fine grid: 1 = editable, or NULL: every cell */
     double origin_x, origin_y, dx, dy; /* GeoTransform: x = origin_x + col dx, y = origin_y - row dy */
     double max_abs_dz, z_wall;
     double* volumes;           /* out: [n_cand][2] cut_m3, fill_m3 of each edit, or NULL */

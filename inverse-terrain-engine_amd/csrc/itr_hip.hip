@@ -2343,7 +2343,8 @@ struct AdjArgs {
     const float* gn2;      // [ny][P]
     const int32_t* ghost;  // [2nx + 2ny]
     int32_t nx, ny, P, R;    // R: rows per band of the reverse march
-    int32_t waves, pad;      // waves per row (62 columns each)
+    int32_t waves;           // waves per row (62 columns each)
+    int32_t tail_groups;     // row groups of the last wave (as the forward step; 1: off)
     size_t plane, cstride; // plane = ny*P, cstride = 3*plane
     size_t dzx_stride, dzy_stride;
     StepK K;
@@ -2542,8 +2543,18 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
     const int nx = a.nx, ny = a.ny;
     const int a0 = band * a.R, b0 = min(a0 + a.R, ny);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int col = wave * kColsPerWave - 1 + lane;
-    const bool upd = lane >= 1 && lane <= kColsPerWave && col < nx;
+    // The last wave of a row that updates few columns marches tail_groups row groups of the
+    // band side by side (lane groups with their own halo lanes), as the forward step does:
+    // group g covers rows a0 + g*t_rows .. of the band; lr is its offset.
+    const int S = a.tail_groups;
+    const bool tail = S > 1 && uni(wave) == a.waves - 1;
+    const int tgw = tail ? 64 / S : 64;
+    const int tg = tail ? lane / tgw : 0;
+    const int gl = lane - tg * tgw;
+    const int t_rows = tail ? (b0 - a0 + S - 1) / S : b0 - a0;
+    const int lr = tg * t_rows;
+    const int col = wave * kColsPerWave - 1 + gl;
+    const bool upd = gl >= 1 && gl <= tgw - 2 && tg < S && col < nx;
     const bool cin = col >= 0 && col < nx;
     const int cc = min(max(col, 0), nx - 1), xc = min(max(col, 0), nx);
     const float* U = a.U + (size_t)cand * a.cstride;
@@ -2609,16 +2620,20 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         const adj::F<float> b{dtdy * (lr.h - ll.h), -dtdy * ll.qy, dtdy * lr.qy, dtdy * (lr.qx - ll.qx)};
         return adj::face_adj<false>(L.h, vl, R.h, vr, dz, k, b);
     };
-    const RevIn ip = load(a0 - 1), ic = load(a0);
-    RevIn nxt = load(a0 + 1);
-    NbW Pn = cell(ip, a0 - 1), Cn = cell(ic, a0);
+    const int g0 = a0 + lr;   // this lane's first row
+    const RevIn ip = load(g0 - 1), ic = load(g0);
+    RevIn nxt = load(g0 + 1);
+    NbW Pn = cell(ip, g0 - 1), Cn = cell(ic, g0);
     adj::V<float> vC = velo(Cn);
-    adj::S3<float> lC = lam1(ic, a0);
+    adj::S3<float> lC = lam1(ic, g0);
     float dzw = ic.dzw;   // west face jump of the current row
     adj::FaceBar<float> fN =
-        yface(Pn, velo(Pn), lam1(ip, a0 - 1), Cn, vC, lC, ic.dzs);
-    if (upd && a0 >= 1) gy[(size_t)a0 * a.P + col] += (double)fN.dz;
-    for (int j = a0; j < b0; j++) {
+        yface(Pn, velo(Pn), lam1(ip, g0 - 1), Cn, vC, lC, ic.dzs);
+    // (a group's north face inside the band is the previous group's south face, counted there)
+    if (upd && a0 >= 1 && lr == 0) gy[(size_t)a0 * a.P + col] += (double)fN.dz;
+    for (int jb = a0; jb < a0 + t_rows; jb++) {
+        const int j = jb + lr;   // this lane's row
+        const bool rv = j < b0;
         const RevIn in = nxt;   // row j+1
         if (j + 1 < b0) nxt = load(j + 2);
         const NbW Sn = cell(in, j + 1);
@@ -2632,7 +2647,7 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         const adj::S3<float> lW{from_left(lC.h), from_left(lC.qx), from_left(lC.qy)};
         const adj::F<float> bw{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
         const adj::FaceBar<float> fW = adj::face_adj<true>(Wn.h, vW, Cn.h, vC, dzw, k, bw);
-        if (upd && col >= 1 && col <= nx - 1) gx[(size_t)j * a.P + col] += (double)fW.dz;
+        if (upd && rv && col >= 1 && col <= nx - 1) gx[(size_t)j * a.P + col] += (double)fW.dz;
         dzw = in.dzw;   // row j+1's west face
         // East face: the right lane's west face.
         adj::FaceBar<float> fE;
@@ -2661,7 +2676,7 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         bv.u += fS.ul; bv.v += fS.vl; bv.c += fS.cl;
         if (j == ny - 1) fold(Sn, false, fS.hr, adj::V<float>{fS.ur, fS.vr, fS.cr});
         const adj::S3<float> q = adj::vel_adj(Cn.h, Cn.qx, Cn.qy, k, bv);
-        if (upd) {
+        if (upd && rv) {
             const size_t o = (size_t)j * a.P + col;
             LO[o] = bC.h + q.h;
             LO[a.plane + o] = bC.qx + q.qx;
@@ -4263,6 +4278,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     // batches keep the per-cell faces kernel (one thread per cell).
     A.R = std::min(16, ny);
     A.waves = (nx + kColsPerWave - 1) / kColsPerWave;
+    A.tail_groups = tail_group_count(nx, A.waves);
     const dim3 bgrd((unsigned)((ny + A.R - 1) / A.R), (unsigned)nc);
     const bool band_mode = ctx->adjoint_mode == 1 ||
                            (ctx->adjoint_mode == 0 && (size_t)bgrd.x * bgrd.y * (size_t)A.waves >= 2048);

@@ -1911,7 +1911,9 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     __shared__ double shd[48];
     __shared__ CandCtl sS;
     __shared__ int s_go;
-    // XCD-aware task order: consecutive bands of a candidate share one XCD's L2.
+    // XCD-aware task order: each XCD takes a contiguous share of the task list (whole runs
+    // of a candidate's bands, so band halo rows stay in one L2), ordered band-major by
+    // order_tasks so that resident workgroups read the same rows of the shared planes.
     const int nblk = gridDim.x;
     const int per = nblk >> 3;
     const int task = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -2835,6 +2837,21 @@ itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* er
     return ITR_OK;
 }
 
+// Task order within each XCD's share of the launch (the kernel maps block i to XCD i % 8
+// and XCD x to tasks [x*per, (x+1)*per)): band-major, so that workgroups resident together
+// on one XCD read the same rows of the shared wet / gn2 planes.
+void order_tasks(std::vector<int4>& tasks) {
+    const size_t n = tasks.size();
+    if (n == 0) return;
+    const size_t per = (n + 7) / 8;
+    for (size_t x0 = 0; x0 < n; x0 += per) {
+        const size_t x1 = std::min(n, x0 + per);
+        std::stable_sort(tasks.begin() + x0, tasks.begin() + x1, [](const int4& a, const int4& b) {
+            return a.x != b.x ? a.x < b.x : a.z < b.z;
+        });
+    }
+}
+
 // A grid given as designs (itr_design), built on the device in three parts: design_begin
 // uploads the designs and computes every candidate's fine dz (and at level >= 2 its coarse
 // level z); design_levels hands the candidates' level z to the host (Stage ghosts and
@@ -3746,6 +3763,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         else max_lds = std::max(max_lds, lds_layout(R, nx, D.special != 0).total * sizeof(float));
     }
     b->ncand = total;
+    order_tasks(tasks);
     b->ntasks = (int)tasks.size();
     b->ntasks_li = (int)tasks_li.size();
     b->modes = 0;
@@ -4398,6 +4416,7 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
                 (D.li ? tasks_li : tasks).push_back(make_int4((int)g, k, band, D.cand0 + k));
         dg.push_back(D);
     }
+    order_tasks(tasks);
     HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (!tasks_li.empty())

@@ -3764,6 +3764,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     }
     b->ncand = total;
     order_tasks(tasks);
+    order_tasks(tasks_li);
     b->ntasks = (int)tasks.size();
     b->ntasks_li = (int)tasks_li.size();
     b->modes = 0;
@@ -4417,6 +4418,7 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
         dg.push_back(D);
     }
     order_tasks(tasks);
+    order_tasks(tasks_li);
     HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (!tasks_li.empty())

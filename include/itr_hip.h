@@ -266,9 +266,12 @@ void itr_hip_batch_destroy(itr_batch* b);
  * frozen schedule makes the discrete step a fixed function of the state), stores
  * checkpoints every ceil(sqrt(N)) steps, replays each segment and sweeps it backwards;
  * adjoint arithmetic is f32, face-gradient accumulation f64.  p->max_steps > 0 bounds the
- * horizon (the run then ends after that many steps).  Signed face mode only
- * (crest levels return ITR_DECLINED).  The earthwork term and the chain rule to
- * primitive heights are host work on the returned cell gradient. */
+ * horizon (the run then ends after that many steps).  Signed face mode only: crest
+ * levels, Stage / Inflow ghosts, an initial stage (its depth depends on the bed), local
+ * inertial physics, replay grids and device designs return ITR_DECLINED.  A replayed
+ * segment that differs bitwise from its checkpoint returns ITR_EDEVICE (no gradient).
+ * The earthwork term and the chain rule to primitive heights are host work on the
+ * returned cell gradient. */
 typedef struct {
     int32_t n_assets;
     int32_t reserved;

@@ -2724,6 +2724,25 @@ void set_err(char* err, size_t errlen, const char* fmt, ...) {
         }                                                                                    \
     } while (0)
 
+// Two timing events destroyed on every return path.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    hipError_t create() {
+        for (auto& x : e) {
+            const hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    ~EventPair() {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 // FixedSum<f32> of one row, then the f64 sum over rows (solver.rs:250-262).
 double host_volume(const std::vector<float>& h, int nx, int ny, double area) {
     double v = 0.0;
@@ -2830,8 +2849,9 @@ void free_batch_mem(itr_batch* b) {
 
 template <class T>
 itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* err, size_t errlen) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    HIP_TRY(hipMalloc(&p, bytes));
     allocs.push_back(p);
     *out = (T*)p;
     return ITR_OK;
@@ -3862,11 +3882,10 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     if (s != ITR_OK) return s;
     std::vector<CandCtl> c0;
     if ((s = read_ctl(b, c0, err, errlen)) != ITR_OK) return s;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_TRY(ev.create());
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    HIP_TRY(hipEventRecord(ev.e[0], ctx->stream));
     for (;;) {
         // Complete every in-flight step first: a chunk always ends on the launch grid.
         if ((s = launch_steps(b, K, err, errlen)) != ITR_OK) return s;
@@ -3909,13 +3928,11 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
             if (!(b->h_status[c] & F_DONE)) all_done = false;
         if (all_done) break;
     }
-    HIP_TRY(hipEventRecord(e1, ctx->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e[1], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ev.e[1]));
     const auto t1 = std::chrono::steady_clock::now();
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     b->stats.device_ms = ms;
     b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
     std::vector<CandCtl> c1;
@@ -3938,20 +3955,17 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     if (s != ITR_OK) return s;
     std::vector<CandCtl> c0, c1;
     if ((s = read_ctl(b, c0, err, errlen)) != ITR_OK) return s;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_TRY(ev.create());
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    HIP_TRY(hipEventRecord(ev.e[0], ctx->stream));
     if (b->ntasks + b->ntasks_li > 0 && n_steps > 0)
         if ((s = launch_steps(b, n_steps, err, errlen)) != ITR_OK) return s;
-    HIP_TRY(hipEventRecord(e1, ctx->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e[1], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ev.e[1]));
     const auto t1 = std::chrono::steady_clock::now();
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     if ((s = read_ctl(b, c1, err, errlen)) != ITR_OK) return s;
     HIP_TRY(hipMemcpy(b->h_status, b->d_status, sizeof(uint32_t) * b->ncand, hipMemcpyDeviceToHost));
     b->stats.device_ms = ms;
@@ -4085,6 +4099,16 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "the adjoint differentiates the HLL step only");
         return ITR_DECLINED;
     }
+    // An initial stage makes the initial depth max(stage - z_ref - z, h0) a function of the
+    // bed, a term the reverse sweep (face jumps only) does not carry.
+    if (p->has_initial_stage) {
+        set_err(err, errlen, "the adjoint does not differentiate an initial stage (its depth depends on the bed)");
+        return ITR_DECLINED;
+    }
+    if (grid->replay || grid->design) {
+        set_err(err, errlen, "the adjoint runs on prepared faces of a full grid (no replay, no device designs)");
+        return ITR_DECLINED;
+    }
     if (grid->ghost)
         for (int i = 0; i < 2 * grid->nx + 2 * grid->ny; i++)
             if (grid->ghost[i] >= 2) {
@@ -4189,9 +4213,9 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         }
         return hipSuccess;
     };
-    hipEvent_t ev[2];
-    ADJ_TRY(hipEventCreate(&ev[0]));
-    ADJ_TRY(hipEventCreate(&ev[1]));
+    EventPair evp;   // destroyed on every return path
+    ADJ_TRY(evp.create());
+    hipEvent_t* ev = evp.e;
     auto elapsed = [&](float* ms) -> hipError_t {
         hipError_t e = hipEventSynchronize(ev[1]);
         return e == hipSuccess ? hipEventElapsedTime(ms, ev[0], ev[1]) : e;
@@ -4301,8 +4325,10 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     const dim3 bgrd((unsigned)((ny + A.R - 1) / A.R), (unsigned)nc);
     const bool band_mode = ctx->adjoint_mode == 1 ||
                            (ctx->adjoint_mode == 0 && (size_t)bgrd.x * bgrd.y * (size_t)A.waves >= 2048);
-    hipEvent_t er[4];
-    for (auto& e : er) ADJ_TRY(hipEventCreate(&e));
+    EventPair erp0, erp1;
+    ADJ_TRY(erp0.create());
+    ADJ_TRY(erp1.create());
+    hipEvent_t er[4] = {erp0.e[0], erp0.e[1], erp1.e[0], erp1.e[1]};
     double replay_ms = 0.0, reverse_ms = 0.0;
     int li = 0;
     for (int k = nseg; k >= 1; k--) {
@@ -4380,11 +4406,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         o.replay_ms = replay_ms;
         o.reverse_ms = reverse_ms;
     }
-    for (auto& e : er) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
 #undef ADJ_TRY
     cleanup();
+    if (mis != 0) {   // a replayed segment left the checkpointed trajectory: no gradient
+        set_err(err, errlen, "%llu replayed segment-end values differ from the checkpoints", mis);
+        return ITR_EDEVICE;
+    }
     return ITR_OK;
 }
 

@@ -117,6 +117,19 @@ def test_adjoint_declines_stage_boundaries(solver):
         solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)
 
 
+def test_adjoint_declines_initial_stage(solver):
+    """An initial stage makes the initial depth max(stage - z, h0) a function of the bed,
+    which the reverse sweep does not differentiate: ITR_DECLINED, not a gradient missing
+    that term."""
+    from inverse_terrain_engine_amd.gpu import Declined
+    base = sc.validation_terrain(20, 12, 2.0, lambda x, y: 1.0 + 0.01 * x, n=0.03,
+                                 segments=[{"kind": "transmissive", "side": "east"}])
+    p = sc.SolverParams(20.0, 10.0, initial_stage=1.1)
+    p.locked_dt = [0.1] * 200
+    with pytest.raises(Declined, match="initial stage"):
+        solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)
+
+
 def test_adjoint_band_march_equals_per_cell(solver):
     """The two reverse sweeps (band march, per-cell kernels) give the same bits: gradient of
     every face and cell, J, peaks.  Three candidates on a 130x70 grid (three waves per row,

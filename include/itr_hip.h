@@ -244,6 +244,19 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value);
 itr_status itr_hip_run_batch(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
                              itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * itr_hip_run_batch with every device buffer of the batch (state, faces, control blocks,
+ * partial results, monitors, ledgers) carved from a static 64 MiB array that each device
+ * holds from module load until the process ends.  It calls no hipMalloc, hipFree,
+ * hipHostMalloc, graph instantiation or event creation: steps launch directly and the
+ * run is not timed on the device (itr_stats is not reported).  Results are bitwise
+ * those of itr_hip_run_batch.  A batch whose buffers exceed the array returns
+ * ITR_ENOMEM with a message (use itr_hip_run_batch).  Every call on one device shares
+ * that device's array: calls that overlap on one device race. */
+itr_status itr_hip_run_batch_noalloc(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                                     itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen);
+
 /* The same, split so that inputs can stay resident in HBM across runs (serving loops,
  * benchmarks): create uploads, run executes the step loop, results downloads. */
 itr_status itr_hip_batch_create(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,

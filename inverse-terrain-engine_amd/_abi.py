@@ -93,6 +93,7 @@ SYNC_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C
                       C.c_int32, _dp)
 
 EXPORTS = ["itr_hip_create", "itr_hip_destroy", "itr_hip_device_info", "itr_hip_set_option", "itr_hip_run_batch",
+           "itr_hip_run_batch_noalloc",
            "itr_hip_batch_create", "itr_hip_batch_run", "itr_hip_batch_results", "itr_hip_batch_stats",
            "itr_hip_batch_destroy", "itr_hip_batch_advance", "itr_hip_adjoint",
            "itr_hip_dt_schedule", "itr_hip_autotune", "itr_hip_math_selfcheck"]
@@ -121,6 +122,8 @@ def load():
     L.itr_hip_run_batch.restype = C.c_int32
     L.itr_hip_run_batch.argtypes = [vp, C.POINTER(SolverParamsC), C.POINTER(GridC), C.c_int32, SYNC_FN, vp,
                                     C.POINTER(RunOutC), C.c_char_p, C.c_size_t]
+    L.itr_hip_run_batch_noalloc.restype = C.c_int32
+    L.itr_hip_run_batch_noalloc.argtypes = L.itr_hip_run_batch.argtypes
     L.itr_hip_batch_create.restype = C.c_int32
     L.itr_hip_batch_create.argtypes = [vp, C.POINTER(SolverParamsC), C.POINTER(GridC), C.c_int32, C.POINTER(vp),
                                        C.c_char_p, C.c_size_t]

@@ -2706,6 +2706,15 @@ __global__ void __launch_bounds__(256) itr_adj_compare_kernel(const float* A, co
     if (bad) atomicAdd(count, bad);
 }
 
+// ------------------------------------------------------------------ static scratch
+// Device memory of itr_hip_run_batch_noalloc: every buffer of that entry point's batch
+// (state, faces, control blocks, partial results, monitors, ledgers, staging) is carved
+// from this array, so the call allocates nothing.  Each device the library runs on
+// holds its own copy until the process ends.  Host code never names it after set-up:
+// itr_hip_create maps it once per device with hipGetSymbolAddress and keeps the address.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 // ------------------------------------------------------------------ host side
 void set_err(char* err, size_t errlen, const char* fmt, ...) {
     if (!err || errlen == 0) return;
@@ -2742,6 +2751,25 @@ struct EventPair {
             if (x) (void)hipEventDestroy(x);
     }
 };
+
+// Bump allocator over the static scratch of one device.  While an Arena is installed on
+// the calling thread (ArenaScope), dev_alloc carves from it instead of calling hipMalloc,
+// and the carved buffers are never freed (the next call reuses the array).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+thread_local Arena* tl_arena = nullptr;
+struct ArenaScope {
+    Arena* prev;
+    explicit ArenaScope(Arena* a) : prev(tl_arena) { tl_arena = a; }
+    ~ArenaScope() { tl_arena = prev; }
+};
+
+// The scratch address of each device, mapped once (hipGetSymbolAddress with the device
+// current) by the first itr_hip_create on it.
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch[kMaxDevices];
 
 // FixedSum<f32> of one row, then the f64 sum over rows (solver.rs:250-262).
 double host_volume(const std::vector<float>& h, int nx, int ny, double area) {
@@ -2782,6 +2810,7 @@ struct itr_ctx {
     int adjoint_mode = 0;  // reverse sweep: 0 by batch size, 1 band march, 2 per-cell kernels
     struct Tuned { int nx, ny, crest, R; };
     std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
+    void* scratch = nullptr;    // this device's static scratch (itr_hip_run_batch_noalloc)
 };
 
 namespace {
@@ -2833,23 +2862,44 @@ struct itr_batch {
     int graph_steps = 0;
     itr_stats stats{};
     std::vector<std::vector<double>> cb_mon;   // scratch for callbacks
+    // Built by itr_hip_run_batch_noalloc: device buffers live in the static scratch (never
+    // freed), the status word is polled into host memory, launches are direct (no graph)
+    // and untimed (no events).
+    bool no_alloc = false;
+    std::vector<uint32_t> h_status_host;
 };
 
 namespace {
 
 void free_batch_mem(itr_batch* b) {
+    // (buffers carved from the static scratch were never added to these lists)
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
-    void* ps[] = {b->d_grids, b->d_tasks, b->d_tasks_li, b->d_snap, b->d_acc, b->d_status};
-    for (void* p : ps)
-        if (p) (void)hipFree(p);
-    if (b->h_status) (void)hipHostFree(b->h_status);
+    if (!b->no_alloc) {
+        void* ps[] = {b->d_grids, b->d_tasks, b->d_tasks_li, b->d_snap, b->d_acc, b->d_status};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (b->h_status) (void)hipHostFree(b->h_status);
+    }
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
 }
 
 template <class T>
 itr_status dev_alloc(std::vector<void*>& allocs, T** out, size_t count, char* err, size_t errlen) {
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (Arena* A = tl_arena) {
+        const size_t off = (A->used + 255) & ~(size_t)255;
+        if (off > A->cap || bytes > A->cap - off) {
+            set_err(err, errlen,
+                    "batch does not fit the %zu MiB static scratch of the no-allocation entry point "
+                    "(a buffer of %zu bytes at offset %zu); use itr_hip_run_batch",
+                    A->cap >> 20, bytes, off);
+            return ITR_ENOMEM;
+        }
+        A->used = off + bytes;
+        *out = (T*)(A->base + off);
+        return ITR_OK;
+    }
     void* p = nullptr;
     HIP_TRY(hipMalloc(&p, bytes));
     allocs.push_back(p);
@@ -3201,8 +3251,16 @@ itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errle
         set_err(err, errlen, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
         return ITR_DECLINED;
     }
+    // The static scratch of this device: mapped once, with the device current.
+    void* sp = device < kMaxDevices ? g_scratch[device].load() : nullptr;
+    if (device < kMaxDevices && !sp) {
+        HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        void* expected = nullptr;
+        if (!g_scratch[device].compare_exchange_strong(expected, sp)) sp = expected;
+    }
     auto* c = new itr_ctx();
     c->device = device;
+    c->scratch = sp;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -3384,6 +3442,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     }
     auto* b = new itr_batch();
     b->ctx = ctx;
+    b->no_alloc = tl_arena != nullptr;
     auto fail = [&](itr_status s) {
         free_batch_mem(b);
         delete b;
@@ -3822,7 +3881,11 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->prm = b->grids[0].prm;
     if ((s = dev_alloc(misc, &d_status, std::max(total, 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_status = d_status;
-    if (hipHostMalloc((void**)&b->h_status, sizeof(uint32_t) * std::max(total, 1), hipHostMallocDefault) != hipSuccess) {
+    if (b->no_alloc) {
+        b->h_status_host.assign((size_t)std::max(total, 1), 0u);
+        b->h_status = b->h_status_host.data();
+    } else if (hipHostMalloc((void**)&b->h_status, sizeof(uint32_t) * std::max(total, 1), hipHostMallocDefault) !=
+               hipSuccess) {
         set_err(err, errlen, "pinned allocation failed");
         return fail(ITR_ENOMEM);
     }
@@ -3872,7 +3935,10 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     }
     int K = ctx->steps_per_chunk > 0 ? ctx->steps_per_chunk : 32;
     K = (K + 3) / 4 * 4;   // chunks keep the launch sequence aligned to the 4-deep ring
-    const bool graph = ctx->use_graph != 0;
+    // (a no-allocation batch launches directly and is not timed on the device: graph
+    // instantiation and events are runtime allocations)
+    const bool graph = ctx->use_graph != 0 && !b->no_alloc;
+    const bool timed = !b->no_alloc;
     b->stats.steps_per_chunk = K;
     b->stats.use_graph = graph ? 1 : 0;
     b->stats.launches = 0;
@@ -3883,9 +3949,9 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     std::vector<CandCtl> c0;
     if ((s = read_ctl(b, c0, err, errlen)) != ITR_OK) return s;
     EventPair ev;
-    HIP_TRY(ev.create());
+    if (timed) HIP_TRY(ev.create());
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipEventRecord(ev.e[0], ctx->stream));
+    if (timed) HIP_TRY(hipEventRecord(ev.e[0], ctx->stream));
     for (;;) {
         // Complete every in-flight step first: a chunk always ends on the launch grid.
         if ((s = launch_steps(b, K, err, errlen)) != ITR_OK) return s;
@@ -3928,11 +3994,15 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
             if (!(b->h_status[c] & F_DONE)) all_done = false;
         if (all_done) break;
     }
-    HIP_TRY(hipEventRecord(ev.e[1], ctx->stream));
-    HIP_TRY(hipEventSynchronize(ev.e[1]));
-    const auto t1 = std::chrono::steady_clock::now();
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    if (timed) {
+        HIP_TRY(hipEventRecord(ev.e[1], ctx->stream));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
+        HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    } else {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    const auto t1 = std::chrono::steady_clock::now();
     b->stats.device_ms = ms;
     b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
     std::vector<CandCtl> c1;
@@ -4073,6 +4143,31 @@ itr_status itr_hip_run_batch(itr_ctx* ctx, const itr_solver_params* p, const itr
                              itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen) {
     itr_batch* b = nullptr;
     itr_status s = itr_hip_batch_create(ctx, p, grids, n_grids, &b, err, errlen);
+    if (s != ITR_OK) return s;
+    s = itr_hip_batch_run(b, on_sync, user, err, errlen);
+    if (s == ITR_OK) s = itr_hip_batch_results(b, outs, err, errlen);
+    itr_hip_batch_destroy(b);
+    return s;
+}
+
+itr_status itr_hip_run_batch_noalloc(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grids, int32_t n_grids,
+                                     itr_sync_fn on_sync, void* user, itr_run_out* outs, char* err, size_t errlen) {
+    if (!ctx) {
+        set_err(err, errlen, "null argument");
+        return ITR_EINVAL;
+    }
+    if (!ctx->scratch) {
+        set_err(err, errlen, "device %d has no static scratch mapped", ctx->device);
+        return ITR_EDEVICE;
+    }
+    // Every device buffer of the batch comes from the static scratch; the calls of one
+    // device share it, so calls that overlap on one device race.
+    Arena A;
+    A.base = (char*)ctx->scratch;
+    A.cap = kScratchBytes;
+    ArenaScope scope(&A);
+    itr_batch* b = nullptr;
+    itr_status s = batch_create_impl(ctx, p, grids, n_grids, &b, 0, err, errlen);
     if (s != ITR_OK) return s;
     s = itr_hip_batch_run(b, on_sync, user, err, errlen);
     if (s == ITR_OK) s = itr_hip_batch_results(b, outs, err, errlen);

@@ -468,6 +468,23 @@ class GpuSolver:
         tpeak = np.where(wall | (run.monitor_max <= 0.0), np.nan, run.monitor_tmax).reshape(base.ny, base.nx)
         return run, peak, tpeak
 
+    def run_batch_noalloc(self, p: SolverParams, groups, on_sync=None, want_state=True):
+        """itr_hip_run_batch_noalloc: the batch of ``groups`` (as for ``session``) run with
+        every device buffer in the library's static scratch (no device allocation); the
+        results equal run_batch's bitwise.  Raises DeviceError (out of memory) when the
+        batch does not fit the scratch."""
+        keep = _Keep()
+        q = _params_c(p, keep)
+        grids = (_abi.GridC * len(groups))()
+        for i, (base, terrs, mons, guard) in enumerate(groups):
+            grids[i] = _grid_c(base, terrs, mons, keep, guard, p.initial_stage is not None)
+        outs, bufs = _result_buffers(groups, [(g[0].nx, g[0].ny) for g in groups], want_state)
+        cb = _sync_fn(on_sync) if on_sync is not None else C.cast(None, _abi.SYNC_FN)
+        err = C.create_string_buffer(1024)
+        _check(self._L.itr_hip_run_batch_noalloc(self._h, C.byref(q), grids, len(groups), cb, None, outs, err, 1024),
+               err)
+        return _collect_results(outs, bufs)
+
     def run_batch(self, p: SolverParams, terrains, monitors, guard=None, on_sync=None, want_state=True):
         """GpuSolver::run_batch: every candidate (sharing one TerrainBase) to its end."""
         if len(terrains) == 0:
@@ -478,6 +495,60 @@ class GpuSolver:
             return s.results(want_state)
         finally:
             s.close()
+
+
+def _result_buffers(groups, shapes, want_state):
+    """itr_run_out entries (grids concatenated) with their host buffers."""
+    n = sum(len(g[1]) for g in groups)
+    outs = (_abi.RunOutC * max(n, 1))()
+    bufs = []
+    k = 0
+    for (base, terrs, mons, _), (gnx, gny) in zip(groups, shapes):
+        for _ in terrs:
+            nm = max(len(mons), 1)
+            b = {"mm": np.zeros(nm), "mt": np.zeros(nm), "led": np.zeros((4096, 8))}
+            if want_state:
+                b["h"] = np.zeros((gny, gnx), np.float32)
+                b["qx"] = np.zeros((gny, gnx), np.float32)
+                b["qy"] = np.zeros((gny, gnx), np.float32)
+            o = outs[k]
+            o.monitor_max = b["mm"].ctypes.data_as(_abi._dp)
+            o.monitor_tmax = b["mt"].ctypes.data_as(_abi._dp)
+            o.ledger = b["led"].ctypes.data_as(_abi._dp)
+            o.max_ledger = 4096
+            if want_state:
+                o.depth = b["h"].ctypes.data_as(_abi._fp)
+                o.qx = b["qx"].ctypes.data_as(_abi._fp)
+                o.qy = b["qy"].ctypes.data_as(_abi._fp)
+            bufs.append((b, len(mons)))
+            k += 1
+    return outs, bufs
+
+
+def _collect_results(outs, bufs):
+    res = []
+    for i in range(len(bufs)):
+        o = outs[i]
+        b, nm = bufs[i]
+        res.append(GpuRun(b["mm"][:nm].copy(), b["mt"][:nm].copy(), o.steps, o.t_end, o.volume_m3, b.get("h"),
+                          b.get("qx"), b.get("qy"), bool(o.bad), o.cell_updates, o.rows_skipped, o.dt_min,
+                          o.dt_max, o.dt_mean, o.lts, o.v0, o.rel_residual, o.aborted, o.abort_cell,
+                          o.abort_sync, o.abort_value, o.sync_count, b["led"][:min(o.n_ledger, 4096)].copy(),
+                          o.dt_fallback_t))
+    return res
+
+
+def _sync_fn(on_sync):
+    """The Observer callback (itr_sync_fn) around a Python function of (grid, cand, t,
+    step, sync_index, monitor_max) that returns None / False to continue or an abort value."""
+    def cb(user, grid, cand, t, step, sync_index, mon_ptr, n_mon, abort_value):
+        mon = np.ctypeslib.as_array(mon_ptr, shape=(n_mon,)).copy() if n_mon > 0 else np.zeros(0)
+        r = on_sync(int(grid), int(cand), float(t), int(step), int(sync_index), mon)
+        if r is None or r is False:
+            return 0
+        abort_value[0] = float(r)
+        return 1
+    return _abi.SYNC_FN(cb)
 
 
 class BatchSession:
@@ -513,16 +584,7 @@ class BatchSession:
         b = C.c_void_p()
         _check(L.itr_hip_batch_create(solver._h, C.byref(q), grids, len(groups), C.byref(b), err, 1024), err)
         self._b = b
-        self._cb = None
-        if on_sync is not None:
-            def cb(user, grid, cand, t, step, sync_index, mon_ptr, n_mon, abort_value):
-                mon = np.ctypeslib.as_array(mon_ptr, shape=(n_mon,)).copy() if n_mon > 0 else np.zeros(0)
-                r = on_sync(int(grid), int(cand), float(t), int(step), int(sync_index), mon)
-                if r is None or r is False:
-                    return 0
-                abort_value[0] = float(r)
-                return 1
-            self._cb = _abi.SYNC_FN(cb)
+        self._cb = _sync_fn(on_sync) if on_sync is not None else None
 
     def run(self):
         err = C.create_string_buffer(1024)
@@ -544,41 +606,10 @@ class BatchSession:
         return self.stats
 
     def results(self, want_state=True):
-        n = sum(len(g[1]) for g in self.groups)
-        outs = (_abi.RunOutC * max(n, 1))()
-        bufs = []
-        k = 0
-        for (base, terrs, mons, _), (gnx, gny) in zip(self.groups, self._shapes):
-            for _ in terrs:
-                nm = max(len(mons), 1)
-                b = {"mm": np.zeros(nm), "mt": np.zeros(nm), "led": np.zeros((4096, 8))}
-                if want_state:
-                    b["h"] = np.zeros((gny, gnx), np.float32)
-                    b["qx"] = np.zeros((gny, gnx), np.float32)
-                    b["qy"] = np.zeros((gny, gnx), np.float32)
-                o = outs[k]
-                o.monitor_max = b["mm"].ctypes.data_as(_abi._dp)
-                o.monitor_tmax = b["mt"].ctypes.data_as(_abi._dp)
-                o.ledger = b["led"].ctypes.data_as(_abi._dp)
-                o.max_ledger = 4096
-                if want_state:
-                    o.depth = b["h"].ctypes.data_as(_abi._fp)
-                    o.qx = b["qx"].ctypes.data_as(_abi._fp)
-                    o.qy = b["qy"].ctypes.data_as(_abi._fp)
-                bufs.append((b, len(mons)))
-                k += 1
+        outs, bufs = _result_buffers(self.groups, self._shapes, want_state)
         err = C.create_string_buffer(1024)
         _check(self._L.itr_hip_batch_results(self._b, outs, err, 1024), err)
-        res = []
-        for i in range(n):
-            o = outs[i]
-            b, nm = bufs[i]
-            res.append(GpuRun(b["mm"][:nm].copy(), b["mt"][:nm].copy(), o.steps, o.t_end, o.volume_m3, b.get("h"),
-                              b.get("qx"), b.get("qy"), bool(o.bad), o.cell_updates, o.rows_skipped, o.dt_min,
-                              o.dt_max, o.dt_mean, o.lts, o.v0, o.rel_residual, o.aborted, o.abort_cell,
-                              o.abort_sync, o.abort_value, o.sync_count, b["led"][:min(o.n_ledger, 4096)].copy(),
-                              o.dt_fallback_t))
-        return res
+        return _collect_results(outs, bufs)
 
     def close(self):
         if getattr(self, "_b", None):

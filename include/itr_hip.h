@@ -295,6 +295,25 @@ typedef struct {
     double tau_m;
 } itr_objective;
 
+/* Earthwork term of one edit: TerrainEdit::cut_m3 / fill_m3 (crates/itr-core/src/design.rs:372-379)
+ * and the ObjectiveDef costs and weight lambda (objective.rs:35-45). */
+typedef struct {
+    double cut_m3, fill_m3;
+    double cut_cost, fill_cost;
+    double earth_weight;
+} itr_earthwork;
+
+/* Objective::evaluate's J (objective.rs:95-114) of one candidate from its monitor maxima
+ * (monitor_max[n_mon], positions as in itr_objective), which with running maxima is
+ * Objective::running_lower_bound (objective.rs:146-149), the incumbent abort's bound
+ * (crates/itr-opt/src/evaluator.rs:100-105).  Host only (no device work): a host without
+ * the reference's Objective (this library's Python mirror, sync callbacks) uses it, and the
+ * adjoint computes its J_risk with the same code (include/itr_objective.h).  earth may be
+ * NULL (no earthwork term); j_risk_out and peaks_out[n_assets] may be NULL. */
+itr_status itr_hip_objective_j(const itr_objective* obj, const itr_earthwork* earth, const double* monitor_max,
+                               int32_t n_mon, double* j_out, double* j_risk_out, double* peaks_out, char* err,
+                               size_t errlen);
+
 typedef struct {
     double j_risk;                /* J_risk of the forward run */
     double* peak_m;               /* [n_assets] asset peaks, or NULL */

@@ -78,6 +78,11 @@ class ObjectiveC(C.Structure):
                 ("threshold_m", _dp), ("weight", _dp), ("tau_m", C.c_double)]
 
 
+class EarthworkC(C.Structure):
+    _fields_ = [("cut_m3", C.c_double), ("fill_m3", C.c_double), ("cut_cost", C.c_double), ("fill_cost", C.c_double),
+                ("earth_weight", C.c_double)]
+
+
 class AdjointOutC(C.Structure):
     _fields_ = [("j_risk", C.c_double), ("peak_m", _dp), ("dJ_dz_face_x", _dp), ("dJ_dz_face_y", _dp),
                 ("dJ_dz_cell", _dp), ("steps", C.c_uint64), ("segment_steps", C.c_int32), ("checkpoints", C.c_int32),
@@ -96,7 +101,7 @@ EXPORTS = ["itr_hip_create", "itr_hip_destroy", "itr_hip_device_info", "itr_hip_
            "itr_hip_run_batch_noalloc",
            "itr_hip_batch_create", "itr_hip_batch_run", "itr_hip_batch_results", "itr_hip_batch_stats",
            "itr_hip_batch_destroy", "itr_hip_batch_advance", "itr_hip_adjoint",
-           "itr_hip_dt_schedule", "itr_hip_autotune", "itr_hip_math_selfcheck"]
+           "itr_hip_dt_schedule", "itr_hip_autotune", "itr_hip_math_selfcheck", "itr_hip_objective_j"]
 
 _lib = None
 
@@ -146,6 +151,9 @@ def load():
                                    C.POINTER(TuneResultC), C.c_char_p, C.c_size_t]
     L.itr_hip_math_selfcheck.restype = C.c_int32
     L.itr_hip_math_selfcheck.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
+    L.itr_hip_objective_j.restype = C.c_int32
+    L.itr_hip_objective_j.argtypes = [C.POINTER(ObjectiveC), C.POINTER(EarthworkC), _dp, C.c_int32, _dp, _dp, _dp,
+                                      C.c_char_p, C.c_size_t]
     L.itr_hip_batch_destroy.restype = None
     L.itr_hip_batch_destroy.argtypes = [vp]
     _lib = L

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "../../include/itr_hip.h"
+#include "../../include/itr_objective.h"
 #include "itr_numerics.h"
 #include "itr_adjoint.h"
 
@@ -4175,6 +4176,24 @@ itr_status itr_hip_run_batch_noalloc(itr_ctx* ctx, const itr_solver_params* p, c
     return s;
 }
 
+itr_status itr_hip_objective_j(const itr_objective* obj, const itr_earthwork* earth, const double* monitor_max,
+                               int32_t n_mon, double* j_out, double* j_risk_out, double* peaks_out, char* err,
+                               size_t errlen) {
+    if (!obj || !j_out || (n_mon > 0 && !monitor_max) || obj->n_assets < 0 ||
+        (obj->n_assets > 0 && (!obj->asset_start || !obj->threshold_m || !obj->weight))) {
+        set_err(err, errlen, "null argument");
+        return ITR_EINVAL;
+    }
+    for (int a = 0; a < obj->n_assets; a++)
+        for (int q = obj->asset_start[a]; q < obj->asset_start[a + 1]; q++)
+            if (!obj->asset_pos || obj->asset_pos[q] < 0 || obj->asset_pos[q] >= n_mon) {
+                set_err(err, errlen, "asset position out of the monitor list");
+                return ITR_EINVAL;
+            }
+    *j_out = itr_objective_value(obj, monitor_max, earth, peaks_out, j_risk_out);
+    return ITR_OK;
+}
+
 itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,
                            itr_adjoint_out* outs, char* err, size_t errlen) {
     if (!ctx || !p || !grid || !obj || !outs) {
@@ -4352,17 +4371,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             ADJ_TRY(hipMemcpy(mm.data(), D.mon_max + off, sizeof(float) * nm, hipMemcpyDeviceToHost));
             ADJ_TRY(hipMemcpy(mt.data(), D.mon_tmax + off, sizeof(double) * nm, hipMemcpyDeviceToHost));
         }
+        std::vector<double> md(mm.begin(), mm.begin() + std::max(nm, 1));
+        o.j_risk = itr_j_risk(obj, md.data(), o.peak_m);
         for (int a = 0; a < obj->n_assets; a++) {
-            double pk = 0.0;
-            int kstar = -1;
-            for (int qq = obj->asset_start[a]; qq < obj->asset_start[a + 1]; qq++) {
-                const int kk = obj->asset_pos[qq];
-                if ((double)mm[kk] > pk) { pk = (double)mm[kk]; kstar = kk; }
-            }
+            int32_t kstar = -1;
+            const double pk = itr_asset_peak(obj, a, md.data(), &kstar);
             const double x = (pk - obj->threshold_m[a]) / obj->tau_m;
-            const double sp = x > 30.0 ? x : (x < -30.0 ? std::exp(x) : std::log1p(std::exp(x)));
-            o.j_risk += obj->weight[a] * obj->tau_m * sp;
-            if (o.peak_m) o.peak_m[a] = pk;
             if (kstar < 0) continue;   // never wet: the peak does not move to first order
             uint64_t nstar = 0;
             for (uint64_t i = 0; i < N; i++)

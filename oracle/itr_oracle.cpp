@@ -15,18 +15,22 @@
 //   crates/itr-hydro/src/terrain.rs      FineTerrain, TerrainBase, apply_edit
 //   crates/itr-hydro/src/solver.rs       Solver::run_impl
 //   crates/itr-core/src/scenario.rs      Hyetograph
-//   crates/itr-core/src/objective.rs     Objective
+//   crates/itr-core/src/objective.rs     Objective (J and the incumbent lower bound come
+//                                        from include/itr_objective.h, the one
+//                                        implementation of J, pinned by closed forms)
 //   crates/itr-core/src/design.rs        Primitive, materialize
 //   crates/itr-hydro/src/inertial.rs     step_local_inertial (kernel 2, screening physics)
 //
 // Pins of the local-inertial step: the reference's own tests (lake at rest, rain mass,
 // tilted-plane outflow, peak depths within 15 % of HLL), the kinematic normal depth
 // (friction exponent, g n^2 and its face mean), the time step of a still lake and of a
-// steady plane.  Partially unpinned: the value of the theta weighting (0.8) and the
-// max(u, v) of the 2-D CFL rate follow the published formula; no check tells them from
-// nearby alternatives.
+// steady plane, a one-step hand expansion of the theta = 0.8 face update, and the dt of
+// a state whose fastest cells are v-bound (max(u, v) against u, u + v, |(u, v)|).
+// Stage ghosts: a lake at rest at the stage level over a wavy bed, a basin filling from
+// dry to the stage level, and a one-step expansion of a ghost clamped to depth 0.
 
 #include "itr_oracle.h"
+#include "../include/itr_objective.h"
 
 #include <algorithm>
 #include <atomic>
@@ -1090,19 +1094,19 @@ StepStats step_local_inertial(const Prepared<T>& pt, Workspace<T>& ws, const Ste
     return st;
 }
 
-double softplus(double x) { return x > 30.0 ? x : (x < -30.0 ? std::exp(x) : std::log1p(std::exp(x))); }
-
+// Objective::running_lower_bound (objective.rs:146-149) of the observer's edit.
 double objective_j(const orc_observer* o, const double* mon) {
-    double j_risk = 0.0;
-    int off = 0;
-    for (int a = 0; a < o->n_assets; a++) {
-        double pk = 0.0;
-        for (int q = 0; q < o->asset_len[a]; q++) pk = std::fmax(pk, mon[o->asset_pos[off + q]]);
-        off += o->asset_len[a];
-        j_risk += o->asset_w[a] * o->tau * softplus((pk - o->asset_thr[a]) / o->tau);
-    }
-    double j_earth = o->cut_cost * o->edit_cut_m3 + o->fill_cost * o->edit_fill_m3;
-    return j_risk + o->earth_weight * j_earth;
+    std::vector<int32_t> start((size_t)o->n_assets + 1, 0);
+    for (int a = 0; a < o->n_assets; a++) start[a + 1] = start[a] + o->asset_len[a];
+    itr_objective ob{};
+    ob.n_assets = o->n_assets;
+    ob.asset_start = start.data();
+    ob.asset_pos = o->asset_pos;
+    ob.threshold_m = o->asset_thr;
+    ob.weight = o->asset_w;
+    ob.tau_m = o->tau;
+    itr_earthwork ew{o->edit_cut_m3, o->edit_fill_m3, o->cut_cost, o->fill_cost, o->earth_weight};
+    return itr_objective_value(&ob, mon, &ew, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------- solver.rs:250-660

@@ -1,0 +1,160 @@
+"""The named workloads at full size on the GPU, bitwise against the CPU oracle (the
+reference's f32 fused kernel restated), through the C ABI:
+
+* the batched population forward (64 candidates x 120x96, 3 primitives each, the whole
+  60 min storm, guard early abort at 0.01 m against the unedited baseline),
+* the incumbent early abort of the optimizer's GuardObserver
+  (crates/itr-opt/src/evaluator.rs:100-105) driven from a host sync callback,
+* the large batched forward (512x512 at 2 m, 4 primitives per candidate): 8 candidates
+  over 2500 steps bitwise, then all 128 through the storm's first peak with the guard on
+  and a sample of abort decisions checked,
+* the uneven fidelity ladder (256 instances of 45x35 face-crest, 90x70 and 180x140 in one
+  launch per step) with oracle checks of instances of every shape.
+
+The oracle runs candidate-parallel on the host cores (one candidate per thread, as the
+reference's evaluator does, crates/itr-opt/src/evaluator.rs:337-352)."""
+from __future__ import annotations
+
+import copy
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from conftest import assert_run_equal, oracle_params, oracle_terrain, po
+from inverse_terrain_engine_amd import objective as ob
+from inverse_terrain_engine_amd import scenario as sc
+from inverse_terrain_engine_amd import terrain as tr
+
+pytestmark = pytest.mark.gpu
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def oracle_many(base, p, edits, monitors, observers=None, max_steps=0):
+    """The oracle's f32 run of each edit (None = unedited), one per host thread."""
+    ot, op = oracle_terrain(base), oracle_params(p)
+    observers = observers or [None] * len(edits)
+
+    def one(k):
+        e = edits[k]
+        return po.run(ot, op, 32, edit=None if e is None else e.as_dict(), monitors=monitors,
+                      observer=observers[k], max_steps=max_steps)
+    with ThreadPoolExecutor(THREADS) as ex:
+        return list(ex.map(one, range(len(edits))))
+
+
+def test_config2_population_with_guard(solver):
+    w = sc.config2()
+    _say(f"case: config2, {len(w.candidates)} candidates x 120x96, {w.params.duration_s:g} s, guard 0.01 m")
+    base_run = solver.run_batch(w.params, [tr.PreparedTerrain(w.base)], w.monitors)[0]
+    guard = {"baseline_max": base_run.monitor_max, "guard_pos": w.guard_pos, "tol": 0.01}
+    runs = solver.run_batch(w.params, w.candidates, w.monitors, guard=guard)
+    obs = [{"baseline_max": base_run.monitor_max, "guard_pos": w.guard_pos, "tol": 0.01, "require": True}
+           for _ in w.edits]
+    orc = oracle_many(w.base, w.params, w.edits, w.monitors, obs)
+    base_o = oracle_many(w.base, w.params, [None], w.monitors)[0]
+    assert_run_equal(base_run, base_o, "config2 baseline")
+    for i, (r, o) in enumerate(zip(runs, orc)):
+        assert_run_equal(r, o, f"config2 cand {i}")
+    n_abort = sum(r.aborted == 1 for r in runs)
+    _say(f"config2: {n_abort} guard aborts of {len(runs)}, steps {min(r.steps for r in runs)}..{max(r.steps for r in runs)}")
+    assert 0 < n_abort < len(runs), "the workload should exercise both outcomes"
+
+
+def test_incumbent_abort_from_sync_callback(solver):
+    """Elitist incumbent abort: at every sync point the host computes the running lower
+    bound J(running maxima, the edit's cut/fill) with the library's J and aborts the
+    candidate when it reaches the incumbent; equal to the oracle's GuardObserver with
+    has_incumbent (abort sync, steps, state, monitors and the bound, bitwise)."""
+    w = sc.config2(n_cand=12, seed=7)
+    assets = [(list(a), 0.02, 1.0) for a in w.asset_pos]
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, wt) for a, th, wt in assets], 0.01, earth_weight=1e-3)
+    _say("case: incumbent abort, 12 candidates x 120x96")
+    free = solver.run_batch(w.params, w.candidates, w.monitors, want_state=False)
+    js = [objv.j(r.monitor_max, e.cut_m3, e.fill_m3) for r, e in zip(free, w.edits)]
+    inc = float(np.median(js))
+
+    def cb(grid, cand, t, step, sync_index, mon_max):
+        e = w.edits[cand]
+        lb = objv.j(mon_max, e.cut_m3, e.fill_m3)
+        return lb if lb >= inc else None
+
+    runs = solver.run_batch(w.params, w.candidates, w.monitors, on_sync=cb)
+    obs = [{"assets": assets, "tau": 0.01, "earth_weight": 1e-3, "cut_m3": e.cut_m3, "fill_m3": e.fill_m3,
+            "incumbent": inc} for e in w.edits]
+    orc = oracle_many(w.base, w.params, w.edits, w.monitors, obs)
+    for i, (r, o) in enumerate(zip(runs, orc)):
+        assert_run_equal(r, o, f"incumbent cand {i}")
+    n = sum(r.aborted == 2 for r in runs)
+    _say(f"incumbent {inc:.6g}: {n} of {len(runs)} aborted, syncs {[r.abort_sync for r in runs if r.aborted]}")
+    assert 0 < n < len(runs)
+
+
+def test_config5_candidates_bitwise(solver):
+    w = sc.config5(n_cand=8)
+    p = copy.copy(w.params)
+    p.max_steps = 2500
+    _say("case: config5, 8 candidates x 512x512, 2500 steps")
+    runs = solver.run_batch(p, w.candidates, w.monitors)
+    orc = oracle_many(w.base, w.params, w.edits, w.monitors, max_steps=2500)
+    for i, (r, o) in enumerate(zip(runs, orc)):
+        assert_run_equal(r, o, f"config5 cand {i}")
+    d = runs[0].depth
+    _say(f"config5: t_end {runs[0].t_end:.1f} s, max depth {d.max():.3f} m, cells deeper than 1 cm {int((d > 0.01).sum())}")
+    assert d.max() > 0.1 and (d > 0.01).sum() > 1000
+
+
+def test_config5_first_storm_peak_guard_decisions(solver):
+    """All 128 candidates through the storm's first peak (0-3600 s, about 33000 steps;
+    the whole 2 h storm takes about 515000 steps, its thin films holding dt near 1e-4 s)
+    with the guard (0.01 m against the unedited baseline over the same horizon) on the
+    device.  The oracle checks, bitwise, up to eight guard aborts (the earliest) and four
+    candidates that finish: state, steps, monitors and every abort decision."""
+    w = sc.config5()
+    p = copy.copy(w.params)
+    p.duration_s = 3600.0
+    _say("case: config5, 128 candidates x 512x512, 0-3600 s, guard 0.01 m")
+    base_run = solver.run_batch(p, [tr.PreparedTerrain(w.base)], w.monitors, want_state=False)[0]
+    guard = {"baseline_max": base_run.monitor_max, "guard_pos": w.guard_pos, "tol": 0.01}
+    runs = solver.run_batch(p, w.candidates, w.monitors, guard=guard)
+    aborted = [i for i, r in enumerate(runs) if r.aborted == 1]
+    done = [i for i, r in enumerate(runs) if r.aborted == 0]
+    _say(f"config5: {len(aborted)} guard aborts (steps {sorted(runs[i].steps for i in aborted)[:8]}), "
+         f"{len(done)} finished, baseline {base_run.steps} steps")
+    assert all(r.aborted in (0, 1) for r in runs) and len(aborted) + len(done) == 128
+    assert all(runs[i].t_end == p.duration_s for i in done)
+    obs = {"baseline_max": base_run.monitor_max, "guard_pos": w.guard_pos, "tol": 0.01, "require": True}
+    sample = sorted(aborted, key=lambda i: runs[i].steps)[:8] + done[:4]
+    orc = oracle_many(w.base, p, [w.edits[i] for i in sample], w.monitors, [obs] * len(sample))
+    for i, o in zip(sample, orc):
+        assert_run_equal(runs[i], o, f"config5 cand {i} (aborted {runs[i].aborted})")
+    _say(f"config5 oracle checks: {len(sample)} candidates")
+
+
+def test_config3_ladder_all_instances(solver):
+    lad = sc.config3()
+    n = sum(len(g[1]) for g in lad.groups)
+    _say(f"case: config3 ladder, {n} instances in one launch per step")
+    s = solver.session(lad.params, lad.groups)
+    try:
+        s.run()
+        runs = s.results()
+    finally:
+        s.close()
+    k = 0
+    per = 0
+    for gi, (base, cands, mon, _) in enumerate(lad.groups):
+        picks = sorted({0, len(cands) // 2, len(cands) - 1})
+        edits = [lad.edits[k + i] for i in picks]
+        orc = oracle_many(base, lad.params, edits, mon)
+        for i, o in zip(picks, orc):
+            assert_run_equal(runs[k + i], o, f"ladder grid {gi} ({base.nx}x{base.ny}) cand {i}")
+            per += 1
+        k += len(cands)
+    assert k == n == 256 and per >= 9
+    assert all(r.aborted == 0 and r.t_end == lad.params.duration_s for r in runs)

@@ -161,6 +161,43 @@ class Workload:
     designs: list = field(default_factory=list)   # each candidate's primitives
 
 
+def _ring_contains(ring, x, y):
+    """Even-odd crossing test of one ring (crates/itr-core/src/raster.rs:122-138)."""
+    inside = False
+    n = len(ring)
+    if n < 3:
+        return False
+    j = n - 1
+    for i in range(n):
+        xi, yi = ring[i]
+        xj, yj = ring[j]
+        if (yi > y) != (yj > y) and x < (xj - xi) * (y - yi) / (yj - yi) + xi:
+            inside = not inside
+        j = i
+    return inside
+
+
+def rasterize(polys, nx, ny, geo) -> np.ndarray:
+    """Polygons (dicts with "exterior" and "holes" rings of (x, y)) to a [ny][nx] mask by
+    the cell-centre rule (raster.rs:151-177): a cell belongs to a polygon iff its centre
+    lies inside the exterior and outside every hole."""
+    m = np.zeros((ny, nx), bool)
+    for p in polys:
+        ext = [tuple(v) for v in p["exterior"]]
+        holes = [[tuple(v) for v in h] for h in p.get("holes", [])]
+        xs, ys = [v[0] for v in ext], [v[1] for v in ext]
+        c0f, r1f = geo.to_grid(min(xs), min(ys))
+        c1f, r0f = geo.to_grid(max(xs), max(ys))
+        c0, r0 = max(int(math.floor(c0f)), 0), max(int(math.floor(r0f)), 0)
+        c1, r1 = min(max(int(math.ceil(c1f)), 0), nx), min(max(int(math.ceil(r1f)), 0), ny)
+        for r in range(r0, r1):
+            for c in range(c0, c1):
+                x, y = geo.cell_center(c, r)
+                if _ring_contains(ext, x, y) and not any(_ring_contains(h, x, y) for h in holes):
+                    m[r, c] = True
+    return m
+
+
 def asset_and_guard_cells(nx, ny, boxes, guard_cols):
     cells = set()
     assets = []
@@ -173,6 +210,39 @@ def asset_and_guard_cells(nx, ny, boxes, guard_cols):
     mon = np.array(sorted(cells), np.uint32)
     pos = {int(c): i for i, c in enumerate(mon)}
     return mon, np.array([pos[c] for c in guard], np.int32), [[pos[c] for c in a] for a in assets]
+
+
+def example_workload(spec: dict, dem: np.ndarray):
+    """A scenario given as data (the reference's example directory decoded: DEM, its
+    georeferencing, hydrology, boundary segments, protected and guard polygons), assembled
+    as Problem::build does for level 1 (crates/itr-opt/src/problem.rs:103-160): walls where
+    the DEM is not finite, uniform Manning n, assets and guard rasterized by the cell-centre
+    rule, monitors = asset cells and guard cells, the SolverParams defaults
+    (crates/itr-core/src/scenario.rs:334-345, solver.rs:52-75).  Returns (Workload,
+    assets [(positions, threshold_m, weight)], tau_m)."""
+    dem = np.asarray(dem, np.float64)
+    ny, nx = dem.shape
+    g = spec["geo"]
+    geo = GeoTransform(g["origin_x"], g["origin_y"], g["dx"], g["dy"])
+    walls = ~np.isfinite(dem)
+    fine = FineTerrain.new(np.where(walls, 0.0, dem), walls, spec["manning_n"], geo)
+    base = TerrainBase.build(fine, 1, [dict(sg) for sg in spec["boundaries"]])
+    p = SolverParams(float(spec["duration_s"]), float(spec["sync_interval_s"]), rain=mm_h(spec["rain_mm_h"]),
+                     infil_capacity=spec["infil_capacity_mm_h"] / 1000.0 / 3600.0)
+    amasks = [rasterize([a], nx, ny, geo) for a in spec["protected"]]
+    gmask = rasterize(spec["guard"], nx, ny, geo)
+    cells = set(int(k) for k in np.flatnonzero(gmask))
+    for m in amasks:
+        cells.update(int(k) for k in np.flatnonzero(m))
+    mon = np.array(sorted(cells), np.uint32)
+    pos = {int(c): i for i, c in enumerate(mon)}
+    thr = spec["depth_threshold_m"]
+    assets = [(np.array([pos[int(k)] for k in np.flatnonzero(m)], np.int32),
+               a["properties"].get("threshold_m", thr), a["properties"].get("weight", 1.0))
+              for a, m in zip(spec["protected"], amasks)]
+    gpos = np.array([pos[int(k)] for k in np.flatnonzero(gmask)], np.int32)
+    w = Workload("example", base, p, [PreparedTerrain(base)], mon, gpos, [a[0] for a in assets])
+    return w, assets, spec["tau_m"]
 
 
 def config1(cfl=0.9) -> Workload:

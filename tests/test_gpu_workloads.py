@@ -158,3 +158,21 @@ def test_config3_ladder_all_instances(solver):
         k += len(cands)
     assert k == n == 256 and per >= 9
     assert all(r.aborted == 0 and r.t_end == lad.params.duration_s for r in runs)
+
+
+def test_diverted_flood_example(solver):
+    """The reference's examples/diverted-flood no-change run (fixtures decoded from its
+    dem.tif, scenario.toml and polygons): every monitor maximum bitwise equal to the
+    oracle's, so the J the library computes from them equals the oracle's, and both give
+    the published J = 0.0279 and asset peaks 0.033 / 0.062 m (README.md:15,90)."""
+    from test_oracle import diverted_flood
+    spec, w, assets, tau = diverted_flood()
+    _say("case: diverted-flood, 90x70, 2400 s")
+    r = solver.run_batch(w.params, w.candidates, w.monitors)[0]
+    o = oracle_many(w.base, w.params, [None], w.monitors)[0]
+    assert_run_equal(r, o, "diverted-flood")
+    objv = ob.Objective([ob.Asset(a, th, wt) for a, th, wt in assets], tau)
+    j = objv.j(r.monitor_max, 0.0, 0.0)
+    assert j == objv.j(o["monitor_max"], 0.0, 0.0)
+    assert float(f"{j:.3g}") == spec["published"]["j_no_change"]
+    assert [round(p, 3) for p in objv.peaks(r.monitor_max)] == spec["published"]["asset_peaks_m"]

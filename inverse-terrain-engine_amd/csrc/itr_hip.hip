@@ -693,7 +693,7 @@ __device__ __forceinline__ int band_index(int i, int j, int w, int h) {
     return w + mid * (h - 2) + (i - 1);
 }
 
-template <bool CREST, int DEPTH, bool SPECIAL>
+template <bool CREST, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
                                           double t, double t_new, bool sync, int par, bool from_snap, uint64_t step,
                                           uint32_t sync_idx, StepAcc* acc_cur, float* lds) {
@@ -832,7 +832,10 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         v.h = bload(rs_h, cb, so);
         v.qx = bload(rs_h, cb, so + planeb);
         v.qy = bload(rs_h, cb, so + 2 * planeb);
-        v.code_we = edge_wave ? GHOST[2 * nx + (col < 0 ? 0 : ny) + rr] : 1;
+        // The row's west / east boundary codes are wave-uniform: scalar loads, selected per
+        // lane (only the edge lanes use them), so no vector load waits in the march.
+        const int code_w = GHOST[2 * nx + rr], code_e = GHOST[2 * nx + ny + rr];
+        v.code_we = col < 0 ? code_w : code_e;
         return v;
     };
     // Stage / Inflow ghosts over the wall / transmissive transform (boundary.rs:43-47, 66-70).
@@ -942,7 +945,13 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     if (has_m) l_moff[tid] = m_off0;   // read back after the march
 
     int smax_bits = 0;
-    bool bad = false;
+    // Health flag of the band, as a wave mask (ballots keep it in scalar registers).
+    uint64_t bad_mask = 0;
+    // Lanes that update a cell, and per-row lane masks of a comparison: v_cmp straight into
+    // a lane mask (LLVM fcmp predicates: 2 ogt, 12 ult, 8 uno, 14 une), with every lane active.
+    const uint64_t upd_mask = __builtin_amdgcn_ballot_w64(upd);
+    auto lanes_ogt0 = [](float x) -> uint64_t { return __builtin_amdgcn_fcmpf(x, 0.0f, 2); };
+    auto lanes_une0 = [](float x) -> uint64_t { return __builtin_amdgcn_fcmpf(x, 0.0f, 14); };
     // The march down the band.  TL: the tail wave's form, whose lanes are at row j + lrow
     // (their group's row) when the march is at row j; everything else is the same code.
     auto march = [&](auto tag) {
@@ -977,57 +986,39 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         else fn = face_signed<false>(cp.h, vp, ch, vc, DZY(rs_dzy, a0), K);
         if (a0 == 0 && owner && lr == 0) yb_out[col] = fn.m;
 
-        // Row inputs are loaded DEPTH rows ahead of their use (software pipelining; two rows
-        // cover HBM latency at two resident workgroups per CU, one row keeps the register
-        // budget of the crest variants).
-        Cell pre_c = FETCH(a0 + 1), pre_c2{};
-        float pre_dzs = DZY(rs_dzy, a0 + 1), pre_dzs2 = CREST ? DZY(rs_dzy2, a0 + 1) : 0.0f;
-        float pre_dzw = DZX(rs_dzx, a0), pre_dzw2 = CREST ? DZX(rs_dzx2, a0) : 0.0f;
-        float pre_wet = ROW(rs_wet, a0), pre_gn2 = ROW(rs_gn2, a0);
-        float nxt_dzs = 0.0f, nxt_dzs2 = 0.0f, nxt_dzw = 0.0f, nxt_dzw2 = 0.0f, nxt_wet = 0.0f, nxt_gn2 = 0.0f;
-        if (DEPTH == 2) {
-            pre_c2 = FETCH(a0 + 2);
-            nxt_dzs = DZY(rs_dzy, a0 + 2);
-            if (CREST) nxt_dzs2 = DZY(rs_dzy2, a0 + 2);
-            nxt_dzw = DZX(rs_dzx, min(a0 + 1, ny - 1));
-            if (CREST) nxt_dzw2 = DZX(rs_dzx2, min(a0 + 1, ny - 1));
-            nxt_wet = ROW(rs_wet, a0 + 1);
-            nxt_gn2 = ROW(rs_gn2, a0 + 1);
-        }
+        // Row inputs of iteration j: the old state of row j+1, the y-face jumps of row j+1
+        // (south faces of row j), the x-face jumps, wet and g n^2 of row j.
+        struct RowIn {
+            Cell c;
+            float dzs, dzs2, dzw, dzw2, wet, gn2;
+        };
+        auto load_in = [&](int jj) -> RowIn {
+            RowIn q;
+            q.c = FETCH(jj + 1);
+            q.dzs = DZY(rs_dzy, jj + 1);
+            q.dzs2 = CREST ? DZY(rs_dzy2, jj + 1) : 0.0f;
+            q.dzw = DZX(rs_dzx, min(jj, ny - 1));
+            q.dzw2 = CREST ? DZX(rs_dzx2, min(jj, ny - 1)) : 0.0f;
+            q.wet = ROW(rs_wet, jj);
+            q.gn2 = ROW(rs_gn2, jj);
+            return q;
+        };
 
-        const int j_end = TL ? a0 + t_rows : b0;
-        for (int j = a0; j < j_end; j++) {
+        // One row of the march: faces, update, stores, row flags; the row below becomes the
+        // current one.  `prefetch` issues the next row's loads once this row's inputs have
+        // arrived (the scheduling barrier keeps them after that wait: the march's stores make
+        // every wait a full drain of the vector-memory counter, so a load issued before it
+        // would be waited for at once).
+        auto row_step = [&](int j, const RowIn& in, auto&& prefetch) {
             const int jl = j + lr;          // this lane's row
             const bool rv = !TL || jl < b0;   // a row of the band (a tail group's last rows may not be)
-            const Cell cn = fix(pre_c, jl + 1);
+            const Cell cn = fix(in.c, jl + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            prefetch();
+            __builtin_amdgcn_sched_barrier(0);
             const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
-            const float dzs = pre_dzs, dzs2 = pre_dzs2, dzw = pre_dzw, dzw2 = pre_dzw2;
-            const float wet = pre_wet, gn2 = pre_gn2;
-            if (DEPTH == 2) {
-                pre_c = pre_c2;
-                pre_dzs = nxt_dzs;
-                pre_dzs2 = nxt_dzs2;
-                pre_dzw = nxt_dzw;
-                pre_dzw2 = nxt_dzw2;
-                pre_wet = nxt_wet;
-                pre_gn2 = nxt_gn2;
-                pre_c2 = FETCH(j + 3);
-                nxt_dzs = DZY(rs_dzy, j + 3);
-                if (CREST) nxt_dzs2 = DZY(rs_dzy2, j + 3);
-                nxt_dzw = DZX(rs_dzx, min(j + 2, ny - 1));
-                if (CREST) nxt_dzw2 = DZX(rs_dzx2, min(j + 2, ny - 1));
-                nxt_wet = ROW(rs_wet, j + 2);
-                nxt_gn2 = ROW(rs_gn2, j + 2);
-            } else {
-                pre_c = FETCH(j + 2);
-                pre_dzs = DZY(rs_dzy, j + 2);
-                if (CREST) pre_dzs2 = DZY(rs_dzy2, j + 2);
-                pre_dzw = DZX(rs_dzx, min(j + 1, ny - 1));
-                if (CREST) pre_dzw2 = DZX(rs_dzx2, min(j + 1, ny - 1));
-                pre_wet = ROW(rs_wet, j + 1);
-                pre_gn2 = ROW(rs_gn2, j + 1);
-            }
-
+            const float dzs = in.dzs, dzs2 = in.dzs2, dzw = in.dzw, dzw2 = in.dzw2;
+            const float wet = in.wet, gn2 = in.gn2;
             const Vel vn = cell_velocity(nh, nqx, nqy, K);
             Flux fs;
             if (CREST) fs = face_crest<false>(ch, vc, nh, vn, dzs, dzs2, K);
@@ -1076,24 +1067,26 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
                 }
                 l_infil[r * nx + col] = u.infil;
                 smax_bits = max(smax_bits, __float_as_int(u.s));
-                bad = bad || u.bad;
                 if (sync) sbuf[(size_t)jl * P + ccol] = u.s;
             }
+            if constexpr (TL) bad_mask |= __builtin_amdgcn_ballot_w64(upd && rv && u.bad);
+            else   // u.bad = !(h1 >= -h_dry) || qm is NaN
+                bad_mask |= (__builtin_amdgcn_fcmpf(u.h1, -K.h_dry, 12) | __builtin_amdgcn_fcmpf(u.qm, u.qm, 8)) & upd_mask;
             if (edge_wave) {
                 if (col == 0 && gl >= 1 && rv) l_bw[r] = fw.m;
                 if (col == nx && gl >= 1 && rv) l_be[r] = fw.m;
             }
             if constexpr (TL) {   // the groups' rows differ: per-lane row flags
                 if (upd && rv && u.h > 0.0f) atomicOr(&l_wet[r], 1u);
-                if (__builtin_expect(__ballot(upd && rv && u.fix != 0.0f) != 0ull, 0)) {
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(upd && rv && u.fix != 0.0f) != 0ull, 0)) {
                     if (upd && rv) fixbuf[(size_t)jl * P + ccol] = u.fix;
                     if (upd && rv && u.fix != 0.0f) atomicOr(&l_fixmask[r], 1u << wave);
                 }
                 if (b0 == ny && owner && jl == ny - 1) yb_out[nx + col] = fs.m;
             } else {
-                const unsigned long long wb = __ballot(upd && u.h > 0.0f);
+                const uint64_t wb = lanes_ogt0(u.h) & upd_mask;
                 if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
-                if (__builtin_expect(__ballot(upd && u.fix != 0.0f) != 0ull, 0)) {   // rare: a cell went negative
+                if (__builtin_expect((lanes_une0(u.fix) & upd_mask) != 0ull, 0)) {   // rare: a cell went negative
                     if (upd) fixbuf[(size_t)j * P + ccol] = u.fix;
                     if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
                 }
@@ -1103,7 +1096,18 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             cqy = nqy;
             vc = vn;
             fn = fs;
+        };
+
+        // Two input sets in ping-pong: a row's loads are issued one row of work ahead, and
+        // the unrolled pair needs no register copies to rotate the pipeline.
+        const int j_end = TL ? a0 + t_rows : b0;
+        RowIn in0 = load_in(a0), in1;
+        int j = a0;
+        for (; j + 1 < j_end; j += 2) {
+            row_step(j, in0, [&] { in1 = load_in(j + 1); });
+            row_step(j + 1, in1, [&] { in0 = load_in(j + 2); });   // (clamped loads; unused past the band)
         }
+        if (j < j_end) row_step(j, in0, [] {});
         if constexpr (!TL) {
             if (b0 == ny && owner) yb_out[nx + col] = fn.m;
             if (SPECIAL && rec && ring_owner && b0 == ny) ring_put(ny, ch, cqx, cqy);   // the south ghost row
@@ -1129,7 +1133,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     // Reductions of the band: CFL rate max (as ordered int bits; -0.0 never wins over
     // +0.0) and the health flag.
     smax_bits = wave_max_i32(smax_bits);
-    const bool bad_any = __any(bad);
+    const bool bad_any = bad_mask != 0ull;
     if (lane == 0) {
         l_red[wave] = smax_bits;
         l_red[kMaxWaves + wave] = bad_any ? 1 : 0;
@@ -2012,13 +2016,13 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const uint32_t sync_idx = sS.sync_index;
         if (MODES == 3) {
             if (G.crest)
-                band_step<true, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx, acc_cur,
+                band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx, acc_cur,
                                             lds);
             else
-                band_step<false, 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx,
+                band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx,
                                              acc_cur, lds);
         } else {
-            band_step<MODES == 2, MODES == 1 ? 2 : 1, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
+            band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
                                                               step, sync_idx, acc_cur, lds);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
@@ -3092,6 +3096,12 @@ itr_status set_params(const itr_solver_params* p, DevParams& P, int& max_ledger,
     P.h_dry = (float)p->h_dry;
     P.heps2 = (float)(p->h_eps * p->h_eps);
     P.q_min = (float)1e-30;
+    // The step's short reciprocal of the velocity denominator h^2 + max(h^2, h_eps^2) relies
+    // on h_eps^2 being a normal f32 (h_eps >= about 1.1e-19 m; the reference's default is 1e-5).
+    if (!(P.heps2 >= FLT_MIN)) {
+        set_err(err, errlen, "h_eps = %g: h_eps^2 is below the normal f32 range", p->h_eps);
+        return ITR_DECLINED;
+    }
     double *rt = nullptr, *rr = nullptr;
     itr_status s;
     if ((s = dev_alloc(allocs, &rt, (size_t)p->n_rain, err, errlen)) != ITR_OK) return s;

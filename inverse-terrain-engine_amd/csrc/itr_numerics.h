@@ -62,6 +62,19 @@ __device__ __forceinline__ float crecip(float x) {
     return q;
 }
 
+// 1/x for operands the step bounds below: x >= 2^-126 whenever x is not NaN (the
+// velocity denominator h^2 + max(h^2, h_eps^2) >= h_eps^2, which the host requires to be a
+// normal f32; the friction denominator 1 + dt g n^2 |q| r^7 >= 1).  Then one comparison
+// separates the verified range [2^-126, 2^126) from NaN and values at or above 2^126,
+// which take the IEEE operation.
+__device__ __forceinline__ float crecip_lower_bounded(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    float q = __builtin_fmaf(e, r, r);
+    if (__builtin_expect(!(x < 0x1p126f), 0)) q = 1.0f / x;
+    return q;
+}
+
 // x^(-1/3): integer seed then three multiplication-only Newton steps (real.rs:116-129).
 __device__ __forceinline__ float inv_cbrt(float x) {
     const uint32_t seed = 0x54A23280u - __float_as_uint(x) / 3u;
@@ -79,7 +92,9 @@ __device__ __forceinline__ float inv_cbrt(float x) {
 // Desingularised velocity and celerity of one cell (numerics.rs:36-43).
 __device__ __forceinline__ Vel cell_velocity(float h, float qx, float qy, const StepK& k) {
     const float h2 = h * h;
-    const float inv = crecip(h2 + smax(h2, k.heps2));
+    // smax(h2, heps2): h2 is an arithmetic result (no signalling NaN) and never -0, and
+    // heps2 > 0, so the hardware max (NaN -> heps2) is the same select.
+    const float inv = crecip_lower_bounded(h2 + __builtin_fmaxf(h2, k.heps2));
     const float t = (h + h) * inv;
     Vel o;
     o.u = qx * t;
@@ -128,9 +143,15 @@ __device__ __forceinline__ Flux hll_flux(float hl, float ul, float tl, float cl,
 template <bool NORMAL_X>
 __device__ __forceinline__ Flux face_signed(float hl, const Vel& vl, float hr, const Vel& vr, float dz,
                                             const StepK& k) {
+    // smax(dz, 0) and smax(-dz, 0) as selects on pos (bit selects, so that they stay one
+    // instruction each).  For every non-NaN dz (bed jumps are finite) they give the
+    // reference's values, except smax(-dz, 0) = -0 instead of +0 at dz = 0, which
+    // smax(hr - rise_r, 0) below cannot tell apart.
     const bool pos = dz > 0.0f;
-    const float hsl = smax(hl - smax(dz, 0.0f), 0.0f);
-    const float hsr = smax(hr - smax(-dz, 0.0f), 0.0f);
+    const float rise_l = __int_as_float(pos ? __float_as_int(dz) : 0);
+    const float rise_r = pos ? 0.0f : -dz;
+    const float hsl = smax(hl - rise_l, 0.0f);
+    const float hsr = smax(hr - rise_r, 0.0f);
     const float hred = pos ? hsl : hsr;
     const float cred = csqrt(k.g * hred);
     const float csl = pos ? cred : vl.c;
@@ -153,6 +174,7 @@ __device__ __forceinline__ Flux face_crest(float hl, const Vel& vl, float hr, co
 
 struct CellNew {
     float h, qx, qy, s, infil, fix;
+    float h1, qm;   // the depth after the flux divergence and |q| (the health test's operands)
     bool bad;
 };
 
@@ -165,6 +187,7 @@ __device__ __forceinline__ CellNew cell_update(float h, float qx, float qy, floa
     const float qx1 = qx - k.dtdx * (enl - wnr) - k.dtdy * (st - nt);
     const float qy1 = qy - k.dtdx * (et - wt) - k.dtdy * (snl - nnr);
     CellNew o;
+    o.h1 = h1;
     bool bad = !(h1 >= -k.h_dry);
     o.fix = h1 < 0.0f ? -h1 : 0.0f;
     if (h1 < 0.0f) h1 = 0.0f;
@@ -176,7 +199,7 @@ __device__ __forceinline__ CellNew cell_update(float h, float qx, float qy, floa
     const float r3 = r * r * r;
     const float r7 = r3 * r3 * r;
     const float qm = csqrt(qx1 * qx1 + qy1 * qy1);
-    const float inv = crecip(1.0f + k.dt * gn2 * qm * r7);
+    const float inv = crecip_lower_bounded(1.0f + k.dt * gn2 * qm * r7);
     const float keep = dry ? 0.0f : 1.0f;
     float qx2 = qx1 * inv * keep;
     float qy2 = qy1 * inv * keep;
@@ -185,9 +208,14 @@ __device__ __forceinline__ CellNew cell_update(float h, float qx, float qy, floa
     const float c = csqrt(k.g * h3);
     o.s = (sabs(qx2) * r3 + c) * k.inv_dx + (sabs(qy2) * r3 + c) * k.inv_dy;
     bad = bad || !(qm == qm);
-    o.h = canon(h3);
-    o.qx = canon(qx2);
-    o.qy = canon(qy2);
+    o.qm = qm;
+    // canon_zero (real.rs:56-58) of the new state is the identity here, so it is not
+    // evaluated: h3 is never -0 (the old depth is never -0, x - y is -0 only for x = -0,
+    // and the rain and infiltration terms keep that), and qx2 / qy2 are +0 or at least
+    // q_min in magnitude (or NaN) after the q_min test.
+    o.h = h3;
+    o.qx = qx2;
+    o.qy = qy2;
     o.bad = bad;
     return o;
 }

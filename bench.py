@@ -3,8 +3,11 @@
 A "step" is one time step of every candidate of the batch: one launch of the HIP step
 kernel over 128 candidates of a 512x512 grid at 2 m (the large batched-forward
 workload; seeded DEM with 40 hills and 8 gullies, 4 primitives per candidate, bimodal
-2 h storm).  W untimed warmup steps, then exactly K timed steps with the inputs already
-resident in HBM.  Prints one JSON line.
+2 h storm), with the guard early abort against the unedited baseline on.  The batch
+first runs untimed from t = 0 to about the storm peak (reported as the whole-run figure,
+early aborts and host round trips included), then W untimed warmup steps, then exactly
+K timed steps with the inputs already resident in HBM.  Prints one JSON line (progress
+on stderr).
 
 The cpu_baseline leg times the CPU oracle (the reference's f32 fused kernel restated in
 C++, bitwise equal to it) candidate-parallel on the host cores, one candidate per thread
@@ -14,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -104,7 +108,8 @@ def _pmc_pass(counter, steps, band_rows):
     out = tempfile.mkdtemp(prefix="itr_pmc_", dir="/tmp")
     cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc",
            "--", sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", "2", "--no-cpu",
-           "--no-traffic", "--copy-probe", "--no-adjoint", "--no-ladder", "--band-rows", str(band_rows)]
+           "--no-traffic", "--copy-probe", "--no-adjoint", "--no-ladder", "--band-rows", str(band_rows),
+           "--advance", "0", "--guard-tol", "0"]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
     path = None
@@ -124,10 +129,13 @@ def _pmc_pass(counter, steps, band_rows):
     return vals
 
 
-def measure_traffic(kernel_us, band_rows, steps=6):
-    """HBM bytes per step launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
-    scaled by what the same counter reads for a plain 1 GiB device copy; and the VALU
-    instruction count per launch (SQ_INSTS_VALU, wave64 instructions)."""
+def measure_traffic(band_rows, cells, steps=6):
+    """HBM bytes and VALU instructions per cell-step of the step kernel, from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch, each scaled by what the same counter reads for a plain
+    1 GiB device copy) and SQ_INSTS_VALU (wave64 instructions), in counter passes over a
+    short run of the same batch from t = 0 (every candidate running: `cells` cells per
+    launch).  Counting the timed window itself would need the untimed advance in every
+    pass; the per-cell-step figures do not depend on the window."""
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
         v = _pmc_pass(counter, steps, band_rows)
@@ -144,11 +152,10 @@ def measure_traffic(kernel_us, band_rows, steps=6):
         return None
     kf = COPY_PROBE_BYTES / (f_copy * 1024.0)
     kw = COPY_PROBE_BYTES / (w_copy * 1024.0)
-    fetch_b = f_kib * 1024.0 * kf
-    write_b = w_kib * 1024.0 * kw
-    valu = res["SQ_INSTS_VALU"][0]
-    return {"fetch_bytes": fetch_b, "write_bytes": write_b, "copy_scale_fetch": kf, "copy_scale_write": kw,
-            "valu_insts": valu, "valu_frac": valu / (kernel_us * 1e-6 * VALU_PEAK)}
+    return {"fetch_per_cell_step": f_kib * 1024.0 * kf / cells, "write_per_cell_step": w_kib * 1024.0 * kw / cells,
+            "copy_scale_fetch": kf, "copy_scale_write": kw,
+            "valu_lane_insts_per_cell_step": res["SQ_INSTS_VALU"][0] * 64 / cells,
+            "note": f"counter passes over {steps} steps from t = 0, all candidates running"}
 
 
 def copy_probe():
@@ -203,17 +210,31 @@ def adjoint_config4(solver, sc, ob):
             "replay_ms": res["replay_ms"], "reverse_ms": res["reverse_ms"]}
 
 
-def adjoint_best16(solver, sc, ob, w, res, horizon):
-    """Adjoint of the 16 best candidates of the batched forward (J from its monitor
-    maxima), on the best candidate's dt schedule over a bounded horizon."""
+def adjoint_best16(solver, sc, ob, w, res, guard, horizon, t_horizon=1800.0):
+    """Gradient refinement of the batch's incumbents: the adjoint of the 16 best feasible
+    candidates (not guard-aborted; ranked by the running lower bound of J from the batch's
+    monitor maxima, objective.rs:146-149) on the best one's dt schedule from t = 0 to
+    t_horizon (through the first storm peak, so that the assets pond), and a central-
+    difference check of its primitive-height gradient with device forward runs."""
+    import copy
     import numpy as np
+    from inverse_terrain_engine_amd import terrain as tr
     base = w.base
-    area = base.dx * base.dy
+    nx, ny = base.nx, base.ny
     objv = ob.Objective([ob.Asset(np.asarray(a), 0.05, 1.0) for a in w.asset_pos], 0.01, 1e-3)
     js = [objv.j(r.monitor_max, e.cut_m3, e.fill_m3) for r, e in zip(res, w.edits)]
-    best = [int(i) for i in np.argsort(js)[:16]]
-    import copy
+    feasible = [i for i, r in enumerate(res) if r.aborted == 0]
+    best = sorted(feasible, key=lambda i: js[i])[:16]
+    # The gradient check needs a candidate whose primitives reach the assets: the feasible
+    # one with a primitive nearest an asset box takes the 16th place if it is not among them.
+    geo = base.fine.geo
+    centres = [geo.cell_center(c0 + bw / 2.0 - 0.5, r0 + bh / 2.0 - 0.5) for c0, r0, bw, bh in w.asset_boxes]
+    dist = lambda i: min(math.hypot(pr.cx - x, pr.cy - y) for pr in w.designs[i] for x, y in centres)
+    near = min(feasible, key=dist)
+    if near not in best:
+        best = best[:15] + [near]
     p = copy.copy(w.params)   # (the workload's own parameters serve the later legs)
+    p.duration_s = t_horizon
     p.cfl_hard = 1.0
     sched = solver.dt_schedule(p, base, w.candidates[best[0]], w.monitors, max_steps=horizon)
     p.locked_dt = sched
@@ -224,11 +245,41 @@ def adjoint_best16(solver, sc, ob, w, res, horizon):
     out = solver.adjoint(p, base, cands, w.monitors, assets, objv.tau_m)
     r0 = out[0]
     total = r0["forward_ms"] + r0["replay_ms"] + r0["reverse_ms"]
-    return {"workload": f"16 best of the batch, {base.nx}x{base.ny}, first {len(sched)} steps on the best one's dt",
+    # d J_risk / d height of the best candidate's primitives: the adjoint through the 4x4
+    # quadrature profile, against central differences of device f32 forward runs on the
+    # same locked schedule (two steps; f32 limits how small a step can be).
+    k_near = best.index(near)
+    prims = w.designs[near]
+    jac = tr.materialize_jacobian(prims, geo, nx, ny, None, 1.0)
+    grad = [float(np.sum(out[k_near]["dJ_dz_cell"] * J)) for J in jac]
+    fd = {}
+    for step in (1e-2, 3e-3):
+        terrs = []
+        for k in range(len(prims)):
+            for sgn in (1.0, -1.0):
+                q = [copy.copy(pr) for pr in prims]
+                q[k].height += sgn * step
+                terrs.append(tr.PreparedTerrain(base).apply_edit(tr.materialize(q, geo, nx, ny, None, 1.0)))
+        runs = solver.run_batch(p, terrs, w.monitors, want_state=False)
+        jr = [objv.j_risk(r.monitor_max) for r in runs]
+        fd[step] = [(jr[2 * k] - jr[2 * k + 1]) / (2.0 * step) for k in range(len(prims))]
+    scale = max(abs(g) for g in grad) or 1.0
+    rel = lambda a, b: abs(a - b) / max(abs(b), 1e-3 * scale)
+    return {"workload": f"16 best feasible candidates of the batch, {nx}x{ny}, t = 0 - {t_horizon:g} s "
+                        f"({len(sched)} steps on the best one's dt schedule)",
             "steps": r0["steps"], "cost_vs_forward": total / r0["plain_forward_ms"],
             "replay_mismatches": r0["replay_mismatches"], "plain_forward_ms": r0["plain_forward_ms"],
             "forward_ms": r0["forward_ms"], "replay_ms": r0["replay_ms"], "reverse_ms": r0["reverse_ms"],
-            "cell_steps_per_s_reverse": 16 * base.nx * base.ny * r0["steps"] / (r0["reverse_ms"] * 1e-3)}
+            "cell_steps_per_s_reverse": 16 * nx * ny * r0["steps"] / (r0["reverse_ms"] * 1e-3),
+            "best_asset_peaks_m": [float(x) for x in r0["peak"]], "best_j_risk": r0["j_risk"],
+            "fd_check": {"what": "d J_risk / d height of the primitives of the candidate with a primitive nearest "
+                                 "an asset: adjoint vs central differences of device f32 forward runs on the same "
+                                 "locked schedule",
+                         "candidate": near, "asset_peaks_m": [float(x) for x in out[k_near]["peak"]],
+                         "j_risk": out[k_near]["j_risk"], "nearest_primitive_m": dist(near),
+                         "adjoint": grad, "fd_1e-2": fd[1e-2], "fd_3e-3": fd[3e-3],
+                         "rel_err_vs_fd_3e-3": [rel(g, f) for g, f in zip(grad, fd[3e-3])],
+                         "fd_steps_rel_diff": [rel(a, b) for a, b in zip(fd[1e-2], fd[3e-3])]}}
 
 
 def prep_leg(solver, sc, tr, w):
@@ -285,11 +336,13 @@ def ladder(solver_cls, sc, steps):
             "band_rows": [r for r, _ in tuned], "steps": steps}
 
 
-def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_cands=16, cpu_steps=300):
+def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_cands=16, cpu_steps=300, t_wet=1800.0):
     """The same candidates on the face-crest level-2 grid (2x coarser), run with the
     local-inertial screening physics (inertial.rs; problem.rs:178-181) and with HLL.
     Reports device rates and simulated seconds per step (the screening physics takes
-    larger steps), and the oracle's local-inertial f32 rate on the host cores."""
+    larger steps) in two windows, the early mostly dry one and one at the first storm peak
+    (t_wet), the steps both physics take to reach t_wet, and the oracle's local-inertial
+    f32 rate on the host cores."""
     import copy
     import numpy as np
     base2 = tr.TerrainBase.build(w.base.fine, 2, w.base.segments)
@@ -298,6 +351,18 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
     mon = np.unique((w.monitors // nx // 2) * base2.nx + (w.monitors % nx) // 2).astype(np.uint32)
     out = {"workload": f"{len(terrs)} candidates x {base2.nx}x{base2.ny} face-crest cells at "
                        f"{base2.dx:g} m (level 2 of the forward workload)", "steps": steps}
+    def window(sess, n):
+        t_a = np.array([r.t_end for r in sess.results(want_state=False)])
+        t0 = time.perf_counter()
+        st = sess.advance(n)
+        wall = time.perf_counter() - t0
+        t_b = np.array([r.t_end for r in sess.results(want_state=False)])
+        kern_s = st["device_ms"] / 1e3 / n
+        cells = st["cell_steps"] / n
+        return {"t_s": [float(t_a.mean()), float(t_b.mean())], "cell_steps_per_s": st["cell_steps"] / wall,
+                "us_per_step": kern_s * 1e6, "sim_s_per_step": float((t_b - t_a).mean()) / n,
+                "sim_s_per_wall_s": float((t_b - t_a).mean()) / wall, "hbm_frac_40B": 40.0 * cells / kern_s / HBM_PEAK}
+
     for kernel in ("inertial", "fused"):
         p = copy.copy(w.params)
         p.kernel = kernel
@@ -305,22 +370,25 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
         tuned = s.autotune(p, [(base2, terrs, mon, None)], trial_steps=16)
         sess = s.session(p, [(base2, terrs, mon, None)])
         sess.advance(warmup)
-        t_a = np.array([r.t_end for r in sess.results(want_state=False)])
-        t0 = time.perf_counter()
-        st = sess.advance(steps)
-        wall = time.perf_counter() - t0
-        t_b = np.array([r.t_end for r in sess.results(want_state=False)])
+        early = window(sess, steps)
+        # through the first storm peak: wall time to reach t_wet from the start of the run
+        n_run, t0 = warmup + steps, time.perf_counter()
+        while min(r.t_end for r in sess.results(want_state=False)) < t_wet:
+            sess.advance(1000)
+            n_run += 1000
+        reach = time.perf_counter() - t0
+        wet = window(sess, steps)
         sess.close()
         s.close()
-        kern_s = st["device_ms"] / 1e3 / steps
-        cells = st["cell_steps"] / steps
         out["local_inertial" if kernel == "inertial" else "hll"] = {
-            "cell_steps_per_s": st["cell_steps"] / wall, "us_per_step": kern_s * 1e6,
-            "sim_s_per_step": float((t_b - t_a).mean()) / steps,
-            "sim_s_per_wall_s": float((t_b - t_a).mean()) / wall,
-            "hbm_frac_40B": 40.0 * cells / kern_s / HBM_PEAK, "band_rows": tuned[0][0]}
+            "band_rows": tuned[0][0], "early": early, "wet": wet,
+            "steps_to_t_wet": n_run, "wall_to_t_wet_s": reach}
+        say(f"screening {kernel}: {n_run} steps to t = {t_wet:g} s")
     li, hl = out["local_inertial"], out["hll"]
-    out["sim_time_speedup_vs_hll"] = li["sim_s_per_wall_s"] / hl["sim_s_per_wall_s"]
+    out["sim_time_speedup_vs_hll"] = {
+        "early": li["early"]["sim_s_per_wall_s"] / hl["early"]["sim_s_per_wall_s"],
+        "wet": li["wet"]["sim_s_per_wall_s"] / hl["wet"]["sim_s_per_wall_s"],
+        "steps_to_t_wet": hl["steps_to_t_wet"] / li["steps_to_t_wet"]}
     if cpu_threads:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po  # the checker, timed here as the CPU baseline only
@@ -334,7 +402,7 @@ def screening_leg(solver_cls, sc, tr, w, steps, warmup=20, cpu_threads=0, cpu_ca
         cw, _, cu = _run_cands(po, ot, params, [e.as_dict() for e in w.edits[:cpu_cands]], cpu_threads, mon)
         out["cpu_local_inertial"] = {"value": float(cu.sum()) / cw, "unit": "cell-steps/s", "cores": cpu_threads,
                                      "sample": f"{cpu_cands} candidates x {cpu_steps} steps, one per thread"}
-        out["gpu_over_cpu"] = li["cell_steps_per_s"] / out["cpu_local_inertial"]["value"]
+        out["gpu_over_cpu"] = li["early"]["cell_steps_per_s"] / out["cpu_local_inertial"]["value"]
     return out
 
 
@@ -393,6 +461,24 @@ def replay_leg(solver_cls, sc, tr, n_cand=128, duration=1800.0):
             "speedup_vs_full_domain": t_full / t_rep}
 
 
+def say(*a):
+    """Progress lines on stderr (the JSON result is the one stdout line)."""
+    print(*a, file=sys.stderr, flush=True)
+
+
+def advance_to(sess, n_steps, chunk=16384, label=""):
+    """Run n_steps step launches of a resident batch (no sync callbacks), in chunks that
+    each report progress; returns (cell-steps run, wall seconds)."""
+    done, cs, t0 = 0, 0.0, time.perf_counter()
+    while done < n_steps:
+        k = min(chunk, n_steps - done)
+        st = sess.advance(k)
+        done += k
+        cs += st["cell_steps"]
+        say(f"{label}: {done}/{n_steps} steps, {cs / (time.perf_counter() - t0) / 1e9:.1f} G cell-steps/s")
+    return cs, time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,12 +486,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cands", type=int, default=128)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--advance", type=int, default=165000,
+                    help="untimed steps before the timed window (165000 reach about t = 5400 s, the storm peak)")
+    ap.add_argument("--guard-tol", type=float, default=0.01, help="guard early-abort tolerance (m); 0: no guard")
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--min-lds", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
-    ap.add_argument("--adjoint-horizon", type=int, default=120)
+    ap.add_argument("--adjoint-horizon", type=int, default=0,
+                    help="steps of the best-16 adjoint (0: to t = 1800 s on the best candidate's schedule)")
     ap.add_argument("--no-ladder", action="store_true", help="skip the fidelity-ladder leg")
     ap.add_argument("--no-screening", action="store_true", help="skip the local-inertial screening leg")
     ap.add_argument("--no-replay", action="store_true", help="skip the subdomain replay leg")
@@ -426,29 +516,43 @@ def main():
     from __graft_entry__ import load_package
     load_package()
     from inverse_terrain_engine_amd import scenario as sc
+    from inverse_terrain_engine_amd import terrain as tr
     from inverse_terrain_engine_amd.gpu import GpuSolver
+    import numpy as np
 
     t0 = time.time()
     w = sc.config5(n_cand=a.cands, n=a.n)
     t_prep = time.time() - t0
     if a.replay_only:
-        from inverse_terrain_engine_amd import terrain as tr
         print(json.dumps(replay_leg(GpuSolver, sc, tr)), flush=True)
         return
     if a.screening_only:
-        from inverse_terrain_engine_amd import terrain as tr
         print(json.dumps(screening_leg(GpuSolver, sc, tr, w, a.steps, warmup=a.warmup)), flush=True)
         return
     solver = GpuSolver(0, band_rows=a.band_rows)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
-    groups = [(w.base, w.candidates, w.monitors, None)]
+    # The guard's baseline: the unedited terrain over the whole storm (objective.rs:132-144).
+    guard = None
+    base_steps = None
+    if a.guard_tol > 0:
+        tb = time.perf_counter()
+        base_run = solver.run_batch(w.params, [tr.PreparedTerrain(w.base)], w.monitors, want_state=False)[0]
+        base_steps = int(base_run.steps)
+        say(f"guard baseline: {base_steps} steps in {time.perf_counter() - tb:.1f} s")
+        guard = {"baseline_max": base_run.monitor_max, "guard_pos": w.guard_pos, "tol": a.guard_tol}
+    groups = [(w.base, w.candidates, w.monitors, guard)]
     tuned = None
     if not a.band_rows and not a.no_autotune:
         tuned = solver.autotune(w.params, groups, trial_steps=16)
     sess = solver.session(w.params, groups)
+    # The whole run up to the timed window: every step from t = 0 with the guard's early
+    # aborts, host round trips between graph chunks included.
+    whole_cs, whole_wall = advance_to(sess, a.advance, label="advance") if a.advance else (0.0, 0.0)
+    mid = sess.results(want_state=False)
     if a.warmup:
         sess.advance(a.warmup)
+    t_a = np.array([r.t_end for r in sess.results(want_state=False)])
     # advance() returns after synchronizing the library's stream (HIP event), so the
     # timed region is bracketed by device synchronization on both sides.
     t1 = time.perf_counter()
@@ -461,12 +565,14 @@ def main():
     achieved = BYTES_PER_CELL_STEP * cells_per_launch / kernel_s
     res = sess.results(want_state=False)
     sess.close()
+    active = [i for i, r in enumerate(res) if r.aborted == 0 and r.t_end < w.params.duration_s]
+    t_b = np.array([r.t_end for r in res])
     adj = adj16 = None
     if not a.no_adjoint and not a.copy_probe:
         from inverse_terrain_engine_amd import objective as ob
         adj = adjoint_config4(solver, sc, ob)
         if a.cands >= 16:
-            adj16 = adjoint_best16(solver, sc, ob, w, res, a.adjoint_horizon)
+            adj16 = adjoint_best16(solver, sc, ob, w, mid if a.advance else res, guard, a.adjoint_horizon)
 
     out = {
         "metric": METRIC,
@@ -482,14 +588,25 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched forward: {a.cands} candidates x {a.n}x{a.n} cells at 2 m, 4 primitives each, "
-                        "seeded DEM (40 hills, 8 gullies), bimodal 2 h hyetograph (50/90 mm/h), infiltration 5 mm/h",
+            "workload": f"batched forward with guard early abort: {a.cands} candidates x {a.n}x{a.n} cells at 2 m, "
+                        "4 primitives each, seeded DEM (40 hills, 8 gullies), bimodal 2 h hyetograph (50/90 mm/h), "
+                        f"infiltration 5 mm/h, guard {a.guard_tol:g} m against the unedited baseline; timed window "
+                        f"after {a.advance} untimed steps",
             "candidates": a.cands, "nx": a.n, "ny": a.n, "band_rows": st["band_rows"],
             "waves_per_task": st["waves_per_task"], "device": solver.adapter_info(),
             "autotuned": tuned is not None,
             "hbm_fraction": achieved / HBM_PEAK, "prep_s": round(t_prep, 2),
-            "steps_run_per_candidate": int(res[0].steps),
+            "timed_window_t_s": [float(t_a.min()), float(t_b.max())],
+            "candidates_running_in_window": len(active),
+            "guard_aborted": sum(1 for r in res if r.aborted == 1),
+            "guard_baseline_steps": base_steps,
         },
+        "whole_run": {
+            "what": f"the same batch from t = 0 through the {a.advance} untimed steps (to t = "
+                    f"{float(np.mean([r.t_end for r in mid])):.0f} s on average), guard aborts and host round trips "
+                    "between graph chunks included",
+            "cell_steps_per_s": whole_cs / whole_wall if whole_wall else None, "wall_s": whole_wall,
+            "guard_aborted": sum(1 for r in mid if r.aborted == 1)},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved / 1e9,
@@ -505,7 +622,6 @@ def main():
     if not a.no_ladder and not a.copy_probe:
         out["ladder"] = ladder(GpuSolver, sc, 200)
     if not a.copy_probe:
-        from inverse_terrain_engine_amd import terrain as tr
         out["prep"] = prep_leg(solver, sc, tr, w)
         if not a.no_replay:
             out["replay"] = replay_leg(GpuSolver, sc, tr)
@@ -520,18 +636,25 @@ def main():
     if a.copy_probe:
         copy_probe()
     if not a.no_traffic:
-        t = measure_traffic(kernel_s * 1e6, st["band_rows"])
+        t = measure_traffic(st["band_rows"], float(a.cands) * a.n * a.n)
         if t is not None:
             cells = cells_per_launch
-            out["roofline"]["traffic"] = t["fetch_bytes"] + t["write_bytes"]
+            fetch_b, write_b = t["fetch_per_cell_step"] * cells, t["write_per_cell_step"] * cells
+            dram = (fetch_b + write_b) / kernel_s
+            valu = t["valu_lane_insts_per_cell_step"] * cells / 64.0
+            valu_frac = valu / (kernel_s * VALU_PEAK)
+            out["roofline"]["traffic"] = fetch_b + write_b
             out["roofline"]["traffic_detail"] = {
-                "fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
-                "bytes_per_cell_step": (t["fetch_bytes"] + t["write_bytes"]) / cells,
+                "fetch_bytes": fetch_b, "write_bytes": write_b,
+                "bytes_per_cell_step": t["fetch_per_cell_step"] + t["write_per_cell_step"],
                 "copy_scale_fetch": t["copy_scale_fetch"], "copy_scale_write": t["copy_scale_write"],
-                "dram_gbs": (t["fetch_bytes"] + t["write_bytes"]) / kernel_s / 1e9}
+                "dram_gbs": dram / 1e9, "note": t["note"]}
             out["roofline"]["valu"] = {
-                "insts_per_launch": t["valu_insts"], "insts_per_cell_step_lane": t["valu_insts"] * 64 / cells,
-                "peak_insts_per_s": VALU_PEAK, "frac": t["valu_frac"]}
+                "insts_per_launch": valu, "insts_per_cell_step_lane": t["valu_lane_insts_per_cell_step"],
+                "peak_insts_per_s": VALU_PEAK, "frac": valu_frac}
+            # The resource the kernel is closest to (the metric is priced against HBM; the
+            # step also issues VALU instructions at a measured fraction of the peak).
+            out["roofline"]["nearest_limit"] = "valu_issue" if valu_frac > dram / HBM_PEAK else "hbm"
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         v, cw, cs = cpu_baseline(w, threads, a.cpu_cands, a.cpu_steps)

@@ -159,6 +159,7 @@ class Workload:
     asset_pos: list
     edits: list = field(default_factory=list)
     designs: list = field(default_factory=list)   # each candidate's primitives
+    asset_boxes: list = field(default_factory=list)   # (c0, r0, w, h) cell boxes of the assets
 
 
 def _ring_contains(ring, x, y):
@@ -392,7 +393,8 @@ def config5(n_cand=128, seed=5, n=512, duration=7200.0) -> Workload:
                              [{"kind": "transmissive", "side": "east"}])
     p = SolverParams(duration, 60.0, rain=mm_h([[0, 0], [1800, 50], [3600, 10], [5400, 90], [7200, 0]]),
                      infil_capacity=5.0 / 1000.0 / 3600.0)
-    mon, gpos, apos = asset_and_guard_cells(nx, ny, [(200, 200, 8, 8), (300, 350, 8, 8)], range(nx - 16, nx))
+    boxes = [(200, 200, 8, 8), (300, 350, 8, 8)]
+    mon, gpos, apos = asset_and_guard_cells(nx, ny, boxes, range(nx - 16, nx))
     cands, edits, designs = [], [], []
     for _ in range(n_cand):
         prims = random_primitives(rng, 4, geo, nx, ny)
@@ -400,4 +402,4 @@ def config5(n_cand=128, seed=5, n=512, duration=7200.0) -> Workload:
         designs.append(prims)
         edits.append(e)
         cands.append(PreparedTerrain(base).apply_edit(e))
-    return Workload("config5", base, p, cands, mon, gpos, apos, edits, designs)
+    return Workload("config5", base, p, cands, mon, gpos, apos, edits, designs, boxes)

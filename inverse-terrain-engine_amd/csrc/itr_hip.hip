@@ -2804,9 +2804,14 @@ struct GridHost {
 
 }  // namespace
 
+constexpr int kMaxLists = 4;   // launch lists per batch (streams per context)
+
 struct itr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[kMaxLists - 1] = {};   // the streams of launch lists 1.. (default priority)
+    hipEvent_t ev_fork = nullptr;          // fork / join of the lists' streams (no timing)
+    hipEvent_t ev_join[kMaxLists - 1] = {};
     std::string info;
     int band_rows = 0;
     int steps_per_chunk = 0;
@@ -2840,28 +2845,36 @@ int choose_band_rows(const itr_ctx* ctx, const itr_grid* grids, int n_grids, int
 
 }  // namespace
 
+// The step tasks of one class of grids (physics, face mode, ghosts, waves per row), run
+// as one kernel launch per step.  No candidate is in two lists, so the lists of a batch
+// are independent chains of launches: each list after the first runs on a stream of its
+// own, forked from and joined to the context's stream around every run of launches, and
+// a list of small grids does not wait each step for the slowest band of a large one.
+struct LaunchList {
+    KArgs args{};
+    std::vector<int> grids;
+    size_t task_off = 0;            // its slice of the batch's task buffer
+    int ntasks = 0;
+    int block = 64;
+    size_t lds = 0;
+    int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts or tape
+    bool li = false;                // local-inertial physics
+};
+
 struct itr_batch {
     itr_ctx* ctx = nullptr;
     std::vector<GridHost> grids;
     DevParams prm{};                // the first grid's parameters (single-grid users: adjoint, schedules)
-    KArgs args{};
-    KArgs args_li{};                // local-inertial grids: their own task list and kernel
+    KArgs args{};                   // grids, control and accumulators (the lists add their tasks)
+    std::vector<LaunchList> lists;
     void* d_grids = nullptr;
     void* d_tasks = nullptr;
-    void* d_tasks_li = nullptr;
     void* d_snap = nullptr;         // CandCtl [2][ncand]
     void* d_acc = nullptr;          // StepAcc [4][ncand]
     void* d_status = nullptr;
     uint32_t* h_status = nullptr;   // pinned
     int ncand = 0;
-    int ntasks = 0;
-    int block = 64;
-    size_t lds_bytes = 0;
-    int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts
-    int ntasks_li = 0;
-    int block_li = 64;
-    size_t lds_li = 0;
-    int modes_li = 0;
+    int ntasks = 0;                 // over all lists
     uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -2881,7 +2894,7 @@ void free_batch_mem(itr_batch* b) {
     for (auto& g : b->grids)
         for (void* p : g.allocs) (void)hipFree(p);
     if (!b->no_alloc) {
-        void* ps[] = {b->d_grids, b->d_tasks, b->d_tasks_li, b->d_snap, b->d_acc, b->d_status};
+        void* ps[] = {b->d_grids, b->d_tasks, b->d_snap, b->d_acc, b->d_status};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         if (b->h_status) (void)hipHostFree(b->h_status);
@@ -2924,6 +2937,80 @@ void order_tasks(std::vector<int4>& tasks) {
         std::stable_sort(tasks.begin() + x0, tasks.begin() + x1, [](const int4& a, const int4& b) {
             return a.x != b.x ? a.x < b.x : a.z < b.z;
         });
+    }
+}
+
+// Launch lists of a batch: one per class of grids (physics, face mode, Stage / Inflow
+// ghosts or tape, waves per row) when there are at most kMaxLists classes, else one per
+// physics (whose kernels then carry every face mode present).
+void assign_lists(itr_batch* b, const std::vector<DevGrid>& dg) {
+    b->lists.clear();
+    std::vector<uint64_t> keys;
+    std::vector<int> which(dg.size());
+    for (size_t g = 0; g < dg.size(); g++) {
+        const DevGrid& D = dg[g];
+        const uint64_t k = (uint64_t)D.li | (uint64_t)(D.crest ? 1 : 0) << 1 | (uint64_t)(D.special || D.tape_mode ? 1 : 0) << 2 |
+                           (uint64_t)D.waves << 3;
+        size_t i = 0;
+        while (i < keys.size() && keys[i] != k) i++;
+        if (i == keys.size()) keys.push_back(k);
+        which[g] = (int)i;
+    }
+    if (keys.size() > (size_t)kMaxLists) {
+        keys.clear();
+        for (size_t g = 0; g < dg.size(); g++) {
+            const uint64_t k = (uint64_t)dg[g].li;
+            size_t i = 0;
+            while (i < keys.size() && keys[i] != k) i++;
+            if (i == keys.size()) keys.push_back(k);
+            which[g] = (int)i;
+        }
+    }
+    b->lists.resize(keys.size());
+    for (size_t g = 0; g < dg.size(); g++) {
+        LaunchList& L = b->lists[which[g]];
+        L.grids.push_back((int)g);
+        L.li = dg[g].li != 0;
+    }
+}
+
+// The lists' tasks (concatenated into `all`, each list's slice in XCD order), workgroup
+// size, face modes and LDS for the grids' current band heights.  Returns the largest LDS.
+size_t fill_lists(itr_batch* b, const std::vector<DevGrid>& dg, std::vector<int4>& all) {
+    all.clear();
+    size_t lds_max = 0;
+    for (LaunchList& L : b->lists) {
+        std::vector<int4> t;
+        int waves = 1;
+        size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0);
+        L.modes = 0;
+        for (int g : L.grids) {
+            const DevGrid& D = dg[g];
+            for (int k = 0; k < D.ncand; k++)
+                for (int band = 0; band < D.nbands; band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
+            waves = std::max(waves, D.waves);
+            L.modes |= (D.crest ? 2 : 1) | (D.special || D.tape_mode ? 4 : 0);
+            lds = std::max(lds, (D.li ? li_lds_layout(D.R, D.nx, D.special != 0).total
+                                      : lds_layout(D.R, D.nx, D.special != 0).total) * sizeof(float));
+        }
+        order_tasks(t);
+        L.task_off = all.size();
+        L.ntasks = (int)t.size();
+        L.block = waves * kWave;
+        L.lds = lds;
+        all.insert(all.end(), t.begin(), t.end());
+        lds_max = std::max(lds_max, lds);
+    }
+    b->ntasks = (int)all.size();
+    return lds_max;
+}
+
+// Point each list's kernel arguments at its slice of the task buffer.
+void bind_lists(itr_batch* b) {
+    for (LaunchList& L : b->lists) {
+        L.args = b->args;
+        L.args.tasks = (const int4*)b->d_tasks + L.task_off;
+        L.args.ntasks = L.ntasks;
     }
 }
 
@@ -3140,33 +3227,48 @@ itr_status set_callback_flag(itr_batch* b, int has_cb, char* err, size_t errlen)
     return ITR_OK;
 }
 
-// Enqueue n step launches whose launch indices start at lk0 (mod 4).
-itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
-    const int nblk = ((b->ntasks + 7) / 8) * 8;
-    const int nblk_li = ((b->ntasks_li + 7) / 8) * 8;
-    for (int s = 0; s < n; s++) {
-        const uint32_t lk = (uint32_t)((lk0 + (uint64_t)s) & 3u);
-        if (b->ntasks_li > 0) {
-            if (b->modes_li & 4)
-                hipLaunchKernelGGL(itr_step_li_general, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
-            else if (b->modes_li == 1)
-                hipLaunchKernelGGL(itr_step_li_signed, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
-            else if (b->modes_li == 2)
-                hipLaunchKernelGGL(itr_step_li_crest, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
-            else
-                hipLaunchKernelGGL(itr_step_li_general, dim3(nblk_li), dim3(b->block_li), b->lds_li, b->ctx->stream, b->args_li, lk);
-        }
-        if (b->ntasks == 0) continue;
-        if (b->modes & 4)
-            hipLaunchKernelGGL(itr_step_special, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
-        else if (b->modes == 1)
-            hipLaunchKernelGGL(itr_step_signed, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
-        else if (b->modes == 2)
-            hipLaunchKernelGGL(itr_step_crest, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+// One step launch of a list.
+void launch_list(const LaunchList& L, hipStream_t st, uint32_t lk) {
+    const dim3 grid((unsigned)(((L.ntasks + 7) / 8) * 8)), blk((unsigned)L.block);
+    if (L.li) {
+        if (L.modes == 1)
+            hipLaunchKernelGGL(itr_step_li_signed, grid, blk, L.lds, st, L.args, lk);
+        else if (L.modes == 2)
+            hipLaunchKernelGGL(itr_step_li_crest, grid, blk, L.lds, st, L.args, lk);
         else
-            hipLaunchKernelGGL(itr_step_mixed, dim3(nblk), dim3(b->block), b->lds_bytes, b->ctx->stream, b->args, lk);
+            hipLaunchKernelGGL(itr_step_li_general, grid, blk, L.lds, st, L.args, lk);
+    } else if (L.modes & 4) {
+        hipLaunchKernelGGL(itr_step_special, grid, blk, L.lds, st, L.args, lk);
+    } else if (L.modes == 1) {
+        hipLaunchKernelGGL(itr_step_signed, grid, blk, L.lds, st, L.args, lk);
+    } else if (L.modes == 2) {
+        hipLaunchKernelGGL(itr_step_crest, grid, blk, L.lds, st, L.args, lk);
+    } else {
+        hipLaunchKernelGGL(itr_step_mixed, grid, blk, L.lds, st, L.args, lk);
+    }
+}
+
+// Enqueue n step launches of every list, launch indices from lk0 (mod 4).  Lists after
+// the first run on their own streams between a fork and a join on the context's stream
+// (also under graph capture, where they become parallel branches of the graph).
+itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
+    itr_ctx* ctx = b->ctx;
+    const int nl = (int)b->lists.size();
+    if (nl > 1) {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        for (int i = 1; i < nl; i++) HIP_TRY(hipStreamWaitEvent(ctx->aux[i - 1], ctx->ev_fork, 0));
+    }
+    for (int i = 0; i < nl; i++) {
+        const LaunchList& L = b->lists[i];
+        if (L.ntasks == 0) continue;
+        const hipStream_t st = i == 0 ? ctx->stream : ctx->aux[i - 1];
+        for (int s = 0; s < n; s++) launch_list(L, st, (uint32_t)((lk0 + (uint64_t)s) & 3u));
     }
     HIP_TRY(hipGetLastError());
+    for (int i = 1; i < nl; i++) {
+        HIP_TRY(hipEventRecord(ctx->ev_join[i - 1], ctx->aux[i - 1]));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join[i - 1], 0));
+    }
     return ITR_OK;
 }
 
@@ -3212,7 +3314,7 @@ itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
         b->graph = nullptr;
     }
     b->graph_steps = 0;
-    if (K == 0 || b->ntasks + b->ntasks_li == 0) return ITR_OK;
+    if (K == 0 || b->ntasks == 0) return ITR_OK;
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
     itr_status ls = enqueue_steps(b, K, 0, err, errlen);
@@ -3273,9 +3375,12 @@ itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errle
     c->device = device;
     c->scratch = sp;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < kMaxLists - 1 && e == hipSuccess; i++) e = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    for (int i = 0; i < kMaxLists - 1 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete c;
-        set_err(err, errlen, "hipStreamCreate: %s", hipGetErrorString(e));
+        itr_hip_destroy(c);
+        set_err(err, errlen, "stream / event creation: %s", hipGetErrorString(e));
         return ITR_EDEVICE;
     }
     char buf[512];
@@ -3290,6 +3395,11 @@ itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errle
 void itr_hip_destroy(itr_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < kMaxLists - 1; i++) {
+        if (ctx->aux[i]) (void)hipStreamDestroy(ctx->aux[i]);
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -3461,9 +3571,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     };
     // ---- grids
     int total = 0;
-    int max_waves = 1, max_waves_li = 1;
-    size_t max_lds = 0, max_lds_li = 0;
-    std::vector<int4> tasks, tasks_li;
     std::vector<CandCtl> ctl;
     std::vector<DevGrid> dgrids;
     b->grids.resize(n_grids);
@@ -3488,7 +3595,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.li = p->physics == ITR_PHYSICS_LOCAL_INERTIAL ? 1 : 0;
         D.waves = D.li ? (nx + kLiCols - 1) / kLiCols : (nx + kColsPerWave - 1) / kColsPerWave;
         D.tail_groups = D.li ? 1 : tail_group_count(nx, D.waves);
-        (D.li ? max_waves_li : max_waves) = std::max(D.li ? max_waves_li : max_waves, D.waves);
         const int R = choose_band_rows(ctx, grids, n_grids, g);
         D.R = R;
         D.nbands = (ny + R - 1) / R;
@@ -3646,7 +3752,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
             if (!(0.0 < dur && 0u < n_sync)) C.flags = F_DONE;
             if (p->max_steps < 0) C.flags = F_DONE;
             ctl.push_back(C);
-            for (int band = 0; band < D.nbands; band++) (D.li ? tasks_li : tasks).push_back(make_int4(g, k, band, total + k));
         }
         total += nc;
         // device buffers
@@ -3849,24 +3954,13 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         D.hyd_q = d_hyd_q;
         D.max_log = log_steps > 0 ? (int32_t)std::min<int64_t>(log_steps, 1 << 30) : 0;
         dgrids.push_back(D);
-        if (D.li) max_lds_li = std::max(max_lds_li, li_lds_layout(R, nx, D.special != 0).total * sizeof(float));
-        else max_lds = std::max(max_lds, lds_layout(R, nx, D.special != 0).total * sizeof(float));
     }
     b->ncand = total;
-    order_tasks(tasks);
-    order_tasks(tasks_li);
-    b->ntasks = (int)tasks.size();
-    b->ntasks_li = (int)tasks_li.size();
-    b->modes = 0;
-    b->modes_li = 0;
-    for (auto& D : dgrids) (D.li ? b->modes_li : b->modes) |= (D.crest ? 2 : 1) | (D.special || D.tape_mode ? 4 : 0);
-    if (b->modes == 0) b->modes = 1;
-    b->block = max_waves * kWave;
-    b->block_li = max_waves_li * kWave;
-    b->lds_bytes = std::max(max_lds, (size_t)std::max<int64_t>(ctx->min_lds, 0));
-    b->lds_li = std::max(max_lds_li, (size_t)std::max<int64_t>(ctx->min_lds, 0));
-    if (std::max(b->lds_bytes, b->lds_li) > 160 * 1024 - 1024) {   // static shared memory of the kernel: < 1 KiB
-        set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", std::max(b->lds_bytes, b->lds_li));
+    std::vector<int4> tasks;
+    assign_lists(b, dgrids);
+    const size_t lds_max = fill_lists(b, dgrids, tasks);
+    if (lds_max > 160 * 1024 - 1024) {   // static shared memory of the kernel: < 1 KiB
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit; lower band_rows", lds_max);
         return fail(ITR_DECLINED);
     }
     std::vector<void*> misc;
@@ -3882,9 +3976,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)D.ny;
     if ((s = dev_alloc(misc, &d_tasks, std::max(task_cap, tasks.size()), err, errlen)) != ITR_OK) return fail(s);
     b->d_tasks = d_tasks;
-    int4* d_tasks_li;
-    if ((s = dev_alloc(misc, &d_tasks_li, std::max(task_cap, tasks_li.size()), err, errlen)) != ITR_OK) return fail(s);
-    b->d_tasks_li = d_tasks_li;
     if ((s = dev_alloc(misc, &d_snap, 2 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
     b->d_snap = d_snap;
     if ((s = dev_alloc(misc, &d_acc, 4 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);
@@ -3902,8 +3993,6 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     }
     bool ok = hipMemcpy(d_grids, dgrids.data(), dgrids.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
               (tasks.empty() || hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
-              (tasks_li.empty() ||
-               hipMemcpy(d_tasks_li, tasks_li.data(), tasks_li.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
               (ctl.empty() || (hipMemcpy(d_snap, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess &&
                                hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
                                          hipMemcpyHostToDevice) == hipSuccess)) &&
@@ -3919,16 +4008,15 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->args.status = d_status;
     b->args.ntasks = b->ntasks;
     b->args.ncand = total;
-    b->args_li = b->args;
-    b->args_li.tasks = d_tasks_li;
-    b->args_li.ntasks = b->ntasks_li;
+    bind_lists(b);
     if (total > 0) {
         hipLaunchKernelGGL(itr_init_kernel, dim3(total), dim3(64), 0, ctx->stream, b->args);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     b->stats.band_rows = dgrids.empty() ? 0 : dgrids[0].R;
-    b->stats.waves_per_task = max_waves;
+    b->stats.waves_per_task = 1;
+    for (const LaunchList& L : b->lists) b->stats.waves_per_task = std::max(b->stats.waves_per_task, L.block / kWave);
     *out = b;
     return ITR_OK;
 }
@@ -3954,7 +4042,7 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     b->stats.use_graph = graph ? 1 : 0;
     b->stats.launches = 0;
     b->stats.syncs = 0;
-    if (b->ncand == 0 || b->ntasks + b->ntasks_li == 0) return ITR_OK;
+    if (b->ncand == 0 || b->ntasks == 0) return ITR_OK;
     itr_status s = prepare_graph(b, graph ? K : 0, err, errlen);
     if (s != ITR_OK) return s;
     std::vector<CandCtl> c0;
@@ -4040,7 +4128,7 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     HIP_TRY(ev.create());
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ev.e[0], ctx->stream));
-    if (b->ntasks + b->ntasks_li > 0 && n_steps > 0)
+    if (b->ntasks > 0 && n_steps > 0)
         if ((s = launch_steps(b, n_steps, err, errlen)) != ITR_OK) return s;
     HIP_TRY(hipEventRecord(ev.e[1], ctx->stream));
     HIP_TRY(hipEventSynchronize(ev.e[1]));
@@ -4539,42 +4627,29 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
 // Re-tile a resident batch: new band heights per grid (tasks, band monitor masks and
 // LDS size follow).  State and control blocks are untouched.
 itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t errlen) {
-    size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0), lds_li = lds;
-    for (size_t g = 0; g < b->grids.size(); g++) {
-        const DevGrid& D = b->grids[g].d;
-        const int R = std::max(1, std::min(Rs[g], D.ny));
-        if (D.li) lds_li = std::max(lds_li, li_lds_layout(R, D.nx, D.special != 0).total * sizeof(float));
-        else lds = std::max(lds, lds_layout(R, D.nx, D.special != 0).total * sizeof(float));
-    }
-    if (std::max(lds, lds_li) > 160 * 1024 - 1024) {
-        set_err(err, errlen, "band of %zu bytes of LDS does not fit", std::max(lds, lds_li));
-        return ITR_DECLINED;
-    }
-    std::vector<int4> tasks, tasks_li;
     std::vector<DevGrid> dg;
     for (size_t g = 0; g < b->grids.size(); g++) {
-        GridHost& GH = b->grids[g];
-        DevGrid& D = GH.d;
+        DevGrid D = b->grids[g].d;
         const int R = std::max(1, std::min(Rs[g], D.ny));
         D.R = R;
         D.nbands = (D.ny + R - 1) / R;
-        for (int k = 0; k < D.ncand; k++)
-            for (int band = 0; band < D.nbands; band++)
-                (D.li ? tasks_li : tasks).push_back(make_int4((int)g, k, band, D.cand0 + k));
         dg.push_back(D);
     }
-    order_tasks(tasks);
-    order_tasks(tasks_li);
+    std::vector<int4> tasks;
+    std::vector<LaunchList> keep = b->lists;
+    const size_t lds_max = fill_lists(b, dg, tasks);
+    if (lds_max > 160 * 1024 - 1024) {
+        b->lists = keep;
+        b->ntasks = 0;
+        for (const LaunchList& L : keep) b->ntasks += L.ntasks;
+        set_err(err, errlen, "band of %zu bytes of LDS does not fit", lds_max);
+        return ITR_DECLINED;
+    }
+    for (size_t g = 0; g < b->grids.size(); g++) b->grids[g].d = dg[g];
     HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
-    if (!tasks_li.empty())
-        HIP_TRY(hipMemcpy(b->d_tasks_li, tasks_li.data(), tasks_li.size() * sizeof(int4), hipMemcpyHostToDevice));
-    b->ntasks = (int)tasks.size();
     b->args.ntasks = b->ntasks;
-    b->ntasks_li = (int)tasks_li.size();
-    b->args_li.ntasks = b->ntasks_li;
-    b->lds_li = lds_li;
-    b->lds_bytes = lds;
+    bind_lists(b);
     b->stats.band_rows = dg.empty() ? 0 : dg[0].R;
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);

@@ -13,8 +13,9 @@
 //     by completing its candidate's previous step from that step's results (CFL max and
 //     health flag, monitors): f64 time, dt, sync clipping, exact hyetograph integral,
 //     guard abort, termination and the next step's f32 constants.  Every workgroup of
-//     the candidate computes the same decision from the same inputs; the first band
-//     also does the row-ordered ledger sums and writes the new control block.
+//     the candidate computes the same decision from the same inputs; the candidate's
+//     control task (a workgroup with no band) also does the row-ordered ledger sums and
+//     writes the new control block, while the bands march.
 //   * Everything a launch reads from the previous one is double-buffered (control
 //     blocks by launch parity; state, row partials, boundary masses and monitors by the
 //     candidate's state parity; per-step atomics in a ring of four by launch index).
@@ -551,7 +552,7 @@ __device__ __forceinline__ double step_smax(const DevGrid& G, const StepAcc& A) 
 // and the sync point, 554-603, except the ledger sums), then the termination test and
 // the head of the next step (prepare_step).  Every wave of every workgroup of the
 // candidate evaluates it on the same inputs, so all reach the same decision; only the
-// candidate's first band writes the result.  Returns true if this launch runs a step.
+// candidate's control task writes the result.  Returns true if this launch runs a step.
 __device__ __forceinline__ bool complete_step(const DevParams& p, const DevGrid& G, CandCtl& S, const StepAcc& A,
                                               int lane) {
     if (S.flags & F_DONE) return false;
@@ -1256,7 +1257,7 @@ __device__ __forceinline__ void ledger_finish(const DevGrid& G, const CandCtl& S
 }
 
 // Ledger accounting of the completed step (kernel.rs:453-500 sums, solver.rs:529-540,
-// 554-592), run by the candidate's first band after its own step.  Row-ordered f64
+// 554-592), run by the candidate's control task while the bands march the next step.  Row-ordered f64
 // sums of the row partials and the FixedSum<f64> of the boundary y-faces, then thread
 // 0 adds them to the new control block (in LDS).  `Sin` is the block of the previous
 // launch (the completed step's dt, rain, dry-row count).
@@ -1903,9 +1904,12 @@ __device__ __forceinline__ void snapshot_band(const DevGrid& G, int local, int b
 // One launch = one step of every active candidate.  A workgroup first completes the
 // candidate's previous step from its results (identically in every workgroup of the
 // candidate), then, unless the candidate finished or waits for the host, runs its band
-// of the next step.  The candidate's first band also runs the ledger accounting and
-// writes the new control block.  The control block is worked on in LDS (it is 60 words;
-// held in registers it would set the kernel's register budget).  lk = launch index & 3.
+// of the next step.  One more workgroup per candidate, its control task (band index -1),
+// marches no band: it runs the ledger accounting of the completed step and writes the
+// new control block, side by side with the bands (the ledger reads the completed step's
+// row partials, the bands write the other parity).  The control block is worked on in
+// LDS (it is 60 words; held in registers it would set the kernel's register budget).
+// lk = launch index & 3.
 // MODES: face modes present in the batch (1 signed, 2 crest, 3 both); a batch of one
 // mode gets a kernel without the other mode's code and its register budget.  SPECIAL:
 // the batch has Stage / Inflow ghosts (their code stays out of the other kernels).
@@ -1930,7 +1934,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     const int nthr = G.waves * kWave;
     const int local = tk.y;
     const int cand = tk.w;
-    const bool designated = tk.z == 0;
+    const bool designated = tk.z < 0;   // the candidate's control task (no band)
     const DevParams& p = G.prm;
     const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
@@ -1966,7 +1970,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         }
     }
     __syncthreads();
-    if (LI && s_go) {
+    if (LI && s_go && !designated) {
         LiK K;
         K.dt = uni(sS.dt);
         K.rx = K.dt / G.dx;
@@ -1991,7 +1995,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
             li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
-    } else if (!LI && s_go) {
+    } else if (!LI && s_go && !designated) {
         // The step's constants are wave-uniform: keep them in scalar registers.
         StepK K;
         K.g = G.g32;
@@ -2986,8 +2990,10 @@ size_t fill_lists(itr_batch* b, const std::vector<DevGrid>& dg, std::vector<int4
         L.modes = 0;
         for (int g : L.grids) {
             const DevGrid& D = dg[g];
-            for (int k = 0; k < D.ncand; k++)
+            for (int k = 0; k < D.ncand; k++) {
+                t.push_back(make_int4(g, k, -1, D.cand0 + k));   // the control task
                 for (int band = 0; band < D.nbands; band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
+            }
             waves = std::max(waves, D.waves);
             L.modes |= (D.crest ? 2 : 1) | (D.special || D.tape_mode ? 4 : 0);
             lds = std::max(lds, (D.li ? li_lds_layout(D.R, D.nx, D.special != 0).total
@@ -3973,7 +3979,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
     b->d_grids = d_grids;
     size_t task_cap = 0;   // room for any band height (re-tiling never reallocates)
-    for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)D.ny;
+    for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)(D.ny + 1);
     if ((s = dev_alloc(misc, &d_tasks, std::max(task_cap, tasks.size()), err, errlen)) != ITR_OK) return fail(s);
     b->d_tasks = d_tasks;
     if ((s = dev_alloc(misc, &d_snap, 2 * std::max<size_t>(ctl.size(), 1), err, errlen)) != ITR_OK) return fail(s);

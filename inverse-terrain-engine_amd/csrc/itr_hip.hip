@@ -206,6 +206,7 @@ struct KArgs {
     uint32_t* status;          // [ncand] flags of the latest control block, for the host
     int32_t ntasks;
     int32_t ncand;
+    int32_t ctl_task;          // 1: each candidate has a control task (band -1); 0: its first band is it
 };
 
 // Dynamic LDS of the step kernel, in floats (offsets 16-byte aligned).
@@ -1904,10 +1905,12 @@ __device__ __forceinline__ void snapshot_band(const DevGrid& G, int local, int b
 // One launch = one step of every active candidate.  A workgroup first completes the
 // candidate's previous step from its results (identically in every workgroup of the
 // candidate), then, unless the candidate finished or waits for the host, runs its band
-// of the next step.  One more workgroup per candidate, its control task (band index -1),
-// marches no band: it runs the ledger accounting of the completed step and writes the
-// new control block, side by side with the bands (the ledger reads the completed step's
-// row partials, the bands write the other parity).  The control block is worked on in
+// of the next step.  The candidate's designated workgroup also runs the ledger accounting
+// of the completed step and writes the new control block: in a launch whose bands fit the
+// chip at once (latency-bound), a control task of its own (band index -1, no band) that
+// runs side by side with the bands (the ledger reads the completed step's row partials,
+// the bands write the other parity); in a launch of several rounds of workgroups
+// (throughput-bound), the first band, after its march.  The control block is worked on in
 // LDS (it is 60 words; held in registers it would set the kernel's register budget).
 // lk = launch index & 3.
 // MODES: face modes present in the batch (1 signed, 2 crest, 3 both); a batch of one
@@ -1934,7 +1937,8 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     const int nthr = G.waves * kWave;
     const int local = tk.y;
     const int cand = tk.w;
-    const bool designated = tk.z < 0;   // the candidate's control task (no band)
+    const bool ctl_only = tk.z < 0;   // a control task (no band)
+    const bool designated = a.ctl_task ? ctl_only : tk.z == 0;
     const DevParams& p = G.prm;
     const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
@@ -1970,7 +1974,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         }
     }
     __syncthreads();
-    if (LI && s_go && !designated) {
+    if (LI && s_go && !ctl_only) {
         LiK K;
         K.dt = uni(sS.dt);
         K.rx = K.dt / G.dx;
@@ -1995,7 +1999,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
             li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
-    } else if (!LI && s_go && !designated) {
+    } else if (!LI && s_go && !ctl_only) {
         // The step's constants are wave-uniform: keep them in scalar registers.
         StepK K;
         K.g = G.g32;
@@ -2812,6 +2816,7 @@ constexpr int kMaxLists = 4;   // launch lists per batch (streams per context)
 
 struct itr_ctx {
     int device = 0;
+    int cus = 256;
     hipStream_t stream = nullptr;
     hipStream_t aux[kMaxLists - 1] = {};   // the streams of launch lists 1.. (default priority)
     hipEvent_t ev_fork = nullptr;          // fork / join of the lists' streams (no timing)
@@ -2822,6 +2827,7 @@ struct itr_ctx {
     int use_graph = 1;
     int64_t min_lds = 0;   // reserve at least this much LDS per step workgroup (caps occupancy)
     int adjoint_mode = 0;  // reverse sweep: 0 by batch size, 1 band march, 2 per-cell kernels
+    int control_task = -1; // ledger and control block: -1 by launch shape, 0 first band, 1 control task
     struct Tuned { int nx, ny, crest, R; };
     std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
     void* scratch = nullptr;    // this device's static scratch (itr_hip_run_batch_noalloc)
@@ -2863,6 +2869,7 @@ struct LaunchList {
     size_t lds = 0;
     int modes = 1;                  // face modes present: 1 signed, 2 crest, 3 both; | 4 Stage / Inflow ghosts or tape
     bool li = false;                // local-inertial physics
+    bool ctl_task = false;          // control tasks (band -1) carry the ledger and control block
 };
 
 struct itr_batch {
@@ -2978,26 +2985,46 @@ void assign_lists(itr_batch* b, const std::vector<DevGrid>& dg) {
     }
 }
 
+// Whether a list's candidates get control tasks: when its bands fit the chip in one round
+// of workgroups (by waves at the kernel's register budget and by LDS), the step is
+// latency-bound and the ledger would sit on its critical path.  A list of several rounds
+// is throughput-bound: a control task would only hold a workgroup's LDS for the length of
+// the ledger, which the first band runs as well after its march.
+bool use_control_task(int cus, size_t nbands, int waves, size_t lds, int modes, bool li) {
+    const int waves_per_simd = (!li && modes == 1) ? 6 : 4;   // 80 VGPRs (signed HLL), else 128
+    const int by_waves = 4 * waves_per_simd / std::max(1, waves);
+    const int by_lds = lds > 0 ? (int)((size_t)160 * 1024 / lds) : 64;
+    const size_t per_cu = (size_t)std::max(1, std::min(by_waves, by_lds));
+    return nbands <= (size_t)cus * per_cu;
+}
+
 // The lists' tasks (concatenated into `all`, each list's slice in XCD order), workgroup
-// size, face modes and LDS for the grids' current band heights.  Returns the largest LDS.
+// size, face modes, LDS and control-task mode for the grids' current band heights.
+// Returns the largest LDS.
 size_t fill_lists(itr_batch* b, const std::vector<DevGrid>& dg, std::vector<int4>& all) {
     all.clear();
     size_t lds_max = 0;
     for (LaunchList& L : b->lists) {
         std::vector<int4> t;
         int waves = 1;
-        size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0);
+        size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0), nbands = 0;
         L.modes = 0;
         for (int g : L.grids) {
             const DevGrid& D = dg[g];
-            for (int k = 0; k < D.ncand; k++) {
-                t.push_back(make_int4(g, k, -1, D.cand0 + k));   // the control task
-                for (int band = 0; band < D.nbands; band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
-            }
+            nbands += (size_t)std::max(D.ncand, 0) * (size_t)D.nbands;
             waves = std::max(waves, D.waves);
             L.modes |= (D.crest ? 2 : 1) | (D.special || D.tape_mode ? 4 : 0);
             lds = std::max(lds, (D.li ? li_lds_layout(D.R, D.nx, D.special != 0).total
                                       : lds_layout(D.R, D.nx, D.special != 0).total) * sizeof(float));
+        }
+        L.ctl_task = b->ctx->control_task >= 0 ? b->ctx->control_task == 1
+                                               : use_control_task(b->ctx->cus, nbands, waves, lds, L.modes, L.li);
+        for (int g : L.grids) {
+            const DevGrid& D = dg[g];
+            for (int k = 0; k < D.ncand; k++) {
+                if (L.ctl_task) t.push_back(make_int4(g, k, -1, D.cand0 + k));
+                for (int band = 0; band < D.nbands; band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
+            }
         }
         order_tasks(t);
         L.task_off = all.size();
@@ -3017,6 +3044,7 @@ void bind_lists(itr_batch* b) {
         L.args = b->args;
         L.args.tasks = (const int4*)b->d_tasks + L.task_off;
         L.args.ntasks = L.ntasks;
+        L.args.ctl_task = L.ctl_task ? 1 : 0;
     }
 }
 
@@ -3379,6 +3407,7 @@ itr_status itr_hip_create(int32_t device, itr_ctx** out, char* err, size_t errle
     }
     auto* c = new itr_ctx();
     c->device = device;
+    c->cus = std::max(1, prop.multiProcessorCount);
     c->scratch = sp;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < kMaxLists - 1 && e == hipSuccess; i++) e = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking);
@@ -3420,6 +3449,7 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     else if (n == "use_graph") ctx->use_graph = value != 0;
     else if (n == "min_lds") ctx->min_lds = value;
     else if (n == "adjoint_mode" && value >= 0 && value <= 2) ctx->adjoint_mode = (int)value;
+    else if (n == "control_task" && value >= -1 && value <= 1) ctx->control_task = (int)value;
     else return ITR_EINVAL;
     return ITR_OK;
 }

@@ -146,11 +146,13 @@ def test_li_launch_modes_and_guard(use_graph, chunk):
         assert_run_equal(r, o, f"li guard cand {i}", li=True)
 
 
-def test_li_locked_dt_and_restart(solver):
+@pytest.mark.parametrize("control_task", [0, 1])
+def test_li_locked_dt_and_restart(solver, control_task):
     """A locked local-inertial schedule with the adaptive restart from the last sync
     snapshot (solver.rs:456-480), which for local inertial includes the boundary faces:
-    natural failures on edited terrain and a failure forced mid-interval."""
-    _say("case: local inertial, locked dt with restarts, 2 groups of 5 candidates on 60x48")
+    natural failures on edited terrain and a failure forced mid-interval; the ledger and
+    control block on each candidate's first band and on a control task of its own."""
+    _say(f"case: local inertial, locked dt with restarts, 2 groups of 5 candidates on 60x48, control_task={control_task}")
     from inverse_terrain_engine_amd.gpu import BatchSession
     nx, ny, dx = 60, 48, 5.0
     rng = sc.SplitMix64(31)
@@ -171,12 +173,16 @@ def test_li_locked_dt_and_restart(solver):
     edits = [tr.materialize(sc.random_primitives(rng, 3, geo, nx, ny), geo, nx, ny, None, 1.0) for _ in range(5)]
     terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
     mon = np.arange(0, nx * ny, 11, dtype=np.uint32)
-    sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 2, None, group_params=[pa, pb])
+    solver.set_option("control_task", control_task)
     try:
-        sess.run()
-        res = sess.results()
+        sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 2, None, group_params=[pa, pb])
+        try:
+            sess.run()
+            res = sess.results()
+        finally:
+            sess.close()
     finally:
-        sess.close()
+        solver.set_option("control_task", -1)
     falls = []
     for gi, q in enumerate((pa, pb)):
         for ci, e in enumerate(edits):

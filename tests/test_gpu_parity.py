@@ -285,17 +285,24 @@ def test_declines_unsupported(solver):
         solver.run_batch(sc.SolverParams(10.0, 5.0), [tr.PreparedTerrain(base)], np.zeros(0, np.uint32))
 
 
-def test_fidelity_ladder_one_launch(solver):
+@pytest.mark.parametrize("control_task", [0, 1])
+def test_fidelity_ladder_one_launch(solver, control_task):
     """Grids of three widths and both face modes (45x35 crest, 90x70 and 180x140 signed)
-    in one batch, one launch per step: every instance bitwise equal to the oracle."""
-    _say("case: fidelity ladder 4/2/1 instances, 900 s")
+    in one batch (three launch lists on three streams), with the ledger and control block
+    on each candidate's first band or on a control task of its own: every instance
+    bitwise equal to the oracle."""
+    _say(f"case: fidelity ladder 4/2/1 instances, 900 s, control_task={control_task}")
     lad = sc.config3(n_inst=7, duration=900.0)
-    sess = solver.session(lad.params, lad.groups)
+    solver.set_option("control_task", control_task)
     try:
-        sess.run()
-        res = sess.results()
+        sess = solver.session(lad.params, lad.groups)
+        try:
+            sess.run()
+            res = sess.results()
+        finally:
+            sess.close()
     finally:
-        sess.close()
+        solver.set_option("control_task", -1)
     k = 0
     for base, cands, mon, _ in lad.groups:
         for _ in cands:
@@ -532,14 +539,16 @@ def test_device_designs_with_stage_levels(solver, level):
     assert not np.array_equal(runs[3].depth, runs[4].depth), "the south edit should change the run"
 
 
-def test_locked_dt_schedule_and_adaptive_restart(solver):
+@pytest.mark.parametrize("control_task", [0, 1])
+def test_locked_dt_schedule_and_adaptive_restart(solver, control_task):
     """Baseline-locked dt (solver.rs:452-455) and its fallback: a step whose locked dt
     fails the hard-CFL check restarts adaptively from the last sync snapshot
     (solver.rs:456-480).  Failures from edited terrain (cfl_hard 0.37), forced right after
     a sync step (the snapshot is the state just completed), forced mid-interval (state,
     monitors, ledger and control roll back), and at the first step; each group has its
-    own parameters in one resident batch."""
-    _say("case: locked dt schedules with restarts, 4 groups of 6 candidates on 60x48")
+    own parameters in one resident batch.  Run with the ledger and control block on each
+    candidate's first band and on a control task of its own."""
+    _say(f"case: locked dt schedules with restarts, 4 groups of 6 candidates on 60x48, control_task={control_task}")
     from inverse_terrain_engine_amd.gpu import BatchSession
     nx, ny, dx = 60, 48, 5.0
     rng = sc.SplitMix64(31)
@@ -561,12 +570,16 @@ def test_locked_dt_schedule_and_adaptive_restart(solver):
     terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
     mon = np.arange(0, nx * ny, 11, dtype=np.uint32)
     params = [pa, pb, pc, pd]
-    sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 4, None, group_params=params)
+    solver.set_option("control_task", control_task)
     try:
-        sess.run()
-        res = sess.results()
+        sess = BatchSession(solver, pa, [(base, terrs, mon, None)] * 4, None, group_params=params)
+        try:
+            sess.run()
+            res = sess.results()
+        finally:
+            sess.close()
     finally:
-        sess.close()
+        solver.set_option("control_task", -1)
     fallbacks = []
     for gi, q in enumerate(params):
         for ci, e in enumerate(edits):

@@ -283,12 +283,12 @@ void itr_hip_batch_destroy(itr_batch* b);
  * frozen schedule makes the discrete step a fixed function of the state), stores
  * checkpoints every ceil(sqrt(N)) steps, replays each segment and sweeps it backwards;
  * adjoint arithmetic is f32, face-gradient accumulation f64.  p->max_steps > 0 bounds the
- * horizon (the run then ends after that many steps).  Signed face mode only: crest
- * levels, Stage / Inflow ghosts, an initial stage (its depth depends on the bed), local
- * inertial physics, replay grids and device designs return ITR_DECLINED.  A replayed
- * segment that differs bitwise from its checkpoint returns ITR_EDEVICE (no gradient).
- * The earthwork term and the chain rule to primitive heights are host work on the
- * returned cell gradient. */
+ * horizon (the run then ends after that many steps).  Signed levels and face-crest levels
+ * (numerics.rs:117-129 with the faces of terrain.rs:378-429); Stage / Inflow ghosts, an
+ * initial stage (their depths depend on the bed), local inertial physics, replay grids and
+ * device designs return ITR_DECLINED.  A replayed segment that differs bitwise from its
+ * checkpoint returns ITR_EDEVICE (no gradient).  The earthwork term and the chain rule to
+ * the fine raster and to primitive heights are host work on the returned gradients. */
 typedef struct {
     int32_t n_assets;
     int32_t reserved;
@@ -321,15 +321,21 @@ itr_status itr_hip_objective_j(const itr_objective* obj, const itr_earthwork* ea
 typedef struct {
     double j_risk;                /* J_risk of the forward run */
     double* peak_m;               /* [n_assets] asset peaks, or NULL */
-    double* dJ_dz_face_x;         /* [ny][nx + 1] dJ_risk / d(z_R - z_L) of x-faces, or NULL */
+    double* dJ_dz_face_x;         /* [ny][nx + 1] dJ_risk / d(z_R - z_L) of x-faces; on a crest level
+                                     dJ_risk / d(crest line of the face), 0 where a cell's level is
+                                     the face's maximum; or NULL */
     double* dJ_dz_face_y;         /* [ny + 1][nx] of y-faces, or NULL */
-    double* dJ_dz_cell;           /* [ny][nx] dJ_risk / dz of each cell (0 on walls), or NULL */
+    double* dJ_dz_cell;           /* [ny][nx] dJ_risk / dz of each cell's level (0 on walls; on a crest
+                                     level with the crest lines held), or NULL */
     uint64_t steps;               /* N */
     int32_t segment_steps;        /* checkpoint spacing ceil(sqrt(N)) */
     int32_t checkpoints;
     uint64_t replay_mismatches;   /* replayed segment ends that differ bitwise from the checkpoint (0) */
     double plain_forward_ms;      /* the same forward without checkpoints (device time) */
     double forward_ms, replay_ms, reverse_ms;
+    double* dJ_drise_face_x;      /* crest levels: [ny][nx + 1][2] dJ_risk / d(d_l), d(d_r) of x-faces,
+                                     or NULL (not written on signed levels) */
+    double* dJ_drise_face_y;      /* crest levels: [ny + 1][nx][2] of y-faces, or NULL */
 } itr_adjoint_out;
 
 itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_grid* grid, const itr_objective* obj,

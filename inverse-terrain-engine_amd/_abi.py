@@ -87,7 +87,8 @@ class AdjointOutC(C.Structure):
     _fields_ = [("j_risk", C.c_double), ("peak_m", _dp), ("dJ_dz_face_x", _dp), ("dJ_dz_face_y", _dp),
                 ("dJ_dz_cell", _dp), ("steps", C.c_uint64), ("segment_steps", C.c_int32), ("checkpoints", C.c_int32),
                 ("replay_mismatches", C.c_uint64), ("plain_forward_ms", C.c_double), ("forward_ms", C.c_double),
-                ("replay_ms", C.c_double), ("reverse_ms", C.c_double)]
+                ("replay_ms", C.c_double), ("reverse_ms", C.c_double), ("dJ_drise_face_x", _dp),
+                ("dJ_drise_face_y", _dp)]
 
 
 class TuneResultC(C.Structure):

@@ -24,7 +24,9 @@ template <class T> struct V { T u, v, c; };
 template <class T> struct F { T m, nl, nr, t; };
 template <class T> struct S3 { T h, qx, qy; };
 template <class T> struct HllBar { T hl, ul, tl, cl, hr, ur, tr, cr; };
-template <class T> struct FaceBar { T hl, ul, vl, cl, hr, ur, vr, cr, dz; };
+// dz: the adjoint of the signed jump, or of the left rise d_l on a crest face; dz2: of
+// the right rise d_r (crest faces only).
+template <class T> struct FaceBar { T hl, ul, vl, cl, hr, ur, vr, cr, dz, dz2; };
 
 template <class T> __host__ __device__ inline T smx(T a, T b) { return a > b ? a : b; }   // real.rs:44-46
 template <class T> __host__ __device__ inline T smn(T a, T b) { return a < b ? a : b; }   // real.rs:48-50
@@ -233,6 +235,56 @@ __host__ __device__ inline FaceBar<T> face_adj(T hl, const V<T>& vl, T hr, const
     if (ar > T(0)) {
         o.hr += bhsr;
         if (-dz > T(0)) o.dz += bhsr;
+    }
+    return o;
+}
+
+// ------------------------------------------------------------ face_crest (numerics.rs:117-129)
+// dl, dr: the rises from the left / right bed to the face crest (both >= 0); a side with
+// a rise reconstructs its celerity from its reduced depth.
+template <bool NX, class T>
+__host__ __device__ inline F<T> face_cr(T hl, const V<T>& vl, T hr, const V<T>& vr, T dl, T dr, const Consts<T>& k) {
+    const T hsl = smx(hl - dl, T(0));
+    const T hsr = smx(hr - dr, T(0));
+    const T csl = dl > T(0) ? rt(k.g * hsl) : vl.c;
+    const T csr = dr > T(0) ? rt(k.g * hsr) : vr.c;
+    if (NX) return hll(hsl, vl.u, vl.v, csl, hsr, vr.u, vr.v, csr, k);
+    return hll(hsl, vl.v, vl.u, csl, hsr, vr.v, vr.u, csr, k);
+}
+
+template <bool NX, class T>
+__host__ __device__ inline FaceBar<T> face_cr_adj(T hl, const V<T>& vl, T hr, const V<T>& vr, T dl, T dr,
+                                                  const Consts<T>& k, const F<T>& b) {
+    const T al = hl - dl, ar = hr - dr;
+    const T hsl = smx(al, T(0)), hsr = smx(ar, T(0));
+    const T csl = dl > T(0) ? rt(k.g * hsl) : vl.c;
+    const T csr = dr > T(0) ? rt(k.g * hsr) : vr.c;
+    HllBar<T> hb = NX ? hll_adj(hsl, vl.u, vl.v, csl, hsr, vr.u, vr.v, csr, k, b)
+                      : hll_adj(hsl, vl.v, vl.u, csl, hsr, vr.v, vr.u, csr, k, b);
+    FaceBar<T> o{};
+    if (NX) {
+        o.ul = hb.ul; o.vl = hb.tl; o.ur = hb.ur; o.vr = hb.tr;
+    } else {
+        o.vl = hb.ul; o.ul = hb.tl; o.vr = hb.ur; o.ur = hb.tr;
+    }
+    T bhsl = hb.hl, bhsr = hb.hr;
+    if (dl > T(0)) {
+        if (csl > T(0)) bhsl += hb.cl * k.g / (T(2) * csl);
+    } else {
+        o.cl = hb.cl;
+    }
+    if (dr > T(0)) {
+        if (csr > T(0)) bhsr += hb.cr * k.g / (T(2) * csr);
+    } else {
+        o.cr = hb.cr;
+    }
+    if (al > T(0)) {
+        o.hl += bhsl;
+        o.dz -= bhsl;
+    }
+    if (ar > T(0)) {
+        o.hr += bhsr;
+        o.dz2 -= bhsr;
     }
     return o;
 }

@@ -2352,8 +2352,13 @@ struct AdjArgs {
     float* lamo;           // adjoint of the old state
     double* gx;            // [cand][ny][P] dJ/d(dzx) (west face of each cell)
     double* gy;            // [cand][ny+1][P] dJ/d(dzy) (north face of each cell)
-    const float* dzx;      // [cand][ny][P]
+    const float* dzx;      // [cand][ny][P] signed jump, or the left rise d_l of a crest face
     const float* dzy;      // [cand][ny+1][P]
+    const float* dzx2;     // crest levels: [cand][ny][P] the right rise d_r
+    const float* dzy2;     // crest levels: [cand][ny+1][P]
+    double* gx2;           // crest levels: dJ/d(d_r) of the west faces
+    double* gy2;           // crest levels: dJ/d(d_r) of the north faces
+    int32_t crest;         // face-crest level (numerics.rs:117-129)
     const float* wet;      // [ny][P]
     const float* gn2;      // [ny][P]
     const int32_t* ghost;  // [2nx + 2ny]
@@ -2409,6 +2414,7 @@ struct Stencil {
     bool wW, wE, wN, wS;
     Vel vC, vW, vE, vN, vS;
     float dzW, dzE, dzN, dzS, wet, gn2;
+    float dzW2, dzE2, dzN2, dzS2;   // crest levels: the right rises
 };
 
 __device__ __forceinline__ Stencil stencil(const AdjArgs& a, const float* U, int cand, int r, int c) {
@@ -2429,6 +2435,15 @@ __device__ __forceinline__ Stencil stencil(const AdjArgs& a, const float* U, int
     s.dzE = dzx[(size_t)r * a.P + c + 1];
     s.dzN = dzy[(size_t)r * a.P + c];
     s.dzS = dzy[(size_t)(r + 1) * a.P + c];
+    s.dzW2 = s.dzE2 = s.dzN2 = s.dzS2 = 0.0f;
+    if (a.crest) {
+        const float* dzx2 = a.dzx2 + (size_t)cand * a.dzx_stride;
+        const float* dzy2 = a.dzy2 + (size_t)cand * a.dzy_stride;
+        s.dzW2 = dzx2[(size_t)r * a.P + c];
+        s.dzE2 = dzx2[(size_t)r * a.P + c + 1];
+        s.dzN2 = dzy2[(size_t)r * a.P + c];
+        s.dzS2 = dzy2[(size_t)(r + 1) * a.P + c];
+    }
     s.wet = a.wet[(size_t)r * a.P + c];
     s.gn2 = a.gn2[(size_t)r * a.P + c];
     return s;
@@ -2440,10 +2455,18 @@ __global__ void __launch_bounds__(256) itr_replay_kernel(AdjArgs a) {
     if (c >= a.nx) return;
     const float* U = a.U + (size_t)cand * a.cstride;
     const Stencil s = stencil(a, U, cand, r, c);
-    const Flux fw = face_signed<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, a.K);
-    const Flux fe = face_signed<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, a.K);
-    const Flux fn = face_signed<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, a.K);
-    const Flux fs = face_signed<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, a.K);
+    Flux fw, fe, fn, fs;
+    if (a.crest) {
+        fw = face_crest<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, s.dzW2, a.K);
+        fe = face_crest<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, s.dzE2, a.K);
+        fn = face_crest<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, s.dzN2, a.K);
+        fs = face_crest<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, s.dzS2, a.K);
+    } else {
+        fw = face_signed<true>(s.W.h, s.vW, s.C.h, s.vC, s.dzW, a.K);
+        fe = face_signed<true>(s.C.h, s.vC, s.E.h, s.vE, s.dzE, a.K);
+        fn = face_signed<false>(s.N.h, s.vN, s.C.h, s.vC, s.dzN, a.K);
+        fs = face_signed<false>(s.C.h, s.vC, s.S.h, s.vS, s.dzS, a.K);
+    }
     const CellNew u = cell_update(s.C.h, s.C.qx, s.C.qy, fw.m, fw.nr, fw.t, fe.m, fe.nl, fe.t, fn.m, fn.nr, fn.t, fs.m,
                                   fs.nl, fs.t, s.wet, s.gn2, a.K);
     float* Uo = a.Uo + (size_t)cand * a.cstride;
@@ -2505,32 +2528,49 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
     };
     double* gx = a.gx + (size_t)cand * a.dzx_stride;
     double* gy = a.gy + (size_t)cand * a.dzy_stride;
+    double* gx2 = a.crest ? a.gx2 + (size_t)cand * a.dzx_stride : nullptr;
+    double* gy2 = a.crest ? a.gy2 + (size_t)cand * a.dzy_stride : nullptr;
+    // the face's adjoint: signed jump, or the two rises of a crest face
+    auto fadj_x = [&](const Nb& L, const Vel& vl, const Nb& R, const Vel& vr, float d, float d2, const adj::F<float>& b) {
+        return a.crest ? adj::face_cr_adj<true>(L.h, V3(vl), R.h, V3(vr), d, d2, k, b)
+                       : adj::face_adj<true>(L.h, V3(vl), R.h, V3(vr), d, k, b);
+    };
+    auto fadj_y = [&](const Nb& L, const Vel& vl, const Nb& R, const Vel& vr, float d, float d2, const adj::F<float>& b) {
+        return a.crest ? adj::face_cr_adj<false>(L.h, V3(vl), R.h, V3(vr), d, d2, k, b)
+                       : adj::face_adj<false>(L.h, V3(vl), R.h, V3(vr), d, k, b);
+    };
     {   // west face: L = W, R = C
         const adj::F<float> b{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
-        const adj::FaceBar<float> f = adj::face_adj<true>(s.W.h, V3(s.vW), s.C.h, V3(s.vC), s.dzW, k, b);
+        const adj::FaceBar<float> f = fadj_x(s.W, s.vW, s.C, s.vC, s.dzW, s.dzW2, b);
         bC.h += f.hr;
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
         if (c == 0) fold(s.W, s.wW, true, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
-        else gx[(size_t)r * a.P + c] += (double)f.dz;
+        else {
+            gx[(size_t)r * a.P + c] += (double)f.dz;
+            if (gx2) gx2[(size_t)r * a.P + c] += (double)f.dz2;
+        }
     }
     {   // east face: L = C, R = E
         const adj::F<float> b{dtdx * (lE.h - lC.h), -dtdx * lC.qx, dtdx * lE.qx, dtdx * (lE.qy - lC.qy)};
-        const adj::FaceBar<float> f = adj::face_adj<true>(s.C.h, V3(s.vC), s.E.h, V3(s.vE), s.dzE, k, b);
+        const adj::FaceBar<float> f = fadj_x(s.C, s.vC, s.E, s.vE, s.dzE, s.dzE2, b);
         bC.h += f.hl;
         bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
         if (c == a.nx - 1) fold(s.E, s.wE, true, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
     }
     {   // north face: L = N, R = C (normal velocity v)
         const adj::F<float> b{dtdy * (lC.h - lN.h), -dtdy * lN.qy, dtdy * lC.qy, dtdy * (lC.qx - lN.qx)};
-        const adj::FaceBar<float> f = adj::face_adj<false>(s.N.h, V3(s.vN), s.C.h, V3(s.vC), s.dzN, k, b);
+        const adj::FaceBar<float> f = fadj_y(s.N, s.vN, s.C, s.vC, s.dzN, s.dzN2, b);
         bC.h += f.hr;
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
         if (r == 0) fold(s.N, s.wN, false, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
-        else gy[(size_t)r * a.P + c] += (double)f.dz;
+        else {
+            gy[(size_t)r * a.P + c] += (double)f.dz;
+            if (gy2) gy2[(size_t)r * a.P + c] += (double)f.dz2;
+        }
     }
     {   // south face: L = C, R = S
         const adj::F<float> b{dtdy * (lS.h - lC.h), -dtdy * lC.qy, dtdy * lS.qy, dtdy * (lS.qx - lC.qx)};
-        const adj::FaceBar<float> f = adj::face_adj<false>(s.C.h, V3(s.vC), s.S.h, V3(s.vS), s.dzS, k, b);
+        const adj::FaceBar<float> f = fadj_y(s.C, s.vC, s.S, s.vS, s.dzS, s.dzS2, b);
         bC.h += f.hl;
         bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
         if (r == a.ny - 1) fold(s.S, s.wS, false, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
@@ -2552,7 +2592,7 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
 // the same.  Each interior face's jump gradient is written by the lane that owns its
 // east / south cell.  A row's loads are issued one row ahead.
 struct NbW { float h, qx, qy, wall; };
-struct RevIn { float h, qx, qy, lh, lqx, lqy, ph, pqx, pqy, wet, gn2, dzs, dzw; int code; };
+struct RevIn { float h, qx, qy, lh, lqx, lqy, ph, pqx, pqy, wet, gn2, dzs, dzw, dzs2, dzw2; int code; };
 __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
     const int band = blockIdx.x, cand = blockIdx.y;
     const int nx = a.nx, ny = a.ny;
@@ -2580,6 +2620,11 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
     const float* dzy = a.dzy + (size_t)cand * a.dzy_stride;
     double* gx = a.gx + (size_t)cand * a.dzx_stride;
     double* gy = a.gy + (size_t)cand * a.dzy_stride;
+    const bool crest = a.crest != 0;
+    const float* dzx2 = crest ? a.dzx2 + (size_t)cand * a.dzx_stride : nullptr;
+    const float* dzy2 = crest ? a.dzy2 + (size_t)cand * a.dzy_stride : nullptr;
+    double* gx2 = crest ? a.gx2 + (size_t)cand * a.dzx_stride : nullptr;
+    double* gy2 = crest ? a.gy2 + (size_t)cand * a.dzy_stride : nullptr;
     const adj::Consts<float> k = adj_consts(a.K);
     const float dtdx = a.K.dtdx, dtdy = a.K.dtdy;
     // Raw inputs of row k1: its old state (the interior neighbour for a ghost), the adjoint
@@ -2607,6 +2652,8 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         v.gn2 = a.gn2[ow];
         v.dzs = dzy[(size_t)min(max(k1, 0), ny) * a.P + cc];
         v.dzw = dzx[(size_t)min(max(k1, 0), ny - 1) * a.P + xc];
+        v.dzs2 = crest ? dzy2[(size_t)min(max(k1, 0), ny) * a.P + cc] : 0.0f;
+        v.dzw2 = crest ? dzx2[(size_t)min(max(k1, 0), ny - 1) * a.P + xc] : 0.0f;
         return v;
     };
     // The ghost rule (boundary.rs:12-87) on a loaded row: wall mirrors the normal
@@ -2631,9 +2678,9 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         return adj::sources_adj(v.ph, v.pqx, v.pqy, v.wet, v.gn2, k, adj::S3<float>{v.lh, v.lqx, v.lqy});
     };
     auto yface = [&](const NbW& L, const adj::V<float>& vl, const adj::S3<float>& ll, const NbW& R,
-                     const adj::V<float>& vr, const adj::S3<float>& lr, float dz) {
+                     const adj::V<float>& vr, const adj::S3<float>& lr, float dz, float dz2) {
         const adj::F<float> b{dtdy * (lr.h - ll.h), -dtdy * ll.qy, dtdy * lr.qy, dtdy * (lr.qx - ll.qx)};
-        return adj::face_adj<false>(L.h, vl, R.h, vr, dz, k, b);
+        return crest ? adj::face_cr_adj<false>(L.h, vl, R.h, vr, dz, dz2, k, b) : adj::face_adj<false>(L.h, vl, R.h, vr, dz, k, b);
     };
     const int g0 = a0 + lr;   // this lane's first row
     const RevIn ip = load(g0 - 1), ic = load(g0);
@@ -2641,11 +2688,14 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
     NbW Pn = cell(ip, g0 - 1), Cn = cell(ic, g0);
     adj::V<float> vC = velo(Cn);
     adj::S3<float> lC = lam1(ic, g0);
-    float dzw = ic.dzw;   // west face jump of the current row
+    float dzw = ic.dzw, dzw2 = ic.dzw2;   // west face jump (rises) of the current row
     adj::FaceBar<float> fN =
-        yface(Pn, velo(Pn), lam1(ip, g0 - 1), Cn, vC, lC, ic.dzs);
+        yface(Pn, velo(Pn), lam1(ip, g0 - 1), Cn, vC, lC, ic.dzs, ic.dzs2);
     // (a group's north face inside the band is the previous group's south face, counted there)
-    if (upd && a0 >= 1 && lr == 0) gy[(size_t)a0 * a.P + col] += (double)fN.dz;
+    if (upd && a0 >= 1 && lr == 0) {
+        gy[(size_t)a0 * a.P + col] += (double)fN.dz;
+        if (crest) gy2[(size_t)a0 * a.P + col] += (double)fN.dz2;
+    }
     for (int jb = a0; jb < a0 + t_rows; jb++) {
         const int j = jb + lr;   // this lane's row
         const bool rv = j < b0;
@@ -2654,16 +2704,24 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         const NbW Sn = cell(in, j + 1);
         const adj::V<float> vS = velo(Sn);
         const adj::S3<float> lS = lam1(in, j + 1);
-        const adj::FaceBar<float> fS = yface(Cn, vC, lC, Sn, vS, lS, in.dzs);
-        if (upd && j + 1 <= ny - 1 && j + 1 < b0) gy[(size_t)(j + 1) * a.P + col] += (double)fS.dz;
+        const adj::FaceBar<float> fS = yface(Cn, vC, lC, Sn, vS, lS, in.dzs, in.dzs2);
+        if (upd && j + 1 <= ny - 1 && j + 1 < b0) {
+            gy[(size_t)(j + 1) * a.P + col] += (double)fS.dz;
+            if (crest) gy2[(size_t)(j + 1) * a.P + col] += (double)fS.dz2;
+        }
         // West face: L = the left lane's cell, R = this lane's.
         const NbW Wn{from_left(Cn.h), from_left(Cn.qx), from_left(Cn.qy), from_left(Cn.wall)};
         const adj::V<float> vW{from_left(vC.u), from_left(vC.v), from_left(vC.c)};
         const adj::S3<float> lW{from_left(lC.h), from_left(lC.qx), from_left(lC.qy)};
         const adj::F<float> bw{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
-        const adj::FaceBar<float> fW = adj::face_adj<true>(Wn.h, vW, Cn.h, vC, dzw, k, bw);
-        if (upd && rv && col >= 1 && col <= nx - 1) gx[(size_t)j * a.P + col] += (double)fW.dz;
+        const adj::FaceBar<float> fW = crest ? adj::face_cr_adj<true>(Wn.h, vW, Cn.h, vC, dzw, dzw2, k, bw)
+                                             : adj::face_adj<true>(Wn.h, vW, Cn.h, vC, dzw, k, bw);
+        if (upd && rv && col >= 1 && col <= nx - 1) {
+            gx[(size_t)j * a.P + col] += (double)fW.dz;
+            if (crest) gx2[(size_t)j * a.P + col] += (double)fW.dz2;
+        }
         dzw = in.dzw;   // row j+1's west face
+        dzw2 = in.dzw2;
         // East face: the right lane's west face.
         adj::FaceBar<float> fE;
         fE.hl = from_right(fW.hl); fE.ul = from_right(fW.ul); fE.vl = from_right(fW.vl); fE.cl = from_right(fW.cl);
@@ -4339,10 +4397,6 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "the adjoint runs on a locked dt schedule (locked_dt)");
         return ITR_EINVAL;
     }
-    if (grid->crest) {
-        set_err(err, errlen, "the adjoint supports signed-face levels only");
-        return ITR_DECLINED;
-    }
     if (p->physics != ITR_PHYSICS_HLL) {
         set_err(err, errlen, "the adjoint differentiates the HLL step only");
         return ITR_DECLINED;
@@ -4443,8 +4497,9 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         return e;
     };
     float *ckpt, *seg, *pre, *lam, *lam1;
-    double *gx, *gy;
+    double *gx, *gy, *gx2 = nullptr, *gy2 = nullptr;
     unsigned long long* d_mis;
+    const bool crest = grid->crest != 0;
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(nseg + 1), (void**)&ckpt));
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(C + 1), (void**)&seg));
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)C, (void**)&pre));   // post-divergence state of each step
@@ -4452,6 +4507,10 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     ADJ_TRY(dalloc(sizeof(float) * state_n, (void**)&lam1));
     ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * plane, (void**)&gx));
     ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * D.plane_y, (void**)&gy));
+    if (crest) {
+        ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * plane, (void**)&gx2));
+        ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * D.plane_y, (void**)&gy2));
+    }
     ADJ_TRY(dalloc(sizeof(unsigned long long), (void**)&d_mis));
     auto copy_states = [&](float* dst, int parity) -> hipError_t {
         for (int c = 0; c < nc; c++) {
@@ -4525,10 +4584,19 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     ADJ_TRY(hipMemsetAsync(lam, 0, sizeof(float) * state_n * 2, st));
     ADJ_TRY(hipMemsetAsync(gx, 0, sizeof(double) * (size_t)nc * plane, st));
     ADJ_TRY(hipMemsetAsync(gy, 0, sizeof(double) * (size_t)nc * D.plane_y, st));
+    if (crest) {
+        ADJ_TRY(hipMemsetAsync(gx2, 0, sizeof(double) * (size_t)nc * plane, st));
+        ADJ_TRY(hipMemsetAsync(gy2, 0, sizeof(double) * (size_t)nc * D.plane_y, st));
+    }
     ADJ_TRY(hipMemsetAsync(d_mis, 0, sizeof(unsigned long long), st));
     AdjArgs A{};
     A.dzx = D.dzx;
     A.dzy = D.dzy;
+    A.crest = crest ? 1 : 0;
+    A.dzx2 = crest ? D.dzx2 : nullptr;
+    A.dzy2 = crest ? D.dzy2 : nullptr;
+    A.gx2 = gx2;
+    A.gy2 = gy2;
     A.wet = D.wet;
     A.gn2 = D.gn2;
     A.ghost = D.ghost;
@@ -4618,28 +4686,92 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     unsigned long long mis = 0;
     ADJ_TRY(hipMemcpy(&mis, d_mis, sizeof mis, hipMemcpyDeviceToHost));
     // 5. face and cell gradients
-    std::vector<double> hx((size_t)nc * plane), hy((size_t)nc * D.plane_y);
+    std::vector<double> hx((size_t)nc * plane), hy((size_t)nc * D.plane_y), hx2, hy2;
     ADJ_TRY(hipMemcpy(hx.data(), gx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost));
     ADJ_TRY(hipMemcpy(hy.data(), gy, sizeof(double) * hy.size(), hipMemcpyDeviceToHost));
+    if (crest) {
+        hx2.resize(hx.size());
+        hy2.resize(hy.size());
+        ADJ_TRY(hipMemcpy(hx2.data(), gx2, sizeof(double) * hx2.size(), hipMemcpyDeviceToHost));
+        ADJ_TRY(hipMemcpy(hy2.data(), gy2, sizeof(double) * hy2.size(), hipMemcpyDeviceToHost));
+    }
     for (int c = 0; c < nc; c++) {
         itr_adjoint_out& o = outs[c];
         const double* X = hx.data() + (size_t)c * plane;
         const double* Y = hy.data() + (size_t)c * D.plane_y;
+        // interior faces only: boundary faces have fixed zero jumps / rises
         auto fx = [&](int r, int f) { return (f >= 1 && f <= nx - 1) ? X[(size_t)r * P + f] : 0.0; };
         auto fy = [&](int f, int col) { return (f >= 1 && f <= ny - 1) ? Y[(size_t)f * P + col] : 0.0; };
-        if (o.dJ_dz_face_x)
-            for (int r = 0; r < ny; r++)
-                for (int f = 0; f <= nx; f++) o.dJ_dz_face_x[(size_t)r * (nx + 1) + f] = fx(r, f);
-        if (o.dJ_dz_face_y)
-            for (int f = 0; f <= ny; f++)
-                for (int col = 0; col < nx; col++) o.dJ_dz_face_y[(size_t)f * nx + col] = fy(f, col);
-        if (o.dJ_dz_cell)
-            for (int r = 0; r < ny; r++)
-                for (int col = 0; col < nx; col++) {
-                    const size_t i = (size_t)r * nx + col;
-                    o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0
-                                      : fx(r, col) - fx(r, col + 1) + fy(r, col) - fy(r + 1, col);
+        if (!crest) {
+            if (o.dJ_dz_face_x)
+                for (int r = 0; r < ny; r++)
+                    for (int f = 0; f <= nx; f++) o.dJ_dz_face_x[(size_t)r * (nx + 1) + f] = fx(r, f);
+            if (o.dJ_dz_face_y)
+                for (int f = 0; f <= ny; f++)
+                    for (int col = 0; col < nx; col++) o.dJ_dz_face_y[(size_t)f * nx + col] = fy(f, col);
+            if (o.dJ_dz_cell)
+                for (int r = 0; r < ny; r++)
+                    for (int col = 0; col < nx; col++) {
+                        const size_t i = (size_t)r * nx + col;
+                        o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0
+                                          : fx(r, col) - fx(r, col + 1) + fy(r, col) - fy(r + 1, col);
+                    }
+        } else {
+            // Crest faces (terrain.rs:378-429): zf = max(crest, z_l, z_r), d_l = zf - z_l,
+            // d_r = zf - z_r.  The rises tell which term is the maximum (a zero rise: that
+            // side's level), so the rise gradients split into the crest line's gradient (it
+            // moves zf only while it is the strict maximum) and the two cells' levels.
+            const double* X2 = hx2.data() + (size_t)c * plane;
+            const double* Y2 = hy2.data() + (size_t)c * D.plane_y;
+            const float* dl_x = grid->dzx + (size_t)c * ny * (nx + 1);
+            const float* dr_x = grid->dzx_r + (size_t)c * ny * (nx + 1);
+            const float* dl_y = grid->dzy + (size_t)c * (ny + 1) * nx;
+            const float* dr_y = grid->dzy_r + (size_t)c * (ny + 1) * nx;
+            std::vector<double> cell((size_t)nx * ny, 0.0);
+            // one face: rise gradients (bl, br), rises (dl, dr); returns the crest gradient
+            // and adds the levels' gradients to cells l and r
+            auto split = [&](double bl, double br, float dl, float dr, size_t l, size_t r) -> double {
+                if (dl > 0.0f && dr > 0.0f) {
+                    cell[l] -= bl;
+                    cell[r] -= br;
+                    return bl + br;
                 }
+                if (dl == 0.0f) {   // zf = z_l: d_l stays 0, d_r = z_l - z_r
+                    cell[l] += br;
+                    cell[r] -= br;
+                } else {            // zf = z_r: d_l = z_r - z_l
+                    cell[r] += bl;
+                    cell[l] -= bl;
+                }
+                return 0.0;
+            };
+            for (int r = 0; r < ny; r++)
+                for (int f = 0; f <= nx; f++) {
+                    const size_t i = (size_t)r * (nx + 1) + f;
+                    const bool in = f >= 1 && f <= nx - 1;
+                    const double bl = in ? X[(size_t)r * P + f] : 0.0, br = in ? X2[(size_t)r * P + f] : 0.0;
+                    const double gc = in ? split(bl, br, dl_x[i], dr_x[i], (size_t)r * nx + f - 1, (size_t)r * nx + f) : 0.0;
+                    if (o.dJ_dz_face_x) o.dJ_dz_face_x[i] = gc;
+                    if (o.dJ_drise_face_x) {
+                        o.dJ_drise_face_x[2 * i] = bl;
+                        o.dJ_drise_face_x[2 * i + 1] = br;
+                    }
+                }
+            for (int f = 0; f <= ny; f++)
+                for (int col = 0; col < nx; col++) {
+                    const size_t i = (size_t)f * nx + col;
+                    const bool in = f >= 1 && f <= ny - 1;
+                    const double bl = in ? Y[(size_t)f * P + col] : 0.0, br = in ? Y2[(size_t)f * P + col] : 0.0;
+                    const double gc = in ? split(bl, br, dl_y[i], dr_y[i], (size_t)(f - 1) * nx + col, (size_t)f * nx + col) : 0.0;
+                    if (o.dJ_dz_face_y) o.dJ_dz_face_y[i] = gc;
+                    if (o.dJ_drise_face_y) {
+                        o.dJ_drise_face_y[2 * i] = bl;
+                        o.dJ_drise_face_y[2 * i + 1] = br;
+                    }
+                }
+            if (o.dJ_dz_cell)
+                for (size_t i = 0; i < (size_t)nx * ny; i++) o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0 : cell[i];
+        }
         o.steps = N;
         o.segment_steps = C;
         o.checkpoints = nseg + 1;

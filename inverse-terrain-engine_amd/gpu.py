@@ -284,7 +284,11 @@ class GpuSolver:
         dict per candidate: j_risk, peak (per asset), dJ_dz_cell (ny, nx), dJ_dz_face_x
         (ny, nx+1), dJ_dz_face_y (ny+1, nx), steps, segment_steps, checkpoints,
         replay_mismatches and the device times of the plain forward, the checkpointed
-        forward, the replay and the reverse sweep (ms)."""
+        forward, the replay and the reverse sweep (ms).  On a face-crest level the face
+        entries are the gradients of the faces' crest lines, dJ_dz_cell holds the crest lines
+        fixed, and dJ_drise_face_x (ny, nx+1, 2) / dJ_drise_face_y (ny+1, nx, 2) are the
+        gradients of the rises (d_l, d_r); terrain.fine_gradient maps them to the fine
+        raster."""
         if p.locked_dt is None:
             raise ValueError("the adjoint needs p.locked_dt")
         keep = _Keep()
@@ -309,6 +313,11 @@ class GpuSolver:
         for i in range(n):
             b = {"peak": np.zeros(max(len(assets), 1)), "cell": np.zeros((ny, nx)), "fx": np.zeros((ny, nx + 1)),
                  "fy": np.zeros((ny + 1, nx))}
+            if base.crest:
+                b["rx"] = np.zeros((ny, nx + 1, 2))
+                b["ry"] = np.zeros((ny + 1, nx, 2))
+                outs[i].dJ_drise_face_x = b["rx"].ctypes.data_as(_abi._dp)
+                outs[i].dJ_drise_face_y = b["ry"].ctypes.data_as(_abi._dp)
             outs[i].peak_m = b["peak"].ctypes.data_as(_abi._dp)
             outs[i].dJ_dz_cell = b["cell"].ctypes.data_as(_abi._dp)
             outs[i].dJ_dz_face_x = b["fx"].ctypes.data_as(_abi._dp)
@@ -324,6 +333,9 @@ class GpuSolver:
                         "segment_steps": o.segment_steps, "checkpoints": o.checkpoints,
                         "replay_mismatches": int(o.replay_mismatches), "plain_forward_ms": o.plain_forward_ms,
                         "forward_ms": o.forward_ms, "replay_ms": o.replay_ms, "reverse_ms": o.reverse_ms})
+            if base.crest:
+                res[-1]["dJ_drise_face_x"] = b["rx"]
+                res[-1]["dJ_drise_face_y"] = b["ry"]
         return res
 
     def record_tape(self, p: SolverParams, base: TerrainBase, roi, monitors=(), max_steps: int = 100000,

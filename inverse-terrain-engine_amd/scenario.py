@@ -297,21 +297,22 @@ class AdjointCase:
     earth_weight: float
 
 
-def config4(seed=4, duration=2400.0) -> AdjointCase:
+def config4(seed=4, duration=2400.0, level=1) -> AdjointCase:
     """120x96 at 5 m (10 hills + 3 gullies), 40 min storm peaking at 80 mm/h at 15 min,
     infiltration 5 mm/h, CFL 0.9 (about 2000 steps), one design of 3 primitives; two 4x4
     assets (thresholds 0.02 and 0.04 m); tau 0.01 m, lambda 1e-3 per m^3.  The dt schedule is frozen before differentiating (the caller
-    sets params.locked_dt from a forward run)."""
+    sets params.locked_dt from a forward run).  level=2: the same terrain and design on
+    the 60x48 face-crest level at 10 m (assets 2x2 level cells over the same ground)."""
     nx, ny, dx = 120, 96, 5.0
     rng = SplitMix64(seed)
     dem = hills_dem(nx, ny, dx, rng, 10, n_gullies=3)
     geo = GeoTransform(0.0, ny * dx, dx, dx)
-    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1, [{"kind": "transmissive", "side": "east"}])
+    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), level, [{"kind": "transmissive", "side": "east"}])
     p = SolverParams(duration, 60.0, cfl=0.9, rain=mm_h([[0, 0], [900, 80], [2400, 0]]),
                      infil_capacity=5.0 / 1000.0 / 3600.0)
     p.cfl_hard = 1.0   # the locked schedule runs at CFL 0.9
-    boxes = [(40, 30, 4, 4), (70, 60, 4, 4)]
-    mon, _, apos = asset_and_guard_cells(nx, ny, boxes, range(nx - 10, nx))
+    boxes = [(c // level, r // level, w // level, h // level) for c, r, w, h in [(40, 30, 4, 4), (70, 60, 4, 4)]]
+    mon, _, apos = asset_and_guard_cells(base.nx, base.ny, boxes, range(base.nx - 10 // level, base.nx))
     prims = random_primitives(rng, 3, geo, nx, ny)
     edit = materialize(prims, geo, nx, ny, None, 1.0)
     assets = [(np.array(a, np.int32), th, 1.0) for a, th in zip(apos, (0.02, 0.04))]

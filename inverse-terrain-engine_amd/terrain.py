@@ -195,13 +195,12 @@ class TerrainBase:
         return bool((self.ghost_codes() == GHOST_STAGE).any())
 
 
-def _crest_x_block(fine: FineTerrain, lv, zfine, z, r0, r1, ia, ib):
-    """face_x crest (terrain.rs:378-402) for the x-faces i in [ia, ib) of coarse rows
-    [r0, r1), as arrays: the max over the non-wall fine cells either side of each face
-    (fine columns i*lv-1 and min(i*lv, nx-1); walls and rows past the grid read -inf),
-    z_wall where all of them are walls, then max with both coarse levels; the rises to the
-    crest from the left and right levels, rounded to f32.  Maxima of finite values do not
-    depend on their order, so this equals the reference's loop."""
+def _crest_line_x(fine: FineTerrain, lv, zfine, r0, r1, ia, ib):
+    """The crest lines of face_x (terrain.rs:383-397) for the x-faces i in [ia, ib) of
+    coarse rows [r0, r1), as an array: the max over the non-wall fine cells either side of
+    each face (fine columns i*lv-1 and min(i*lv, nx-1); walls and rows past the grid read
+    -inf), z_wall where all of them are walls.  Maxima of finite values do not depend on
+    their order, so this equals the reference's loop."""
     rr = np.arange(r0, r1)[:, None] * lv + np.arange(lv)[None, :]          # [nr][lv] fine rows
     out = rr >= fine.ny
     rri = np.minimum(rr, fine.ny - 1)
@@ -210,16 +209,13 @@ def _crest_x_block(fine: FineTerrain, lv, zfine, z, r0, r1, ia, ib):
     R, Cc = rri[:, :, None, None], cols[None, None, :, :]
     wall = fine.wall[R, Cc] | out[:, :, None, None]
     crest = np.where(wall, -np.inf, zfine[R, Cc]).max(axis=(1, 3))
-    crest = np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
-    zl, zr = z[r0:r1, ia - 1:ib - 1], z[r0:r1, ia:ib]
-    zf = np.maximum(np.maximum(crest, zl), zr)
-    return (zf - zl).astype(np.float32), (zf - zr).astype(np.float32)
+    return np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
 
 
-def _crest_y_block(fine: FineTerrain, lv, zfine, z, fa, fb, c0, c1):
-    """face_y crest (terrain.rs:406-429) for the y-faces fa .. fb-1 (face f joins coarse
-    rows f-1 and f; fine rows f*lv-1 and min(f*lv, ny-1)) of columns [c0, c1), as arrays,
-    the same way as _crest_x_block."""
+def _crest_line_y(fine: FineTerrain, lv, zfine, fa, fb, c0, c1):
+    """The crest lines of face_y (terrain.rs:410-424) for the y-faces fa .. fb-1 (face f
+    joins coarse rows f-1 and f; fine rows f*lv-1 and min(f*lv, ny-1)) of columns [c0, c1),
+    the same way as _crest_line_x."""
     cc = np.arange(c0, c1)[:, None] * lv + np.arange(lv)[None, :]          # [nc][lv] fine columns
     out = cc >= fine.nx
     cci = np.minimum(cc, fine.nx - 1)
@@ -228,7 +224,23 @@ def _crest_y_block(fine: FineTerrain, lv, zfine, z, fa, fb, c0, c1):
     R, Cc = rows[:, :, None, None], cci[None, None, :, :]                     # [nf][2][nc][lv]
     wall = fine.wall[R, Cc] | out[None, None, :, :]
     crest = np.where(wall, -np.inf, zfine[R, Cc]).max(axis=(1, 3))
-    crest = np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
+    return np.where(wall.all(axis=(1, 3)), fine.z_wall, crest)
+
+
+def _crest_x_block(fine: FineTerrain, lv, zfine, z, r0, r1, ia, ib):
+    """face_x crest (terrain.rs:378-402) for the x-faces i in [ia, ib) of coarse rows
+    [r0, r1), as arrays: the crest line (_crest_line_x), then max with both coarse levels;
+    the rises to the crest from the left and right levels, rounded to f32."""
+    crest = _crest_line_x(fine, lv, zfine, r0, r1, ia, ib)
+    zl, zr = z[r0:r1, ia - 1:ib - 1], z[r0:r1, ia:ib]
+    zf = np.maximum(np.maximum(crest, zl), zr)
+    return (zf - zl).astype(np.float32), (zf - zr).astype(np.float32)
+
+
+def _crest_y_block(fine: FineTerrain, lv, zfine, z, fa, fb, c0, c1):
+    """face_y crest (terrain.rs:406-429) for the y-faces fa .. fb-1 of columns [c0, c1), as
+    arrays, the same way as _crest_x_block."""
+    crest = _crest_line_y(fine, lv, zfine, fa, fb, c0, c1)
     zt, zb = z[fa - 1:fb - 1, c0:c1], z[fa:fb, c0:c1]
     zf = np.maximum(np.maximum(crest, zt), zb)
     return (zf - zt).astype(np.float32), (zf - zb).astype(np.float32)
@@ -335,6 +347,58 @@ class PreparedTerrain:
         self.fx0, self.fx1, self.fy0, self.fy1 = fx0, fx1, fy0, fy1
         self.z_level = z
         return self
+
+
+def fine_gradient(base: TerrainBase, edit: TerrainEdit | None, dJ_dz_cell, dJ_dz_face_x=None, dJ_dz_face_y=None):
+    """d J / d(fine z) on the fine raster from the gradients the adjoint reports on a level.
+
+    Level 1 (signed faces): the level is the fine raster, so this is dJ_dz_cell.  A
+    face-crest level (terrain.rs:357-429): a level z is the mean of its block's open fine
+    cells, so dJ_dz_cell (crest lines held) spreads evenly over them; a face's crest line is
+    the maximum of the open fine cells either side of the face, so its gradient goes to the
+    cell that holds that maximum (the first in the reference's loop order, rows then the two
+    columns, on a tie).  ``edit``: the candidate's edit on the fine raster (the maxima are
+    taken over the edited terrain)."""
+    fine, lv = base.fine, base.level
+    if not base.crest:
+        return np.asarray(dJ_dz_cell, np.float64).copy()
+    nx, ny = base.nx, base.ny
+    zfine = fine.z + (edit.dense(fine.nx, fine.ny) if edit is not None and not edit.is_empty() else 0.0)
+    cnt = np.zeros((ny, nx), np.int64)
+    for dr in range(lv):
+        for dc in range(lv):
+            rr = np.arange(ny) * lv + dr
+            cc = np.arange(nx) * lv + dc
+            valid = (rr < fine.ny)[:, None] & (cc < fine.nx)[None, :]
+            ww = fine.wall[np.ix_(np.minimum(rr, fine.ny - 1), np.minimum(cc, fine.nx - 1))]
+            cnt += valid & ~ww
+    g = np.zeros((fine.ny, fine.nx))
+    fr, fc = np.arange(fine.ny) // lv, np.arange(fine.nx) // lv
+    share = np.where(cnt > 0, np.asarray(dJ_dz_cell, np.float64) / np.maximum(cnt, 1), 0.0)
+    g += np.where(fine.wall, 0.0, share[np.ix_(fr, fc)])
+    gx = np.zeros((ny, nx + 1)) if dJ_dz_face_x is None else np.asarray(dJ_dz_face_x, np.float64)
+    gy = np.zeros((ny + 1, nx)) if dJ_dz_face_y is None else np.asarray(dJ_dz_face_y, np.float64)
+    for r, i in zip(*np.nonzero(gx)):
+        if not 1 <= i <= nx - 1:
+            continue
+        best, at = -np.inf, None
+        for rr in range(r * lv, min((r + 1) * lv, fine.ny)):
+            for cc in (i * lv - 1, min(i * lv, fine.nx - 1)):
+                if not fine.wall[rr, cc] and zfine[rr, cc] > best:
+                    best, at = zfine[rr, cc], (rr, cc)
+        if at is not None:
+            g[at] += gx[r, i]
+    for f, c in zip(*np.nonzero(gy)):
+        if not 1 <= f <= ny - 1:
+            continue
+        best, at = -np.inf, None
+        for cc in range(c * lv, min((c + 1) * lv, fine.nx)):
+            for rr in (f * lv - 1, min(f * lv, fine.ny - 1)):
+                if not fine.wall[rr, cc] and zfine[rr, cc] > best:
+                    best, at = zfine[rr, cc], (rr, cc)
+        if at is not None:
+            g[at] += gy[f, c]
+    return g
 
 
 # ---------------------------------------------------------------- design.rs:95-385

@@ -109,6 +109,35 @@ int main() {
             };
             dir_check(nm, {0.45, 0.08, -0.03, 0.3, 0.05, 0.02, dz}, 4, fwd, adj, 1e-8);
         }
+    // crest faces: (hl, qxl, qyl, hr, qxr, qyr, d_l, d_r), x and y normals; a rise on one
+    // side (the other at the crest), on both, and a left side drained below the crest
+    struct CP { const char* n; double dl, dr, hl; };
+    std::vector<CP> cp = {{"left rise", 0.12, 0.0, 0.45}, {"right rise", 0.0, 0.09, 0.45},
+                          {"both rises", 0.05, 0.2, 0.45}, {"left below crest", 0.6, 0.1, 0.45}};
+    for (int nx = 0; nx < 2; nx++)
+        for (auto& q : cp) {
+            char nm[64];
+            std::snprintf(nm, sizeof nm, "crest face %s %s", nx ? "x" : "y", q.n);
+            auto fwd = [&, nx](const std::vector<double>& x) {
+                V<T> vl = vel(x[0], x[1], x[2], k), vr = vel(x[3], x[4], x[5], k);
+                F<T> f = nx ? face_cr<true>(x[0], vl, x[3], vr, x[6], x[7], k) : face_cr<false>(x[0], vl, x[3], vr, x[6], x[7], k);
+                return std::vector<double>{f.m, f.nl, f.nr, f.t};
+            };
+            auto adj = [&, nx](const std::vector<double>& x, const std::vector<double>& b) {
+                V<T> vl = vel(x[0], x[1], x[2], k), vr = vel(x[3], x[4], x[5], k);
+                F<T> fb{b[0], b[1], b[2], b[3]};
+                FaceBar<T> o = nx ? face_cr_adj<true>(x[0], vl, x[3], vr, x[6], x[7], k, fb)
+                                  : face_cr_adj<false>(x[0], vl, x[3], vr, x[6], x[7], k, fb);
+                S3<T> l = vel_adj(x[0], x[1], x[2], k, V<T>{o.ul, o.vl, o.cl});
+                S3<T> r = vel_adj(x[3], x[4], x[5], k, V<T>{o.ur, o.vr, o.cr});
+                return std::vector<double>{l.h + o.hl, l.qx, l.qy, r.h + o.hr, r.qx, r.qy, o.dz, o.dz2};
+            };
+            // a zero rise sits on the select's switch: held fixed (one-sided there)
+            std::vector<int> fixed;
+            if (q.dl == 0.0) fixed.push_back(6);
+            if (q.dr == 0.0) fixed.push_back(7);
+            dir_check(nm, {q.hl, 0.08, -0.03, 0.3, 0.05, 0.02, q.dl, q.dr}, 4, fwd, adj, 1e-8, fixed);
+        }
     // sources: (h1, qx1, qy1) with wet cells, capacity-limited and not, q flushing
     struct SP { const char* n; std::vector<double> x; double rain, cap; };
     std::vector<SP> sp = {{"sources wet capped", {0.3, 0.05, -0.02}, 2e-6, 3e-7},

@@ -1,7 +1,9 @@
 """Writes tests/golden/adjoint_fd_config4.json: f64 central finite differences of the
 objective J = J_risk + lambda * J_earth on the 120x96 adjoint workload (scenario.config4)
 with the dt schedule frozen, at 32 seeded cells (d J / d dz of the cell) and for every
-primitive height.  The schedule is the f32 forward's own (the oracle, which is bitwise
+primitive height.  With --level 2, tests/golden/adjoint_fd_config4_level2.json: the same
+design on the 60x48 face-crest level (the differences are of the fine raster's dz, which
+reaches the level through its block means and crest lines).  The schedule is the f32 forward's own (the oracle, which is bitwise
 the reference's f32 fused kernel); every differenced run is the oracle in f64 on that
 schedule.
 
@@ -10,7 +12,7 @@ rain-fed films of a few mm switch the hydrostatic reconstruction max(h - dz, 0) 
 dry-cell test within +-1e-4 m of the bed, so that difference is not the derivative
 (it is kept for reference).  1e-6 and 1e-7 agree to ~1e-7 relative: J is smooth at that
 scale and the 1e-6 difference is the derivative the adjoint is checked against.
-Usage: python tests/golden/make_adjoint_fd.py  (about a minute on 8 cores)."""
+Usage: python tests/golden/make_adjoint_fd.py [--level 2]  (about a minute on 8 cores)."""
 from __future__ import annotations
 
 import json
@@ -37,14 +39,27 @@ STEPS = (1e-4, 1e-6, 1e-7)   # FD steps on dz and on heights (m)
 
 
 def fd_cells(case, per_asset=16, seed=41):
-    """16 seeded cells within 4 cells of each 4x4 asset box (rows and columns
-    first-4 .. first+7)."""
+    """Level 1: 16 seeded cells within 4 cells of each 4x4 asset box (rows and columns
+    first-4 .. first+7).  A coarser level: every fine cell of the level cells of each asset
+    box and the ring of level cells around it (their block means and the crest lines of
+    their faces)."""
     rng = np.random.default_rng(seed)
-    nx, ny = case.base.nx, case.base.ny
+    lv, nxl, nyl = case.base.level, case.base.nx, case.base.ny
+    nx, ny = case.base.fine.nx, case.base.fine.ny
     cells = []
+    if lv > 1:
+        for pos, _, _ in case.assets:
+            cs = case.monitors[pos]
+            ra, rb = int(cs.min() // nxl) - 1, int(cs.max() // nxl) + 1
+            ca, cb = int((cs % nxl).min()) - 1, int((cs % nxl).max()) + 1
+            for r in range(max(ra, 0), min(rb, nyl - 1) + 1):
+                for c in range(max(ca, 0), min(cb, nxl - 1) + 1):
+                    cells += [(r * lv + i, c * lv + j) for i in range(lv) for j in range(lv)
+                              if r * lv + i < ny and c * lv + j < nx and (r * lv + i, c * lv + j) not in cells]
+        return cells
     for pos, _, _ in case.assets:
         cs = case.monitors[pos]
-        r0, c0 = int(cs.min() // nx), int((cs % nx).min())
+        r0, c0 = int(cs.min() // nxl) * lv, int((cs % nxl).min()) * lv
         picked = []
         while len(picked) < per_asset:
             r = int(np.clip(r0 + rng.integers(-4, 8), 0, ny - 1))
@@ -55,11 +70,11 @@ def fd_cells(case, per_asset=16, seed=41):
     return cells
 
 
-def main():
-    case = sc.config4()
+def main(level=1):
+    case = sc.config4(level=level)
     base, p = case.base, case.params
-    nx, ny = base.nx, base.ny
-    area = base.dx * base.dy
+    nx, ny = base.fine.nx, base.fine.ny   # the edit's raster
+    area = base.fine.geo.dx * base.fine.geo.dy
     objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
     terr = cf.oracle_terrain(base)
     f32 = cf.po.run(terr, cf.oracle_params(p), precision=32, edit=case.edit.as_dict(), monitors=case.monitors,
@@ -101,12 +116,15 @@ def main():
         g_h = [(v[off + 2 * i] - v[off + 2 * i + 1]) / (2 * H) for i in range(len(prims))]
         fd[repr(H)] = {"dJ_dz_cell": g_cells, "dJ_dheight": g_h}
     out = {
-        "workload": "scenario.config4(): 120x96 at 5 m, 40 min storm, CFL 0.9, locked dt from the f32 forward",
+        "workload": (f"scenario.config4(level={level}): " + ("120x96 at 5 m" if level == 1 else
+                     "the 60x48 face-crest level at 10 m of the 120x96 5 m raster") +
+                     ", 40 min storm, CFL 0.9, locked dt from the f32 forward"),
         "steps": int(steps), "j0": j0, "locked_dt": [float(x) for x in sched],
         "cells": [[int(r), int(c)] for r, c in cells], "heights": [float(x.height) for x in prims],
         "fd": fd, "fd_step_checked": repr(1e-6),
     }
-    with open(os.path.join(HERE, "adjoint_fd_config4.json"), "w") as f:
+    name = "adjoint_fd_config4.json" if level == 1 else f"adjoint_fd_config4_level{level}.json"
+    with open(os.path.join(HERE, name), "w") as f:
         json.dump(out, f)
     for H in STEPS:
         g = fd[repr(H)]
@@ -115,4 +133,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(int(sys.argv[sys.argv.index("--level") + 1]) if "--level" in sys.argv else 1)

@@ -1006,17 +1006,40 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             return q;
         };
 
-        // One row of the march: faces, update, stores, row flags; the row below becomes the
-        // current one.  `prefetch` issues the next row's loads once this row's inputs have
-        // arrived (the scheduling barrier keeps them after that wait: the march's stores make
-        // every wait a full drain of the vector-memory counter, so a load issued before it
-        // would be waited for at once).
+        // The new state of a row is stored one row late, right after the next row's wait:
+        // the vector-memory counter counts loads and stores together, and with loads and
+        // stores both pending every wait is a full drain, so a row's stores issued at its end
+        // would hold up the next row's start by their whole latency.  Issued after the wait,
+        // they complete under a row of work like the prefetched loads.
+        float held_h = 0.0f, held_qx = 0.0f, held_qy = 0.0f;
+        int held_row = -1;        // the row whose new state is held (-1: none)
+        bool held_rv = false;     // tail wave: the lane's held row is a row of the band
+        auto flush = [&]() {
+            if (held_row >= 0 && upd && (!TL || held_rv)) {
+                if constexpr (TL) {
+                    const uint32_t vo = cb + (uint32_t)(held_row + lr) * rowb;
+                    bstore(held_h, rs_ho, vo, 0u);
+                    bstore(held_qx, rs_ho, vo, planeb);
+                    bstore(held_qy, rs_ho, vo, 2 * planeb);
+                } else {
+                    const uint32_t so = (uint32_t)held_row * rowb;
+                    bstore(held_h, rs_ho, cb, so);
+                    bstore(held_qx, rs_ho, cb, so + planeb);
+                    bstore(held_qy, rs_ho, cb, so + 2 * planeb);
+                }
+            }
+        };
+        // One row of the march: faces, update, row flags; the row below becomes the current
+        // one.  `prefetch` issues the next row's loads once this row's inputs have arrived,
+        // then the previous row's stores go out (the scheduling barrier keeps both after
+        // that wait).
         auto row_step = [&](int j, const RowIn& in, auto&& prefetch) {
             const int jl = j + lr;          // this lane's row
             const bool rv = !TL || jl < b0;   // a row of the band (a tail group's last rows may not be)
             const Cell cn = fix(in.c, jl + 1);
             __builtin_amdgcn_sched_barrier(0);
             prefetch();
+            flush();
             __builtin_amdgcn_sched_barrier(0);
             const float nh = cn.h, nqx = cn.qx, nqy = cn.qy;
             const float dzs = in.dzs, dzs2 = in.dzs2, dzw = in.dzw, dzw2 = in.dzw2;
@@ -1055,18 +1078,12 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
                     }
                 }
             }
+            held_h = u.h;
+            held_qx = u.qx;
+            held_qy = u.qy;
+            held_row = j;
+            held_rv = rv;
             if (upd && rv) {
-                if constexpr (TL) {
-                    const uint32_t vo = cb + (uint32_t)jl * rowb;
-                    bstore(u.h, rs_ho, vo, 0u);
-                    bstore(u.qx, rs_ho, vo, planeb);
-                    bstore(u.qy, rs_ho, vo, 2 * planeb);
-                } else {
-                    const uint32_t so = (uint32_t)j * rowb;
-                    bstore(u.h, rs_ho, cb, so);
-                    bstore(u.qx, rs_ho, cb, so + planeb);
-                    bstore(u.qy, rs_ho, cb, so + 2 * planeb);
-                }
                 l_infil[r * nx + col] = u.infil;
                 smax_bits = max(smax_bits, __float_as_int(u.s));
                 if (sync) sbuf[(size_t)jl * P + ccol] = u.s;
@@ -1110,6 +1127,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             row_step(j + 1, in1, [&] { in0 = load_in(j + 2); });   // (clamped loads; unused past the band)
         }
         if (j < j_end) row_step(j, in0, [] {});
+        flush();
         if constexpr (!TL) {
             if (b0 == ny && owner) yb_out[nx + col] = fn.m;
             if (SPECIAL && rec && ring_owner && b0 == ny) ring_put(ny, ch, cqx, cqy);   // the south ghost row

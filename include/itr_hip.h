@@ -235,7 +235,8 @@ const char* itr_hip_device_info(itr_ctx* ctx);
  * per host round trip), "use_graph" (0/1: capture the chunk in a hipGraph), "min_lds"
  * (bytes of LDS reserved per step workgroup, which caps occupancy), "adjoint_mode"
  * (reverse sweep of itr_hip_adjoint: 1 band march, 2 per-cell kernels; both give the
- * same bits), "control_task" (who runs each candidate's ledger sums and writes its
+ * same bits), "adjoint_rows" (rows per band of the reverse band march; 0: 16; the same
+ * bits at any height), "control_task" (who runs each candidate's ledger sums and writes its
  * control block: 1 a workgroup of its own beside the bands, 0 its first band after its
  * march, -1 chosen per launch list: 1 when the list's bands fit the chip at once; both
  * give the same bits).  0 = default (adjoint_mode: band march for large batches;

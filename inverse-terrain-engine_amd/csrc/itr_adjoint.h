@@ -31,14 +31,22 @@ template <class T> struct FaceBar { T hl, ul, vl, cl, hr, ur, vr, cr, dz, dz2; }
 template <class T> __host__ __device__ inline T smx(T a, T b) { return a > b ? a : b; }   // real.rs:44-46
 template <class T> __host__ __device__ inline T smn(T a, T b) { return a < b ? a : b; }   // real.rs:48-50
 template <class T> __host__ __device__ inline T sab(T x) { return x < T(0) ? -x : x; }    // real.rs:39-41
+template <class T> __host__ __device__ inline T rcp(T x) { return T(1) / x; }
 template <class T> __host__ __device__ inline T rt(T x) { return sqrt(x); }
 // x^(-1/3) to working precision; the derivative blocks use its exact derivative -r/(3x).
 template <class T> __host__ __device__ inline T icbrt(T x) { return T(1) / cbrt(x); }
+// On the device (f32) the reciprocal and square root are the hardware estimates
+// (v_rcp_f32, v_sqrt_f32, about 1 ulp): a derivative needs no correct rounding (gradients
+// are checked against finite differences to 1e-3), and the forward replay, which must
+// equal the forward bitwise, does not use these blocks.  The host checks run in double.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float rt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float icbrt(float x) { return __builtin_amdgcn_rcpf(cbrtf(x)); }
 
 // ------------------------------------------------------------ cell_vel (numerics.rs:36-43)
 template <class T> __host__ __device__ inline V<T> vel(T h, T qx, T qy, const Consts<T>& k) {
     const T h2 = h * h;
-    const T inv = T(1) / (h2 + smx(h2, k.heps2));
+    const T inv = rcp(h2 + smx(h2, k.heps2));
     const T t = (h + h) * inv;
     return V<T>{qx * t, qy * t, rt(k.g * h)};
 }
@@ -47,7 +55,7 @@ __host__ __device__ inline S3<T> vel_adj(T h, T qx, T qy, const Consts<T>& k, co
     const T h2 = h * h;
     const bool big = h2 > k.heps2;
     const T D = h2 + (big ? h2 : k.heps2);
-    const T inv = T(1) / D;
+    const T inv = rcp(D);
     const T t = (h + h) * inv;
     const T dD = (h + h) + (big ? h + h : T(0));
     const T bt = b.u * qx + b.v * qy;
@@ -56,7 +64,7 @@ __host__ __device__ inline S3<T> vel_adj(T h, T qx, T qy, const Consts<T>& k, co
     o.qy = b.v * t;
     o.h = bt * (T(2) * inv - (h + h) * dD * inv * inv);
     const T c = rt(k.g * h);
-    if (c > T(0)) o.h += b.c * k.g / (T(2) * c);
+    if (c > T(0)) o.h += b.c * k.g * rcp(T(2) * c);
     return o;
 }
 
@@ -75,7 +83,7 @@ __host__ __device__ inline F<T> hll(T hl, T ul, T tl, T cl, T hr, T ur, T tr, T 
     const T pl = ml * ul + pl_, pr = mr * ur + pr_;
     const bool both_dry = dry_l && dry_r, use_l = sl >= T(0), use_r = sr <= T(0);
     const T den = (both_dry || use_l || use_r) ? T(1) : sr - sl;
-    const T inv = T(1) / den;
+    const T inv = rcp(den);
     const T slsr = sl * sr;
     const T fm_star = (sr * ml - sl * mr + slsr * (hr - hl)) * inv;
     const T fn_star = (sr * pl - sl * pr + slsr * (mr - ml)) * inv;
@@ -102,7 +110,7 @@ __host__ __device__ inline HllBar<T> hll_adj(T hl, T ul, T tl, T cl, T hr, T ur,
     const T pl = ml * ul + pl_, pr = mr * ur + pr_;
     const bool both_dry = dry_l && dry_r, use_l = sl >= T(0), use_r = sr <= T(0);
     const bool star = !(both_dry || use_l || use_r);
-    const T inv = star ? T(1) / (sr - sl) : T(1);
+    const T inv = star ? rcp(sr - sl) : T(1);
     const T slsr = sl * sr;
     const T A = sr * ml - sl * mr + slsr * (hr - hl);
     const T B = sr * pl - sl * pr + slsr * (mr - ml);
@@ -224,7 +232,7 @@ __host__ __device__ inline FaceBar<T> face_adj(T hl, const V<T>& vl, T hr, const
         bcred = hb.cr;
         o.cl = hb.cl;
     }
-    const T bhred = cred > T(0) ? bcred * k.g / (T(2) * cred) : T(0);
+    const T bhred = cred > T(0) ? bcred * k.g * rcp(T(2) * cred) : T(0);
     T bhsl = hb.hl, bhsr = hb.hr;
     if (pos) bhsl += bhred;
     else bhsr += bhred;
@@ -269,12 +277,12 @@ __host__ __device__ inline FaceBar<T> face_cr_adj(T hl, const V<T>& vl, T hr, co
     }
     T bhsl = hb.hl, bhsr = hb.hr;
     if (dl > T(0)) {
-        if (csl > T(0)) bhsl += hb.cl * k.g / (T(2) * csl);
+        if (csl > T(0)) bhsl += hb.cl * k.g * rcp(T(2) * csl);
     } else {
         o.cl = hb.cl;
     }
     if (dr > T(0)) {
-        if (csr > T(0)) bhsr += hb.cr * k.g / (T(2) * csr);
+        if (csr > T(0)) bhsr += hb.cr * k.g * rcp(T(2) * csr);
     } else {
         o.cr = hb.cr;
     }
@@ -300,7 +308,7 @@ __host__ __device__ inline S3<T> sources(T h1, T qx1, T qy1, T wet, T gn2, const
     const T r = icbrt(dry ? T(1) : h3);
     const T r3 = r * r * r, r7 = r3 * r3 * r;
     const T qm = rt(qx1 * qx1 + qy1 * qy1);
-    const T inv = T(1) / (T(1) + k.dt * gn2 * qm * r7);
+    const T inv = rcp(T(1) + k.dt * gn2 * qm * r7);
     const T keep = dry ? T(0) : T(1);
     T qx2 = qx1 * inv * keep, qy2 = qy1 * inv * keep;
     if (sab(qx2) < k.q_min) qx2 = T(0);
@@ -323,7 +331,7 @@ __host__ __device__ inline S3<T> sources_adj(T h1, T qx1, T qy1, T wet, T gn2, c
         const T r3 = r * r * r, r7 = r3 * r3 * r;
         const T qm = rt(qx1 * qx1 + qy1 * qy1);
         const T Kf = k.dt * gn2;
-        const T inv = T(1) / (T(1) + Kf * qm * r7);
+        const T inv = rcp(T(1) + Kf * qm * r7);
         const T qx2 = qx1 * inv, qy2 = qy1 * inv;
         const T bqx2 = sab(qx2) < k.q_min ? T(0) : b.qx;
         const T bqy2 = sab(qy2) < k.q_min ? T(0) : b.qy;
@@ -334,10 +342,11 @@ __host__ __device__ inline S3<T> sources_adj(T h1, T qx1, T qy1, T wet, T gn2, c
         const T bqm = bden * Kf * r7;
         const T br7 = bden * Kf * qm;
         if (qm > T(0)) {
-            o.qx += bqm * qx1 / qm;
-            o.qy += bqm * qy1 / qm;
+            const T rq = rcp(qm);
+            o.qx += bqm * qx1 * rq;
+            o.qy += bqm * qy1 * rq;
         }
-        bh3 += br7 * (T(-7) / T(3)) * r7 / h3;
+        bh3 += br7 * (T(-7) / T(3)) * r7 * rcp(h3);
     }
     const T bh2 = capped ? bh3 : T(0);
     o.h = pos1 ? bh2 : T(0);

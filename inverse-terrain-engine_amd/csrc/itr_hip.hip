@@ -2903,6 +2903,7 @@ struct itr_ctx {
     int use_graph = 1;
     int64_t min_lds = 0;   // reserve at least this much LDS per step workgroup (caps occupancy)
     int adjoint_mode = 0;  // reverse sweep: 0 by batch size, 1 band march, 2 per-cell kernels
+    int adjoint_rows = 0;  // rows per band of the reverse band march (0: 16)
     int control_task = -1; // ledger and control block: -1 by launch shape, 0 first band, 1 control task
     struct Tuned { int nx, ny, crest, R; };
     std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
@@ -3526,6 +3527,7 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     else if (n == "min_lds") ctx->min_lds = value;
     else if (n == "adjoint_mode" && value >= 0 && value <= 2) ctx->adjoint_mode = (int)value;
     else if (n == "control_task" && value >= -1 && value <= 1) ctx->control_task = (int)value;
+    else if (n == "adjoint_rows" && value >= 0 && value <= 1 << 20) ctx->adjoint_rows = (int)value;
     else return ITR_EINVAL;
     return ITR_OK;
 }
@@ -4648,7 +4650,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     const dim3 grd((unsigned)((nx + 63) / 64), (unsigned)ny, (unsigned)nc), blk(64);
     // The band march needs enough bands in flight to hide its row-to-row latency; small
     // batches keep the per-cell faces kernel (one thread per cell).
-    A.R = std::min(16, ny);
+    A.R = std::min(ctx->adjoint_rows > 0 ? ctx->adjoint_rows : 16, ny);
     A.waves = (nx + kColsPerWave - 1) / kColsPerWave;
     A.tail_groups = tail_group_count(nx, A.waves);
     const dim3 bgrd((unsigned)((ny + A.R - 1) / A.R), (unsigned)nc);

@@ -2564,8 +2564,8 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
         if (c == 0) fold(s.W, s.wW, true, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
         else {
-            gx[(size_t)r * a.P + c] += (double)f.dz;
-            if (gx2) gx2[(size_t)r * a.P + c] += (double)f.dz2;
+            if (f.dz != 0.0f) gx[(size_t)r * a.P + c] += (double)f.dz;
+            if (gx2 && f.dz2 != 0.0f) gx2[(size_t)r * a.P + c] += (double)f.dz2;
         }
     }
     {   // east face: L = C, R = E
@@ -2582,8 +2582,8 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
         if (r == 0) fold(s.N, s.wN, false, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
         else {
-            gy[(size_t)r * a.P + c] += (double)f.dz;
-            if (gy2) gy2[(size_t)r * a.P + c] += (double)f.dz2;
+            if (f.dz != 0.0f) gy[(size_t)r * a.P + c] += (double)f.dz;
+            if (gy2 && f.dz2 != 0.0f) gy2[(size_t)r * a.P + c] += (double)f.dz2;
         }
     }
     {   // south face: L = C, R = S
@@ -2608,7 +2608,9 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
 // itr_adj_faces_kernel does it twice.  Each cell adds its faces' contributions in that
 // kernel's order (west, east, north, south, ghost folds included), so lamo is bitwise
 // the same.  Each interior face's jump gradient is written by the lane that owns its
-// east / south cell.  A row's loads are issued one row ahead.
+// east / south cell.  A row's loads are issued one row ahead.  A zero face adjoint is not
+// added: the f64 sums start at +0 and adding ±0 leaves any value but −0 unchanged, so
+// skipping it changes no bit and saves the read-modify-write where λ has not reached.
 struct NbW { float h, qx, qy, wall; };
 struct RevIn { float h, qx, qy, lh, lqx, lqy, ph, pqx, pqy, wet, gn2, dzs, dzw, dzs2, dzw2; int code; };
 __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
@@ -2700,19 +2702,26 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         const adj::F<float> b{dtdy * (lr.h - ll.h), -dtdy * ll.qy, dtdy * lr.qy, dtdy * (lr.qx - ll.qx)};
         return crest ? adj::face_cr_adj<false>(L.h, vl, R.h, vr, dz, dz2, k, b) : adj::face_adj<false>(L.h, vl, R.h, vr, dz, k, b);
     };
+    // Rows where λ has not reached: with λ1 of a cell and of its four neighbours exactly 0,
+    // every bar of its faces and of its update is a product of zeros with finite values,
+    // so its adjoint and its faces' contributions are 0.  A wave whose three rows hold no
+    // non-zero λ1 writes zeros for the row and carries zero bars (exact: only the sign of
+    // a zero adjoint can differ, and zero face contributions are not added).
+    auto zero3 = [](const adj::S3<float>& l) { return l.h == 0.0f && l.qx == 0.0f && l.qy == 0.0f; };
     const int g0 = a0 + lr;   // this lane's first row
     const RevIn ip = load(g0 - 1), ic = load(g0);
     RevIn nxt = load(g0 + 1);
     NbW Pn = cell(ip, g0 - 1), Cn = cell(ic, g0);
     adj::V<float> vC = velo(Cn);
     adj::S3<float> lC = lam1(ic, g0);
+    bool zN = zero3(lam1(ip, g0 - 1)), zC = zero3(lC);
     float dzw = ic.dzw, dzw2 = ic.dzw2;   // west face jump (rises) of the current row
     adj::FaceBar<float> fN =
         yface(Pn, velo(Pn), lam1(ip, g0 - 1), Cn, vC, lC, ic.dzs, ic.dzs2);
     // (a group's north face inside the band is the previous group's south face, counted there)
     if (upd && a0 >= 1 && lr == 0) {
-        gy[(size_t)a0 * a.P + col] += (double)fN.dz;
-        if (crest) gy2[(size_t)a0 * a.P + col] += (double)fN.dz2;
+        if (fN.dz != 0.0f) gy[(size_t)a0 * a.P + col] += (double)fN.dz;
+        if (crest && fN.dz2 != 0.0f) gy2[(size_t)a0 * a.P + col] += (double)fN.dz2;
     }
     for (int jb = a0; jb < a0 + t_rows; jb++) {
         const int j = jb + lr;   // this lane's row
@@ -2721,11 +2730,31 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         if (j + 1 < b0) nxt = load(j + 2);
         const NbW Sn = cell(in, j + 1);
         const adj::V<float> vS = velo(Sn);
-        const adj::S3<float> lS = lam1(in, j + 1);
+        const bool raw0 = in.lh == 0.0f && in.lqx == 0.0f && in.lqy == 0.0f;
+        const adj::S3<float> lS = raw0 ? adj::S3<float>{0.0f, 0.0f, 0.0f} : lam1(in, j + 1);
+        const bool zS = zero3(lS);
+        if (__builtin_amdgcn_ballot_w64(!(zN && zC && zS)) == 0ull) {   // (wave-uniform)
+            if (upd && rv) {
+                const size_t o = (size_t)j * a.P + col;
+                LO[o] = 0.0f;
+                LO[a.plane + o] = 0.0f;
+                LO[2 * a.plane + o] = 0.0f;
+            }
+            dzw = in.dzw;
+            dzw2 = in.dzw2;
+            Pn = Cn;
+            Cn = Sn;
+            vC = vS;
+            lC = lS;
+            fN = adj::FaceBar<float>{};
+            zN = zC;
+            zC = zS;
+            continue;
+        }
         const adj::FaceBar<float> fS = yface(Cn, vC, lC, Sn, vS, lS, in.dzs, in.dzs2);
         if (upd && j + 1 <= ny - 1 && j + 1 < b0) {
-            gy[(size_t)(j + 1) * a.P + col] += (double)fS.dz;
-            if (crest) gy2[(size_t)(j + 1) * a.P + col] += (double)fS.dz2;
+            if (fS.dz != 0.0f) gy[(size_t)(j + 1) * a.P + col] += (double)fS.dz;
+            if (crest && fS.dz2 != 0.0f) gy2[(size_t)(j + 1) * a.P + col] += (double)fS.dz2;
         }
         // West face: L = the left lane's cell, R = this lane's.
         const NbW Wn{from_left(Cn.h), from_left(Cn.qx), from_left(Cn.qy), from_left(Cn.wall)};
@@ -2735,8 +2764,8 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         const adj::FaceBar<float> fW = crest ? adj::face_cr_adj<true>(Wn.h, vW, Cn.h, vC, dzw, dzw2, k, bw)
                                              : adj::face_adj<true>(Wn.h, vW, Cn.h, vC, dzw, k, bw);
         if (upd && rv && col >= 1 && col <= nx - 1) {
-            gx[(size_t)j * a.P + col] += (double)fW.dz;
-            if (crest) gx2[(size_t)j * a.P + col] += (double)fW.dz2;
+            if (fW.dz != 0.0f) gx[(size_t)j * a.P + col] += (double)fW.dz;
+            if (crest && fW.dz2 != 0.0f) gx2[(size_t)j * a.P + col] += (double)fW.dz2;
         }
         dzw = in.dzw;   // row j+1's west face
         dzw2 = in.dzw2;
@@ -2778,6 +2807,132 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         vC = vS;
         lC = lS;
         fN = fS;
+        zN = zC;
+        zC = zS;
+    }
+}
+
+// Forward replay of one step for a band of R rows of one candidate (large batches),
+// marched like the forward band: lanes are columns with a halo lane on each side,
+// neighbours by DPP; each face is evaluated once (the west face's values reach the left
+// lane by DPP as its east face, the south face carries to the next row as its north
+// face) and each cell velocity once, where itr_replay_kernel evaluates four faces and
+// five velocities per cell.  Every value is the same IEEE expression of the same inputs
+// (ghosts by the same rule, corners unused), so both give the same bits; the segment
+// check against the checkpoints holds either to it.
+struct FwdIn { float h, qx, qy, wet, gn2, dzs, dzw, dzs2, dzw2; int code; };
+__global__ void __launch_bounds__(1024) itr_replay_band_kernel(AdjArgs a) {
+    const int band = blockIdx.x, cand = blockIdx.y;
+    const int nx = a.nx, ny = a.ny;
+    const int a0 = band * a.R, b0 = min(a0 + a.R, ny);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int S = a.tail_groups;   // the last wave's row groups, as the forward step
+    const bool tail = S > 1 && uni(wave) == a.waves - 1;
+    const int tgw = tail ? 64 / S : 64;
+    const int tg = tail ? lane / tgw : 0;
+    const int gl = lane - tg * tgw;
+    const int t_rows = tail ? (b0 - a0 + S - 1) / S : b0 - a0;
+    const int lr = tg * t_rows;
+    const int col = wave * kColsPerWave - 1 + gl;
+    const bool upd = gl >= 1 && gl <= tgw - 2 && tg < S && col < nx;
+    const bool cin = col >= 0 && col < nx;
+    const int cc = min(max(col, 0), nx - 1), xc = min(max(col, 0), nx);
+    const float* U = a.U + (size_t)cand * a.cstride;
+    float* Uo = a.Uo + (size_t)cand * a.cstride;
+    float* Pr = a.pre + (size_t)cand * a.cstride;
+    const float* dzx = a.dzx + (size_t)cand * a.dzx_stride;
+    const float* dzy = a.dzy + (size_t)cand * a.dzy_stride;
+    const bool crest = a.crest != 0;
+    const float* dzx2 = crest ? a.dzx2 + (size_t)cand * a.dzx_stride : nullptr;
+    const float* dzy2 = crest ? a.dzy2 + (size_t)cand * a.dzy_stride : nullptr;
+    const StepK& K = a.K;
+    // Raw inputs of row k1: its old state (the interior neighbour for a ghost), wet and
+    // g n^2, the jumps of its north face and of its west face.
+    auto load = [&](int k1) -> FwdIn {
+        FwdIn v;
+        const int rr = min(max(k1, 0), ny - 1);
+        const size_t o = (size_t)rr * a.P + cc;
+        v.h = U[o];
+        v.qx = U[a.plane + o];
+        v.qy = U[2 * a.plane + o];
+        const bool rin = k1 >= 0 && k1 < ny;
+        v.code = 1;
+        if (rin && !cin) v.code = a.ghost[2 * nx + (col < 0 ? 0 : ny) + k1];
+        else if (!rin && cin) v.code = a.ghost[(k1 < 0 ? 0 : nx) + col];
+        v.wet = a.wet[o];
+        v.gn2 = a.gn2[o];
+        v.dzs = dzy[(size_t)min(max(k1, 0), ny) * a.P + cc];
+        v.dzw = dzx[(size_t)min(max(k1, 0), ny - 1) * a.P + xc];
+        v.dzs2 = crest ? dzy2[(size_t)min(max(k1, 0), ny) * a.P + cc] : 0.0f;
+        v.dzw2 = crest ? dzx2[(size_t)min(max(k1, 0), ny - 1) * a.P + xc] : 0.0f;
+        return v;
+    };
+    // The ghost rule (boundary.rs:12-87), as nb_at: wall mirrors the normal discharge,
+    // transmissive copies; corners (never read by an interior cell) are 0.
+    auto cell = [&](const FwdIn& v, int k1) -> Nb {
+        const bool rin = k1 >= 0 && k1 < ny;
+        if (!rin && !cin) return Nb{0.0f, 0.0f, 0.0f};
+        Nb w{v.h, v.qx, v.qy};
+        const bool wall = (!rin || !cin) && v.code == 0;
+        if (!cin && wall) w.qx = -w.qx;
+        if (!rin && wall) w.qy = -w.qy;
+        return w;
+    };
+    auto yface = [&](const Nb& L, const Vel& vl, const Nb& R, const Vel& vr, float dz, float dz2) -> Flux {
+        return crest ? face_crest<false>(L.h, vl, R.h, vr, dz, dz2, K) : face_signed<false>(L.h, vl, R.h, vr, dz, K);
+    };
+    const int g0 = a0 + lr;   // this lane's first row
+    const FwdIn ip = load(g0 - 1), ic = load(g0);
+    FwdIn nxt = load(g0 + 1);
+    const Nb Pn = cell(ip, g0 - 1);
+    Nb Cn = cell(ic, g0);
+    Vel vC = cell_velocity(Cn.h, Cn.qx, Cn.qy, K);
+    float wet = ic.wet, gn2 = ic.gn2, dzw = ic.dzw, dzw2 = ic.dzw2;
+    Flux fN;
+    {
+        const Vel vP = cell_velocity(Pn.h, Pn.qx, Pn.qy, K);
+        fN = yface(Pn, vP, Cn, vC, ic.dzs, ic.dzs2);
+    }
+    for (int jb = a0; jb < a0 + t_rows; jb++) {
+        const int j = jb + lr;   // this lane's row
+        const bool rv = j < b0;
+        const FwdIn in = nxt;    // row j+1
+        if (j + 1 < b0) nxt = load(j + 2);
+        const Nb Sn = cell(in, j + 1);
+        const Vel vS = cell_velocity(Sn.h, Sn.qx, Sn.qy, K);
+        const Flux fS = yface(Cn, vC, Sn, vS, in.dzs, in.dzs2);
+        // West face: L = the left lane's cell, R = this lane's; the east face is the right
+        // lane's west face.
+        const float hw = from_left(Cn.h);
+        Vel vW;
+        vW.u = from_left(vC.u);
+        vW.v = from_left(vC.v);
+        vW.c = from_left(vC.c);
+        const Flux fW = crest ? face_crest<true>(hw, vW, Cn.h, vC, dzw, dzw2, K) : face_signed<true>(hw, vW, Cn.h, vC, dzw, K);
+        Flux fE;
+        fE.m = from_right(fW.m);
+        fE.nl = from_right(fW.nl);
+        fE.nr = from_right(fW.nr);
+        fE.t = from_right(fW.t);
+        const CellNew u = cell_update(Cn.h, Cn.qx, Cn.qy, fW.m, fW.nr, fW.t, fE.m, fE.nl, fE.t, fN.m, fN.nr, fN.t, fS.m,
+                                      fS.nl, fS.t, wet, gn2, K);
+        if (upd && rv) {
+            const size_t o = (size_t)j * a.P + col;
+            Uo[o] = u.h;
+            Uo[a.plane + o] = u.qx;
+            Uo[2 * a.plane + o] = u.qy;
+            // the state after the flux divergence (numerics.rs:148-153), as itr_replay_kernel
+            Pr[o] = Cn.h - K.dtdx * (fE.m - fW.m) - K.dtdy * (fS.m - fN.m);
+            Pr[a.plane + o] = Cn.qx - K.dtdx * (fE.nl - fW.nr) - K.dtdy * (fS.t - fN.t);
+            Pr[2 * a.plane + o] = Cn.qy - K.dtdx * (fE.t - fW.t) - K.dtdy * (fS.nl - fN.nr);
+        }
+        Cn = Sn;
+        vC = vS;
+        fN = fS;
+        wet = in.wet;
+        gn2 = in.gn2;
+        dzw = in.dzw;
+        dzw2 = in.dzw2;
     }
 }
 
@@ -4671,7 +4826,10 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             A.Uo = seg + (size_t)(n - n0) * state_n;
             A.pre = pre + (size_t)(n - 1 - n0) * state_n;
             A.K = consts(n);
-            hipLaunchKernelGGL(itr_replay_kernel, grd, blk, 0, st, A);
+            if (band_mode)
+                hipLaunchKernelGGL(itr_replay_band_kernel, bgrd, dim3((unsigned)(A.waves * 64)), 0, st, A);
+            else
+                hipLaunchKernelGGL(itr_replay_kernel, grd, blk, 0, st, A);
         }
         hipLaunchKernelGGL(itr_adj_compare_kernel, dim3(256), dim3(256), 0, st, seg + (size_t)(n1 - n0) * state_n,
                            ckpt + (size_t)k * state_n, state_n, d_mis);

@@ -285,9 +285,10 @@ void itr_hip_batch_destroy(itr_batch* b);
  * checkpoints every ceil(sqrt(N)) steps, replays each segment and sweeps it backwards;
  * adjoint arithmetic is f32, face-gradient accumulation f64.  p->max_steps > 0 bounds the
  * horizon (the run then ends after that many steps).  Signed levels and face-crest levels
- * (numerics.rs:117-129 with the faces of terrain.rs:378-429); Stage / Inflow ghosts, an
- * initial stage (their depths depend on the bed), local inertial physics, replay grids and
- * device designs return ITR_DECLINED.  A replayed segment that differs bitwise from its
+ * (numerics.rs:117-129 with the faces of terrain.rs:378-429), wall, transmissive, Stage and
+ * Inflow boundaries (a Stage ghost's depth moves with its boundary cell's level, which
+ * dJ_dz_cell includes); an initial stage (its depth depends on the bed), local inertial
+ * physics, replay grids and device designs return ITR_DECLINED.  A replayed segment that differs bitwise from its
  * checkpoint returns ITR_EDEVICE (no gradient).  The earthwork term and the chain rule to
  * the fine raster and to primitive heights are host work on the returned gradients. */
 typedef struct {

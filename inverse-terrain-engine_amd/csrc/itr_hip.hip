@@ -2377,6 +2377,10 @@ struct AdjArgs {
     double* gx2;           // crest levels: dJ/d(d_r) of the west faces
     double* gy2;           // crest levels: dJ/d(d_r) of the north faces
     int32_t crest;         // face-crest level (numerics.rs:117-129)
+    int32_t special;       // the grid has Stage / Inflow ghosts
+    const DevGrid* G;      // the grid's descriptor (its ghost data), for special ghosts
+    double t;              // the step's start time (Inflow discharges)
+    double* gz;            // [cand][2nx + 2ny] dJ/d(level z of the boundary cell) through its Stage ghost
     const float* wet;      // [ny][P]
     const float* gn2;      // [ny][P]
     const int32_t* ghost;  // [2nx + 2ny]
@@ -2392,23 +2396,46 @@ struct Nb { float h, qx, qy; };
 
 // Old-state value of cell (r, c) or of the ghost next to the interior (boundary.rs:12-87:
 // wall mirrors the normal discharge, transmissive copies); never a corner here.
-__device__ __forceinline__ Nb nb_at(const float* U, const AdjArgs& a, int r, int c, bool* wall) {
-    int rr = r, cc = c, code = 1;
-    bool gx_ = false, gy_ = false;
+// A special ghost (boundary.rs:43-47, 66-70) as the forward step builds it from the
+// interior neighbour v: Stage holds the candidate's stage depth f32(max(stage - z, 0)) with
+// the interior discharges; Inflow the larger of the interior depth and the critical depth
+// of the hydrograph's discharge at the step's start, with that discharge normal to the
+// boundary.  k: ghost index (north[nx], south[nx], west[ny], east[ny]).
+__device__ __forceinline__ Nb special_nb(const AdjArgs& a, int cand, int k, int code, Nb v) {
+    const bool xdir = k >= 2 * a.nx;
+    const bool inward = k < a.nx || (xdir && k < 2 * a.nx + a.ny);   // north or west
+    const GhostSp g = special_ghost(*a.G, a.G->prm.inflow_scale, code, k, cand, a.t, inward, 0);
+    if (code == 2) {
+        v.h = g.hv;
+        return v;
+    }
+    v.h = inflow_depth(v.h, g.hv);
+    v.qx = xdir ? g.qv : 0.0f;
+    v.qy = xdir ? 0.0f : g.qv;
+    return v;
+}
+
+// Old-state value of cell (r, c) or of the ghost next to the interior (boundary.rs:12-87:
+// wall mirrors the normal discharge, transmissive copies, Stage / Inflow as special_nb);
+// never a corner here.  *code / *kg: the ghost's code and index (-1 for an interior cell).
+__device__ __forceinline__ Nb nb_at(const float* U, const AdjArgs& a, int cand, int r, int c, int* code, int* kg) {
+    int rr = r, cc = c, cd = -1, k = -1;
     if (c < 0 || c >= a.nx) {
         cc = c < 0 ? 0 : a.nx - 1;
-        code = a.ghost[2 * a.nx + (c < 0 ? 0 : a.ny) + r];
-        gx_ = true;
+        k = 2 * a.nx + (c < 0 ? 0 : a.ny) + r;
+        cd = a.ghost[k];
     } else if (r < 0 || r >= a.ny) {
         rr = r < 0 ? 0 : a.ny - 1;
-        code = a.ghost[(r < 0 ? 0 : a.nx) + c];
-        gy_ = true;
+        k = (r < 0 ? 0 : a.nx) + c;
+        cd = a.ghost[k];
     }
     const size_t o = (size_t)rr * a.P + cc;
     Nb v{U[o], U[a.plane + o], U[2 * a.plane + o]};
-    if (gx_ && code == 0) v.qx = -v.qx;
-    if (gy_ && code == 0) v.qy = -v.qy;
-    if (wall) *wall = code == 0;
+    if (cd == 0 && k >= 2 * a.nx) v.qx = -v.qx;
+    if (cd == 0 && k >= 0 && k < 2 * a.nx) v.qy = -v.qy;
+    if (cd >= 2) v = special_nb(a, cand, k, cd, v);
+    if (code) *code = cd;
+    if (kg) *kg = k;
     return v;
 }
 
@@ -2429,7 +2456,8 @@ __device__ __forceinline__ adj::Consts<float> adj_consts(const StepK& K) {
 
 struct Stencil {
     Nb C, W, E, N, S;
-    bool wW, wE, wN, wS;
+    int cW, cE, cN, cS;   // ghost codes of the neighbours (-1: interior)
+    int kW, kE, kN, kS;   // their ghost indices
     Vel vC, vW, vE, vN, vS;
     float dzW, dzE, dzN, dzS, wet, gn2;
     float dzW2, dzE2, dzN2, dzS2;   // crest levels: the right rises
@@ -2437,11 +2465,11 @@ struct Stencil {
 
 __device__ __forceinline__ Stencil stencil(const AdjArgs& a, const float* U, int cand, int r, int c) {
     Stencil s;
-    s.C = nb_at(U, a, r, c, nullptr);
-    s.W = nb_at(U, a, r, c - 1, &s.wW);
-    s.E = nb_at(U, a, r, c + 1, &s.wE);
-    s.N = nb_at(U, a, r - 1, c, &s.wN);
-    s.S = nb_at(U, a, r + 1, c, &s.wS);
+    s.C = nb_at(U, a, cand, r, c, nullptr, nullptr);
+    s.W = nb_at(U, a, cand, r, c - 1, &s.cW, &s.kW);
+    s.E = nb_at(U, a, cand, r, c + 1, &s.cE, &s.kE);
+    s.N = nb_at(U, a, cand, r - 1, c, &s.cN, &s.kN);
+    s.S = nb_at(U, a, cand, r + 1, c, &s.cS, &s.kS);
     s.vC = cell_velocity(s.C.h, s.C.qx, s.C.qy, a.K);
     s.vW = cell_velocity(s.W.h, s.W.qx, s.W.qy, a.K);
     s.vE = cell_velocity(s.E.h, s.E.qx, s.E.qy, a.K);
@@ -2465,6 +2493,28 @@ __device__ __forceinline__ Stencil stencil(const AdjArgs& a, const float* U, int
     s.wet = a.wet[(size_t)r * a.P + c];
     s.gn2 = a.gn2[(size_t)r * a.P + c];
     return s;
+}
+
+// The adjoint of a ghost (bh: of its depth; q: of its discharges) into the boundary cell:
+// wall and transmissive ghosts mirror or copy the cell; a Stage ghost copies the
+// discharges and its depth f32(max(stage - z, 0)) moves with the cell's level z (-1 while
+// positive), which goes to gz[k]; an Inflow ghost's depth is the cell's own where that
+// exceeds the critical depth (hc: the ghost's depth; hi: the cell's), its discharges are
+// the hydrograph's.
+__device__ __forceinline__ void fold_ghost(const AdjArgs& a, int cand, int code, int kg, bool xdir, float hg, float hi,
+                                           float bh, const adj::S3<float>& q, adj::S3<float>& bC) {
+    if (code == 2) {
+        if (hg > 0.0f && bh != 0.0f) a.gz[(size_t)cand * (2 * a.nx + 2 * a.ny) + kg] -= (double)bh;
+        bC.qx += q.qx;
+        bC.qy += q.qy;
+    } else if (code == 3) {
+        if (hg == hi) bC.h += bh;
+    } else {
+        const bool wall = code == 0;
+        bC.h += bh;
+        bC.qx += (xdir && wall) ? -q.qx : q.qx;
+        bC.qy += (!xdir && wall) ? -q.qy : q.qy;
+    }
 }
 
 // Forward replay of one step (kernel.rs:590-727 semantics, bitwise).
@@ -2537,12 +2587,11 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
     auto V3 = [](const Vel& v) { return adj::V<float>{v.u, v.v, v.c}; };
     adj::S3<float> bC = lC;   // h1 = h - ..., qx1 = qx - ..., qy1 = qy - ...
     adj::V<float> bv{0.0f, 0.0f, 0.0f};
-    // A ghost's adjoint folds back into this cell (its mirror image).
-    auto fold = [&](const Nb& g, bool wall, bool xdir, float bh, const adj::V<float>& gv) {
+    // A ghost's adjoint folds back into this cell (its mirror image), or, for a Stage
+    // ghost's depth, into the boundary cell's level.
+    auto fold = [&](const Nb& g, int code, int kg, bool xdir, float bh, const adj::V<float>& gv) {
         adj::S3<float> q = adj::vel_adj(g.h, g.qx, g.qy, k, gv);
-        bC.h += bh + q.h;
-        bC.qx += (xdir && wall) ? -q.qx : q.qx;
-        bC.qy += (!xdir && wall) ? -q.qy : q.qy;
+        fold_ghost(a, cand, code, kg, xdir, g.h, s.C.h, bh + q.h, q, bC);
     };
     double* gx = a.gx + (size_t)cand * a.dzx_stride;
     double* gy = a.gy + (size_t)cand * a.dzy_stride;
@@ -2562,7 +2611,7 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
         const adj::FaceBar<float> f = fadj_x(s.W, s.vW, s.C, s.vC, s.dzW, s.dzW2, b);
         bC.h += f.hr;
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
-        if (c == 0) fold(s.W, s.wW, true, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
+        if (c == 0) fold(s.W, s.cW, s.kW, true, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
         else {
             if (f.dz != 0.0f) gx[(size_t)r * a.P + c] += (double)f.dz;
             if (gx2 && f.dz2 != 0.0f) gx2[(size_t)r * a.P + c] += (double)f.dz2;
@@ -2573,14 +2622,14 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
         const adj::FaceBar<float> f = fadj_x(s.C, s.vC, s.E, s.vE, s.dzE, s.dzE2, b);
         bC.h += f.hl;
         bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
-        if (c == a.nx - 1) fold(s.E, s.wE, true, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
+        if (c == a.nx - 1) fold(s.E, s.cE, s.kE, true, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
     }
     {   // north face: L = N, R = C (normal velocity v)
         const adj::F<float> b{dtdy * (lC.h - lN.h), -dtdy * lN.qy, dtdy * lC.qy, dtdy * (lC.qx - lN.qx)};
         const adj::FaceBar<float> f = fadj_y(s.N, s.vN, s.C, s.vC, s.dzN, s.dzN2, b);
         bC.h += f.hr;
         bv.u += f.ur; bv.v += f.vr; bv.c += f.cr;
-        if (r == 0) fold(s.N, s.wN, false, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
+        if (r == 0) fold(s.N, s.cN, s.kN, false, f.hl, adj::V<float>{f.ul, f.vl, f.cl});
         else {
             if (f.dz != 0.0f) gy[(size_t)r * a.P + c] += (double)f.dz;
             if (gy2 && f.dz2 != 0.0f) gy2[(size_t)r * a.P + c] += (double)f.dz2;
@@ -2591,7 +2640,7 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
         const adj::FaceBar<float> f = fadj_y(s.C, s.vC, s.S, s.vS, s.dzS, s.dzS2, b);
         bC.h += f.hl;
         bv.u += f.ul; bv.v += f.vl; bv.c += f.cl;
-        if (r == a.ny - 1) fold(s.S, s.wS, false, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
+        if (r == a.ny - 1) fold(s.S, s.cS, s.kS, false, f.hr, adj::V<float>{f.ur, f.vr, f.cr});
     }
     const adj::S3<float> q = adj::vel_adj(s.C.h, s.C.qx, s.C.qy, k, bv);
     const size_t o = (size_t)cand * a.cstride + (size_t)r * a.P + c;
@@ -2611,7 +2660,7 @@ __global__ void __launch_bounds__(256) itr_adj_faces_kernel(AdjArgs a) {
 // east / south cell.  A row's loads are issued one row ahead.  A zero face adjoint is not
 // added: the f64 sums start at +0 and adding ±0 leaves any value but −0 unchanged, so
 // skipping it changes no bit and saves the read-modify-write where λ has not reached.
-struct NbW { float h, qx, qy, wall; };
+struct NbW { float h, qx, qy, code; };   // code: the ghost code of a ghost cell, -1 in the grid
 struct RevIn { float h, qx, qy, lh, lqx, lqy, ph, pqx, pqy, wet, gn2, dzs, dzw, dzs2, dzw2; int code; };
 __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
     const int band = blockIdx.x, cand = blockIdx.y;
@@ -2677,14 +2726,22 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         return v;
     };
     // The ghost rule (boundary.rs:12-87) on a loaded row: wall mirrors the normal
-    // discharge, transmissive copies, corners are 0.
+    // discharge, transmissive copies, Stage / Inflow as special_nb; corners are 0.
     auto cell = [&](const RevIn& v, int k1) -> NbW {
         const bool rin = k1 >= 0 && k1 < ny;
-        if (!rin && !cin) return NbW{0.0f, 0.0f, 0.0f, 0.0f};
-        const bool wall = (!rin || !cin) && v.code == 0;
-        NbW w{v.h, v.qx, v.qy, wall ? 1.0f : 0.0f};
+        if (!rin && !cin) return NbW{0.0f, 0.0f, 0.0f, -1.0f};
+        const bool ghost = !rin || !cin;
+        const bool wall = ghost && v.code == 0;
+        NbW w{v.h, v.qx, v.qy, ghost ? (float)v.code : -1.0f};
         if (!cin && wall) w.qx = -w.qx;
         if (!rin && wall) w.qy = -w.qy;
+        if (ghost && v.code >= 2) {
+            const int kg = !cin ? 2 * nx + (col < 0 ? 0 : ny) + k1 : (k1 < 0 ? 0 : nx) + col;
+            const Nb g = special_nb(a, cand, kg, v.code, Nb{w.h, w.qx, w.qy});
+            w.h = g.h;
+            w.qx = g.qx;
+            w.qy = g.qy;
+        }
         return w;
     };
     auto velo = [&](const NbW& c) {
@@ -2757,7 +2814,7 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
             if (crest && fS.dz2 != 0.0f) gy2[(size_t)(j + 1) * a.P + col] += (double)fS.dz2;
         }
         // West face: L = the left lane's cell, R = this lane's.
-        const NbW Wn{from_left(Cn.h), from_left(Cn.qx), from_left(Cn.qy), from_left(Cn.wall)};
+        const NbW Wn{from_left(Cn.h), from_left(Cn.qx), from_left(Cn.qy), from_left(Cn.code)};
         const adj::V<float> vW{from_left(vC.u), from_left(vC.v), from_left(vC.c)};
         const adj::S3<float> lW{from_left(lC.h), from_left(lC.qx), from_left(lC.qy)};
         const adj::F<float> bw{dtdx * (lC.h - lW.h), -dtdx * lW.qx, dtdx * lC.qx, dtdx * (lC.qy - lW.qy)};
@@ -2773,28 +2830,25 @@ __global__ void __launch_bounds__(1024) itr_adj_band_kernel(AdjArgs a) {
         adj::FaceBar<float> fE;
         fE.hl = from_right(fW.hl); fE.ul = from_right(fW.ul); fE.vl = from_right(fW.vl); fE.cl = from_right(fW.cl);
         fE.hr = from_right(fW.hr); fE.ur = from_right(fW.ur); fE.vr = from_right(fW.vr); fE.cr = from_right(fW.cr);
-        const NbW En{from_right(Cn.h), from_right(Cn.qx), from_right(Cn.qy), from_right(Cn.wall)};
+        const NbW En{from_right(Cn.h), from_right(Cn.qx), from_right(Cn.qy), from_right(Cn.code)};
         adj::S3<float> bC = lC;
         adj::V<float> bv{0.0f, 0.0f, 0.0f};
-        auto fold = [&](const NbW& g, bool xdir, float bh, const adj::V<float>& gv) {
+        auto fold = [&](const NbW& g, int kg, bool xdir, float bh, const adj::V<float>& gv) {
             const adj::S3<float> q = adj::vel_adj(g.h, g.qx, g.qy, k, gv);
-            const bool wall = g.wall != 0.0f;
-            bC.h += bh + q.h;
-            bC.qx += (xdir && wall) ? -q.qx : q.qx;
-            bC.qy += (!xdir && wall) ? -q.qy : q.qy;
+            fold_ghost(a, cand, (int)g.code, kg, xdir, g.h, Cn.h, bh + q.h, q, bC);
         };
         bC.h += fW.hr;
         bv.u += fW.ur; bv.v += fW.vr; bv.c += fW.cr;
-        if (col == 0) fold(Wn, true, fW.hl, adj::V<float>{fW.ul, fW.vl, fW.cl});
+        if (col == 0) fold(Wn, 2 * nx + j, true, fW.hl, adj::V<float>{fW.ul, fW.vl, fW.cl});
         bC.h += fE.hl;
         bv.u += fE.ul; bv.v += fE.vl; bv.c += fE.cl;
-        if (col == nx - 1) fold(En, true, fE.hr, adj::V<float>{fE.ur, fE.vr, fE.cr});
+        if (col == nx - 1) fold(En, 2 * nx + ny + j, true, fE.hr, adj::V<float>{fE.ur, fE.vr, fE.cr});
         bC.h += fN.hr;
         bv.u += fN.ur; bv.v += fN.vr; bv.c += fN.cr;
-        if (j == 0) fold(Pn, false, fN.hl, adj::V<float>{fN.ul, fN.vl, fN.cl});
+        if (j == 0) fold(Pn, col, false, fN.hl, adj::V<float>{fN.ul, fN.vl, fN.cl});
         bC.h += fS.hl;
         bv.u += fS.ul; bv.v += fS.vl; bv.c += fS.cl;
-        if (j == ny - 1) fold(Sn, false, fS.hr, adj::V<float>{fS.ur, fS.vr, fS.cr});
+        if (j == ny - 1) fold(Sn, nx + col, false, fS.hr, adj::V<float>{fS.ur, fS.vr, fS.cr});
         const adj::S3<float> q = adj::vel_adj(Cn.h, Cn.qx, Cn.qy, k, bv);
         if (upd && rv) {
             const size_t o = (size_t)j * a.P + col;
@@ -2868,14 +2922,18 @@ __global__ void __launch_bounds__(1024) itr_replay_band_kernel(AdjArgs a) {
         return v;
     };
     // The ghost rule (boundary.rs:12-87), as nb_at: wall mirrors the normal discharge,
-    // transmissive copies; corners (never read by an interior cell) are 0.
+    // transmissive copies, Stage / Inflow as special_nb; corners (never read by an
+    // interior cell) are 0.
     auto cell = [&](const FwdIn& v, int k1) -> Nb {
         const bool rin = k1 >= 0 && k1 < ny;
         if (!rin && !cin) return Nb{0.0f, 0.0f, 0.0f};
         Nb w{v.h, v.qx, v.qy};
-        const bool wall = (!rin || !cin) && v.code == 0;
+        const bool ghost = !rin || !cin;
+        const bool wall = ghost && v.code == 0;
         if (!cin && wall) w.qx = -w.qx;
         if (!rin && wall) w.qy = -w.qy;
+        if (ghost && v.code >= 2)
+            w = special_nb(a, cand, !cin ? 2 * nx + (col < 0 ? 0 : ny) + k1 : (k1 < 0 ? 0 : nx) + col, v.code, w);
         return w;
     };
     auto yface = [&](const Nb& L, const Vel& vl, const Nb& R, const Vel& vr, float dz, float dz2) -> Flux {
@@ -4586,12 +4644,15 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "the adjoint runs on prepared faces of a full grid (no replay, no device designs)");
         return ITR_DECLINED;
     }
+    bool special = false;
     if (grid->ghost)
-        for (int i = 0; i < 2 * grid->nx + 2 * grid->ny; i++)
-            if (grid->ghost[i] >= 2) {
-                set_err(err, errlen, "the adjoint supports wall and transmissive boundaries only");
+        for (int i = 0; i < 2 * grid->nx + 2 * grid->ny; i++) {
+            if (grid->ghost[i] > 3) {
+                set_err(err, errlen, "the adjoint supports wall, transmissive, Stage and Inflow boundaries");
                 return ITR_DECLINED;
             }
+            special = special || grid->ghost[i] >= 2;
+        }
     if (grid->n_cand <= 0 || !(obj->tau_m > 0.0) || obj->n_assets < 0 || (obj->n_assets > 0 && !obj->asset_start)) {
         set_err(err, errlen, "invalid adjoint arguments");
         return ITR_EINVAL;
@@ -4672,9 +4733,10 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         return e;
     };
     float *ckpt, *seg, *pre, *lam, *lam1;
-    double *gx, *gy, *gx2 = nullptr, *gy2 = nullptr;
+    double *gx, *gy, *gx2 = nullptr, *gy2 = nullptr, *gz = nullptr;
     unsigned long long* d_mis;
     const bool crest = grid->crest != 0;
+    const size_t nb = 2 * (size_t)grid->nx + 2 * (size_t)grid->ny;
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(nseg + 1), (void**)&ckpt));
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)(C + 1), (void**)&seg));
     ADJ_TRY(dalloc(sizeof(float) * state_n * (size_t)C, (void**)&pre));   // post-divergence state of each step
@@ -4686,6 +4748,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * plane, (void**)&gx2));
         ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * D.plane_y, (void**)&gy2));
     }
+    if (special) ADJ_TRY(dalloc(sizeof(double) * (size_t)nc * nb, (void**)&gz));
     ADJ_TRY(dalloc(sizeof(unsigned long long), (void**)&d_mis));
     auto copy_states = [&](float* dst, int parity) -> hipError_t {
         for (int c = 0; c < nc; c++) {
@@ -4763,6 +4826,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         ADJ_TRY(hipMemsetAsync(gx2, 0, sizeof(double) * (size_t)nc * plane, st));
         ADJ_TRY(hipMemsetAsync(gy2, 0, sizeof(double) * (size_t)nc * D.plane_y, st));
     }
+    if (special) ADJ_TRY(hipMemsetAsync(gz, 0, sizeof(double) * (size_t)nc * nb, st));
     ADJ_TRY(hipMemsetAsync(d_mis, 0, sizeof(unsigned long long), st));
     AdjArgs A{};
     A.dzx = D.dzx;
@@ -4772,6 +4836,9 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     A.dzy2 = crest ? D.dzy2 : nullptr;
     A.gx2 = gx2;
     A.gy2 = gy2;
+    A.special = special ? 1 : 0;
+    A.G = (const DevGrid*)b->d_grids;
+    A.gz = gz;
     A.wet = D.wet;
     A.gn2 = D.gn2;
     A.ghost = D.ghost;
@@ -4785,6 +4852,8 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     A.gx = gx;
     A.gy = gy;
     A.lam1 = lam1;
+    // the start time of step n (1-based): the previous step's end, bitwise as the forward
+    auto t_start = [&](uint64_t n) { return n <= 1 ? 0.0 : log[(size_t)(n - 2)].t_new; };
     auto consts = [&](uint64_t n) {   // constants of step n (1-based)
         StepK K;
         const StepRec& r = log[(size_t)(n - 1)];
@@ -4826,6 +4895,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             A.Uo = seg + (size_t)(n - n0) * state_n;
             A.pre = pre + (size_t)(n - 1 - n0) * state_n;
             A.K = consts(n);
+            A.t = t_start(n);
             if (band_mode)
                 hipLaunchKernelGGL(itr_replay_band_kernel, bgrd, dim3((unsigned)(A.waves * 64)), 0, st, A);
             else
@@ -4844,6 +4914,7 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
             A.lam = lam_n;
             A.lamo = lam + (size_t)(li ^ 1) * state_n;
             A.K = consts(n);
+            A.t = t_start(n);
             if (band_mode) {
                 hipLaunchKernelGGL(itr_adj_band_kernel, bgrd, dim3((unsigned)(A.waves * 64)), 0, st, A);
             } else {
@@ -4872,6 +4943,11 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         hy2.resize(hy.size());
         ADJ_TRY(hipMemcpy(hx2.data(), gx2, sizeof(double) * hx2.size(), hipMemcpyDeviceToHost));
         ADJ_TRY(hipMemcpy(hy2.data(), gy2, sizeof(double) * hy2.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<double> hz;
+    if (special) {
+        hz.resize((size_t)nc * nb);
+        ADJ_TRY(hipMemcpy(hz.data(), gz, sizeof(double) * hz.size(), hipMemcpyDeviceToHost));
     }
     for (int c = 0; c < nc; c++) {
         itr_adjoint_out& o = outs[c];
@@ -4949,6 +5025,17 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
                 }
             if (o.dJ_dz_cell)
                 for (size_t i = 0; i < (size_t)nx * ny; i++) o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0 : cell[i];
+        }
+        // Stage ghosts: their depth moves with the boundary cell's level (gz, per ghost).
+        if (special && o.dJ_dz_cell) {
+            const double* Z = hz.data() + (size_t)c * nb;
+            for (size_t kg = 0; kg < nb; kg++) {
+                if (Z[kg] == 0.0) continue;
+                const size_t k = kg, snx = (size_t)nx, sny = (size_t)ny;
+                const size_t i = k < snx ? k : k < 2 * snx ? (sny - 1) * snx + (k - snx)
+                               : k < 2 * snx + sny ? (k - 2 * snx) * snx : (k - 2 * snx - sny) * snx + snx - 1;
+                if (grid->wet[i] != 0.0f) o.dJ_dz_cell[i] += Z[kg];
+            }
         }
         o.steps = N;
         o.segment_steps = C;

@@ -319,6 +319,39 @@ def config4(seed=4, duration=2400.0, level=1) -> AdjointCase:
     return AdjointCase(base, p, prims, edit, PreparedTerrain(base).apply_edit(edit), mon, assets, 0.01, 1e-3)
 
 
+def adjoint_boundary_case(kind: str, seed=6, duration=1200.0) -> AdjointCase:
+    """The adjoint on a grid with a special boundary: 60x48 at 5 m (6 hills, 2 gullies,
+    sloping down to the east), a Stage (kind='stage': level 0.3 m above the lowest bed of
+    the segment's boundary cells) or an Inflow (kind='inflow': 0 -> 1.5 -> 0.5 m^3/s)
+    segment over rows 10-29 of the west side, the east side transmissive; 20 min storm
+    peaking at 40 mm/h at 5 min, infiltration 5 mm/h, CFL 0.9.  One design: a mound on
+    the segment's boundary cells, a berm across the inflow path and a swale; two 3x3
+    assets downstream (thresholds 0.02 and 0.04 m); tau 0.01 m, lambda 1e-3 per m^3."""
+    nx, ny, dx = 60, 48, 5.0
+    rng = SplitMix64(seed)
+    dem = hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
+    geo = GeoTransform(0.0, ny * dx, dx, dx)
+    rows = (10, 30)
+    if kind == "stage":
+        seg = {"kind": "stage", "side": "west", "range_m": (rows[0] * dx, rows[1] * dx),
+               "stage_m": float(dem[rows[0]:rows[1], 0].min()) + 0.3}
+    else:
+        seg = {"kind": "inflow", "side": "west", "range_m": (rows[0] * dx, rows[1] * dx),
+               "hydrograph": [[0.0, 0.0], [300.0, 1.5], [1200.0, 0.5]]}
+    base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1, [seg, {"kind": "transmissive", "side": "east"}])
+    p = SolverParams(duration, 60.0, cfl=0.9, rain=mm_h([[0, 0], [300, 40], [1200, 0]]),
+                     infil_capacity=5.0 / 1000.0 / 3600.0)
+    p.cfl_hard = 1.0
+    y = lambda r: geo.origin_y - (r + 0.5) * dx
+    prims = [Primitive("mound", 1.5 * dx, y(20), 0.0, 4.0 * dx, 0.0, 0.25),
+             Primitive("berm", 12.5 * dx, y(22), 12.0 * dx, 2.0 * dx, 80.0, 0.2),
+             Primitive("swale", 20.5 * dx, y(18), 10.0 * dx, 2.0 * dx, 20.0, -0.15)]
+    edit = materialize(prims, geo, nx, ny, None, 1.0)
+    mon, _, apos = asset_and_guard_cells(nx, ny, [(8, 15, 3, 3), (16, 25, 3, 3)], range(nx - 4, nx))
+    assets = [(np.array(a, np.int32), th, 1.0) for a, th in zip(apos, (0.02, 0.04))]
+    return AdjointCase(base, p, prims, edit, PreparedTerrain(base).apply_edit(edit), mon, assets, 0.01, 1e-3)
+
+
 def refine_dem(dem: np.ndarray, f: int) -> np.ndarray:
     """Upsample a DEM by an integer factor, bilinear between cell centres (the refinement
     the reference's revalidation uses, crates/itr-cli/src/revalidate.rs:19-38; every

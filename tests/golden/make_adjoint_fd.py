@@ -3,7 +3,10 @@ objective J = J_risk + lambda * J_earth on the 120x96 adjoint workload (scenario
 with the dt schedule frozen, at 32 seeded cells (d J / d dz of the cell) and for every
 primitive height.  With --level 2, tests/golden/adjoint_fd_config4_level2.json: the same
 design on the 60x48 face-crest level (the differences are of the fine raster's dz, which
-reaches the level through its block means and crest lines).  The schedule is the f32 forward's own (the oracle, which is bitwise
+reaches the level through its block means and crest lines).  With --case stage or --case
+inflow, tests/golden/adjoint_fd_<case>.json: scenario.adjoint_boundary_case, a grid with a
+Stage or Inflow segment on its west side, differenced at the segment's boundary cells
+(whose levels set the Stage ghost depths) and at cells around the assets.  The schedule is the f32 forward's own (the oracle, which is bitwise
 the reference's f32 fused kernel); every differenced run is the oracle in f64 on that
 schedule.
 
@@ -12,7 +15,8 @@ rain-fed films of a few mm switch the hydrostatic reconstruction max(h - dz, 0) 
 dry-cell test within +-1e-4 m of the bed, so that difference is not the derivative
 (it is kept for reference).  1e-6 and 1e-7 agree to ~1e-7 relative: J is smooth at that
 scale and the 1e-6 difference is the derivative the adjoint is checked against.
-Usage: python tests/golden/make_adjoint_fd.py [--level 2]  (about a minute on 8 cores)."""
+Usage: python tests/golden/make_adjoint_fd.py [--level 2 | --case stage | --case inflow]
+(about a minute on 8 cores)."""
 from __future__ import annotations
 
 import json
@@ -70,8 +74,8 @@ def fd_cells(case, per_asset=16, seed=41):
     return cells
 
 
-def main(level=1):
-    case = sc.config4(level=level)
+def main(level=1, kind=None):
+    case = sc.config4(level=level) if kind is None else sc.adjoint_boundary_case(kind)
     base, p = case.base, case.params
     nx, ny = base.fine.nx, base.fine.ny   # the edit's raster
     area = base.fine.geo.dx * base.fine.geo.dy
@@ -92,6 +96,8 @@ def main(level=1):
 
     j0, steps = J(dz0)
     cells = fd_cells(case)
+    if kind is not None:   # the segment's boundary cells (rows 10-29 of column 0), every other one
+        cells = [(r, 0) for r in range(10, 30, 2)] + [c for c in cells if c[1] != 0][:22]
     prims = case.prims
     jobs = []
     for H in STEPS:
@@ -116,14 +122,17 @@ def main(level=1):
         g_h = [(v[off + 2 * i] - v[off + 2 * i + 1]) / (2 * H) for i in range(len(prims))]
         fd[repr(H)] = {"dJ_dz_cell": g_cells, "dJ_dheight": g_h}
     out = {
-        "workload": (f"scenario.config4(level={level}): " + ("120x96 at 5 m" if level == 1 else
+        "workload": (f"scenario.adjoint_boundary_case({kind!r}): 60x48 at 5 m with a {kind} segment on the west "
+                     "side, 20 min storm, CFL 0.9, locked dt from the f32 forward" if kind is not None else
+                     f"scenario.config4(level={level}): " + ("120x96 at 5 m" if level == 1 else
                      "the 60x48 face-crest level at 10 m of the 120x96 5 m raster") +
                      ", 40 min storm, CFL 0.9, locked dt from the f32 forward"),
         "steps": int(steps), "j0": j0, "locked_dt": [float(x) for x in sched],
         "cells": [[int(r), int(c)] for r, c in cells], "heights": [float(x.height) for x in prims],
         "fd": fd, "fd_step_checked": repr(1e-6),
     }
-    name = "adjoint_fd_config4.json" if level == 1 else f"adjoint_fd_config4_level{level}.json"
+    name = (f"adjoint_fd_{kind}.json" if kind is not None else
+            "adjoint_fd_config4.json" if level == 1 else f"adjoint_fd_config4_level{level}.json")
     with open(os.path.join(HERE, name), "w") as f:
         json.dump(out, f)
     for H in STEPS:
@@ -133,4 +142,5 @@ def main(level=1):
 
 
 if __name__ == "__main__":
-    main(int(sys.argv[sys.argv.index("--level") + 1]) if "--level" in sys.argv else 1)
+    main(int(sys.argv[sys.argv.index("--level") + 1]) if "--level" in sys.argv else 1,
+         sys.argv[sys.argv.index("--case") + 1] if "--case" in sys.argv else None)

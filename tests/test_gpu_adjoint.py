@@ -137,6 +137,60 @@ def test_adjoint_crest_level_against_f64_fd(solver):
     assert worst <= REL_TOL
 
 
+@pytest.mark.parametrize("kind", ["stage", "inflow"])
+def test_adjoint_special_boundary_against_f64_fd(solver, kind):
+    """The adjoint on a grid with a Stage or an Inflow segment (scenario.adjoint_boundary_case,
+    fixtures tests/golden/adjoint_fd_<kind>.json): the ghosts' adjoints fold into their
+    boundary cells as the forward builds the ghosts (boundary.rs:43-47, 66-70) -- a Stage
+    ghost's depth f32(max(stage - z, 0)) moves with the boundary cell's level, an Inflow
+    ghost's depth is the cell's own above the critical depth -- against f64 central
+    differences at the segment's boundary cells, cells around the assets and the three
+    primitive heights (one primitive is a mound on the segment's boundary cells)."""
+    with open(os.path.join(HERE, "golden", f"adjoint_fd_{kind}.json")) as f:
+        fx = json.load(f)
+    case = sc.adjoint_boundary_case(kind)
+    p = case.params
+    p.locked_dt = np.array(fx["locked_dt"])
+    _say(f"case: adjoint with a {kind} segment, 60x48, {len(p.locked_dt)} locked steps")
+    res = solver.adjoint(p, case.base, [case.candidate], case.monitors, case.assets, case.tau_m)[0]
+    _say({k: v for k, v in res.items() if not isinstance(v, np.ndarray)})
+    assert res["steps"] == fx["steps"]
+    assert res["replay_mismatches"] == 0, "replayed segments must equal the checkpoints bitwise"
+    nx, ny = case.base.nx, case.base.ny
+    area = case.base.dx * case.base.dy
+    objv = ob.Objective([ob.Asset(np.asarray(a), th, w) for a, th, w in case.assets], case.tau_m, case.earth_weight)
+    dz0 = case.edit.dense(nx, ny)
+    cut, fill = ob.volumes(dz0, area)
+    j = res["j_risk"] + case.earth_weight * objv.j_earth(cut, fill)
+    assert abs(j - fx["j0"]) <= 1e-4 * abs(fx["j0"]), (j, fx["j0"])
+    g = res["dJ_dz_cell"] + objv.earth_gradient(dz0, area)
+    fd6, fd7, fd4 = (fx["fd"][repr(h)] for h in (1e-6, 1e-7, 1e-4))
+    scale = 1e-3 * np.abs(np.array(fd6["dJ_dz_cell"])).max()
+    worst, checked, boundary = 0.0, 0, 0
+    for (r, c), v6, v7, v4 in zip(fx["cells"], fd6["dJ_dz_cell"], fd7["dJ_dz_cell"], fd4["dJ_dz_cell"]):
+        smooth = _rel(v6, v7, scale) <= 1e-4
+        e = _rel(g[r, c], v6, scale)
+        if smooth:
+            worst = max(worst, e)
+            checked += 1
+            boundary += c == 0 and v6 != 0.0
+        _say(f"cell ({r:3d},{c:3d}) adjoint {g[r, c]: .6e} fd(1e-6) {v6: .6e} rel {e:.2e}"
+             f"{'' if smooth else ' (kink within 1e-6: not checked)'}  fd(1e-4) {v4: .6e}")
+    jac = tr.materialize_jacobian(case.prims, case.base.fine.geo, nx, ny, None, 1.0)
+    for i, (Jp, v6, v7, v4) in enumerate(zip(jac, fd6["dJ_dheight"], fd7["dJ_dheight"], fd4["dJ_dheight"])):
+        gh = float(np.sum(g * Jp))
+        e = _rel(gh, v6, 1e-9)
+        smooth = _rel(v6, v7, 1e-9) <= 1e-3
+        if smooth:
+            worst = max(worst, e)
+            checked += 1
+        _say(f"height {i}: adjoint {gh: .8e} fd(1e-6) {v6: .8e} rel {e:.2e}  fd(1e-7) {v7: .8e}"
+             f"{'' if smooth else ' (kink within 1e-6: not checked)'}")
+    _say(f"{kind}: worst rel {worst:.2e} over {checked} checked values ({boundary} non-zero boundary cells)")
+    assert checked >= 28 and boundary >= 3
+    assert worst <= REL_TOL
+
+
 def test_autograd_function_backward(solver):
     """The torch.autograd.Function wrapper: J.backward() gives the adjoint gradient, and
     it matches the f64 differences of the committed fixture.  The tensors live on the
@@ -163,13 +217,12 @@ def test_autograd_function_backward(solver):
     assert worst <= REL_TOL
 
 
-def test_adjoint_declines_stage_boundaries(solver):
-    """The adjoint covers wall and transmissive boundaries; a Stage ghost's depth depends on
-    the bed, so such grids return ITR_DECLINED."""
+def test_adjoint_declines_local_inertial(solver):
+    """The adjoint differentiates the HLL step: a local-inertial grid returns ITR_DECLINED."""
     from inverse_terrain_engine_amd.gpu import Declined
-    base = sc.validation_terrain(20, 12, 2.0, lambda x, y: 1.0 + 0.01 * x, n=0.03,
-                                 segments=[{"kind": "stage", "side": "west", "stage_m": 1.2}])
+    base = sc.validation_terrain(20, 12, 2.0, lambda x, y: 1.0 + 0.01 * x, n=0.03)
     p = sc.SolverParams(20.0, 10.0)
+    p.kernel = "inertial"
     p.locked_dt = [0.1] * 200
     with pytest.raises(Declined):
         solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)
@@ -231,3 +284,35 @@ def test_adjoint_band_march_equals_per_cell(solver, level):
         for key in keys:
             assert np.array_equal(a[key].view(np.uint64), b[key].view(np.uint64)), (c, key)
         assert np.abs(a["dJ_dz_cell"]).max() > 0.0
+
+
+@pytest.mark.parametrize("kind", ["stage", "inflow"])
+def test_adjoint_band_march_equals_per_cell_special(solver, kind):
+    """Band-march and per-cell replay and reverse give the same gradient bits on a grid with
+    a Stage or Inflow segment (special ghosts in the replay, their folds in the reverse)."""
+    import copy
+    case = sc.adjoint_boundary_case(kind)
+    geo = case.base.fine.geo
+    nx, ny = case.base.nx, case.base.ny
+    rng = sc.SplitMix64(21)
+    terrs = [case.candidate]
+    for _ in range(2):
+        prims = sc.random_primitives(rng, 3, geo, nx, ny)
+        terrs.append(tr.PreparedTerrain(case.base).apply_edit(tr.materialize(prims, geo, nx, ny, None, 1.0)))
+    p = copy.copy(case.params)
+    p.duration_s = 600.0
+    dts = solver.dt_schedule(p, case.base, case.candidate, case.monitors, 4000)
+    p.locked_dt = np.asarray(dts)
+    _say(f"case: band vs per-cell reverse with a {kind} segment, 3 x {nx}x{ny}, {len(dts)} steps")
+    out = {}
+    for mode in (1, 2):
+        solver.set_option("adjoint_mode", mode)
+        out[mode] = solver.adjoint(p, case.base, terrs, case.monitors, case.assets, case.tau_m)
+    solver.set_option("adjoint_mode", 0)
+    for c in range(3):
+        a, b = out[1][c], out[2][c]
+        assert a["replay_mismatches"] == 0 and b["replay_mismatches"] == 0
+        assert a["j_risk"] == b["j_risk"]
+        for key in ("dJ_dz_cell", "dJ_dz_face_x", "dJ_dz_face_y"):
+            assert np.array_equal(a[key].view(np.uint64), b[key].view(np.uint64)), (c, key)
+    assert np.abs(out[1][0]["dJ_dz_cell"][:, 0]).max() > 0.0

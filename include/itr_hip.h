@@ -287,8 +287,9 @@ void itr_hip_batch_destroy(itr_batch* b);
  * horizon (the run then ends after that many steps).  Signed levels and face-crest levels
  * (numerics.rs:117-129 with the faces of terrain.rs:378-429), wall, transmissive, Stage and
  * Inflow boundaries (a Stage ghost's depth moves with its boundary cell's level, which
- * dJ_dz_cell includes); an initial stage (its depth depends on the bed), local inertial
- * physics, replay grids and device designs return ITR_DECLINED.  A replayed segment that differs bitwise from its
+ * dJ_dz_cell includes) and an initial stage (the initial depth moves with each cell's level
+ * where the stage term is the maximum; needs z_level); local inertial physics, replay
+ * grids and device designs return ITR_DECLINED.  A replayed segment that differs bitwise from its
  * checkpoint returns ITR_EDEVICE (no gradient).  The earthwork term and the chain rule to
  * the fine raster and to primitive heights are host work on the returned gradients. */
 typedef struct {

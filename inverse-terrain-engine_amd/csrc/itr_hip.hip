@@ -4634,11 +4634,12 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
         set_err(err, errlen, "the adjoint differentiates the HLL step only");
         return ITR_DECLINED;
     }
-    // An initial stage makes the initial depth max(stage - z_ref - z, h0) a function of the
-    // bed, a term the reverse sweep (face jumps only) does not carry.
-    if (p->has_initial_stage) {
-        set_err(err, errlen, "the adjoint does not differentiate an initial stage (its depth depends on the bed)");
-        return ITR_DECLINED;
+    // An initial stage makes the initial depth f32(max(h0, stage - z_ref - z, 0)) a function
+    // of the level z: the adjoint of the initial state adds -lambda_h(0) where the stage
+    // term is the maximum (it needs the levels the initial state was built from).
+    if (p->has_initial_stage && !grid->z_level) {
+        set_err(err, errlen, "initial_stage needs z_level");
+        return ITR_EINVAL;
     }
     if (grid->replay || grid->design) {
         set_err(err, errlen, "the adjoint runs on prepared faces of a full grid (no replay, no device designs)");
@@ -4934,6 +4935,15 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
     }
     unsigned long long mis = 0;
     ADJ_TRY(hipMemcpy(&mis, d_mis, sizeof mis, hipMemcpyDeviceToHost));
+    // lambda of the initial state (the last sweep's output), for an initial stage
+    std::vector<float> lam0h;
+    if (p->has_initial_stage) {
+        lam0h.resize((size_t)nc * plane);
+        const float* L0 = lam + (size_t)li * state_n;
+        for (int c = 0; c < nc; c++)
+            ADJ_TRY(hipMemcpy(lam0h.data() + (size_t)c * plane, L0 + (size_t)c * cstride, sizeof(float) * plane,
+                              hipMemcpyDeviceToHost));
+    }
     // 5. face and cell gradients
     std::vector<double> hx((size_t)nc * plane), hy((size_t)nc * D.plane_y), hx2, hy2;
     ADJ_TRY(hipMemcpy(hx.data(), gx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost));
@@ -5025,6 +5035,19 @@ itr_status itr_hip_adjoint(itr_ctx* ctx, const itr_solver_params* p, const itr_g
                 }
             if (o.dJ_dz_cell)
                 for (size_t i = 0; i < (size_t)nx * ny; i++) o.dJ_dz_cell[i] = grid->wet[i] == 0.0f ? 0.0 : cell[i];
+        }
+        // Initial stage (solver.rs:292-304): h0 = f32(max(h_init, stage - z_ref - z, 0))
+        // moves with the level z (-1) where the stage term is the maximum.
+        if (p->has_initial_stage && o.dJ_dz_cell) {
+            const double* zl = grid->z_level + (size_t)c * nx * ny;
+            const float* L0 = lam0h.data() + (size_t)c * plane;
+            for (int r = 0; r < ny; r++)
+                for (int col = 0; col < nx; col++) {
+                    const size_t i = (size_t)r * nx + col;
+                    const double ds = p->initial_stage - grid->z_ref - zl[i];
+                    if (grid->wet[i] != 0.0f && ds > p->initial_depth && ds > 0.0)
+                        o.dJ_dz_cell[i] -= (double)L0[(size_t)r * P + col];
+                }
         }
         // Stage ghosts: their depth moves with the boundary cell's level (gz, per ghost).
         if (special && o.dJ_dz_cell) {

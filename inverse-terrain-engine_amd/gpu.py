@@ -293,7 +293,7 @@ class GpuSolver:
             raise ValueError("the adjoint needs p.locked_dt")
         keep = _Keep()
         q = _params_c(p, keep)
-        g = _grid_c(base, terrains, monitors, keep)
+        g = _grid_c(base, terrains, monitors, keep, need_z=p.initial_stage is not None)
         starts = np.zeros(len(assets) + 1, np.int32)
         pos = []
         for i, (ps, _, _) in enumerate(assets):

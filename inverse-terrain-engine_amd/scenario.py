@@ -326,12 +326,42 @@ def adjoint_boundary_case(kind: str, seed=6, duration=1200.0) -> AdjointCase:
     segment over rows 10-29 of the west side, the east side transmissive; 20 min storm
     peaking at 40 mm/h at 5 min, infiltration 5 mm/h, CFL 0.9.  One design: a mound on
     the segment's boundary cells, a berm across the inflow path and a swale; two 3x3
-    assets downstream (thresholds 0.02 and 0.04 m); tau 0.01 m, lambda 1e-3 per m^3."""
+    assets downstream (thresholds 0.02 and 0.04 m); tau 0.01 m, lambda 1e-3 per m^3.
+    kind='initial': no special segment; an initial stage 0.05 m above the bed's 30th
+    percentile ponds the low ground, the two assets sit on its shore (initial depths
+    nearest 0.03 and 0.12 m) and the mound on the first of them; the first 10 min of the
+    storm (over all of it the second asset's pond fills past a sill, where its f32 and f64
+    peaks part by 1.5 cm)."""
     nx, ny, dx = 60, 48, 5.0
     rng = SplitMix64(seed)
     dem = hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
     geo = GeoTransform(0.0, ny * dx, dx, dx)
     rows = (10, 30)
+    if kind == "initial":
+        stage = float(np.quantile(dem, 0.3)) + 0.05
+        d0 = stage - dem
+        boxes = []
+        for want in (0.03, 0.12):
+            cost = np.abs(d0 - want)
+            cost[:3, :] = cost[-3:, :] = np.inf
+            cost[:, :3] = cost[:, -5:] = np.inf
+            for (c0, r0, w, h) in boxes:   # keep the boxes apart
+                cost[max(r0 - 4, 0):r0 + h + 4, max(c0 - 4, 0):c0 + w + 4] = np.inf
+            r, c = np.unravel_index(int(np.argmin(cost)), cost.shape)
+            boxes.append((int(c) - 1, int(r) - 1, 3, 3))
+        base = TerrainBase.build(FineTerrain.new(dem, None, 0.035, geo), 1, [{"kind": "transmissive", "side": "east"}])
+        p = SolverParams(min(duration, 600.0), 60.0, cfl=0.9, rain=mm_h([[0, 0], [300, 40], [1200, 0]]),
+                         infil_capacity=5.0 / 1000.0 / 3600.0, initial_stage=stage)
+        p.cfl_hard = 1.0
+        y = lambda r: geo.origin_y - (r + 0.5) * dx
+        (ca, ra, _, _), (cb, rb, _, _) = boxes
+        prims = [Primitive("mound", (ca + 1.5) * dx, y(ra + 1), 0.0, 3.0 * dx, 0.0, 0.2),
+                 Primitive("berm", (cb + 1.5) * dx, y(rb - 2), 8.0 * dx, 2.0 * dx, 10.0, 0.15),
+                 Primitive("swale", (ca + 6.5) * dx, y(ra + 4), 8.0 * dx, 2.0 * dx, 60.0, -0.1)]
+        edit = materialize(prims, geo, nx, ny, None, 1.0)
+        mon, _, apos = asset_and_guard_cells(nx, ny, boxes, range(nx - 4, nx))
+        assets = [(np.array(a, np.int32), th, 1.0) for a, th in zip(apos, (0.05, 0.15))]
+        return AdjointCase(base, p, prims, edit, PreparedTerrain(base).apply_edit(edit), mon, assets, 0.01, 1e-3)
     if kind == "stage":
         seg = {"kind": "stage", "side": "west", "range_m": (rows[0] * dx, rows[1] * dx),
                "stage_m": float(dem[rows[0]:rows[1], 0].min()) + 0.3}

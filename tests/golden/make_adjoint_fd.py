@@ -6,7 +6,9 @@ design on the 60x48 face-crest level (the differences are of the fine raster's d
 reaches the level through its block means and crest lines).  With --case stage or --case
 inflow, tests/golden/adjoint_fd_<case>.json: scenario.adjoint_boundary_case, a grid with a
 Stage or Inflow segment on its west side, differenced at the segment's boundary cells
-(whose levels set the Stage ghost depths) and at cells around the assets.  The schedule is the f32 forward's own (the oracle, which is bitwise
+(whose levels set the Stage ghost depths) and at cells around the assets; --case initial:
+the same grid with an initial stage ponding its low ground, assets on the pond's shore,
+differenced at cells around them (whose levels set their initial depths).  The schedule is the f32 forward's own (the oracle, which is bitwise
 the reference's f32 fused kernel); every differenced run is the oracle in f64 on that
 schedule.
 
@@ -15,7 +17,7 @@ rain-fed films of a few mm switch the hydrostatic reconstruction max(h - dz, 0) 
 dry-cell test within +-1e-4 m of the bed, so that difference is not the derivative
 (it is kept for reference).  1e-6 and 1e-7 agree to ~1e-7 relative: J is smooth at that
 scale and the 1e-6 difference is the derivative the adjoint is checked against.
-Usage: python tests/golden/make_adjoint_fd.py [--level 2 | --case stage | --case inflow]
+Usage: python tests/golden/make_adjoint_fd.py [--level 2 | --case stage | --case inflow | --case initial]
 (about a minute on 8 cores)."""
 from __future__ import annotations
 
@@ -96,7 +98,7 @@ def main(level=1, kind=None):
 
     j0, steps = J(dz0)
     cells = fd_cells(case)
-    if kind is not None:   # the segment's boundary cells (rows 10-29 of column 0), every other one
+    if kind in ("stage", "inflow"):   # the segment's boundary cells (rows 10-29 of column 0), every other one
         cells = [(r, 0) for r in range(10, 30, 2)] + [c for c in cells if c[1] != 0][:22]
     prims = case.prims
     jobs = []

@@ -137,15 +137,21 @@ def test_adjoint_crest_level_against_f64_fd(solver):
     assert worst <= REL_TOL
 
 
-@pytest.mark.parametrize("kind", ["stage", "inflow"])
+@pytest.mark.parametrize("kind", ["stage", "inflow", "initial"])
 def test_adjoint_special_boundary_against_f64_fd(solver, kind):
-    """The adjoint on a grid with a Stage or an Inflow segment (scenario.adjoint_boundary_case,
-    fixtures tests/golden/adjoint_fd_<kind>.json): the ghosts' adjoints fold into their
-    boundary cells as the forward builds the ghosts (boundary.rs:43-47, 66-70) -- a Stage
-    ghost's depth f32(max(stage - z, 0)) moves with the boundary cell's level, an Inflow
-    ghost's depth is the cell's own above the critical depth -- against f64 central
-    differences at the segment's boundary cells, cells around the assets and the three
-    primitive heights (one primitive is a mound on the segment's boundary cells)."""
+    """The adjoint on a grid with a Stage or an Inflow segment, or with an initial stage
+    (scenario.adjoint_boundary_case, fixtures tests/golden/adjoint_fd_<kind>.json): the
+    ghosts' adjoints fold into their boundary cells as the forward builds the ghosts
+    (boundary.rs:43-47, 66-70) -- a Stage ghost's depth f32(max(stage - z, 0)) moves with the
+    boundary cell's level, an Inflow ghost's depth is the cell's own above the critical
+    depth -- and the initial depth f32(max(h0, stage - z_ref - z, 0)) (solver.rs:292-304)
+    moves with each cell's level; against f64 central differences at the segment's boundary
+    cells or the pond's shore cells, cells around the assets and the three primitive heights
+    (one primitive is a mound on the boundary cells or on the first asset).  With the
+    initial stage, cell components below 5% of the largest are held to 3e-2 instead of
+    1e-3: the f32 trajectory the adjoint differentiates and the f64 one the differences
+    follow part slightly at the pond's shore films (the heights and the dominant components
+    agree to 2e-5)."""
     with open(os.path.join(HERE, "golden", f"adjoint_fd_{kind}.json")) as f:
         fx = json.load(f)
     case = sc.adjoint_boundary_case(kind)
@@ -166,12 +172,16 @@ def test_adjoint_special_boundary_against_f64_fd(solver, kind):
     g = res["dJ_dz_cell"] + objv.earth_gradient(dz0, area)
     fd6, fd7, fd4 = (fx["fd"][repr(h)] for h in (1e-6, 1e-7, 1e-4))
     scale = 1e-3 * np.abs(np.array(fd6["dJ_dz_cell"])).max()
-    worst, checked, boundary = 0.0, 0, 0
+    worst, worst_small, checked, boundary = 0.0, 0.0, 0, 0
+    big = 50.0 * scale   # 5% of the largest difference
     for (r, c), v6, v7, v4 in zip(fx["cells"], fd6["dJ_dz_cell"], fd7["dJ_dz_cell"], fd4["dJ_dz_cell"]):
         smooth = _rel(v6, v7, scale) <= 1e-4
         e = _rel(g[r, c], v6, scale)
         if smooth:
-            worst = max(worst, e)
+            if kind == "initial" and abs(v6) < big:
+                worst_small = max(worst_small, e)
+            else:
+                worst = max(worst, e)
             checked += 1
             boundary += c == 0 and v6 != 0.0
         _say(f"cell ({r:3d},{c:3d}) adjoint {g[r, c]: .6e} fd(1e-6) {v6: .6e} rel {e:.2e}"
@@ -186,8 +196,14 @@ def test_adjoint_special_boundary_against_f64_fd(solver, kind):
             checked += 1
         _say(f"height {i}: adjoint {gh: .8e} fd(1e-6) {v6: .8e} rel {e:.2e}  fd(1e-7) {v7: .8e}"
              f"{'' if smooth else ' (kink within 1e-6: not checked)'}")
-    _say(f"{kind}: worst rel {worst:.2e} over {checked} checked values ({boundary} non-zero boundary cells)")
-    assert checked >= 28 and boundary >= 3
+    nonzero = sum(1 for v6, v7 in zip(fd6["dJ_dz_cell"], fd7["dJ_dz_cell"]) if v6 != 0.0 and _rel(v6, v7, scale) <= 1e-4)
+    _say(f"{kind}: worst rel {worst:.2e} over {checked} checked values ({nonzero} non-zero cells, "
+         f"{boundary} on the boundary)")
+    if kind == "initial":
+        _say(f"initial: components below 5% of the largest: worst rel {worst_small:.2e}")
+        assert checked >= 20 and nonzero >= 10 and worst_small <= 3e-2
+    else:
+        assert checked >= 28 and boundary >= 3
     assert worst <= REL_TOL
 
 
@@ -225,19 +241,6 @@ def test_adjoint_declines_local_inertial(solver):
     p.kernel = "inertial"
     p.locked_dt = [0.1] * 200
     with pytest.raises(Declined):
-        solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)
-
-
-def test_adjoint_declines_initial_stage(solver):
-    """An initial stage makes the initial depth max(stage - z, h0) a function of the bed,
-    which the reverse sweep does not differentiate: ITR_DECLINED, not a gradient missing
-    that term."""
-    from inverse_terrain_engine_amd.gpu import Declined
-    base = sc.validation_terrain(20, 12, 2.0, lambda x, y: 1.0 + 0.01 * x, n=0.03,
-                                 segments=[{"kind": "transmissive", "side": "east"}])
-    p = sc.SolverParams(20.0, 10.0, initial_stage=1.1)
-    p.locked_dt = [0.1] * 200
-    with pytest.raises(Declined, match="initial stage"):
         solver.adjoint(p, base, [tr.PreparedTerrain(base)], np.array([5], np.uint32), [([0], 0.05, 1.0)], 0.01)
 
 

@@ -322,7 +322,8 @@ def prep_leg(solver, sc, tr, w):
 
 def ladder(solver_cls, sc, steps):
     """The fidelity-ladder batch (256 instances: 45x35 crest, 90x70 and 180x140 signed,
-    4:2:1) in one launch per step, with band heights from the autotuner."""
+    4:2:1) in one call: three launch lists (one per class of grids) on three streams, one
+    launch per step each, with band heights from the autotuner."""
     lad = sc.config3()
     s = solver_cls(0)
     tuned = s.autotune(lad.params, lad.groups, trial_steps=16)

@@ -1781,8 +1781,8 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
     }
 }
 
-// Ledger sums of a completed local-inertial step (inertial.rs:171-216), by the first
-// band of the candidate: the positivity fixes and the boundary discharges are the
+// Ledger sums of a completed local-inertial step (inertial.rs:171-216), by the
+// candidate's designated workgroup (its control task or first band): the positivity fixes and the boundary discharges are the
 // reference's sequential f64 sums in its order (rows, then the north / south faces by
 // column); infiltration and CFL rate are f64 sums of the band sums in band order (the
 // reference adds cell by cell: equal to within rounding); the volume is the row FixedSums
@@ -2089,7 +2089,7 @@ itr_step_signed(KArgs a, uint32_t lk) { step_body<1, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_special(KArgs a, uint32_t lk) { step_body<3, true, false>(a, lk); }
-// Local-inertial grids (a separate task list, launched after the HLL kernel each step).
+// Local-inertial grids (launch lists of their own).
 __global__ void __launch_bounds__(1024) itr_step_li_signed(KArgs a, uint32_t lk) { step_body<1, false, true>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_li_crest(KArgs a, uint32_t lk) { step_body<2, false, true>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_li_general(KArgs a, uint32_t lk) { step_body<3, true, true>(a, lk); }

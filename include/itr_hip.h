@@ -224,6 +224,7 @@ typedef struct {
     uint64_t launches;   /* step-kernel launches */
     uint64_t syncs;      /* host round trips */
     int32_t band_rows, waves_per_task, steps_per_chunk, use_graph;
+    double sim_s;        /* sum over candidates of the simulated time advanced [s] */
 } itr_stats;
 
 /* GpuSolver::new (lib.rs:85-124).  device: HIP device ordinal. */

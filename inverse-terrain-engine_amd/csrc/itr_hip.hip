@@ -3177,8 +3177,13 @@ struct itr_batch {
     int ncand = 0;
     int ntasks = 0;                 // over all lists
     uint64_t launches = 0;          // step launches so far; control copy (launches-1)&1 is current
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
+    // Captured chunks of step launches, longest first: graph[0] holds the batch's chunk
+    // (graph_steps[0] launches), the others 16, 8 and 4 launches where shorter, so that an
+    // advance by a few steps also runs as graphs (back-to-back graph launches start each
+    // step kernel sooner than separate launches do).
+    static constexpr int kGraphs = 4;
+    hipGraphExec_t graph[kGraphs] = {};
+    int graph_steps[kGraphs] = {};
     itr_stats stats{};
     std::vector<std::vector<double>> cb_mon;   // scratch for callbacks
     // Built by itr_hip_run_batch_noalloc: device buffers live in the static scratch (never
@@ -3190,6 +3195,14 @@ struct itr_batch {
 
 namespace {
 
+void drop_graphs(itr_batch* b) {
+    for (int i = 0; i < itr_batch::kGraphs; i++) {
+        if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+        b->graph[i] = nullptr;
+        b->graph_steps[i] = 0;
+    }
+}
+
 void free_batch_mem(itr_batch* b) {
     // (buffers carved from the static scratch were never added to these lists)
     for (auto& g : b->grids)
@@ -3200,7 +3213,7 @@ void free_batch_mem(itr_batch* b) {
             if (p) (void)hipFree(p);
         if (b->h_status) (void)hipHostFree(b->h_status);
     }
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    drop_graphs(b);
 }
 
 template <class T>
@@ -3596,17 +3609,21 @@ itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t er
     return ITR_OK;
 }
 
-// n launches continuing the batch's launch sequence: graph chunks of `K` launches when
-// a graph is captured and the sequence is at a multiple of 4, single launches otherwise.
+// n launches continuing the batch's launch sequence: while the sequence is at a multiple
+// of 4, the longest captured chunk that fits; single launches otherwise.
 itr_status launch_steps(itr_batch* b, int64_t n, char* err, size_t errlen) {
     while (n > 0) {
-        if (b->graph && b->graph_steps <= n && (b->launches & 3u) == 0) {
-            HIP_TRY(hipGraphLaunch(b->graph, b->ctx->stream));
-            b->launches += (uint64_t)b->graph_steps;
-            n -= b->graph_steps;
+        int gi = -1;
+        if ((b->launches & 3u) == 0)
+            for (int i = 0; i < itr_batch::kGraphs && gi < 0; i++)
+                if (b->graph[i] && b->graph_steps[i] <= n) gi = i;
+        if (gi >= 0) {
+            HIP_TRY(hipGraphLaunch(b->graph[gi], b->ctx->stream));
+            b->launches += (uint64_t)b->graph_steps[gi];
+            n -= b->graph_steps[gi];
         } else {
             int64_t k = n;   // single launches: the remainder, or up to the next multiple of 4
-            if (b->graph && (b->launches & 3u) != 0) k = std::min<int64_t>(n, 4 - (int64_t)(b->launches & 3u));
+            if (b->graph[0] && (b->launches & 3u) != 0) k = std::min<int64_t>(n, 4 - (int64_t)(b->launches & 3u));
             itr_status s = enqueue_steps(b, (int)k, b->launches, err, errlen);
             if (s != ITR_OK) return s;
             b->launches += (uint64_t)k;
@@ -3630,31 +3647,44 @@ CandCtl* cur_ctl_dev(itr_batch* b, int c) {
     return (CandCtl*)b->d_snap + half * b->ncand + c;
 }
 
-// Capture K launches (launch indices 0..K-1 mod 4) in a hipGraph; K = 0 drops it.
-itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
-    if (K == b->graph_steps && (K == 0 || b->graph)) return ITR_OK;
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
-    b->graph_steps = 0;
-    if (K == 0 || b->ntasks == 0) return ITR_OK;
+// Capture n launches (launch indices 0..n-1 mod 4) in a hipGraph.
+itr_status capture_steps(itr_batch* b, int n, hipGraphExec_t* out, char* err, size_t errlen) {
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
-    itr_status ls = enqueue_steps(b, K, 0, err, errlen);
+    itr_status ls = enqueue_steps(b, n, 0, err, errlen);
     hipError_t ce = hipStreamEndCapture(b->ctx->stream, &g);
     if (ls != ITR_OK) return ls;
     if (ce != hipSuccess) {
         set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
         return ITR_EDEVICE;
     }
-    hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess) {
+        *out = nullptr;
         set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
         return ITR_EDEVICE;
     }
-    b->graph_steps = K;
+    return ITR_OK;
+}
+
+// Capture chunks of K launches and, where shorter than K, of 16, 8 and 4 launches;
+// K = 0 drops them.
+itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
+    if (K == b->graph_steps[0] && (K == 0 || b->graph[0])) return ITR_OK;
+    drop_graphs(b);
+    if (K == 0 || b->ntasks == 0) return ITR_OK;
+    int sizes[itr_batch::kGraphs] = {K, 16, 8, 4};
+    int ng = 0;
+    for (int i = 0; i < itr_batch::kGraphs; i++) {
+        if (i > 0 && sizes[i] >= K) continue;
+        itr_status s = capture_steps(b, sizes[i], &b->graph[ng], err, errlen);
+        if (s != ITR_OK) {
+            drop_graphs(b);
+            return s;
+        }
+        b->graph_steps[ng++] = sizes[i];
+    }
     return ITR_OK;
 }
 
@@ -3667,6 +3697,17 @@ double cell_steps_between(const itr_batch* b, const std::vector<CandCtl>& c0, co
         cs += (n1 - n0) * (double)b->grids[c1[c].grid].d.nx * (double)b->grids[c1[c].grid].d.ny;
     }
     return cs;
+}
+
+// Simulated time advanced between two control snapshots, summed over candidates (the
+// step in flight counted to its end time, as cell_steps_between counts it).
+double sim_s_between(const std::vector<CandCtl>& c0, const std::vector<CandCtl>& c1) {
+    double s = 0.0;
+    for (size_t c = 0; c < c1.size(); c++) {
+        const double t1 = c1[c].in_flight ? c1[c].t_new : c1[c].t, t0 = c0[c].in_flight ? c0[c].t_new : c0[c].t;
+        s += t1 - t0;
+    }
+    return s;
 }
 
 }  // namespace
@@ -4434,6 +4475,7 @@ itr_status itr_hip_batch_run(itr_batch* b, itr_sync_fn on_sync, void* user, char
     std::vector<CandCtl> c1;
     if ((s = read_ctl(b, c1, err, errlen)) != ITR_OK) return s;
     b->stats.cell_steps = cell_steps_between(b, c0, c1);
+    b->stats.sim_s = sim_s_between(c0, c1);
     return ITR_OK;
 }
 
@@ -4467,6 +4509,7 @@ itr_status itr_hip_batch_advance(itr_batch* b, int64_t n_steps, char* err, size_
     b->stats.device_ms = ms;
     b->stats.wall_s = std::chrono::duration<double>(t1 - t0).count();
     b->stats.cell_steps = cell_steps_between(b, c0, c1);
+    b->stats.sim_s = sim_s_between(c0, c1);
     b->stats.launches = (uint64_t)n_steps;
     b->stats.syncs = 1;
     b->stats.steps_per_chunk = K;
@@ -5107,11 +5150,7 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
     b->args.ntasks = b->ntasks;
     bind_lists(b);
     b->stats.band_rows = dg.empty() ? 0 : dg[0].R;
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-    }
-    b->graph_steps = 0;
+    drop_graphs(b);
     return ITR_OK;
 }
 

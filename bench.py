@@ -109,7 +109,7 @@ def _pmc_pass(counter, steps, band_rows):
     cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc",
            "--", sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", "2", "--no-cpu",
            "--no-traffic", "--copy-probe", "--no-adjoint", "--no-ladder", "--band-rows", str(band_rows),
-           "--advance", "0", "--guard-tol", "0"]
+           "--advance", "0", "--guard-tol", "0", "--no-graph"]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
     path = None
@@ -512,6 +512,9 @@ def main():
     ap.add_argument("--screening-only", action="store_true",
                     help="run only the local-inertial screening leg (profiling) and print it")
     ap.add_argument("--no-autotune", action="store_true", help="use the default band height")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step directly (no hipGraph chunks): for runs under rocprofv3 --kernel-trace, "
+                         "whose tracing of graph launches ends the process on this ROCm")
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
@@ -538,7 +541,7 @@ def main():
     if a.screening_only:
         print(json.dumps(screening_leg(GpuSolver, sc, tr, w, a.steps, warmup=a.warmup)), flush=True)
         return
-    solver = GpuSolver(0, band_rows=a.band_rows)
+    solver = GpuSolver(0, band_rows=a.band_rows, use_graph=not a.no_graph)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
     # The guard's baseline: the unedited terrain over the whole storm (objective.rs:132-144).

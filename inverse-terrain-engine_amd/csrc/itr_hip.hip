@@ -1377,16 +1377,31 @@ __device__ __forceinline__ void ledger_epilogue(const KArgs& a, const DevGrid& G
 constexpr int kLiCols = 61;
 constexpr double kLiTheta = 0.8;   // de Almeida et al. (2012) weighting (inertial.rs:85-98)
 
-struct LiK { double dt, rx, ry, rain, cap, h_dry, g, dx, dy, dmin; };
+struct LiK {
+    double dt, rx, ry, rain, cap, h_dry, g, dx, dy, dmin, inv_dx, inv_dy, inv_dmin;
+    bool p2;   // dx and dy are powers of two
+};
+
+// x / d for a grid spacing d.  When d is a power of two, 1/d is exact and x * (1/d)
+// rounds the same exact quotient as x / d: the same bits, without an f64 division.
+__device__ __forceinline__ double li_div(double x, double d, double inv_d, const LiK& K) {
+    double q;
+    if (K.p2) q = x * inv_d;
+    else q = x / d;
+    return q;
+}
+__device__ __forceinline__ bool pow2_spacing(double d) {
+    return d > 0.0 && d < 1e300 && (__double_as_longlong(d) & 0x000FFFFFFFFFFFFFll) == 0;
+}
 
 // face_q (inertial.rs:46-58): zero when the face depth is at most h_dry.
-__device__ __forceinline__ double li_face_q(double q, double hl, double hr, double dl, double dr, double d, double gn2,
-                                            const LiK& K) {
+__device__ __forceinline__ double li_face_q(double q, double hl, double hr, double dl, double dr, double d, double inv_d,
+                                            double gn2, const LiK& K) {
     const double el = hl - dl, er = hr - dr;
     const double hf = fmax(fmax(el, er), 0.0);
     double out = 0.0;
     if (hf > K.h_dry) {
-        const double slope = (er - el) / d;
+        const double slope = li_div(er - el, d, inv_d, K);
         const double r = rcbrt64(hf);
         const double r7 = r * r * r * r * r * r * r;
         out = (q - K.g * hf * K.dt * slope) / (1.0 + gn2 * K.dt * fabs(q) * r7);
@@ -1414,19 +1429,19 @@ template <bool CREST> __device__ __forceinline__ double li_jump(float f0, float 
 // ghost discharge; f0, f1: the adjacent interior face (transmissive bed slope).
 template <bool CREST>
 __device__ __forceinline__ double li_boundary(int code, bool lo, float qold, float hc, float hg, float qv, float f0, float f1,
-                                              bool has_face, double d, float n2c, const LiK& K) {
+                                              bool has_face, double d, double inv_d, float n2c, const LiK& K) {
     double q = 0.0;
     if (code == 1) {   // normal-depth outflow only
         double dz = has_face ? li_jump<CREST>(f0, f1) : 0.0;
         dz = lo ? fmax(dz, 0.0) : fmin(dz, 0.0);
         const double h = (double)hc;
-        q = li_face_q((double)qold, h, h, fmax(dz, 0.0), fmax(-dz, 0.0), d, (double)n2c, K);
+        q = li_face_q((double)qold, h, h, fmax(dz, 0.0), fmax(-dz, 0.0), d, inv_d, (double)n2c, K);
         q = lo ? fmin(q, 0.0) : fmax(q, 0.0);
     } else if (code == 3) {
         q = (lo ? 1.0 : -1.0) * fabs((double)qv);
     } else if (code == 2) {
-        q = lo ? li_face_q((double)qold, (double)hg, (double)hc, 0.0, 0.0, d, (double)n2c, K)
-               : li_face_q((double)qold, (double)hc, (double)hg, 0.0, 0.0, d, (double)n2c, K);
+        q = lo ? li_face_q((double)qold, (double)hg, (double)hc, 0.0, 0.0, d, inv_d, (double)n2c, K)
+               : li_face_q((double)qold, (double)hc, (double)hg, 0.0, 0.0, d, inv_d, (double)n2c, K);
     }
     return q;
 }
@@ -1547,7 +1562,7 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
             const float f0 = G.dzx[(size_t)local * plane + ro + max(fc, 1)];
             const float f1 = CREST ? G.dzx2[(size_t)local * plane + ro + max(fc, 1)] : 0.0f;
             const double q = li_boundary<CREST>(code, west, qold, S0[ro + cc], gs.x, gs.y, f0, f1, nx > 1, K.dx,
-                                                G.gn2[ro + cc], K);
+                                                    K.inv_dx, G.gn2[ro + cc], K);
             l_bnd[i] = canon((float)q);
         }
     }
@@ -1599,7 +1614,7 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         const double n2 = 0.5 * ((double)c.gn2 + (double)g2_r);
         double qt = (double)c.qx;
         if (col + 2 < nx) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)q_l + (double)q_r);
-        const float qi = canon((float)li_face_q(qt, (double)c.h, (double)h_r, dl, dr, K.dx, n2, K));
+        const float qi = canon((float)li_face_q(qt, (double)c.h, (double)h_r, dl, dr, K.dx, K.inv_dx, n2, K));
         // boundary faces come from the band's prologue
         return west_lane ? l_bnd[r - a0] : east_lane ? l_bnd[nrows + r - a0] : qi;
     };
@@ -1611,18 +1626,19 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         const double n2 = 0.5 * ((double)u.gn2 + (double)d.gn2);
         double qt = (double)u.qy;
         if (k + 1 < ny) qt = kLiTheta * qt + 0.5 * (1.0 - kLiTheta) * ((double)qp + (double)d.qy);
-        return canon((float)li_face_q(qt, (double)u.h, (double)d.h, dl, dr, K.dy, n2, K));
+        return canon((float)li_face_q(qt, (double)u.h, (double)d.h, dl, dr, K.dy, K.inv_dy, n2, K));
     };
     auto yface_south = [&](const Row& u) -> float {   // k = ny, u = row ny-1
         float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (special && code_s >= 2) gs = gs_ns(false);
-        return canon((float)li_boundary<CREST>(code_s, false, u.qy, u.h, gs.x, gs.y, u.dzy, u.dzy2, ny > 1, K.dy, u.gn2, K));
+        return canon((float)li_boundary<CREST>(code_s, false, u.qy, u.h, gs.x, gs.y, u.dzy, u.dzy2, ny > 1, K.dy,
+                                                   K.inv_dy, u.gn2, K));
     };
     auto yface_north = [&](const Row& d, const Row& d1) -> float {   // k = 0, d = row 0, d1 = row 1
         float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (special && code_n >= 2) gs = gs_ns(true);
-        return canon((float)li_boundary<CREST>(code_n, true, lif_in[ny + ccol], d.h, gs.x, gs.y, d1.dzy, d1.dzy2, ny > 1, K.dy,
-                                               d.gn2, K));
+        return canon((float)li_boundary<CREST>(code_n, true, lif_in[ny + ccol], d.h, gs.x, gs.y, d1.dzy, d1.dzy2,
+                                                   ny > 1, K.dy, K.inv_dy, d.gn2, K));
     };
 
     Row cur = fetch(a0);
@@ -1668,9 +1684,15 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         const float hnew = canon((float)(hn * wet));
         double s = 0.0;
         if (hn > K.h_dry) {
-            const double u = 0.5 * (fabs(dw) + fabs(de)) / hn;
-            const double v = 0.5 * (fabs(dn) + fabs(ds)) / hn;
-            s = (sqrt(K.g * hn) + fmax(u, v)) * 0.7 / K.dmin;
+            // u.max(v) with u = A / hn, v = B / hn: correctly rounded division by a finite
+            // positive hn is monotone, so max(A / hn, B / hn) = max(A, B) / hn bit for bit
+            // (a NaN operand: fmax keeps the other one on both sides), one f64 division
+            // instead of two.  An infinite hn takes the two divisions.
+            const double A = 0.5 * (fabs(dw) + fabs(de)), B = 0.5 * (fabs(dn) + fabs(ds));
+            double uv;
+            if (__builtin_expect(hn < __builtin_inf(), 1)) uv = fmax(A, B) / hn;
+            else uv = fmax(A / hn, B / hn);
+            s = li_div((sqrt(K.g * hn) + uv) * 0.7, K.dmin, K.inv_dmin, K);
         }
         const int r = j - a0;
         if (upd) {
@@ -2004,6 +2026,10 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         K.dx = G.dx;
         K.dy = G.dy;
         K.dmin = fmin(G.dx, G.dy);
+        K.inv_dx = 1.0 / G.dx;
+        K.inv_dy = 1.0 / G.dy;
+        K.inv_dmin = 1.0 / K.dmin;
+        K.p2 = pow2_spacing(G.dx) && pow2_spacing(G.dy);
         const double t_new = uni(sS.t_new);
         const double t_now = uni(sS.t);
         const bool sync = uni(sS.hits_sync) != 0;
@@ -2089,9 +2115,13 @@ itr_step_signed(KArgs a, uint32_t lk) { step_body<1, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_crest(KArgs a, uint32_t lk) { step_body<2, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_mixed(KArgs a, uint32_t lk) { step_body<3, false, false>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_special(KArgs a, uint32_t lk) { step_body<3, true, false>(a, lk); }
-// Local-inertial grids (launch lists of their own).
-__global__ void __launch_bounds__(1024) itr_step_li_signed(KArgs a, uint32_t lk) { step_body<1, false, true>(a, lk); }
-__global__ void __launch_bounds__(1024) itr_step_li_crest(KArgs a, uint32_t lk) { step_body<2, false, true>(a, lk); }
+// Local-inertial grids (launch lists of their own).  The signed and crest forms are held to
+// 96 VGPRs (5 waves per SIMD): their f64 chains are latency-bound, and at 4 waves the
+// step ran 22% slower.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+itr_step_li_signed(KArgs a, uint32_t lk) { step_body<1, false, true>(a, lk); }
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+itr_step_li_crest(KArgs a, uint32_t lk) { step_body<2, false, true>(a, lk); }
 __global__ void __launch_bounds__(1024) itr_step_li_general(KArgs a, uint32_t lk) { step_body<3, true, true>(a, lk); }
 
 // Head of the first step of every candidate: control copy 1 (read by launch 0).

@@ -420,7 +420,8 @@ def replay_leg(solver_cls, sc, tr, n_cand=128, duration=1800.0):
     candidates' primitives, the assets and the guard strip lie in a 64x64-cell window, and
     the region of interest is that window with a 48-cell margin.  The baseline records the
     tape; the candidates then run with the baseline's locked schedule on the full domain
-    and replayed on the region (replay-invalid ones are the evaluator's escalations)."""
+    and replayed on the region (replay-invalid ones are the evaluator's escalations).  Both
+    runs use band heights tuned for them beforehand (untimed)."""
     import numpy as np
     nx = ny = 512
     dx = 2.0
@@ -446,14 +447,16 @@ def replay_leg(solver_cls, sc, tr, n_cand=128, duration=1800.0):
     tape, b0 = s.record_tape(p, base, roi, mon)
     t_rec = time.perf_counter() - t0
     pl = sc.SolverParams(duration, 60.0, rain=p.rain, infil_capacity=p.infil_capacity, locked_dt=dts)
+    s.autotune(pl, [(base, terrs, mon, None)], trial_steps=16)   # untimed: band height of the full-domain batch
     t0 = time.perf_counter()
     sess = s.session(pl, [(base, terrs, mon, None)])
     sess.run()
     full = sess.results(want_state=False)
     sess.close()
     t_full = time.perf_counter() - t0
+    rows = s.tune_replay_rows(pl, tape, terrs, sub_mon)   # untimed: band height of the replay batch
     t0 = time.perf_counter()
-    rep = s.run_replay(pl, tape, terrs, sub_mon, want_state=False)
+    rep = s.run_replay(pl, tape, terrs, sub_mon, want_state=False, band_rows=rows)
     t_rep = time.perf_counter() - t0
     s.close()
     valid = [r for r in rep if r.aborted == 0]
@@ -465,6 +468,7 @@ def replay_leg(solver_cls, sc, tr, n_cand=128, duration=1800.0):
                         f"{duration:g} s of the storm, region {rw}x{rh} cells ({100.0 * rw * rh / (nx * ny):.1f}% of the grid)",
             "steps": int(b0.steps), "record_s": t_rec, "full_domain_s": t_full, "replay_s": t_rep,
             "full_cell_steps_per_s": cs_full / t_full, "replay_cell_steps_per_s": cs_rep / t_rep,
+            "replay_band_rows": rows,
             "valid": len(valid), "invalid": len(rep) - len(valid), "valid_monitors_equal_full": same,
             "valid_max_peak_depth_diff_m": dmax,
             "speedup_vs_full_domain": t_full / t_rep}

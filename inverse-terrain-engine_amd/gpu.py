@@ -9,6 +9,7 @@ Every call goes to the HIP library; there is no CPU fallback in this module.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 from dataclasses import dataclass, field
 
@@ -258,6 +259,7 @@ class GpuSolver:
         for k, v in (("band_rows", band_rows), ("steps_per_chunk", steps_per_chunk),
                      ("use_graph", 1 if use_graph else 0)):
             self._L.itr_hip_set_option(self._h, k.encode(), int(v))
+        self._band_rows = int(band_rows)
         self.last_stats = None
 
     def close(self):
@@ -276,6 +278,8 @@ class GpuSolver:
 
     def set_option(self, name: str, value: int):
         _check(self._L.itr_hip_set_option(self._h, name.encode(), int(value)), None)
+        if name == "band_rows":
+            self._band_rows = int(value)
 
     def adjoint(self, p: SolverParams, base: TerrainBase, terrains, monitors, assets, tau_m: float):
         """Discrete adjoint of J_risk for each candidate on the locked schedule p.locked_dt.
@@ -367,18 +371,45 @@ class GpuSolver:
         return Tape((c0, r0, w, h), ring[:ns.value].copy(), band[:nsy.value].copy(), ns.value, nsy.value), run
 
     def run_replay(self, p: SolverParams, tape: Tape, terrains, sub_monitors, tol_h=2e-3, tol_q=1e-3, guard=None,
-                   want_state=True):
+                   want_state=True, band_rows: int = 0):
         """Candidates replayed on the tape's region (evaluator.rs:427-440): aborted = 5 is
-        ReplayInvalid (escalate to a full-domain run).  State arrays are region-sized."""
+        ReplayInvalid (escalate to a full-domain run).  State arrays are region-sized.
+        band_rows > 0 sets the band height for this call (tune_replay_rows finds one); the
+        results are the same bits at any height."""
         keep = _Keep()
         g = _replay_grid_c(tape, terrains, sub_monitors, keep, tol_h, tol_q, guard)
-        s = BatchSession(self, p, [(None, terrains, sub_monitors, None)], None, grids=[g], keep=keep,
-                         shapes=[(tape.roi[2], tape.roi[3])])
+        saved = self._band_rows
+        if band_rows > 0:
+            self._L.itr_hip_set_option(self._h, b"band_rows", int(band_rows))
         try:
-            s.run()
-            return s.results(want_state)
+            s = BatchSession(self, p, [(None, terrains, sub_monitors, None)], None, grids=[g], keep=keep,
+                             shapes=[(tape.roi[2], tape.roi[3])])
+            try:
+                s.run()
+                return s.results(want_state)
+            finally:
+                s.close()
         finally:
-            s.close()
+            if band_rows > 0:
+                self._L.itr_hip_set_option(self._h, b"band_rows", saved)
+
+    def tune_replay_rows(self, p: SolverParams, tape: Tape, terrains, sub_monitors, rows=(8, 12, 16, 20, 24, 28),
+                         trial_steps: int = 512, tol_h=2e-3, tol_q=1e-3, guard=None) -> int:
+        """Band height for replay batches like this one: the first trial_steps steps of the
+        replay at each height in `rows` (after one warm-up trial), the fastest by device
+        time.  The replay kernel carries the tape's ghost ring and band checks, so its best
+        height differs from the plain step's (itr_hip_autotune tunes those)."""
+        q = copy.copy(p)
+        q.max_steps = int(trial_steps)
+        best, best_ms = 0, float("inf")
+        for i, R in enumerate((rows[0],) + tuple(rows)):
+            self.run_replay(q, tape, terrains, sub_monitors, tol_h, tol_q, guard, want_state=False, band_rows=R)
+            if i == 0:
+                continue   # warm-up
+            ms = self.last_stats["device_ms"] if self.last_stats else float("inf")
+            if ms < best_ms:
+                best, best_ms = R, ms
+        return best
 
     def dt_schedule(self, p: SolverParams, base: TerrainBase, terrain, monitors=(), max_steps: int = 0,
                     capacity: int = 1 << 20) -> np.ndarray:

@@ -89,6 +89,17 @@ def test_replay_on_region(solver):
     assert runs[0].aborted == 0 and runs[1].aborted == 0 and runs[2].aborted == 5
     assert np.array_equal(runs[0].depth.view(np.uint32), run0.depth[r0:r0 + h, c0:c0 + w].view(np.uint32))
     assert np.array_equal(runs[0].monitor_max.view(np.uint64), run0.monitor_max.view(np.uint64))
+    # the band-height sweep for replay batches picks one of its heights, and a replay at
+    # any height gives the same bits
+    R = solver.tune_replay_rows(pl, tape, terrs, sub_mon, rows=(3, 7, 26), trial_steps=64)
+    assert R in (3, 7, 26)
+    for rows in (3, 7, 26):
+        _say(f"case: replay at band height {rows}")
+        again = solver.run_replay(pl, tape, terrs, sub_mon, band_rows=rows)
+        for i, (a, b) in enumerate(zip(again, runs)):
+            assert a.aborted == b.aborted and a.steps == b.steps, (rows, i)
+            assert np.array_equal(a.depth.view(np.uint32), b.depth.view(np.uint32)), (rows, i)
+            assert np.array_equal(a.monitor_max.view(np.uint64), b.monitor_max.view(np.uint64)), (rows, i)
 
 
 def test_replay_crest_level_with_guard(solver):

@@ -533,8 +533,10 @@ def main():
     load_package()
     from inverse_terrain_engine_amd import scenario as sc
     from inverse_terrain_engine_amd import terrain as tr
-    from inverse_terrain_engine_amd.gpu import GpuSolver
+    from inverse_terrain_engine_amd.gpu import GpuSolver as _GpuSolver
+    import functools
     import numpy as np
+    GpuSolver = functools.partial(_GpuSolver, use_graph=not a.no_graph)   # every leg's solvers
 
     t0 = time.time()
     w = sc.config5(n_cand=a.cands, n=a.n)
@@ -545,7 +547,7 @@ def main():
     if a.screening_only:
         print(json.dumps(screening_leg(GpuSolver, sc, tr, w, a.steps, warmup=a.warmup)), flush=True)
         return
-    solver = GpuSolver(0, band_rows=a.band_rows, use_graph=not a.no_graph)
+    solver = GpuSolver(0, band_rows=a.band_rows)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
     # The guard's baseline: the unedited terrain over the whole storm (objective.rs:132-144).

@@ -695,10 +695,24 @@ __device__ __forceinline__ int band_index(int i, int j, int w, int h) {
     return w + mid * (h - 2) + (i - 1);
 }
 
+// A band's monitor range (positions q0 .. q1-1 of the sorted monitor list) and the plane
+// offset of the calling thread's first monitored cell: two dependent loads, which the
+// kernel issues before its control prologue so that their latency passes under it.
+struct MonRange { int q0, q1; uint32_t off0; };
+__device__ __forceinline__ MonRange mon_range(const DevGrid& G, int band, int tid) {
+    const int a0 = band * G.R, b0 = min(a0 + G.R, G.ny);
+    MonRange m;
+    m.q0 = G.mon_row[a0];
+    m.q1 = G.mon_row[b0];
+    const int qm = m.q0 + tid;
+    m.off0 = qm < m.q1 ? G.mon_off[qm] : 0u;
+    return m;
+}
+
 template <bool CREST, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
                                           double t, double t_new, bool sync, int par, bool from_snap, uint64_t step,
-                                          uint32_t sync_idx, StepAcc* acc_cur, float* lds) {
+                                          uint32_t sync_idx, StepAcc* acc_cur, float* lds, const MonRange& mr) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -736,11 +750,11 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
     // The band's monitors: positions mon_q0 .. mon_q1-1 of the sorted monitor list; this
     // thread's first one (qm) has its plane offset staged in LDS for the epilogue.
-    const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
+    const int mon_q0 = mr.q0, mon_q1 = mr.q1;
     const ITR_GLOBAL uint32_t* __restrict__ MONOFF = gp(G.mon_off);
     const int qm = mon_q0 + tid;
     const bool has_m = qm < mon_q1;
-    const uint32_t m_off0 = has_m ? MONOFF[qm] : 0u;
+    const uint32_t m_off0 = mr.off0;
 
     ITR_GLOBAL float* __restrict__ sbuf = gp(G.sbuf + (size_t)local * G.plane);
     ITR_GLOBAL float* __restrict__ fixbuf = gp(G.fixbuf + (size_t)local * G.plane);
@@ -1471,7 +1485,7 @@ __host__ __device__ inline LiLds li_lds_layout(int R, int nx, bool special) {
 template <bool CREST, bool SPECIAL>
 __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, int local, int band, const LiK& K, double t,
                                              double t_new, bool sync, int par, bool from_snap, StepAcc* acc_cur,
-                                             float* lds) {
+                                             float* lds, const MonRange& mr) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -1500,11 +1514,11 @@ __device__ __forceinline__ void li_band_step(const KArgs& a, const DevGrid& G, i
         gp(from_snap ? G.snap_tmax + (size_t)local * G.n_mon : G.mon_tmax + slot(G, par, local) * G.n_mon);
     ITR_GLOBAL float* __restrict__ mon_out = gp(G.mon_max + slot(G, npar, local) * G.n_mon);
     ITR_GLOBAL double* __restrict__ tmax_out = gp(G.mon_tmax + slot(G, npar, local) * G.n_mon);
-    const int mon_q0 = G.mon_row[a0], mon_q1 = G.mon_row[b0];
+    const int mon_q0 = mr.q0, mon_q1 = mr.q1;
     const ITR_GLOBAL uint32_t* __restrict__ MONOFF = gp(G.mon_off);
     const int qm = mon_q0 + tid;
     const bool has_m = qm < mon_q1;
-    const uint32_t m_off0 = has_m ? MONOFF[qm] : 0u;
+    const uint32_t m_off0 = mr.off0;
     ITR_GLOBAL double* __restrict__ fix64 = gp(G.fix64 + slot(G, npar, local) * plane);
     LiRowAcc* ra_out = (LiRowAcc*)(G.rowacc + slot(G, npar, local) * ny);
     const uint32_t ccol = (uint32_t)min(max(col, 0), nx - 1);
@@ -1982,6 +1996,7 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     const DevParams& p = G.prm;
     const CandCtl* Sin = a.snap + (size_t)((lk + 1) & 1) * a.ncand + cand;
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
+    const MonRange mr = ctl_only ? MonRange{0, 0, 0u} : mon_range(G, tk.z, tid);
     if ((Sin->flags & F_DONE) && !designated) return;
     const StepAcc A = a.acc[(size_t)((lk + 3) & 3) * a.ncand + cand];
     const bool completed = Sin->in_flight != 0 && !(Sin->flags & F_DONE);
@@ -2037,10 +2052,10 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         const bool from_snap = uni((int)sS.restore) != 0;
         StepAcc* acc_cur = a.acc + (size_t)(lk & 3) * a.ncand + cand;
         if (MODES == 3) {
-            if (G.crest) li_band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
-            else li_band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
+            if (G.crest) li_band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds, mr);
+            else li_band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds, mr);
         } else {
-            li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds);
+            li_band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, acc_cur, lds, mr);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!LI && s_go && !ctl_only) {
@@ -2069,13 +2084,13 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         if (MODES == 3) {
             if (G.crest)
                 band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx, acc_cur,
-                                            lds);
+                                            lds, mr);
             else
                 band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx,
-                                             acc_cur, lds);
+                                             acc_cur, lds, mr);
         } else {
             band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
-                                                              step, sync_idx, acc_cur, lds);
+                                                              step, sync_idx, acc_cur, lds, mr);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!designated) {

@@ -709,10 +709,61 @@ __device__ __forceinline__ MonRange mon_range(const DevGrid& G, int band, int ti
     return m;
 }
 
+// The band march's lane geometry: the lane's column and its group's first row relative to
+// the band (the tail wave, when it updates few columns, splits into `tail_groups` lane
+// groups of 64 / tail_groups lanes; group g marches rows a0 + g * t_rows ..; every other
+// wave is one group of 64 lanes, whose first and last lanes are halos).
+struct LaneGeo { int tail_s, tgw, tg, gl, t_rows, lrow, col; bool tail; };
+template <bool SPECIAL>
+__device__ __forceinline__ LaneGeo lane_geo(const DevGrid& G, int band, int tid) {
+    const int a0 = band * G.R, nrows = min(a0 + G.R, G.ny) - a0;
+    const int lane = tid & 63, wave = tid >> 6;
+    LaneGeo q;
+    q.tail_s = SPECIAL ? 1 : G.tail_groups;
+    q.tail = q.tail_s > 1 && uni(wave) == G.waves - 1;
+    q.tgw = q.tail ? 64 / q.tail_s : 64;
+    q.tg = q.tail ? lane / q.tgw : 0;
+    q.gl = lane - q.tg * q.tgw;
+    q.t_rows = q.tail ? (nrows + q.tail_s - 1) / q.tail_s : nrows;
+    q.lrow = q.tg * q.t_rows;
+    q.col = wave * kColsPerWave - 1 + q.gl;
+    return q;
+}
+
+// The old state of the band's first two rows (a0-1 and a0, plus the lane group's row
+// offset) at the lane's column, read at the kernel's start from the state copy the step
+// will most likely read (the copy the previous launch wrote), so that their latency
+// passes under the control prologue.  The march uses them when the step does read that
+// copy (`par` equal, no restart from a snapshot) and loads the rows itself otherwise.
+// (Reading more of the march's first inputs this way cost registers and time: measured.)
+struct FirstRows { float h0, qx0, qy0, h1, qx1, qy1; int cwe0, cwe1, par; };
+template <bool SPECIAL>
+__device__ __forceinline__ FirstRows first_rows(const DevGrid& G, int local, int band, int par, int tid) {
+    const LaneGeo q = lane_geo<SPECIAL>(G, band, tid);
+    const int nx = G.nx, ny = G.ny;
+    const int a0 = band * G.R;
+    const float* S = G.state + (size_t)local * 6 * G.plane + (size_t)par * 3 * G.plane;
+    const int cc = min(max(q.col, 0), nx - 1);
+    const int r0 = min(max(a0 - 1 + q.lrow, 0), ny - 1), r1 = min(max(a0 + q.lrow, 0), ny - 1);
+    const size_t o0 = (size_t)r0 * G.P + cc, o1 = (size_t)r1 * G.P + cc;
+    FirstRows f;
+    f.h0 = S[o0];
+    f.qx0 = S[G.plane + o0];
+    f.qy0 = S[2 * G.plane + o0];
+    f.h1 = S[o1];
+    f.qx1 = S[G.plane + o1];
+    f.qy1 = S[2 * G.plane + o1];
+    f.cwe0 = G.ghost[2 * nx + (q.col < 0 ? 0 : ny) + r0];
+    f.cwe1 = G.ghost[2 * nx + (q.col < 0 ? 0 : ny) + r1];
+    f.par = par;
+    return f;
+}
+
 template <bool CREST, bool SPECIAL>
 __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int local, int band, const StepK& K,
                                           double t, double t_new, bool sync, int par, bool from_snap, uint64_t step,
-                                          uint32_t sync_idx, StepAcc* acc_cur, float* lds, const MonRange& mr) {
+                                          uint32_t sync_idx, StepAcc* acc_cur, float* lds, const MonRange& mr,
+                                          const FirstRows& pre) {
     const int nx = G.nx, ny = G.ny, P = G.P, R = G.R;
     const int a0 = band * R;
     const int b0 = min(a0 + R, ny);
@@ -722,17 +773,11 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = G.waves;
-    // The tail wave (the last wave, when it updates few columns) splits into `tail_s` lane
-    // groups of `tgw` lanes; group g marches rows a0 + g*t_rows .. of the band.  Every
-    // other wave is one group of 64 lanes.  A group's first and last lanes are halos.
-    const int tail_s = SPECIAL ? 1 : G.tail_groups;
-    const bool tail = tail_s > 1 && uni(wave) == nwaves - 1;
-    const int tgw = tail ? 64 / tail_s : 64;
-    const int tg = tail ? lane / tgw : 0;
-    const int gl = lane - tg * tgw;   // lane within its group
-    const int t_rows = tail ? (nrows + tail_s - 1) / tail_s : nrows;
-    const int lrow = tg * t_rows;     // the group's first row, relative to a0
-    const int col = wave * kColsPerWave - 1 + gl;
+    // The tail wave's lane groups (lane_geo): group tg of tgw lanes marches rows
+    // a0 + lrow .. of the band; gl is the lane within its group.
+    const LaneGeo lg = lane_geo<SPECIAL>(G, band, tid);
+    const int tail_s = lg.tail_s, tgw = lg.tgw, tg = lg.tg, gl = lg.gl, t_rows = lg.t_rows, lrow = lg.lrow, col = lg.col;
+    const bool tail = lg.tail;
     const bool upd = gl >= 1 && gl <= tgw - 2 && tg < tail_s && col < nx;
     const int npar = par ^ 1;
 
@@ -990,9 +1035,24 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
         };
         const int lr = TL ? lrow : 0;
 
-        // Prologue: rows a0-1 and a0, and the y-face between them.
-        const Cell cp = fix(FETCH(a0 - 1), a0 - 1 + lr);
-        const Cell c0 = fix(FETCH(a0), a0 + lr);
+        // Prologue: rows a0-1 and a0 (read at the kernel's start when the step reads the copy
+        // predicted there), and the y-face between them.
+        Cell rp, r0;
+        if (pre.par == par && !from_snap) {
+            rp.h = pre.h0;
+            rp.qx = pre.qx0;
+            rp.qy = pre.qy0;
+            rp.code_we = pre.cwe0;
+            r0.h = pre.h1;
+            r0.qx = pre.qx1;
+            r0.qy = pre.qy1;
+            r0.code_we = pre.cwe1;
+        } else {
+            rp = FETCH(a0 - 1);
+            r0 = FETCH(a0);
+        }
+        const Cell cp = fix(rp, a0 - 1 + lr);
+        const Cell c0 = fix(r0, a0 + lr);
         float ch = c0.h, cqx = c0.qx, cqy = c0.qy;
         if (!TL && rec && ring_owner && a0 == 0) ring_put(-1, cp.h, cp.qx, cp.qy);
         const Vel vp = cell_velocity(cp.h, cp.qx, cp.qy, K);
@@ -1998,6 +2058,12 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     CandCtl* Sout = a.snap + (size_t)(lk & 1) * a.ncand + cand;
     const MonRange mr = ctl_only ? MonRange{0, 0, 0u} : mon_range(G, tk.z, tid);
     if ((Sin->flags & F_DONE) && !designated) return;
+    // The band's first rows from the copy the previous launch wrote (the completed step's
+    // state: parity flipped when a step is in flight), under the control prologue.
+    FirstRows pre{};
+    pre.par = -1;
+    if (!LI && !ctl_only && !(Sin->flags & F_DONE))
+        pre = first_rows<SPECIAL>(G, local, tk.z, Sin->parity ^ (Sin->in_flight ? 1 : 0), tid);
     const StepAcc A = a.acc[(size_t)((lk + 3) & 3) * a.ncand + cand];
     const bool completed = Sin->in_flight != 0 && !(Sin->flags & F_DONE);
     for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)&sS)[w] = ((const uint32_t*)Sin)[w];
@@ -2084,13 +2150,13 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
         if (MODES == 3) {
             if (G.crest)
                 band_step<true, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx, acc_cur,
-                                            lds, mr);
+                                            lds, mr, pre);
             else
                 band_step<false, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap, step, sync_idx,
-                                             acc_cur, lds, mr);
+                                             acc_cur, lds, mr, pre);
         } else {
             band_step<MODES == 2, SPECIAL>(a, G, local, tk.z, K, t_now, t_new, sync, par, from_snap,
-                                                              step, sync_idx, acc_cur, lds, mr);
+                                                              step, sync_idx, acc_cur, lds, mr, pre);
         }
         if (sync && G.snap_state && !sS.locked_off) snapshot_band(G, local, tk.z, par ^ 1);
     } else if (!designated) {

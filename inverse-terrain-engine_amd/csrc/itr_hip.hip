@@ -881,8 +881,9 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     uint32_t* l_wet = (uint32_t*)(lds + LL.wet);
     int* l_red = (int*)(lds + LL.red);            // [2 * kMaxWaves]
     float* l_acc = lds + LL.acc;                  // [R][16][4]
-    for (int r = tid; r < R; r += nthr) { l_wet[r] = 0u; l_fixmask[r] = 0u; l_bw[r] = 0.0f; l_be[r] = 0.0f; }
-    __syncthreads();
+    // (l_wet, l_fixmask, l_bw and l_be were zeroed before the control prologue's barrier;
+    // the special ghosts staged above are read by other threads)
+    if (special) __syncthreads();
 
     // Old-state cell of this lane's column in `row`: raw loads (scalar row base + lane
     // offset) and, on ghost rows or in the edge waves only, the boundary transform.
@@ -2067,6 +2068,14 @@ __device__ __forceinline__ void step_body(KArgs a, uint32_t lk) {
     const StepAcc A = a.acc[(size_t)((lk + 3) & 3) * a.ncand + cand];
     const bool completed = Sin->in_flight != 0 && !(Sin->flags & F_DONE);
     for (int w = tid; w < kCtlWords; w += nthr) ((uint32_t*)&sS)[w] = ((const uint32_t*)Sin)[w];
+    if (!LI && !ctl_only) {   // a band's row flags and boundary masses (band_step), zeroed before this barrier
+        const LdsLayout LL = lds_layout(G.R, G.nx, G.special != 0);
+        uint32_t* l_fixmask = (uint32_t*)(lds + LL.fixmask);
+        uint32_t* l_wet = (uint32_t*)(lds + LL.wet);
+        float* l_bw = lds + LL.bw;
+        float* l_be = lds + LL.be;
+        for (int r = tid; r < G.R; r += nthr) { l_wet[r] = 0u; l_fixmask[r] = 0u; l_bw[r] = 0.0f; l_be[r] = 0.0f; }
+    }
     __syncthreads();
     if (tid < kWave) {
         const bool go = complete_step(p, G, sS, A, tid);

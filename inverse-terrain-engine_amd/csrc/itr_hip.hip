@@ -1013,7 +1013,7 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
     // a lane mask (LLVM fcmp predicates: 2 ogt, 12 ult, 8 uno, 14 une), with every lane active.
     const uint64_t upd_mask = __builtin_amdgcn_ballot_w64(upd);
     auto lanes_ogt0 = [](float x) -> uint64_t { return __builtin_amdgcn_fcmpf(x, 0.0f, 2); };
-    auto lanes_une0 = [](float x) -> uint64_t { return __builtin_amdgcn_fcmpf(x, 0.0f, 14); };
+    auto lanes_olt0 = [](float x) -> uint64_t { return __builtin_amdgcn_fcmpf(x, 0.0f, 4); };
     // The march down the band.  TL: the tail wave's form, whose lanes are at row j + lrow
     // (their group's row) when the march is at row j; everything else is the same code.
     auto march = [&](auto tag) {
@@ -1172,15 +1172,18 @@ __device__ __forceinline__ void band_step(const KArgs& a, const DevGrid& G, int 
             }
             if constexpr (TL) {   // the groups' rows differ: per-lane row flags
                 if (upd && rv && u.h > 0.0f) atomicOr(&l_wet[r], 1u);
-                if (__builtin_expect(__builtin_amdgcn_ballot_w64(upd && rv && u.fix != 0.0f) != 0ull, 0)) {
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(upd && rv && u.h1 < 0.0f) != 0ull, 0)) {
                     if (upd && rv) fixbuf[(size_t)jl * P + ccol] = u.fix;
                     if (upd && rv && u.fix != 0.0f) atomicOr(&l_fixmask[r], 1u << wave);
                 }
                 if (b0 == ny && owner && jl == ny - 1) yb_out[nx + col] = fs.m;
             } else {
+                // (a plain store: every writer of a row's word writes 1, the tail wave with an
+                // atomic or of 1, so the word ends at 1 in any order)
                 const uint64_t wb = lanes_ogt0(u.h) & upd_mask;
-                if (lane == 0 && wb != 0ull) atomicOr(&l_wet[r], 1u);
-                if (__builtin_expect((lanes_une0(u.fix) & upd_mask) != 0ull, 0)) {   // rare: a cell went negative
+                if (lane == 0 && wb != 0ull) l_wet[r] = 1u;
+                // (fix = -h1 where h1 < 0, else +0: nonzero exactly where h1 < 0)
+                if (__builtin_expect((lanes_olt0(u.h1) & upd_mask) != 0ull, 0)) {   // rare: a cell went negative
                     if (upd) fixbuf[(size_t)j * P + ccol] = u.fix;
                     if (lane == 0) atomicOr(&l_fixmask[r], 1u << wave);
                 }
@@ -3238,6 +3241,8 @@ struct itr_ctx {
     int adjoint_mode = 0;  // reverse sweep: 0 by batch size, 1 band march, 2 per-cell kernels
     int adjoint_rows = 0;  // rows per band of the reverse band march (0: 16)
     int control_task = -1; // ledger and control block: -1 by launch shape, 0 first band, 1 control task
+    int tail_rows = -1;     // rows at a grid's end marched in shorter bands (0: none, -1: by launch shape)
+    int tail_band_rows = 0; // their band height (0: a third of the band height)
     struct Tuned { int nx, ny, crest, R; };
     std::vector<Tuned> tuned;   // band rows chosen by itr_hip_autotune, per grid shape
     void* scratch = nullptr;    // this device's static scratch (itr_hip_run_batch_noalloc)
@@ -3261,6 +3266,51 @@ int choose_band_rows(const itr_ctx* ctx, const itr_grid* grids, int n_grids, int
         }
     }
     return std::max(1, std::min(R, G.ny));
+}
+
+// Shorter bands at a grid's end.  A launch dispatches its tasks in band-major order, so the
+// bands at the end of each grid are the launch's last workgroups, and in a launch of
+// several rounds of workgroups the compute units that have no task left wait while they
+// run: with the last rows of a grid (tail_rows or more) in bands of tail_band_rows rows,
+// that tail is made of short workgroups (128 x 512^2 batch, 30-row bands: the last 32 rows
+// in 10-row bands, 365.6 -> 354.8 us per step).  Such a band is a band of a copy of the
+// grid's descriptor with R = tail_band_rows (a shadow grid, placed after the batch's grids
+// in the device array): the step kernel is unchanged.  The grid's own bands
+// 0 .. nb_main-1 end on a multiple of the short height, where the shadow's band `first`
+// starts.  By default (tail_rows -1) a grid in a launch of several rounds gets a tail of
+// about one band (R rows) in bands of R/3 rows.
+struct TailLayout { bool on = false; int nb_main = 0, R2 = 0, first = 0, count = 0; };
+TailLayout tail_layout(const DevGrid& D, const itr_ctx* ctx, bool several_rounds) {
+    TailLayout t;
+    const int R = D.R;
+    const int T = ctx->tail_rows >= 0 ? ctx->tail_rows : (several_rounds ? R : 0);
+    const int R2 = ctx->tail_band_rows > 0 ? ctx->tail_band_rows : (R + 2) / 3;
+    if (T <= 0 || R2 <= 0 || R2 >= R || D.ny - T < R) return t;
+    int nb = (D.ny - T) / R;
+    while (nb > 0 && (nb * R) % R2 != 0) nb--;
+    if (nb <= 0) return t;
+    t.on = true;
+    t.nb_main = nb;
+    t.R2 = R2;
+    t.first = nb * R / R2;
+    t.count = (D.ny + R2 - 1) / R2 - t.first;
+    return t;
+}
+
+// The device array of grid descriptors: the batch's grids, then a shadow for each grid
+// with a tail layout (tails[g], in grid order).
+std::vector<DevGrid> device_grids(const std::vector<DevGrid>& dg, const std::vector<TailLayout>& tails) {
+    std::vector<DevGrid> all = dg;
+    for (size_t g = 0; g < dg.size(); g++) {
+        const DevGrid& D = dg[g];
+        const TailLayout t = g < tails.size() ? tails[g] : TailLayout{};
+        if (!t.on) continue;
+        DevGrid S = D;
+        S.R = t.R2;
+        S.nbands = t.first + t.count;
+        all.push_back(S);
+    }
+    return all;
 }
 
 }  // namespace
@@ -3288,6 +3338,7 @@ struct itr_batch {
     DevParams prm{};                // the first grid's parameters (single-grid users: adjoint, schedules)
     KArgs args{};                   // grids, control and accumulators (the lists add their tasks)
     std::vector<LaunchList> lists;
+    std::vector<TailLayout> tails;  // per grid: shorter bands at its end (fill_lists)
     void* d_grids = nullptr;
     void* d_tasks = nullptr;
     void* d_snap = nullptr;         // CandCtl [2][ncand]
@@ -3427,8 +3478,12 @@ bool use_control_task(int cus, size_t nbands, int waves, size_t lds, int modes, 
 size_t fill_lists(itr_batch* b, const std::vector<DevGrid>& dg, std::vector<int4>& all) {
     all.clear();
     size_t lds_max = 0;
+    // Each list's shape (its grids' bands fit the chip in one round of workgroups or not)
+    // and its grids' tail layouts, then the shadow indices (device_grids' order).
+    std::vector<TailLayout>& tl = b->tails;
+    tl.assign(dg.size(), TailLayout{});
+    std::vector<int> shadow(dg.size(), -1);
     for (LaunchList& L : b->lists) {
-        std::vector<int4> t;
         int waves = 1;
         size_t lds = (size_t)std::max<int64_t>(b->ctx->min_lds, 0), nbands = 0;
         L.modes = 0;
@@ -3440,22 +3495,31 @@ size_t fill_lists(itr_batch* b, const std::vector<DevGrid>& dg, std::vector<int4
             lds = std::max(lds, (D.li ? li_lds_layout(D.R, D.nx, D.special != 0).total
                                       : lds_layout(D.R, D.nx, D.special != 0).total) * sizeof(float));
         }
-        L.ctl_task = b->ctx->control_task >= 0 ? b->ctx->control_task == 1
-                                               : use_control_task(b->ctx->cus, nbands, waves, lds, L.modes, L.li);
+        const bool one_round = use_control_task(b->ctx->cus, nbands, waves, lds, L.modes, L.li);
+        L.ctl_task = b->ctx->control_task >= 0 ? b->ctx->control_task == 1 : one_round;
+        L.block = waves * kWave;
+        L.lds = lds;
+        for (int g : L.grids) tl[g] = tail_layout(dg[g], b->ctx, !one_round);
+    }
+    int nsh = 0;
+    for (size_t g = 0; g < dg.size(); g++)
+        if (tl[g].on) shadow[g] = (int)dg.size() + nsh++;
+    for (LaunchList& L : b->lists) {
+        std::vector<int4> t;
         for (int g : L.grids) {
             const DevGrid& D = dg[g];
             for (int k = 0; k < D.ncand; k++) {
                 if (L.ctl_task) t.push_back(make_int4(g, k, -1, D.cand0 + k));
-                for (int band = 0; band < D.nbands; band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
+                const TailLayout& T = tl[g];
+                for (int band = 0; band < (T.on ? T.nb_main : D.nbands); band++) t.push_back(make_int4(g, k, band, D.cand0 + k));
+                for (int j = 0; j < (T.on ? T.count : 0); j++) t.push_back(make_int4(shadow[g], k, T.first + j, D.cand0 + k));
             }
         }
         order_tasks(t);
         L.task_off = all.size();
         L.ntasks = (int)t.size();
-        L.block = waves * kWave;
-        L.lds = lds;
         all.insert(all.end(), t.begin(), t.end());
-        lds_max = std::max(lds_max, lds);
+        lds_max = std::max(lds_max, L.lds);
     }
     b->ntasks = (int)all.size();
     return lds_max;
@@ -3680,6 +3744,7 @@ itr_status set_callback_flag(itr_batch* b, int has_cb, char* err, size_t errlen)
         h[g] = b->grids[g].d;
     }
     b->prm.has_cb = has_cb;
+    h = device_grids(h, b->tails);
     if (!h.empty()) HIP_TRY(hipMemcpy(b->d_grids, h.data(), sizeof(DevGrid) * h.size(), hipMemcpyHostToDevice));
     return ITR_OK;
 }
@@ -3902,6 +3967,8 @@ itr_status itr_hip_set_option(itr_ctx* ctx, const char* name, int64_t value) {
     else if (n == "adjoint_mode" && value >= 0 && value <= 2) ctx->adjoint_mode = (int)value;
     else if (n == "control_task" && value >= -1 && value <= 1) ctx->control_task = (int)value;
     else if (n == "adjoint_rows" && value >= 0 && value <= 1 << 20) ctx->adjoint_rows = (int)value;
+    else if (n == "tail_rows" && value >= -1 && value <= 1 << 20) ctx->tail_rows = (int)value;
+    else if (n == "tail_band_rows" && value >= 0 && value <= 1 << 20) ctx->tail_band_rows = (int)value;
     else return ITR_EINVAL;
     return ITR_OK;
 }
@@ -4458,7 +4525,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     CandCtl* d_snap;
     StepAcc* d_acc;
     uint32_t* d_status;
-    if ((s = dev_alloc(misc, &d_grids, dgrids.size(), err, errlen)) != ITR_OK) return fail(s);
+    if ((s = dev_alloc(misc, &d_grids, 2 * dgrids.size(), err, errlen)) != ITR_OK) return fail(s);   // room for shadows
     b->d_grids = d_grids;
     size_t task_cap = 0;   // room for any band height (re-tiling never reallocates)
     for (auto& D : dgrids) task_cap += (size_t)std::max(D.ncand, 0) * (size_t)(D.ny + 1);
@@ -4479,7 +4546,8 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
         set_err(err, errlen, "pinned allocation failed");
         return fail(ITR_ENOMEM);
     }
-    bool ok = hipMemcpy(d_grids, dgrids.data(), dgrids.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
+    const std::vector<DevGrid> dall = device_grids(dgrids, b->tails);
+    bool ok = hipMemcpy(d_grids, dall.data(), dall.size() * sizeof(DevGrid), hipMemcpyHostToDevice) == hipSuccess &&
               (tasks.empty() || hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice) == hipSuccess) &&
               (ctl.empty() || (hipMemcpy(d_snap, ctl.data(), ctl.size() * sizeof(CandCtl), hipMemcpyHostToDevice) == hipSuccess &&
                                hipMemcpy(d_snap + ctl.size(), ctl.data(), ctl.size() * sizeof(CandCtl),
@@ -5256,16 +5324,19 @@ itr_status retile(itr_batch* b, const std::vector<int>& Rs, char* err, size_t er
     }
     std::vector<int4> tasks;
     std::vector<LaunchList> keep = b->lists;
+    std::vector<TailLayout> keep_tails = b->tails;
     const size_t lds_max = fill_lists(b, dg, tasks);
     if (lds_max > 160 * 1024 - 1024) {
         b->lists = keep;
+        b->tails = keep_tails;
         b->ntasks = 0;
         for (const LaunchList& L : keep) b->ntasks += L.ntasks;
         set_err(err, errlen, "band of %zu bytes of LDS does not fit", lds_max);
         return ITR_DECLINED;
     }
     for (size_t g = 0; g < b->grids.size(); g++) b->grids[g].d = dg[g];
-    HIP_TRY(hipMemcpy(b->d_grids, dg.data(), dg.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
+    const std::vector<DevGrid> dall = device_grids(dg, b->tails);
+    HIP_TRY(hipMemcpy(b->d_grids, dall.data(), dall.size() * sizeof(DevGrid), hipMemcpyHostToDevice));
     if (!tasks.empty()) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
     b->args.ntasks = b->ntasks;
     bind_lists(b);

@@ -9,7 +9,8 @@
 //     -fhip-fp32-correctly-rounded-divide-sqrt, or the exhaustively verified short
 //     sequences crecip / csqrt below), f32 denormals preserved;
 //   * max/min are selects with the reference's NaN rule (a > b ? a : b), never
-//     fmaxf/fminf; abs is a select that keeps -0.0, never fabsf;
+//     fmaxf/fminf; abs is a select that keeps -0.0 (fabsf only where the sign of a zero
+//     cannot reach the result);
 //   * every expression keeps the reference's association order (left to right).
 #pragma once
 
@@ -39,6 +40,12 @@ __device__ __forceinline__ float canon(float x) { return x == 0.0f ? 0.0f : x; }
 // or tiny, huge, inf, NaN, negative) takes the IEEE operation itself, which the step
 // never needs on a healthy state.
 //
+//
+// The branch to the IEEE operation is taken on a ballot of the guard (a wave-uniform
+// branch on a scalar mask): on the common path that costs one scalar test and branch,
+// where a per-lane branch saves, sets and restores the execution mask around it.
+__device__ __forceinline__ bool any_lane(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
+
 // sqrt: y = rsq(x) (clamped so that x = +0 gives +0), s = x y, s + (x - s^2) y / 2;
 // exact for x = +0 and 2^-102 <= x < inf.
 __device__ __forceinline__ float csqrt(float x) {
@@ -48,7 +55,8 @@ __device__ __forceinline__ float csqrt(float x) {
     const float e = __builtin_fmaf(-s, s, x);
     float r = __builtin_fmaf(e, 0.5f * y, s);
     const uint32_t b = __float_as_uint(x);
-    if (__builtin_expect(b - 0x0C800000u >= 0x7F800000u - 0x0C800000u && b != 0u, 0)) r = __builtin_sqrtf(x);
+    const bool slow = b - 0x0C800000u >= 0x7F800000u - 0x0C800000u && b != 0u;
+    if (__builtin_expect(any_lane(slow), 0)) r = slow ? __builtin_sqrtf(x) : r;
     return r;
 }
 
@@ -58,7 +66,8 @@ __device__ __forceinline__ float crecip(float x) {
     const float e = __builtin_fmaf(-x, r, 1.0f);
     float q = __builtin_fmaf(e, r, r);
     const uint32_t b = __float_as_uint(x);
-    if (__builtin_expect(b - 0x00800000u >= 0x7E000000u - 0x00800000u, 0)) q = 1.0f / x;
+    const bool slow = b - 0x00800000u >= 0x7E000000u - 0x00800000u;
+    if (__builtin_expect(any_lane(slow), 0)) q = slow ? 1.0f / x : q;
     return q;
 }
 
@@ -71,7 +80,8 @@ __device__ __forceinline__ float crecip_lower_bounded(float x) {
     const float r = __builtin_amdgcn_rcpf(x);
     const float e = __builtin_fmaf(-x, r, 1.0f);
     float q = __builtin_fmaf(e, r, r);
-    if (__builtin_expect(!(x < 0x1p126f), 0)) q = 1.0f / x;
+    const bool slow = !(x < 0x1p126f);
+    if (__builtin_expect(any_lane(slow), 0)) q = slow ? 1.0f / x : q;
     return q;
 }
 
@@ -206,7 +216,9 @@ __device__ __forceinline__ CellNew cell_update(float h, float qx, float qy, floa
     if (sabs(qx2) < k.q_min) qx2 = 0.0f;
     if (sabs(qy2) < k.q_min) qy2 = 0.0f;
     const float c = csqrt(k.g * h3);
-    o.s = (sabs(qx2) * r3 + c) * k.inv_dx + (sabs(qy2) * r3 + c) * k.inv_dy;
+    // |qx2| with the sign bit cleared rather than sabs: they differ only at -0 (and in a
+    // NaN's sign), and (-0) r3 + c = (+0) r3 + c for the finite r3 > 0 and c >= +0 here.
+    o.s = (__builtin_fabsf(qx2) * r3 + c) * k.inv_dx + (__builtin_fabsf(qy2) * r3 + c) * k.inv_dy;
     bad = bad || !(qm == qm);
     o.qm = qm;
     // canon_zero (real.rs:56-58) of the new state is the identity here, so it is not

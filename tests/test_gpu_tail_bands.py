@@ -1,6 +1,6 @@
-"""Shorter bands at a grid's end (the tail_rows / tail_band_rows options, on by default in
-launches of several rounds of workgroups): the band layout must not change a bit of the
-run.  Each case sets a layout whose short bands start on a multiple of their height, with
+"""Shorter bands at a grid's end (the tail_rows / tail_band_rows options) and launch
+lists split by candidates (list_split), both by default in launches of several rounds of
+workgroups: the layout must not change a bit of the run.  Each case sets a layout whose short bands start on a multiple of their height, with
 a short last band, and compares against the CPU oracle (the reference's f32 fused kernel
 restated) or against the same batch run without short bands."""
 from __future__ import annotations
@@ -132,22 +132,28 @@ def test_tail_bands_locked_restart():
         assert r.dt_fallback_t == t[syncs[7]]
 
 
-def test_default_tail_in_a_launch_of_many_rounds():
-    """The default layout: a batch whose bands take many rounds of workgroups gets short
-    end bands by itself (here 4-row bands and 2-row ones after); its run equals, bit for
-    bit, the same batch with short bands switched off and with an explicit layout."""
+def test_default_layout_in_a_launch_of_many_rounds():
+    """The defaults for a batch whose bands take many rounds of workgroups: its launch list
+    is split by candidates into two lists on two streams, and a list that is not split gets
+    short end bands.  Every layout gives the same bits: defaults, whole list without short
+    bands, whole list with its default short bands (4-row bands, then 2-row ones), three
+    lists with an explicit short-band layout, and four lists."""
     w = sc.config2()
     cands = (w.candidates * 8)[:400]
     p = copy.copy(w.params)
     p.duration_s = 600.0
-    _say(f"case: {len(cands)} candidates x {w.base.nx}x{w.base.ny}, band_rows 4, default layout vs none vs explicit")
+    _say(f"case: {len(cands)} candidates x {w.base.nx}x{w.base.ny}, band_rows 4, launch-list splits and end-band layouts")
     runs = {}
-    for tag, T, R2 in (("default", -1, 0), ("none", 0, 0), ("explicit", 9, 2)):
+    for tag, split, T, R2 in (("default", 0, -1, 0), ("whole", 1, 0, 0), ("whole, short end bands", 1, -1, 0),
+                              ("3 lists, explicit", 3, 9, 2), ("4 lists", 4, 0, 0)):
         s = _solver(4, T, R2, chunk=16)
+        s.set_option("list_split", split)
         runs[tag] = s.run_batch(p, cands, w.monitors, want_state=True)
         s.close()
-    for tag in ("default", "explicit"):
-        for i, (a, b) in enumerate(zip(runs[tag], runs["none"])):
+    for tag in runs:
+        if tag == "whole":
+            continue
+        for i, (a, b) in enumerate(zip(runs[tag], runs["whole"])):
             assert a.steps == b.steps, f"{tag} cand {i}: steps"
             for k in ("depth", "qx", "qy", "monitor_max", "monitor_tmax", "ledger"):
                 x, y = np.asarray(getattr(a, k)), np.asarray(getattr(b, k))

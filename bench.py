@@ -98,7 +98,9 @@ def _run_cands(po, ot, params, edits, threads, monitors):
 
 def _pmc_pass(counter, steps, band_rows):
     """One rocprofv3 counter pass over a short run of this benchmark (plus the copy
-    probe); returns {kernel name: [per-dispatch values]} or None if it did not run."""
+    probe); returns {kernel name: [per-dispatch values]} or None if it did not run.  The
+    pass runs the batch as one launch list (list_split 1), so that every dispatch covers
+    every candidate; the per-cell-step figures do not depend on the split."""
     import csv
     import shutil
     import tempfile
@@ -109,7 +111,7 @@ def _pmc_pass(counter, steps, band_rows):
     cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc",
            "--", sys.executable, os.path.abspath(__file__), "--steps", str(steps), "--warmup", "2", "--no-cpu",
            "--no-traffic", "--copy-probe", "--no-adjoint", "--no-ladder", "--band-rows", str(band_rows),
-           "--advance", "0", "--guard-tol", "0", "--no-graph"]
+           "--advance", "0", "--guard-tol", "0", "--no-graph", "--list-split", "1"]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
     path = None
@@ -520,6 +522,7 @@ def main():
                     help="launch every step directly (no hipGraph chunks): for runs under rocprofv3 --kernel-trace, "
                          "whose tracing of graph launches ends the process on this ROCm")
     ap.add_argument("--copy-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--list-split", type=int, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-cands", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=600)
@@ -550,6 +553,8 @@ def main():
     solver = GpuSolver(0, band_rows=a.band_rows)
     if a.min_lds:
         solver.set_option("min_lds", a.min_lds)
+    if a.list_split is not None:
+        solver.set_option("list_split", a.list_split)
     # The guard's baseline: the unedited terrain over the whole storm (objective.rs:132-144).
     guard = None
     base_steps = None
@@ -623,7 +628,8 @@ def main():
                         f"infiltration 5 mm/h, guard {a.guard_tol:g} m against the unedited baseline; timed window "
                         f"after {n_adv} untimed steps",
             "candidates": a.cands, "nx": a.n, "ny": a.n, "band_rows": st["band_rows"],
-            "waves_per_task": st["waves_per_task"], "device": solver.adapter_info(),
+            "waves_per_task": st["waves_per_task"], "launch_lists": st["launch_lists"],
+            "device": solver.adapter_info(),
             "autotuned": tuned is not None,
             "hbm_fraction": achieved / HBM_PEAK, "prep_s": round(t_prep, 2),
             # t of the running candidates before the last untimed steps and at the window's end
@@ -645,6 +651,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
+            # per step: with the candidates split into launch lists on their own streams, a
+            # step is one launch of each list, side by side (HIP events on the library stream
+            # after the lists' join)
             "kernel_avg_us": kernel_s * 1e6,
             "bytes_per_cell_step": BYTES_PER_CELL_STEP,
             "cells_per_launch": cells_per_launch,

@@ -225,6 +225,9 @@ typedef struct {
     uint64_t syncs;      /* host round trips */
     int32_t band_rows, waves_per_task, steps_per_chunk, use_graph;
     double sim_s;        /* sum over candidates of the simulated time advanced [s] */
+    int32_t launch_lists; /* step launch lists, each on its own stream (a class of grids or a
+                            part of one, split by candidates) */
+    int32_t reserved;
 } itr_stats;
 
 /* GpuSolver::new (lib.rs:85-124).  device: HIP device ordinal. */

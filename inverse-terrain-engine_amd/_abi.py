@@ -71,7 +71,7 @@ class StatsC(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("device_ms", C.c_double), ("cell_steps", C.c_double),
                 ("launches", C.c_uint64), ("syncs", C.c_uint64), ("band_rows", C.c_int32),
                 ("waves_per_task", C.c_int32), ("steps_per_chunk", C.c_int32), ("use_graph", C.c_int32),
-                ("sim_s", C.c_double)]
+                ("sim_s", C.c_double), ("launch_lists", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ObjectiveC(C.Structure):

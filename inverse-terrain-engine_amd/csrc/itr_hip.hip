@@ -3279,13 +3279,18 @@ int choose_band_rows(const itr_ctx* ctx, const itr_grid* grids, int n_grids, int
 // in the device array): the step kernel is unchanged.  The grid's own bands
 // 0 .. nb_main-1 end on a multiple of the short height, where the shadow's band `first`
 // starts.  By default (tail_rows -1) a grid in a launch of several rounds gets a tail of
-// about one band (R rows) in bands of R/3 rows.
+// about one band (R rows) in bands of the largest divisor of R up to R/3 rows.
 struct TailLayout { bool on = false; int nb_main = 0, R2 = 0, first = 0, count = 0; };
 TailLayout tail_layout(const DevGrid& D, const itr_ctx* ctx, bool several_rounds) {
     TailLayout t;
     const int R = D.R;
     const int T = ctx->tail_rows >= 0 ? ctx->tail_rows : (several_rounds ? R : 0);
-    const int R2 = ctx->tail_band_rows > 0 ? ctx->tail_band_rows : (R + 2) / 3;
+    int R2 = ctx->tail_band_rows;
+    if (R2 <= 0) {   // the largest divisor of R up to a third of it (so that the tail starts on one)
+        R2 = 0;
+        for (int d = (R + 2) / 3; d >= 2 && R2 == 0; d--)
+            if (R % d == 0) R2 = d;
+    }
     if (T <= 0 || R2 <= 0 || R2 >= R || D.ny - T < R) return t;
     int nb = (D.ny - T) / R;
     while (nb > 0 && (nb * R) % R2 != 0) nb--;
@@ -3354,9 +3359,11 @@ struct itr_batch {
     // Captured chunks of step launches, longest first: graph[0] holds the batch's chunk
     // (graph_steps[0] launches), the others 16, 8 and 4 launches where shorter, so that an
     // advance by a few steps also runs as graphs (back-to-back graph launches start each
-    // step kernel sooner than separate launches do).
+    // step kernel sooner than separate launches do).  One graph per launch list, launched
+    // on the list's stream, so that the lists' chunks follow one another on each stream
+    // without a join between chunks.
     static constexpr int kGraphs = 4;
-    hipGraphExec_t graph[kGraphs] = {};
+    hipGraphExec_t graph[kGraphs][kMaxLists] = {};
     int graph_steps[kGraphs] = {};
     itr_stats stats{};
     std::vector<std::vector<double>> cb_mon;   // scratch for callbacks
@@ -3371,8 +3378,10 @@ namespace {
 
 void drop_graphs(itr_batch* b) {
     for (int i = 0; i < itr_batch::kGraphs; i++) {
-        if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
-        b->graph[i] = nullptr;
+        for (int l = 0; l < kMaxLists; l++) {
+            if (b->graph[i][l]) (void)hipGraphExecDestroy(b->graph[i][l]);
+            b->graph[i][l] = nullptr;
+        }
         b->graph_steps[i] = 0;
     }
 }
@@ -3832,52 +3841,64 @@ void launch_list(const LaunchList& L, hipStream_t st, uint32_t lk) {
     }
 }
 
-// Enqueue n step launches of every list, launch indices from lk0 (mod 4).  Lists after
-// the first run on their own streams between a fork and a join on the context's stream
-// (also under graph capture, where they become parallel branches of the graph).
-itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
-    itr_ctx* ctx = b->ctx;
+// The stream of launch list l: the context's stream for the first list, an auxiliary
+// stream (default priority) for each further one.
+hipStream_t list_stream(const itr_batch* b, int l) { return l == 0 ? b->ctx->stream : b->ctx->aux[l - 1]; }
+
+// Lists after the first run on their own streams between a fork from and a join to the
+// context's stream (around a whole run of launches, not each chunk).
+itr_status fork_lists(itr_batch* b, char* err, size_t errlen) {
     const int nl = (int)b->lists.size();
-    if (nl > 1) {
-        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
-        for (int i = 1; i < nl; i++) HIP_TRY(hipStreamWaitEvent(ctx->aux[i - 1], ctx->ev_fork, 0));
+    if (nl <= 1) return ITR_OK;
+    HIP_TRY(hipEventRecord(b->ctx->ev_fork, b->ctx->stream));
+    for (int i = 1; i < nl; i++) HIP_TRY(hipStreamWaitEvent(b->ctx->aux[i - 1], b->ctx->ev_fork, 0));
+    return ITR_OK;
+}
+itr_status join_lists(itr_batch* b, char* err, size_t errlen) {
+    const int nl = (int)b->lists.size();
+    for (int i = 1; i < nl; i++) {
+        HIP_TRY(hipEventRecord(b->ctx->ev_join[i - 1], b->ctx->aux[i - 1]));
+        HIP_TRY(hipStreamWaitEvent(b->ctx->stream, b->ctx->ev_join[i - 1], 0));
     }
-    for (int i = 0; i < nl; i++) {
+    return ITR_OK;
+}
+
+// Enqueue n step launches of every list on its own stream, launch indices from lk0
+// (mod 4); the caller forks and joins the streams.
+itr_status enqueue_steps(itr_batch* b, int n, uint64_t lk0, char* err, size_t errlen) {
+    for (int i = 0; i < (int)b->lists.size(); i++) {
         const LaunchList& L = b->lists[i];
         if (L.ntasks == 0) continue;
-        const hipStream_t st = i == 0 ? ctx->stream : ctx->aux[i - 1];
-        for (int s = 0; s < n; s++) launch_list(L, st, (uint32_t)((lk0 + (uint64_t)s) & 3u));
+        for (int s = 0; s < n; s++) launch_list(L, list_stream(b, i), (uint32_t)((lk0 + (uint64_t)s) & 3u));
     }
     HIP_TRY(hipGetLastError());
-    for (int i = 1; i < nl; i++) {
-        HIP_TRY(hipEventRecord(ctx->ev_join[i - 1], ctx->aux[i - 1]));
-        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join[i - 1], 0));
-    }
     return ITR_OK;
 }
 
 // n launches continuing the batch's launch sequence: while the sequence is at a multiple
 // of 4, the longest captured chunk that fits; single launches otherwise.
 itr_status launch_steps(itr_batch* b, int64_t n, char* err, size_t errlen) {
+    itr_status s = fork_lists(b, err, errlen);
+    if (s != ITR_OK) return s;
     while (n > 0) {
         int gi = -1;
         if ((b->launches & 3u) == 0)
             for (int i = 0; i < itr_batch::kGraphs && gi < 0; i++)
-                if (b->graph[i] && b->graph_steps[i] <= n) gi = i;
+                if (b->graph_steps[i] > 0 && b->graph_steps[i] <= n) gi = i;
         if (gi >= 0) {
-            HIP_TRY(hipGraphLaunch(b->graph[gi], b->ctx->stream));
+            for (int l = 0; l < (int)b->lists.size(); l++)
+                if (b->graph[gi][l]) HIP_TRY(hipGraphLaunch(b->graph[gi][l], list_stream(b, l)));
             b->launches += (uint64_t)b->graph_steps[gi];
             n -= b->graph_steps[gi];
         } else {
             int64_t k = n;   // single launches: the remainder, or up to the next multiple of 4
-            if (b->graph[0] && (b->launches & 3u) != 0) k = std::min<int64_t>(n, 4 - (int64_t)(b->launches & 3u));
-            itr_status s = enqueue_steps(b, (int)k, b->launches, err, errlen);
-            if (s != ITR_OK) return s;
+            if (b->graph_steps[0] > 0 && (b->launches & 3u) != 0) k = std::min<int64_t>(n, 4 - (int64_t)(b->launches & 3u));
+            if ((s = enqueue_steps(b, (int)k, b->launches, err, errlen)) != ITR_OK) return s;
             b->launches += (uint64_t)k;
             n -= k;
         }
     }
-    return ITR_OK;
+    return join_lists(b, err, errlen);
 }
 
 // The current control copy of every candidate.
@@ -3895,22 +3916,34 @@ CandCtl* cur_ctl_dev(itr_batch* b, int c) {
 }
 
 // Capture n launches (launch indices 0..n-1 mod 4) in a hipGraph.
+// One graph per launch list of n step launches (launch indices 0 .. n-1 mod 4), captured
+// on the list's stream; out[l] stays null for a list without tasks.
 itr_status capture_steps(itr_batch* b, int n, hipGraphExec_t* out, char* err, size_t errlen) {
-    hipGraph_t g;
-    HIP_TRY(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
-    itr_status ls = enqueue_steps(b, n, 0, err, errlen);
-    hipError_t ce = hipStreamEndCapture(b->ctx->stream, &g);
-    if (ls != ITR_OK) return ls;
-    if (ce != hipSuccess) {
-        set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
-        return ITR_EDEVICE;
-    }
-    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-        *out = nullptr;
-        set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
-        return ITR_EDEVICE;
+    for (int l = 0; l < (int)b->lists.size(); l++) {
+        const LaunchList& L = b->lists[l];
+        if (L.ntasks == 0) continue;
+        const hipStream_t st = list_stream(b, l);
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < n; s++) launch_list(L, st, (uint32_t)(s & 3));
+        const hipError_t le = hipGetLastError();
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess) {
+            if (ce == hipSuccess) (void)hipGraphDestroy(g);
+            set_err(err, errlen, "graph capture: %s", hipGetErrorString(le));
+            return ITR_EDEVICE;
+        }
+        if (ce != hipSuccess) {
+            set_err(err, errlen, "graph capture failed: %s", hipGetErrorString(ce));
+            return ITR_EDEVICE;
+        }
+        hipError_t ie = hipGraphInstantiate(&out[l], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            out[l] = nullptr;
+            set_err(err, errlen, "graph instantiate failed: %s", hipGetErrorString(ie));
+            return ITR_EDEVICE;
+        }
     }
     return ITR_OK;
 }
@@ -3918,14 +3951,14 @@ itr_status capture_steps(itr_batch* b, int n, hipGraphExec_t* out, char* err, si
 // Capture chunks of K launches and, where shorter than K, of 16, 8 and 4 launches;
 // K = 0 drops them.
 itr_status prepare_graph(itr_batch* b, int K, char* err, size_t errlen) {
-    if (K == b->graph_steps[0] && (K == 0 || b->graph[0])) return ITR_OK;
+    if (K == b->graph_steps[0] && (K == 0 || b->graph_steps[0] > 0)) return ITR_OK;
     drop_graphs(b);
     if (K == 0 || b->ntasks == 0) return ITR_OK;
     int sizes[itr_batch::kGraphs] = {K, 16, 8, 4};
     int ng = 0;
     for (int i = 0; i < itr_batch::kGraphs; i++) {
         if (i > 0 && sizes[i] >= K) continue;
-        itr_status s = capture_steps(b, sizes[i], &b->graph[ng], err, errlen);
+        itr_status s = capture_steps(b, sizes[i], b->graph[ng], err, errlen);
         if (s != ITR_OK) {
             drop_graphs(b);
             return s;
@@ -4636,6 +4669,7 @@ itr_status batch_create_impl(itr_ctx* ctx, const itr_solver_params* p_call, cons
     b->stats.band_rows = dgrids.empty() ? 0 : dgrids[0].R;
     b->stats.waves_per_task = 1;
     for (const LaunchList& L : b->lists) b->stats.waves_per_task = std::max(b->stats.waves_per_task, L.block / kWave);
+    b->stats.launch_lists = (int32_t)b->lists.size();
     *out = b;
     return ITR_OK;
 }

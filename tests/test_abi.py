@@ -57,7 +57,7 @@ int main(void) {
   F(itr_design, bbox) F(itr_design, origin_x) F(itr_design, volumes) F(itr_design, level) F(itr_design, fine_ny)
   F(itr_replay, ring) F(itr_replay, ring_steps) F(itr_replay, tol_q) F(itr_replay, n_syncs_out)
   F(itr_run_out, steps) F(itr_run_out, monitor_max) F(itr_run_out, max_ledger) F(itr_run_out, dt_fallback_t)
-  F(itr_stats, launches) F(itr_stats, use_graph) F(itr_stats, sim_s)
+  F(itr_stats, launches) F(itr_stats, use_graph) F(itr_stats, sim_s) F(itr_stats, launch_lists)
   F(itr_objective, asset_start) F(itr_objective, tau_m) F(itr_adjoint_out, dJ_dz_cell) F(itr_adjoint_out, steps)
   F(itr_adjoint_out, replay_mismatches) F(itr_adjoint_out, reverse_ms) F(itr_tune_result, us_per_step)
   return 0;

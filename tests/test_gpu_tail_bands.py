@@ -73,12 +73,15 @@ def test_tail_bands_hll(case):
     assert_run_equal(g, oracle_run(base, p, monitors=mon), name)
 
 
+@pytest.mark.parametrize("split", [1, 3], ids=["one-list", "three-lists"])
 @pytest.mark.parametrize("li", [False, True], ids=["hll", "local-inertial"])
-def test_tail_bands_stage_inflow(li):
+def test_tail_bands_stage_inflow(li, split):
     """Stage and Inflow ghosts (the special kernel, whose ghosts are staged per band) and
-    edited candidates, with 6-row bands to row 24 and 4-row bands after."""
-    _say(f"case: short end bands, stage + inflow boundaries, {'local inertial' if li else 'HLL'}")
+    edited candidates, with 6-row bands to row 24 and 4-row bands after; the three
+    candidates in one launch list or in three (one per stream)."""
+    _say(f"case: short end bands, stage + inflow boundaries, {'local inertial' if li else 'HLL'}, list_split {split}")
     s = _solver(6, 8, 4, chunk=5)
+    s.set_option("list_split", split)
     base = _equivalence_base()
     p = sc.SolverParams(900.0, 60.0, infil_capacity=10.0 / 1000.0 / 3600.0,
                         rain=sc.mm_h([[0.0, 0.0], [300.0, 0.0], [400.0, 80.0], [700.0, 0.0]]))

@@ -5487,7 +5487,7 @@ itr_status itr_hip_autotune(itr_ctx* ctx, const itr_solver_params* p, const itr_
     // Band heights tried: every height to 16 (the launch's tail wave depends on the
     // band count modulo the resident workgroup slots), then coarser steps, with multiples
     // of 3 among them (the last wave of a 512-column row marches 3 row groups).
-    static const int cand_R[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 21, 24, 27, 28, 30, 32};
+    static const int cand_R[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 18, 20, 21, 24, 27, 28, 30, 32, 33};
     std::vector<char> done(n_grids, 0);
     for (int gi = 0; gi < n_grids && s == ITR_OK; gi++) {
         const int g = order[gi];

@@ -66,11 +66,14 @@ def test_config2_population_with_guard(solver):
     assert 0 < n_abort < len(runs), "the workload should exercise both outcomes"
 
 
-def test_incumbent_abort_from_sync_callback(solver):
+@pytest.mark.parametrize("split", [0, 3], ids=["default-lists", "three-lists"])
+def test_incumbent_abort_from_sync_callback(solver, split):
     """Elitist incumbent abort: at every sync point the host computes the running lower
     bound J(running maxima, the edit's cut/fill) with the library's J and aborts the
     candidate when it reaches the incumbent; equal to the oracle's GuardObserver with
-    has_incumbent (abort sync, steps, state, monitors and the bound, bitwise)."""
+    has_incumbent (abort sync, steps, state, monitors and the bound, bitwise).  Also with
+    the candidates in three launch lists (streams), whose candidates pause at their sync
+    points and resume independently."""
     w = sc.config2(n_cand=12, seed=7)
     assets = [(list(a), 0.02, 1.0) for a in w.asset_pos]
     objv = ob.Objective([ob.Asset(np.asarray(a), th, wt) for a, th, wt in assets], 0.01, earth_weight=1e-3)
@@ -84,7 +87,11 @@ def test_incumbent_abort_from_sync_callback(solver):
         lb = objv.j(mon_max, e.cut_m3, e.fill_m3)
         return lb if lb >= inc else None
 
-    runs = solver.run_batch(w.params, w.candidates, w.monitors, on_sync=cb)
+    solver.set_option("list_split", split)
+    try:
+        runs = solver.run_batch(w.params, w.candidates, w.monitors, on_sync=cb)
+    finally:
+        solver.set_option("list_split", 0)
     obs = [{"assets": assets, "tau": 0.01, "earth_weight": 1e-3, "cut_m3": e.cut_m3, "fill_m3": e.fill_m3,
             "incumbent": inc} for e in w.edits]
     orc = oracle_many(w.base, w.params, w.edits, w.monitors, obs)

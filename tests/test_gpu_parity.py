@@ -91,9 +91,11 @@ def test_batch_matches_single_and_oracle(solver):
     assert not np.array_equal(batch[0].depth, batch[1].depth), "edits should change results"
 
 
-def test_guard_abort_matches_oracle(solver):
-    """Exact device-side guard abort (objective.rs:132-144) on config-2-like candidates."""
-    _say("case: guard abort, 10 candidates on 60x48")
+@pytest.mark.parametrize("split", [0, 4], ids=["default-lists", "four-lists"])
+def test_guard_abort_matches_oracle(solver, split):
+    """Exact device-side guard abort (objective.rs:132-144) on config-2-like candidates,
+    also with the candidates in four launch lists (streams)."""
+    _say(f"case: guard abort, 10 candidates on 60x48, list_split {split}")
     nx, ny, dx = 60, 48, 5.0
     rng = sc.SplitMix64(21)
     dem = sc.hills_dem(nx, ny, dx, rng, 6, n_gullies=2)
@@ -105,7 +107,11 @@ def test_guard_abort_matches_oracle(solver):
     guard = {"baseline_max": base_run.monitor_max, "guard_pos": gpos, "tol": 0.002}
     edits = [tr.materialize(sc.random_primitives(rng, 3, geo, nx, ny), geo, nx, ny, None, 1.0) for _ in range(10)]
     terrs = [tr.PreparedTerrain(base).apply_edit(e) for e in edits]
-    runs = solver.run_batch(p, terrs, mon, guard=guard)
+    solver.set_option("list_split", split)
+    try:
+        runs = solver.run_batch(p, terrs, mon, guard=guard)
+    finally:
+        solver.set_option("list_split", 0)
     n_abort = 0
     for i, (r, e) in enumerate(zip(runs, edits)):
         obs = {"baseline_max": base_run.monitor_max, "guard_pos": gpos, "tol": 0.002, "require": True}
